@@ -1,0 +1,76 @@
+/*
+ * Extensions of libvie_hip.so beyond the reference's interface.  Each entry
+ * point below is new: the reference has no device-resident solve (its GMRES
+ * calls the operator with host pointers, vie_iterative.c:146-148), no
+ * precision choice and no batching.
+ */
+#ifndef VIE_HIP_H
+#define VIE_HIP_H
+#include "dielectric_fft.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  VIE_HIP_FP64 = 0,  /* every matvec, basis vector and reduction in FP64 */
+  VIE_HIP_MIXED = 1  /* FP32 matvecs and Krylov basis inside each cycle; FP64
+                        true residual b - A x, solution update, dot-product
+                        accumulation and Givens least squares */
+} vie_hip_precision;
+
+typedef struct {
+  vie_hip_precision precision;
+  /* MIXED: an FP32 cycle ends once |g_{j+1}| <= max(target, inner_reduction * beta);
+     0 selects the default (1e-5). */
+  double inner_reduction;
+} vie_hip_solve_options;
+
+/* Device-resident solve of a system prepared by dielectric_fft_prepare[_pec]
+   (new; the drop-in for vie_gmres at dielectric_wire.c:1303 and
+   dielectric_solver.c:526).  Same contract and report as vie_gmres. */
+int dielectric_fft_solve(dielectric_fft_system *system, const VIE_CDOUBLE *rhs, VIE_CDOUBLE *x,
+                         const vie_gmres_options *options, vie_gmres_report *report);
+/* Same for a bare volume plan of vie_fft_create. */
+int vie_fft_solve(vie_fft_plan *plan, const VIE_CDOUBLE *rhs, VIE_CDOUBLE *x,
+                  const vie_gmres_options *options, vie_gmres_report *report);
+/* Precision used by dielectric_fft_solve / vie_fft_solve / vie_gmres dispatch
+   (default VIE_HIP_FP64, or VIE_HIP_PRECISION=mixed in the environment). */
+void vie_hip_set_solve_options(const vie_hip_solve_options *options);
+void vie_hip_get_solve_options(vie_hip_solve_options *options);
+
+/* HIP backend storage cap for dielectric_fft_prepare (default 64 GiB or
+   VIE_HIP_BUDGET_BYTES); the reference's is the 512 MiB macro. */
+size_t dielectric_fft_budget(void);
+void dielectric_fft_set_budget(size_t bytes);
+/* Forces the next apply/solve to upload C/B/D again after entry writes. */
+void dielectric_fft_blocks_changed(dielectric_fft_system *system);
+/* Underlying volume plan of a system (NULL when it has no cells). */
+vie_fft_plan *dielectric_fft_plan(dielectric_fft_system *system);
+
+/* Last error text of this thread ("" when none). */
+const char *vie_hip_last_error(void);
+/* Number of HIP devices visible (0 without a GPU; never initialises one
+   when the runtime reports none). */
+int vie_hip_device_count(void);
+
+/* ---- resident work (for benchmarks and servers) ----
+   Right-hand side and solution stay in device memory between calls. */
+int dielectric_fft_upload_rhs(dielectric_fft_system *system, const VIE_CDOUBLE *rhs);
+int dielectric_fft_solve_resident(dielectric_fft_system *system, const vie_gmres_options *options,
+                                  const vie_hip_solve_options *solve, vie_gmres_report *report);
+int dielectric_fft_download_solution(dielectric_fft_system *system, VIE_CDOUBLE *x);
+/* Times `repeats` device-resident operator applications (input resident,
+   Jacobi fused) at the given precision with HIP events on the system's
+   stream.  pass_ms (optional, 6 entries) receives the average per launch of
+   the five volume passes and the conductor coupling. */
+int dielectric_fft_time_apply(dielectric_fft_system *system, vie_hip_precision precision, int repeats,
+                              double *ms_per_apply, double *pass_ms);
+/* Algorithmic HBM bytes of one fused apply at a precision (per pass and total),
+   from the pruned five-pass model: see the implementation for the formula. */
+int dielectric_fft_apply_bytes(const dielectric_fft_system *system, vie_hip_precision precision,
+                               double *pass_bytes, double *total_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1435 @@
+// C-ABI of libvie_hip.so: the reference's vie_fft.h / vie_iterative.h /
+// dielectric_fft.h entry points plus the device-resident solve (vie_hip.h).
+//
+// Host code here owns validation, budgets, layout maps and control flow; all
+// arithmetic on the operator and the Krylov vectors runs in the HIP kernels of
+// vie_passes.hip, spectrum.hip and krylov.hip.  There is no CPU compute path:
+// without a usable GPU every create/prepare fails with a diagnostic.
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/vie_hip.h"
+#include "krylov.h"
+#include "vie_kernels.h"
+
+using vie::c32;
+using vie::c64;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+static void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+extern "C" const char* vie_hip_last_error(void) { return g_last_error.c_str(); }
+
+#define HIP_OK(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return 0;                                                                   \
+    }                                                                             \
+  } while (0)
+
+// ------------------------------------------------------------------ globals
+enum { BACKEND_AUTO, BACKEND_HIP };
+static int g_requested = BACKEND_AUTO;
+static int g_last_used = -1;
+static vie_hip_solve_options g_solve = {VIE_HIP_FP64, 0.0};
+static bool g_solve_init = false;
+static size_t g_budget = 0;
+static std::mutex g_mutex;
+
+static const double kPi = 3.141592653589793238462643383279502884;
+
+static void init_solve_defaults() {
+  if (g_solve_init) return;
+  g_solve_init = true;
+  const char* p = getenv("VIE_HIP_PRECISION");
+  if (p && (!strcmp(p, "mixed") || !strcmp(p, "MIXED"))) g_solve.precision = VIE_HIP_MIXED;
+}
+
+extern "C" int vie_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+static int ensure_device() {
+  int n = vie_hip_device_count();
+  if (n <= 0) {
+    set_error("no HIP device available");
+    return 0;
+  }
+  return 1;
+}
+
+// ------------------------------------------------------------------ device buffers
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t b) {
+    if (bytes >= b && p) return 1;
+    release();
+    if (b == 0) b = 16;
+    if (hipMalloc(&p, b) != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc of %zu bytes failed", b);
+      return 0;
+    }
+    bytes = b;
+    return 1;
+  }
+  template <typename T> T* as() const { return (T*)p; }
+};
+
+static bool mul_ok(size_t a, size_t b, size_t* out) {
+  if (b && a > SIZE_MAX / b) return false;
+  *out = a * b;
+  return true;
+}
+static bool add_ok(size_t a, size_t b, size_t* out) {
+  if (a > SIZE_MAX - b) return false;
+  *out = a + b;
+  return true;
+}
+
+// ------------------------------------------------------------------ plan
+struct vie_fft_plan {
+  vie::Geom g{};
+  size_t cells = 0;
+  size_t bytes = 0;
+  double h = 0, k = 0;
+  std::vector<size_t> map;       // cell -> box index
+  std::vector<c64> eps;          // per cell
+  c64 self[3] = {};
+  vie::SpectrumResult spec_info;
+  bool eps_dirty = true;
+  hipStream_t stream = nullptr;
+  // device
+  DevBuf spec64, spec32, work, tw64, tw32, eps64, eps32, map_dev;
+  DevBuf io_user, io_sys_in, io_sys_out;  // host-pointer apply staging
+  dielectric_fft_system* wrap = nullptr;  // c = 0 system around this plan, for solves
+  ~vie_fft_plan();
+  size_t nbox() const { return g.nbox(); }
+  size_t oct() const { return (size_t)(g.Px / 2 + 1) * (g.Py / 2 + 1) * (g.Pz / 2 + 1); }
+  template <typename T> vie::VolumeView<T> view();
+};
+
+template <> vie::VolumeView<double> vie_fft_plan::view<double>() {
+  vie::VolumeView<double> v;
+  v.g = g;
+  size_t a_elems = 3 * (size_t)g.nx * g.ny * g.Pz;
+  v.A = work.as<c64>();
+  v.B = work.as<c64>() + a_elems;
+  v.spec = spec64.as<c64>();
+  v.eps = eps64.as<c64>();
+  v.tw = tw64.as<c64>();
+  for (int i = 0; i < 3; i++) v.self[i] = self[i];
+  return v;
+}
+template <> vie::VolumeView<float> vie_fft_plan::view<float>() {
+  vie::VolumeView<float> v;
+  v.g = g;
+  size_t a_elems = 3 * (size_t)g.nx * g.ny * g.Pz;
+  v.A = work.as<c32>();
+  v.B = work.as<c32>() + a_elems;
+  v.spec = spec32.as<c32>();
+  v.eps = eps32.as<c32>();
+  v.tw = tw32.as<c32>();
+  for (int i = 0; i < 3; i++) v.self[i] = c32{(float)self[i].re, (float)self[i].im};
+  return v;
+}
+
+static size_t pad_extent(size_t n) {
+  size_t p = 1;
+  while (p < 2 * n - 1) p *= 2;
+  return p;
+}
+
+static bool passive(c64 e) { return std::isfinite(e.re) && std::isfinite(e.im) && e.re >= 1 && e.im <= 0; }
+
+// Device bytes of a plan, counted before anything is allocated.
+static bool plan_bytes(const vie::Geom& g, size_t cells, size_t* out) {
+  size_t nbox = g.nbox(), oct = (size_t)(g.Px / 2 + 1) * (g.Py / 2 + 1) * (g.Pz / 2 + 1), t, b = sizeof(vie_fft_plan);
+  size_t work = 3 * (size_t)g.nx * g.ny * g.Pz + 3 * (size_t)g.nx * g.Py * g.Pz;
+  if (!mul_ok(oct, 6 * (16 + 8), &t) || !add_ok(b, t, &b)) return false;
+  if (!mul_ok(work, 16, &t) || !add_ok(b, t, &b)) return false;
+  if (!mul_ok(nbox, 16 + 8, &t) || !add_ok(b, t, &b)) return false;
+  if (!mul_ok(cells, 2 * sizeof(size_t) + sizeof(c64), &t) || !add_ok(b, t, &b)) return false;
+  if (!add_ok(b, ((size_t)g.Px + g.Py + g.Pz) * 24, &b)) return false;
+  // host-pointer apply staging: user vector + two system vectors (FP64)
+  if (!mul_ok(3 * cells + 6 * nbox, 16, &t) || !add_ok(b, t, &b)) return false;
+  *out = b;
+  return true;
+}
+
+static void twiddles(int P, std::vector<c64>& out) {
+  for (int m = 0; m < P; m++) {
+    double s, c;
+    if (4 * m % P == 0) {
+      static const double C[4] = {1, 0, -1, 0}, S[4] = {0, -1, 0, 1};
+      int q = (4 * m / P) & 3;
+      c = C[q];
+      s = S[q];
+    } else {
+      c = cos(2 * kPi * m / P);
+      s = -sin(2 * kPi * m / P);
+    }
+    out.push_back(c64{c, s});
+  }
+}
+
+static int plan_upload_eps(vie_fft_plan* p) {
+  if (!p->eps_dirty) return 1;
+  size_t nbox = p->nbox();
+  std::vector<c64> box(nbox, c64{1, 0});
+  for (size_t c = 0; c < p->cells; c++) box[p->map[c]] = p->eps[c];
+  std::vector<c32> box32(nbox);
+  for (size_t i = 0; i < nbox; i++) box32[i] = c32{(float)box[i].re, (float)box[i].im};
+  HIP_OK(hipMemcpyAsync(p->eps64.p, box.data(), nbox * sizeof(c64), hipMemcpyHostToDevice, p->stream));
+  HIP_OK(hipMemcpyAsync(p->eps32.p, box32.data(), nbox * sizeof(c32), hipMemcpyHostToDevice, p->stream));
+  HIP_OK(hipStreamSynchronize(p->stream));
+  p->eps_dirty = false;
+  return 1;
+}
+
+// Internal creation shared by vie_fft_create (pec = 0) and the PEC system.
+// xyz: box coordinates of the unknown cells; kshape_z: kernel z extent.
+static vie_fft_plan* plan_create(const size_t box[3], size_t kshape_z, int pec, size_t zoff, size_t cells,
+                                 const size_t* xyz, double h, double k, c64 er, size_t limit) {
+  if (!cells || cells > SIZE_MAX / 3 || !std::isfinite(h) || h <= 0 || !std::isfinite(k) || k < 0 || !passive(er) ||
+      !std::isfinite(k * h) || k * h > 1) {
+    set_error("invalid plan arguments");
+    return nullptr;
+  }
+  size_t kshape[3] = {box[0], box[1], kshape_z};
+  size_t padded[3];
+  for (int d = 0; d < 3; d++) {
+    if (!kshape[d] || kshape[d] > (1u << 20) || !box[d]) {
+      set_error("lattice extent out of range");
+      return nullptr;
+    }
+    padded[d] = pad_extent(kshape[d]);
+    if (!vie::max_fft_extent_ok((int)padded[d])) {
+      set_error("padded FFT extent %zu exceeds the 1024-point pencil kernels", padded[d]);
+      return nullptr;
+    }
+  }
+  vie::Geom g;
+  g.nx = (int)box[0];
+  g.ny = (int)box[1];
+  g.nz = (int)box[2];
+  g.Px = (int)padded[0];
+  g.Py = (int)padded[1];
+  g.Pz = (int)padded[2];
+  g.kz = (int)kshape_z;
+  g.pec = pec;
+  g.zoff = (int)zoff;
+  size_t bytes;
+  if (!plan_bytes(g, cells, &bytes) || bytes > limit) {
+    set_error("plan storage exceeds the byte limit");
+    return nullptr;
+  }
+  // lattice map with duplicate detection (vie_fft.c:418-425)
+  std::vector<size_t> map(cells);
+  {
+    std::vector<unsigned char> seen(g.nbox(), 0);
+    for (size_t c = 0; c < cells; c++) {
+      for (int d = 0; d < 3; d++)
+        if (xyz[3 * c + d] >= box[d]) {
+          set_error("cell outside the lattice");
+          return nullptr;
+        }
+      size_t idx = (xyz[3 * c] * box[1] + xyz[3 * c + 1]) * box[2] + xyz[3 * c + 2];
+      if (seen[idx]) {
+        set_error("duplicate lattice cell");
+        return nullptr;
+      }
+      seen[idx] = 1;
+      map[c] = idx;
+    }
+  }
+  if (!ensure_device()) return nullptr;
+  vie_fft_plan* p = new (std::nothrow) vie_fft_plan();
+  if (!p) return nullptr;
+  p->g = g;
+  p->cells = cells;
+  p->bytes = bytes;
+  p->h = h;
+  p->k = k;
+  p->map.swap(map);
+  p->eps.assign(cells, er);
+  bool ok = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
+  size_t nbox = g.nbox(), oct = p->oct();
+  size_t work = 3 * (size_t)g.nx * g.ny * g.Pz + 3 * (size_t)g.nx * g.Py * g.Pz;
+  size_t ntw = (size_t)g.Px + g.Py + g.Pz;
+  ok = ok && p->spec64.alloc(6 * oct * sizeof(c64)) && p->spec32.alloc(6 * oct * sizeof(c32)) &&
+       p->work.alloc(work * sizeof(c64)) && p->tw64.alloc(ntw * sizeof(c64)) && p->tw32.alloc(ntw * sizeof(c32)) &&
+       p->eps64.alloc(nbox * sizeof(c64)) && p->eps32.alloc(nbox * sizeof(c32)) &&
+       p->map_dev.alloc(cells * sizeof(size_t));
+  if (ok) {
+    std::vector<c64> tw;
+    twiddles(g.Px, tw);
+    twiddles(g.Py, tw);
+    twiddles(g.Pz, tw);
+    std::vector<c32> tw32(tw.size());
+    for (size_t i = 0; i < tw.size(); i++) tw32[i] = c32{(float)tw[i].re, (float)tw[i].im};
+    ok = hipMemcpyAsync(p->tw64.p, tw.data(), tw.size() * sizeof(c64), hipMemcpyHostToDevice, p->stream) ==
+             hipSuccess &&
+         hipMemcpyAsync(p->tw32.p, tw32.data(), tw32.size() * sizeof(c32), hipMemcpyHostToDevice, p->stream) ==
+             hipSuccess &&
+         hipMemcpyAsync(p->map_dev.p, p->map.data(), cells * sizeof(size_t), hipMemcpyHostToDevice, p->stream) ==
+             hipSuccess &&
+         hipStreamSynchronize(p->stream) == hipSuccess;
+  }
+  if (ok) {
+    int ks[3] = {g.nx, g.ny, g.kz}, P[3] = {g.Px, g.Py, g.Pz};
+    ok = vie::build_spectrum(ks, P, h, k, p->spec64.as<c64>(), p->spec32.as<c32>(), &p->spec_info, p->stream) != 0;
+    if (!ok && g_last_error.empty()) set_error("kernel spectrum quadrature failed");
+  }
+  if (ok) {
+    for (int i = 0; i < 3; i++) p->self[i] = p->spec_info.self[i];
+    // every cell's Jacobi diagonal must be usable (vie_fft.c:438, 450-453)
+    for (int i = 0; i < 3 && ok; i++) {
+      std::complex<double> e(er.re, er.im), s(p->self[i].re, p->self[i].im), d = e - (e - 1.0) * s;
+      if (!std::isfinite(std::abs(d)) || std::abs(d) < 1e-30) ok = false;
+    }
+    ok = ok && plan_upload_eps(p);
+  }
+  if (!ok) {
+    delete p;
+    return nullptr;
+  }
+  return p;
+}
+
+extern "C" int vie_fft_backend_set(const char* name) {
+  if (!name || !strcmp(name, "auto")) g_requested = BACKEND_AUTO;
+  else if (!strcmp(name, "hip")) g_requested = BACKEND_HIP;
+  else return 0;  // "builtin" / "fftw": the reference's CPU backends, not in this library
+  return 1;
+}
+extern "C" int vie_fft_fftw_available(void) { return 0; }
+extern "C" const char* vie_fft_backend_requested_name(void) { return g_requested == BACKEND_HIP ? "hip" : "auto"; }
+extern "C" const char* vie_fft_backend_name(const vie_fft_plan* p) { return p ? "hip" : "hip"; }
+extern "C" const char* vie_fft_backend_last_used_name(void) { return g_last_used < 0 ? nullptr : "hip"; }
+extern "C" void vie_fft_backend_clear_last_used(void) { g_last_used = -1; }
+extern "C" int vie_fft_parallel_available(void) { return 1; }
+extern "C" size_t vie_fft_bytes(const vie_fft_plan* p) { return p ? p->bytes : 0; }
+extern "C" size_t vie_fft_threads(const vie_fft_plan* p) { return p ? 1 : 0; }
+extern "C" void vie_fft_destroy(vie_fft_plan* p) { delete p; }
+
+extern "C" int vie_fft_create(vie_fft_plan** out, const size_t shape[3], size_t cells, const size_t* xyz, double h,
+                              double k, double _Complex er, size_t limit) {
+  if (!out) return 0;
+  *out = nullptr;
+  g_last_error.clear();
+  if (!shape || !xyz) {
+    set_error("invalid plan arguments");
+    return 0;
+  }
+  c64 e = {__real__ er, __imag__ er};
+  vie_fft_plan* p = plan_create(shape, shape[2], 0, 0, cells, xyz, h, k, e, limit);
+  if (!p) return 0;
+  g_last_used = BACKEND_HIP;
+  *out = p;
+  return 1;
+}
+
+extern "C" int vie_fft_set_permittivity(vie_fft_plan* p, size_t cell, double _Complex er) {
+  c64 e = {__real__ er, __imag__ er};
+  if (!p || cell >= p->cells || !passive(e)) return 0;
+  std::complex<double> ec(e.re, e.im);
+  for (int i = 0; i < 3; i++) {
+    std::complex<double> d = ec - (ec - 1.0) * std::complex<double>(p->self[i].re, p->self[i].im);
+    if (!std::isfinite(std::abs(d)) || std::abs(d) < 1e-30) return 0;
+  }
+  p->eps[cell] = e;
+  p->eps_dirty = true;
+  return 1;
+}
+
+// ------------------------------------------------------------------ system
+struct SolverWork {
+  size_t n = 0, ldv = 0;
+  int m = 0, prec = -1;
+  DevBuf V, x, b, r, ax, state, dots, partial, counter, scal;
+};
+
+struct dielectric_fft_system {
+  size_t c = 0, v = 0, cells = 0, bytes = 0;
+  bool pec = false;
+  bool owns_plan = true;
+  std::vector<c64> blocks;  // reference layout (dielectric_fft.c:34-40)
+  vie_fft_plan* plan = nullptr;
+  bool blocks_dirty = true;
+  hipStream_t stream = nullptr;  // the plan's stream, or an own one without cells
+  bool own_stream = false;
+  DevBuf C64, B64, D64, C32, B32, D32, zc64, zc32, bz, cdiag;
+  SolverWork sw;
+  bool rhs_loaded = false;
+  ~dielectric_fft_system() {
+    if (owns_plan) delete plan;
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+  size_t nbox() const { return plan ? plan->nbox() : 0; }
+  size_t nsys() const { return c + 3 * nbox(); }  // device system vector length
+};
+
+vie_fft_plan::~vie_fft_plan() {
+  delete wrap;
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+vie_gmres_options dielectric_gmres_settings = {30, 300, 1e-12, 0};
+
+extern "C" size_t dielectric_fft_budget(void) {
+  if (!g_budget) {
+    const char* e = getenv("VIE_HIP_BUDGET_BYTES");
+    g_budget = e ? strtoull(e, nullptr, 10) : ((size_t)64 << 30);
+    if (!g_budget) g_budget = (size_t)64 << 30;
+  }
+  return g_budget;
+}
+extern "C" void dielectric_fft_set_budget(size_t bytes) { g_budget = bytes; }
+
+static int diag_error(dielectric_diagnostic* d, const char* msg) {
+  if (d) {
+    memset(d, 0, sizeof(*d));
+    snprintf(d->message, sizeof(d->message), "FFT: %s", msg);
+  }
+  set_error("FFT: %s", msg);
+  return 0;
+}
+
+// Untabulated and tabulated material permittivity, exp(+i omega t)
+// (dielectric.h:36-56).
+static c64 material_epsilon(const dielectric_material* m, double k0) {
+  double er = m->epsilon_r, ei = m->epsilon_loss, sigma = m->conductivity;
+  if (m->tabulated) {
+    double f = k0 * 299792458.0 / (2 * 3.14159265358979323846 * 1e6), t;
+    size_t j = 1;
+    const double eps16 = 16 * 2.220446049250313e-16 * std::max(1.0, fabs(f));
+    if (m->sample_count < 2 || !std::isfinite(f)) return c64{NAN, NAN};
+    if (f < m->samples[0].frequency_mhz - eps16 || f > m->samples[m->sample_count - 1].frequency_mhz + eps16)
+      return c64{NAN, NAN};
+    f = std::max(m->samples[0].frequency_mhz, std::min(f, m->samples[m->sample_count - 1].frequency_mhz));
+    while (j + 1 < m->sample_count && f > m->samples[j].frequency_mhz) ++j;
+    const dielectric_sample *a = &m->samples[j - 1], *b = &m->samples[j];
+    t = (f - a->frequency_mhz) / (b->frequency_mhz - a->frequency_mhz);
+    er = (1 - t) * a->epsilon_r + t * b->epsilon_r;
+    ei = (1 - t) * a->loss + t * b->loss;
+    sigma = 0;
+    if (a->conductivity_kind) {
+      sigma = ei;
+      ei = 0;
+    }
+  }
+  return c64{er, -(ei + (sigma == 0 ? 0 : sigma / (k0 * 299792458.0 * 8.8541878128e-12)))};
+}
+
+static int prepare(dielectric_fft_system** out, const dielectric_model* model, const dielectric_mesh_result* mesh,
+                   size_t conductors, double k0, size_t extra, dielectric_diagnostic* diag, int pec) {
+  g_last_error.clear();
+  *out = nullptr;
+  size_t cells = mesh->cell_count, v, n, bytes = sizeof(dielectric_fft_system), blocks = 0, t;
+  const size_t m = dielectric_gmres_settings.restart;
+  if (cells > SIZE_MAX / 3 || conductors > SIZE_MAX - 3 * cells) return diag_error(diag, "unknown count overflow");
+  v = 3 * cells;
+  n = v + conductors;
+  // storage inventory (dielectric_fft.c:98-108) for this backend: host blocks,
+  // FP64 + FP32 device copies, FP64 + FP32 Krylov bases and solver vectors.
+  bool ok = mul_ok(conductors, conductors, &t) && add_ok(blocks, t, &blocks) && mul_ok(conductors, v, &t) &&
+            add_ok(blocks, 2 * t, &blocks) && mul_ok(blocks ? blocks : 1, 16 + 24, &t) && add_ok(bytes, t, &bytes) &&
+            mul_ok(n, (m + 1) * 24 + 5 * 16, &t) && add_ok(bytes, t, &bytes) &&
+            mul_ok(cells, sizeof(dielectric_cell), &t) && add_ok(bytes, t, &bytes) && add_ok(bytes, extra, &bytes);
+  if (!ok || bytes >= dielectric_fft_budget()) return diag_error(diag, "solver storage exceeds the HIP memory budget");
+  std::vector<size_t> xyz;
+  size_t shape[3] = {1, 1, 1};
+  double origin[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, h = 0;
+  c64 er = {0, 0};
+  std::vector<c64> cell_eps(cells);
+  if (cells) {
+    h = mesh->cells[0].size[0];
+    for (size_t i = 0; i < cells; i++) {
+      c64 value = {0, 0};
+      bool found = false;
+      if (pec && mesh->cells[i].center[2] - .5 * mesh->cells[i].size[2] < 0)
+        return diag_error(diag, "PEC FFT cells must lie in z >= 0");
+      for (size_t q = 0; q < model->material_count; q++)
+        if (model->materials[q].id == mesh->cells[i].material_id) {
+          value = material_epsilon(&model->materials[q], k0);
+          found = true;
+        }
+      if (!found || (value.re == 0 && value.im == 0) || !passive(value))
+        return diag_error(diag, "unknown or invalid passive material");
+      cell_eps[i] = value;
+      if (i == 0) er = value;
+      for (int d = 0; d < 3; d++) {
+        if (!std::isfinite(h) || h <= 0 || !std::isfinite(mesh->cells[i].size[d]) ||
+            !std::isfinite(mesh->cells[i].center[d]) || fabs(mesh->cells[i].size[d] - h) > 1e-10 * h)
+          return diag_error(diag, "requires equal cubic cells");
+        origin[d] = std::min(origin[d], mesh->cells[i].center[d]);
+      }
+    }
+    xyz.resize(3 * cells);
+    for (size_t i = 0; i < cells; i++)
+      for (int d = 0; d < 3; d++) {
+        double index = (mesh->cells[i].center[d] - origin[d]) / h, nearest = round(index);
+        if (!std::isfinite(index) || nearest < 0 || nearest >= (double)(SIZE_MAX / 2) || fabs(index - nearest) > 1e-8)
+          return diag_error(diag, "cell centers do not share a common lattice");
+        xyz[3 * i + d] = (size_t)nearest;
+        if (xyz[3 * i + d] >= shape[d]) shape[d] = xyz[3 * i + d] + 1;
+      }
+  }
+  size_t kz = shape[2], zoff = 0;
+  if (pec && cells) {
+    // image embedding (dielectric_fft.c:137-154)
+    double gap_index = 2 * (origin[2] - .5 * h) / h, nearest = round(gap_index);
+    if (!std::isfinite(gap_index) || nearest < 0 || nearest >= (double)(SIZE_MAX / 4) ||
+        fabs(gap_index - nearest) > 1e-8)
+      return diag_error(diag, "PEC FFT requires nonnegative bottom gap on the half-cell lattice");
+    size_t gap = (size_t)nearest;
+    if (shape[2] > SIZE_MAX / 2 || gap > SIZE_MAX - 2 * shape[2]) return diag_error(diag, "PEC image embedding overflow");
+    kz = 2 * shape[2] + gap;
+    zoff = shape[2] + gap;
+  }
+  dielectric_fft_system* s = new (std::nothrow) dielectric_fft_system();
+  if (!s) return diag_error(diag, "system allocation failed");
+  s->c = conductors;
+  s->v = v;
+  s->cells = cells;
+  s->pec = pec != 0;
+  try {
+    s->blocks.assign(blocks ? blocks : 1, c64{0, 0});
+  } catch (...) {
+    delete s;
+    return diag_error(diag, "conductor block allocation failed");
+  }
+  if (cells) {
+    size_t limit = dielectric_fft_budget() - bytes;
+    s->plan = plan_create(shape, kz, pec, zoff, cells, xyz.data(), h, k0, er, limit);
+    if (!s->plan) {
+      std::string why = g_last_error;
+      delete s;
+      std::string msg = "kernel/lattice build failed or padded FFT exceeds memory budget";
+      if (!why.empty()) msg += " (" + why + ")";
+      return diag_error(diag, msg.c_str());
+    }
+    for (size_t i = 0; i < cells; i++)
+      if (!vie_fft_set_permittivity(s->plan, i, *(double _Complex*)&cell_eps[i])) {
+        delete s;
+        return diag_error(diag, "kernel/lattice build failed or padded FFT exceeds memory budget");
+      }
+    s->stream = s->plan->stream;
+  } else {
+    if (!ensure_device()) {
+      delete s;
+      return diag_error(diag, "no HIP device available");
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete s;
+      return diag_error(diag, "stream creation failed");
+    }
+    s->own_stream = true;
+  }
+  s->bytes = bytes + (s->plan ? s->plan->bytes : 0);
+  g_last_used = BACKEND_HIP;
+  *out = s;
+  return 1;
+}
+
+extern "C" int dielectric_fft_prepare(dielectric_fft_system** out, const dielectric_model* model,
+                                      const dielectric_mesh_result* mesh, size_t conductors, double k0, size_t extra,
+                                      dielectric_diagnostic* diag) {
+  return prepare(out, model, mesh, conductors, k0, extra, diag, 0);
+}
+extern "C" int dielectric_fft_prepare_pec(dielectric_fft_system** out, const dielectric_model* model,
+                                          const dielectric_mesh_result* mesh, size_t conductors, double k0,
+                                          size_t extra, dielectric_diagnostic* diag) {
+  return prepare(out, model, mesh, conductors, k0, extra, diag, 1);
+}
+extern "C" void dielectric_fft_free(dielectric_fft_system* s) { delete s; }
+extern "C" size_t dielectric_fft_storage(const dielectric_fft_system* s) { return s ? s->bytes : 0; }
+extern "C" size_t dielectric_fft_threads(const dielectric_fft_system* s) { return s ? 1 : 0; }
+extern "C" const char* dielectric_fft_backend(const dielectric_fft_system* s) { return s ? "hip" : "none"; }
+extern "C" vie_fft_plan* dielectric_fft_plan(dielectric_fft_system* s) { return s ? s->plan : nullptr; }
+extern "C" void dielectric_fft_blocks_changed(dielectric_fft_system* s) {
+  if (s) s->blocks_dirty = true;
+}
+
+extern "C" double _Complex* dielectric_fft_entry(dielectric_fft_system* s, size_t row, size_t col) {
+  size_t c = s->c, v = s->v;
+  s->blocks_dirty = true;
+  if (row < c) return (double _Complex*)&s->blocks[row * (c + v) + col];
+  if (col < c) return (double _Complex*)&s->blocks[c * (c + v) + (row - c) * c + col];
+  return nullptr;
+}
+
+// Reorders C, B, D into device layouts (B rows and D columns in box order)
+// and keeps FP64 and FP32 copies.
+static int upload_blocks(dielectric_fft_system* s) {
+  if (!s->blocks_dirty) return 1;
+  const size_t c = s->c, v = s->v, nbox = s->nbox(), nv = 3 * nbox;
+  hipStream_t st = s->stream;
+  if (c) {
+    std::vector<c64> C(c * c), B(c * nv, c64{0, 0}), D(c * nv, c64{0, 0});
+    for (size_t r = 0; r < c; r++)
+      for (size_t q = 0; q < c; q++) C[r * c + q] = s->blocks[r * (c + v) + q];
+    for (size_t cell = 0; cell < s->cells; cell++)
+      for (int comp = 0; comp < 3; comp++) {
+        size_t bi = comp * nbox + s->plan->map[cell], ui = 3 * cell + comp;
+        for (size_t r = 0; r < c; r++) {
+          B[r * nv + bi] = s->blocks[r * (c + v) + c + ui];
+          D[r * nv + bi] = s->blocks[c * (c + v) + ui * c + r];
+        }
+      }
+    if (!(s->C64.alloc(c * c * 16) && s->C32.alloc(c * c * 8) && s->zc64.alloc(c * 16) && s->zc32.alloc(c * 8) &&
+          s->bz.alloc(c * 16) && s->cdiag.alloc(c * 16)))
+      return 0;
+    std::vector<c64> cd(c);
+    for (size_t r = 0; r < c; r++) {
+      cd[r] = C[r * c + r];
+      if (!std::isfinite(hypot(cd[r].re, cd[r].im)) || hypot(cd[r].re, cd[r].im) < DBL_MIN) {
+        set_error("conductor diagonal unusable for the preconditioner");
+        return 0;
+      }
+    }
+    HIP_OK(hipMemcpyAsync(s->C64.p, C.data(), c * c * 16, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(s->cdiag.p, cd.data(), c * 16, hipMemcpyHostToDevice, st));
+    vie::vec_cast<float, double>(s->C32.as<c32>(), s->C64.as<c64>(), c * c, st);
+    if (nv) {
+      if (!(s->B64.alloc(c * nv * 16) && s->D64.alloc(c * nv * 16) && s->B32.alloc(c * nv * 8) &&
+            s->D32.alloc(c * nv * 8)))
+        return 0;
+      HIP_OK(hipMemcpyAsync(s->B64.p, B.data(), c * nv * 16, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(s->D64.p, D.data(), c * nv * 16, hipMemcpyHostToDevice, st));
+      vie::vec_cast<float, double>(s->B32.as<c32>(), s->B64.as<c64>(), c * nv, st);
+      vie::vec_cast<float, double>(s->D32.as<c32>(), s->D64.as<c64>(), c * nv, st);
+    }
+    HIP_OK(hipStreamSynchronize(st));
+  }
+  s->blocks_dirty = false;
+  return 1;
+}
+
+// ------------------------------------------------------------------ operators on device vectors
+template <typename T> struct Prec;
+template <> struct Prec<double> {
+  static DevBuf& C(dielectric_fft_system* s) { return s->C64; }
+  static DevBuf& B(dielectric_fft_system* s) { return s->B64; }
+  static DevBuf& D(dielectric_fft_system* s) { return s->D64; }
+  static DevBuf& zc(dielectric_fft_system* s) { return s->zc64; }
+  static DevBuf& eps(vie_fft_plan* p) { return p->eps64; }
+  static constexpr double eps_b = 32 * 2.220446049250313e-16;
+};
+template <> struct Prec<float> {
+  static DevBuf& C(dielectric_fft_system* s) { return s->C32; }
+  static DevBuf& B(dielectric_fft_system* s) { return s->B32; }
+  static DevBuf& D(dielectric_fft_system* s) { return s->D32; }
+  static DevBuf& zc(dielectric_fft_system* s) { return s->zc32; }
+  static DevBuf& eps(vie_fft_plan* p) { return p->eps32; }
+  static constexpr double eps_b = 32 * 1.1920928955078125e-07;
+};
+
+// y = A z, z = precond ? M^{-1} x : x, on system vectors [c | 3][nbox].
+template <typename T>
+static int sys_apply(dielectric_fft_system* s, const vie::cx<T>* x, vie::cx<T>* y, int precond,
+                     vie::ReduceScratch& rs, hipEvent_t* ev = nullptr) {
+  const size_t c = s->c;
+  if (s->plan) {
+    vie::VolumeView<T> vv = s->plan->view<T>();
+    if (!vie::volume_apply<T>(vv, x + c, y + c, precond, s->stream, ev)) {
+      set_error("volume kernels failed to launch");
+      return 0;
+    }
+  }
+  if (c) {
+    vie::CouplingView<T> cv;
+    cv.c = (int)c;
+    cv.nbox = s->nbox();
+    cv.nv = 3 * cv.nbox;
+    cv.C = Prec<T>::C(s).template as<vie::cx<T>>();
+    cv.B = Prec<T>::B(s).template as<vie::cx<T>>();
+    cv.D = Prec<T>::D(s).template as<vie::cx<T>>();
+    cv.zc = Prec<T>::zc(s).template as<vie::cx<T>>();
+    cv.bz = s->bz.as<c64>();
+    cv.eps = s->plan ? Prec<T>::eps(s->plan).template as<vie::cx<T>>() : nullptr;
+    for (int i = 0; i < 3; i++)
+      cv.self[i] = s->plan ? vie::cx<T>{(T)s->plan->self[i].re, (T)s->plan->self[i].im} : vie::cx<T>{0, 0};
+    vie::coupling_apply<T>(cv, x, x + c, y, y + c, precond, rs, s->stream);
+    if (ev) HIP_OK(hipEventRecord(ev[6], s->stream));
+  }
+  return hipGetLastError() == hipSuccess;
+}
+
+static vie::JacobiView jacobi_view(dielectric_fft_system* s) {
+  vie::JacobiView J;
+  J.c = s->c;
+  J.nbox = s->nbox();
+  J.cdiag = s->cdiag.as<c64>();
+  J.eps = s->plan ? s->plan->eps64.as<c64>() : nullptr;
+  for (int i = 0; i < 3; i++) J.self[i] = s->plan ? s->plan->self[i] : c64{0, 0};
+  return J;
+}
+
+static int ensure_scratch(dielectric_fft_system* s, int m) {
+  SolverWork& w = s->sw;
+  size_t n = s->nsys();
+  size_t blocks = vie::red_blocks(std::max<size_t>(n, 1));
+  size_t need = (size_t)std::max((size_t)m + 2, s->c) * blocks * sizeof(c64);
+  if (!w.partial.alloc(need)) return 0;
+  if (!w.counter.p) {
+    if (!w.counter.alloc(64)) return 0;
+    HIP_OK(hipMemsetAsync(w.counter.p, 0, 64, s->stream));
+  }
+  if (!w.scal.alloc(16 * sizeof(c64))) return 0;
+  return 1;
+}
+
+static int ensure_solver(dielectric_fft_system* s, int m, int prec) {
+  SolverWork& w = s->sw;
+  size_t n = s->nsys();
+  size_t ldv = (n + 63) / 64 * 64;
+  size_t es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
+  if (!ensure_scratch(s, m)) return 0;
+  if (!(w.V.alloc((size_t)(m + 1) * ldv * es) && w.x.alloc(n * 16) && w.b.alloc(n * 16) && w.r.alloc(n * 16) &&
+        w.ax.alloc(n * 16) && w.state.alloc(sizeof(vie::ArnoldiState)) &&
+        w.dots.alloc(2 * (vie::kMaxRestart + 2) * sizeof(c64))))
+    return 0;
+  w.n = n;
+  w.ldv = ldv;
+  w.m = m;
+  w.prec = prec;
+  return 1;
+}
+
+// ------------------------------------------------------------------ device-resident GMRES
+// The restarted GMRES of vie_iterative.c:80-199 with every vector operation on
+// the GPU.  T is the storage / matvec precision of the Krylov cycle; the
+// solution, right-hand side, true residual and all reductions are FP64.
+// FP64: the reference's algorithm (two Gram-Schmidt passes, Givens, true
+// residual acceptance).  MIXED: the same outer loop with FP32 cycles that stop
+// once |g| <= max(target, inner_reduction * beta); each restart's FP64 true
+// residual refines the solution.
+struct StatusBlock {
+  double gabs, beta;
+  int breakdown, nonfinite, singular, used;
+};
+
+static int read_status(dielectric_fft_system* s, StatusBlock* sb) {
+  HIP_OK(hipMemcpyAsync(sb, s->sw.state.p, sizeof(StatusBlock), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return 1;
+}
+
+static int read_norm(dielectric_fft_system* s, const c64* d, double* out) {
+  c64 v;
+  HIP_OK(hipMemcpyAsync(&v, d, sizeof(c64), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  *out = sqrt(v.re);
+  return 1;
+}
+
+template <typename T>
+static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, const vie_hip_solve_options* so,
+                        vie_gmres_report* rep) {
+  SolverWork& w = s->sw;
+  const size_t n = w.n, ldv = w.ldv;
+  const int m = w.m;
+  hipStream_t st = s->stream;
+  vie::ReduceScratch rs;
+  rs.partial = w.partial.as<c64>();
+  rs.counter = w.counter.as<unsigned>();
+  c64* x = w.x.as<c64>();
+  c64* b = w.b.as<c64>();
+  c64* r = w.r.as<c64>();
+  c64* ax = w.ax.as<c64>();
+  vie::cx<T>* V = w.V.as<vie::cx<T>>();
+  vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
+  c64* dots1 = w.dots.as<c64>();
+  c64* dots2 = dots1 + vie::kMaxRestart + 2;
+  c64* scal = w.scal.as<c64>();  // [0] ||b||^2 [1] ||x||^2 [2] ||r||^2 [3] next^2
+  const vie::JacobiView J = jacobi_view(s);
+  const double reduction = so && so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
+  const bool mixed = sizeof(T) == sizeof(float);
+  // m must be set in the state once
+  {
+    int mm = m;
+    HIP_OK(hipMemcpyAsync((char*)sd + offsetof(vie::ArnoldiState, m), &mm, sizeof(int), hipMemcpyHostToDevice, st));
+  }
+  double bnorm, xnorm;
+  vie::norm2<double>(b, n, rs, scal + 0, st);
+  vie::norm2<double>(x, n, rs, scal + 1, st);
+  if (!read_norm(s, scal + 0, &bnorm) || !read_norm(s, scal + 1, &xnorm)) {
+    rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+    return 0;
+  }
+  const double target = std::max(o->absolute_tolerance, o->relative_tolerance * bnorm);
+  if (!std::isfinite(bnorm) || !std::isfinite(xnorm) || !std::isfinite(target)) {
+    rep->status = VIE_ITERATIVE_NONFINITE;
+    return 0;
+  }
+  auto true_residual = [&](double* beta) -> int {
+    rep->operator_applications++;
+    if (!sys_apply<double>(s, x, ax, 0, rs)) {
+      rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+      return 0;
+    }
+    vie::residual(b, ax, r, n, rs, scal + 2, st);
+    if (!read_norm(s, scal + 2, beta)) {
+      rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+      return 0;
+    }
+    rep->residual_norm = *beta;
+    if (!std::isfinite(*beta)) {
+      rep->status = VIE_ITERATIVE_NONFINITE;
+      return 0;
+    }
+    return 1;
+  };
+  double beta;
+  if (!true_residual(&beta)) return 0;
+  for (;;) {
+    rep->relative_residual = bnorm > 0 ? beta / bnorm : beta;
+    if (beta <= target) {
+      rep->status = VIE_ITERATIVE_CONVERGED;
+      return 1;
+    }
+    if (rep->iterations >= o->maximum_iterations) {
+      rep->status = VIE_ITERATIVE_ITERATION_LIMIT;
+      return 0;
+    }
+    const double inner_target = mixed ? std::max(target, reduction * beta) : target;
+    vie::cycle_init(sd, scal + 2, st);
+    vie::scale_from<T>(V, r, n, sd, st);
+    int used = 0;
+    bool breakdown = false;
+    StatusBlock sb;
+    for (int j = 0; j < m && rep->iterations < o->maximum_iterations; j++) {
+      vie::cx<T>* vj = V + (size_t)j * ldv;
+      vie::cx<T>* wv = V + (size_t)(j + 1) * ldv;
+      rep->operator_applications++;
+      if (!sys_apply<T>(s, vj, wv, 1, rs)) {
+        rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+        return 0;
+      }
+      // two classical Gram-Schmidt passes (same role as the reference's two
+      // MGS passes, vie_iterative.c:151-159); ||w|| before and after
+      vie::multidot<T>(V, ldv, nullptr, j + 1, wv, n, 1, rs, dots1, st);
+      vie::update<T>(V, ldv, nullptr, j + 1, dots1, wv, n, 0, rs, nullptr, st);
+      vie::multidot<T>(V, ldv, nullptr, j + 1, wv, n, 0, rs, dots2, st);
+      vie::update<T>(V, ldv, nullptr, j + 1, dots2, wv, n, 1, rs, scal + 3, st);
+      vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, scal + 3, Prec<T>::eps_b, st);
+      vie::scale_next<T>(wv, n, sd, st);
+      if (!read_status(s, &sb)) {
+        rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+        return 0;
+      }
+      if (sb.nonfinite) {
+        rep->status = VIE_ITERATIVE_NONFINITE;
+        return 0;
+      }
+      rep->iterations++;
+      used = j + 1;
+      breakdown = sb.breakdown != 0;
+      if (breakdown || sb.gabs <= inner_target) break;
+    }
+    vie::backsolve(sd, st);
+    if (!read_status(s, &sb)) return 0;
+    if (sb.singular) {
+      rep->status = VIE_ITERATIVE_BREAKDOWN;
+      return 0;
+    }
+    (void)used;
+    vie::combine<T>(x, V, ldv, n, sd, J, 1, st);
+    if (!true_residual(&beta)) return 0;
+    rep->relative_residual = bnorm > 0 ? beta / bnorm : beta;
+    if (breakdown && beta > target) {
+      rep->status = VIE_ITERATIVE_BREAKDOWN;
+      return 0;
+    }
+    if (beta > target && rep->iterations < o->maximum_iterations) rep->restarts++;
+  }
+}
+
+static int validate_options(const vie_gmres_options* o) {
+  return o && o->restart && o->maximum_iterations && std::isfinite(o->relative_tolerance) &&
+         o->relative_tolerance >= 0 && std::isfinite(o->absolute_tolerance) && o->absolute_tolerance >= 0 &&
+         !(o->relative_tolerance == 0 && o->absolute_tolerance == 0);
+}
+
+static void report_init(vie_gmres_report* r) {
+  memset(r, 0, sizeof(*r));
+  r->status = VIE_ITERATIVE_INVALID_INPUT;
+  r->residual_norm = r->relative_residual = INFINITY;
+}
+
+// Solve with b already in the solver's device buffer (system layout) and x
+// holding the initial guess.
+static int solve_resident(dielectric_fft_system* s, const vie_gmres_options* options,
+                          const vie_hip_solve_options* so, vie_gmres_report* rep) {
+  vie_gmres_options defaults = {30, 300, 1e-12, 0};
+  if (!options) options = &defaults;
+  if (!validate_options(options)) return 0;
+  if (!so) {
+    init_solve_defaults();
+    so = &g_solve;
+  }
+  size_t n = s->c + s->v;
+  size_t m = options->restart;
+  if (m > n) m = n;
+  if (m > options->maximum_iterations) m = options->maximum_iterations;
+  if (m > (size_t)vie::kMaxRestart) {
+    set_error("restart %zu exceeds the device solver's %d", m, vie::kMaxRestart);
+    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
+    return 0;
+  }
+  int prec = so->precision == VIE_HIP_MIXED ? VIE_HIP_MIXED : VIE_HIP_FP64;
+  if (!ensure_solver(s, (int)m, prec) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan))) {
+    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
+    return 0;
+  }
+  return prec == VIE_HIP_MIXED ? gmres_device<float>(s, options, so, rep) : gmres_device<double>(s, options, so, rep);
+}
+
+static int stage_vector(dielectric_fft_system* s, const double _Complex* user, c64* dst) {
+  // user layout [c | 3 cells interleaved] -> system layout (holes zero)
+  size_t nu = s->c + s->v, n = s->nsys();
+  DevBuf& tmp = s->plan ? s->plan->io_user : s->sw.ax;
+  if (!tmp.alloc(std::max<size_t>(nu, 1) * 16)) return 0;
+  HIP_OK(hipMemcpyAsync(tmp.p, user, nu * 16, hipMemcpyHostToDevice, s->stream));
+  vie::vec_zero<double>(dst, n, s->stream);
+  vie::to_box<double>(tmp.as<c64>(), s->c, s->cells, s->plan ? s->plan->map_dev.as<size_t>() : nullptr, s->nbox(), dst,
+                      s->stream);
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return 1;
+}
+
+static int unstage_vector(dielectric_fft_system* s, const c64* src, double _Complex* user) {
+  size_t nu = s->c + s->v;
+  DevBuf& tmp = s->plan ? s->plan->io_user : s->sw.r;
+  if (!tmp.alloc(std::max<size_t>(nu, 1) * 16)) return 0;
+  vie::from_box<double>(src, s->c, s->cells, s->plan ? s->plan->map_dev.as<size_t>() : nullptr, s->nbox(),
+                        tmp.as<c64>(), s->stream);
+  HIP_OK(hipMemcpyAsync(user, tmp.p, nu * 16, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return 1;
+}
+
+static int ensure_vectors(dielectric_fft_system* s) {
+  size_t n = s->nsys();
+  return s->sw.x.alloc(n * 16) && s->sw.b.alloc(n * 16) && s->sw.r.alloc(n * 16) && s->sw.ax.alloc(n * 16);
+}
+
+static int solve_host(dielectric_fft_system* s, const double _Complex* b, double _Complex* x,
+                      const vie_gmres_options* options, const vie_hip_solve_options* so, vie_gmres_report* report) {
+  vie_gmres_report local;
+  if (!report) report = &local;
+  report_init(report);
+  g_last_error.clear();
+  if (!s || !b || !x || (const void*)b == (const void*)x) return 0;
+  if (!validate_options(options ? options : &dielectric_gmres_settings)) return 0;
+  if (!ensure_vectors(s) || !stage_vector(s, b, s->sw.b.as<c64>()) || !stage_vector(s, x, s->sw.x.as<c64>())) {
+    report->status = VIE_ITERATIVE_ALLOCATION_FAILED;
+    return 0;
+  }
+  int ok = solve_resident(s, options, so, report);
+  // x is updated even on failure (vie_iterative.h:48-50)
+  if (report->status != VIE_ITERATIVE_INVALID_INPUT && !unstage_vector(s, s->sw.x.as<c64>(), x)) return 0;
+  return ok;
+}
+
+static dielectric_fft_system* plan_system(vie_fft_plan* p) {
+  if (!p->wrap) {
+    dielectric_fft_system* s = new (std::nothrow) dielectric_fft_system();
+    if (!s) return nullptr;
+    s->c = 0;
+    s->v = 3 * p->cells;
+    s->cells = p->cells;
+    s->plan = p;
+    s->owns_plan = false;
+    s->stream = p->stream;
+    s->blocks.assign(1, c64{0, 0});
+    s->blocks_dirty = false;
+    p->wrap = s;
+  }
+  return p->wrap;
+}
+
+extern "C" int dielectric_fft_solve(dielectric_fft_system* s, const double _Complex* b, double _Complex* x,
+                                    const vie_gmres_options* options, vie_gmres_report* report) {
+  return solve_host(s, b, x, options, nullptr, report);
+}
+
+extern "C" int vie_fft_solve(vie_fft_plan* p, const double _Complex* b, double _Complex* x,
+                             const vie_gmres_options* options, vie_gmres_report* report) {
+  if (!p) {
+    if (report) report_init(report);
+    return 0;
+  }
+  dielectric_fft_system* s = plan_system(p);
+  if (!s) return 0;
+  return solve_host(s, b, x, options, nullptr, report);
+}
+
+extern "C" void vie_hip_set_solve_options(const vie_hip_solve_options* o) {
+  init_solve_defaults();
+  if (o) g_solve = *o;
+}
+extern "C" void vie_hip_get_solve_options(vie_hip_solve_options* o) {
+  init_solve_defaults();
+  if (o) *o = g_solve;
+}
+
+extern "C" int dielectric_fft_upload_rhs(dielectric_fft_system* s, const double _Complex* b) {
+  if (!s || !b || !ensure_vectors(s) || !stage_vector(s, b, s->sw.b.as<c64>())) return 0;
+  s->rhs_loaded = true;
+  return 1;
+}
+
+extern "C" int dielectric_fft_solve_resident(dielectric_fft_system* s, const vie_gmres_options* options,
+                                             const vie_hip_solve_options* so, vie_gmres_report* report) {
+  vie_gmres_report local;
+  if (!report) report = &local;
+  report_init(report);
+  if (!s || !s->rhs_loaded) return 0;
+  vie::vec_zero<double>(s->sw.x.as<c64>(), s->nsys(), s->stream);  // x0 = 0 as the callers do
+  return solve_resident(s, options, so, report);
+}
+
+extern "C" int dielectric_fft_download_solution(dielectric_fft_system* s, double _Complex* x) {
+  if (!s || !x || !s->sw.x.p) return 0;
+  return unstage_vector(s, s->sw.x.as<c64>(), x);
+}
+
+// ------------------------------------------------------------------ host-pointer operator callbacks
+// Compatibility path of the reference's operator contract: each call stages
+// through device memory.  vie_gmres recognises these callbacks and runs the
+// whole solve on the device instead.
+static int apply_host(dielectric_fft_system* s, const double _Complex* in, double _Complex* out, bool precond) {
+  if (!s || !in || !out) return 0;
+  size_t n = s->nsys();
+  if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
+      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s, 1))
+    return 0;
+  if (!stage_vector(s, in, s->sw.x.as<c64>())) return 0;
+  if (precond) {
+    vie::precondition_apply(s->sw.x.as<c64>(), s->sw.ax.as<c64>(), n, jacobi_view(s), s->stream);
+  } else {
+    vie::ReduceScratch rs;
+    rs.partial = s->sw.partial.as<c64>();
+    rs.counter = s->sw.counter.as<unsigned>();
+    if (!sys_apply<double>(s, s->sw.x.as<c64>(), s->sw.ax.as<c64>(), 0, rs)) return 0;
+  }
+  return unstage_vector(s, s->sw.ax.as<c64>(), out);
+}
+
+static int cb_plan_apply(void* ctx, const double _Complex* in, double _Complex* out) {
+  vie_fft_plan* p = (vie_fft_plan*)ctx;
+  return p ? apply_host(plan_system(p), in, out, false) : 0;
+}
+static int cb_plan_precondition(void* ctx, const double _Complex* in, double _Complex* out) {
+  vie_fft_plan* p = (vie_fft_plan*)ctx;
+  return p ? apply_host(plan_system(p), in, out, true) : 0;
+}
+static int cb_system_apply(void* ctx, const double _Complex* in, double _Complex* out) {
+  return apply_host((dielectric_fft_system*)ctx, in, out, false);
+}
+static int cb_system_precondition(void* ctx, const double _Complex* in, double _Complex* out) {
+  return apply_host((dielectric_fft_system*)ctx, in, out, true);
+}
+
+extern "C" void vie_fft_operator(vie_fft_plan* p, vie_linear_operator* op) {
+  memset(op, 0, sizeof(*op));
+  if (!p) return;
+  op->order = 3 * p->cells;
+  op->apply = cb_plan_apply;
+  op->context = p;
+  op->precondition = cb_plan_precondition;
+  op->precondition_context = p;
+}
+
+extern "C" void dielectric_fft_get_operator(dielectric_fft_system* s, vie_linear_operator* op) {
+  op->order = s->c + s->v;
+  op->apply = cb_system_apply;
+  op->context = s;
+  op->precondition = cb_system_precondition;
+  op->precondition_context = s;
+  s->blocks_dirty = true;  // callers write entries before asking for the operator
+}
+
+// ------------------------------------------------------------------ vie_gmres
+extern "C" void vie_gmres_default_options(vie_gmres_options* o) {
+  o->restart = 30;
+  o->maximum_iterations = 300;
+  o->relative_tolerance = 1e-12;
+  o->absolute_tolerance = 0.0;
+}
+
+extern "C" const char* vie_iterative_status_name(vie_iterative_status s) {
+  switch (s) {
+    case VIE_ITERATIVE_CONVERGED: return "converged";
+    case VIE_ITERATIVE_ITERATION_LIMIT: return "iteration limit";
+    case VIE_ITERATIVE_BREAKDOWN: return "Krylov breakdown";
+    case VIE_ITERATIVE_CALLBACK_FAILED: return "operator/preconditioner failure";
+    case VIE_ITERATIVE_INVALID_INPUT: return "invalid input";
+    case VIE_ITERATIVE_ALLOCATION_FAILED: return "workspace allocation/size failure";
+    case VIE_ITERATIVE_NONFINITE: return "nonfinite arithmetic";
+  }
+  return "unknown status";
+}
+
+typedef std::complex<double> zc;
+
+// hypot-accumulated norm, NaN on a nonfinite entry (vie_iterative.c:9-19)
+static double hnorm(size_t n, const zc* x) {
+  double r = 0;
+  for (size_t i = 0; i < n; i++) {
+    if (!std::isfinite(x[i].real()) || !std::isfinite(x[i].imag())) return NAN;
+    r = hypot(r, std::abs(x[i]));
+  }
+  return r;
+}
+
+// Host iteration for callback operators that are not this library's (mock
+// operators, dense adapters): the reference's vie_gmres semantics.
+static int gmres_callbacks(const vie_linear_operator* op, const zc* b, zc* x, const vie_gmres_options* o,
+                           vie_gmres_report* rep) {
+  const size_t n = op->order;
+  size_t m = o->restart;
+  if (m > n) m = n;
+  if (m > o->maximum_iterations) m = o->maximum_iterations;
+  if (m == SIZE_MAX || n > SIZE_MAX / (m + 1) / sizeof(zc)) {
+    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
+    return 0;
+  }
+  std::vector<zc> V, Z, H, r, w, g, y, sn;
+  std::vector<double> cs;
+  try {
+    V.resize(n * (m + 1));
+    Z.resize(n * m);
+    H.resize((m + 1) * m);
+    r.resize(n);
+    w.resize(n);
+    g.resize(m + 1);
+    y.resize(m);
+    sn.resize(m);
+    cs.resize(m);
+  } catch (...) {
+    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
+    return 0;
+  }
+  double bnorm = hnorm(n, b);
+  double target = std::max(o->absolute_tolerance, o->relative_tolerance * bnorm);
+  if (!std::isfinite(bnorm) || !std::isfinite(hnorm(n, x)) || !std::isfinite(target)) {
+    rep->status = VIE_ITERATIVE_NONFINITE;
+    return 0;
+  }
+  auto cb = [](vie_vector_map f, void* c, const zc* in, zc* out) {
+    return f(c, (const double _Complex*)in, (double _Complex*)out);
+  };
+  auto residual = [&]() -> int {
+    rep->operator_applications++;
+    if (!cb(op->apply, op->context, x, r.data())) {
+      rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+      return 0;
+    }
+    for (size_t i = 0; i < n; i++) r[i] = b[i] - r[i];
+    rep->residual_norm = hnorm(n, r.data());
+    if (!std::isfinite(rep->residual_norm)) {
+      rep->status = VIE_ITERATIVE_NONFINITE;
+      return 0;
+    }
+    return 1;
+  };
+  if (!residual()) return 0;
+  for (;;) {
+    size_t used = 0;
+    bool breakdown = false;
+    double beta = rep->residual_norm;
+    rep->relative_residual = bnorm > 0 ? beta / bnorm : beta;
+    if (beta <= target) {
+      rep->status = VIE_ITERATIVE_CONVERGED;
+      return 1;
+    }
+    if (rep->iterations >= o->maximum_iterations) {
+      rep->status = VIE_ITERATIVE_ITERATION_LIMIT;
+      return 0;
+    }
+    std::fill(H.begin(), H.end(), zc(0));
+    std::fill(g.begin(), g.end(), zc(0));
+    g[0] = beta;
+    for (size_t i = 0; i < n; i++) V[i] = r[i] / beta;
+    for (size_t j = 0; j < m && rep->iterations < o->maximum_iterations; j++) {
+      zc* vj = &V[j * n];
+      zc* zj = &Z[j * n];
+      if (op->precondition) {
+        if (!cb(op->precondition, op->precondition_context, vj, zj)) {
+          rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+          return 0;
+        }
+      } else
+        std::copy(vj, vj + n, zj);
+      if (!std::isfinite(hnorm(n, zj))) {
+        rep->status = VIE_ITERATIVE_NONFINITE;
+        return 0;
+      }
+      rep->operator_applications++;
+      if (!cb(op->apply, op->context, zj, w.data())) {
+        rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+        return 0;
+      }
+      double before = hnorm(n, w.data());
+      if (!std::isfinite(before)) {
+        rep->status = VIE_ITERATIVE_NONFINITE;
+        return 0;
+      }
+      for (int pass = 0; pass < 2; pass++)
+        for (size_t k = 0; k <= j; k++) {
+          zc p = 0;
+          const zc* vk = &V[k * n];
+          for (size_t i = 0; i < n; i++) p += std::conj(vk[i]) * w[i];
+          H[k * m + j] += p;
+          for (size_t i = 0; i < n; i++) w[i] -= p * vk[i];
+        }
+      double next = hnorm(n, w.data());
+      if (!std::isfinite(next)) {
+        rep->status = VIE_ITERATIVE_NONFINITE;
+        return 0;
+      }
+      H[(j + 1) * m + j] = next;
+      breakdown = next <= 32 * DBL_EPSILON * before;
+      if (!breakdown)
+        for (size_t i = 0; i < n; i++) V[(j + 1) * n + i] = w[i] / next;
+      for (size_t k = 0; k < j; k++) {
+        zc f = H[k * m + j], sc = H[(k + 1) * m + j];
+        H[k * m + j] = cs[k] * f + sn[k] * sc;
+        H[(k + 1) * m + j] = -std::conj(sn[k]) * f + cs[k] * sc;
+      }
+      {
+        zc a = H[j * m + j], bb = H[(j + 1) * m + j];
+        double aa = std::abs(a), bv = std::abs(bb);
+        if (bv == 0.0) {
+          cs[j] = 1.0;
+          sn[j] = 0.0;
+        } else if (aa == 0.0) {
+          cs[j] = 0.0;
+          sn[j] = std::conj(bb) / bv;
+        } else {
+          double len = hypot(aa, bv);
+          cs[j] = aa / len;
+          sn[j] = (a / aa) * (std::conj(bb) / len);
+        }
+      }
+      H[j * m + j] = cs[j] * H[j * m + j] + sn[j] * H[(j + 1) * m + j];
+      H[(j + 1) * m + j] = 0.0;
+      g[j + 1] = -std::conj(sn[j]) * g[j];
+      g[j] *= cs[j];
+      rep->iterations++;
+      used = j + 1;
+      if (breakdown || std::abs(g[j + 1]) <= target) break;
+    }
+    for (size_t i = used; i-- > 0;) {
+      zc val = g[i];
+      for (size_t k = i + 1; k < used; k++) val -= H[i * m + k] * y[k];
+      if (std::abs(H[i * m + i]) == 0.0) {
+        rep->status = VIE_ITERATIVE_BREAKDOWN;
+        return 0;
+      }
+      y[i] = val / H[i * m + i];
+    }
+    for (size_t j = 0; j < used; j++)
+      for (size_t i = 0; i < n; i++) x[i] += Z[j * n + i] * y[j];
+    if (!residual()) return 0;
+    rep->relative_residual = bnorm > 0 ? rep->residual_norm / bnorm : rep->residual_norm;
+    if (breakdown && rep->residual_norm > target) {
+      rep->status = VIE_ITERATIVE_BREAKDOWN;
+      return 0;
+    }
+    if (rep->residual_norm > target && rep->iterations < o->maximum_iterations) rep->restarts++;
+  }
+}
+
+extern "C" int vie_gmres(const vie_linear_operator* op, const double _Complex* b, double _Complex* x,
+                         const vie_gmres_options* options, vie_gmres_report* report) {
+  vie_gmres_report local;
+  vie_gmres_options defaults;
+  if (!report) report = &local;
+  report_init(report);
+  vie_gmres_default_options(&defaults);
+  if (!options) options = &defaults;
+  if (!op || !op->apply || op->order == 0 || !b || !x || (const void*)b == (const void*)x ||
+      !validate_options(options))
+    return 0;
+  // this library's operators: the device-resident solve
+  if (op->apply == cb_system_apply && op->precondition == cb_system_precondition && op->context)
+    return solve_host((dielectric_fft_system*)op->context, b, x, options, nullptr, report);
+  if (op->apply == cb_plan_apply && op->precondition == cb_plan_precondition && op->context) {
+    dielectric_fft_system* s = plan_system((vie_fft_plan*)op->context);
+    if (!s) return 0;
+    return solve_host(s, b, x, options, nullptr, report);
+  }
+  return gmres_callbacks(op, (const zc*)b, (zc*)x, options, report);
+}
+
+// Dense Jacobi-preconditioned adapter (vie_iterative.c:201-244): host callbacks.
+struct DenseCtx {
+  const zc* a;
+  size_t n;
+};
+static int dense_apply(void* ctx, const double _Complex* x, double _Complex* y) {
+  const DenseCtx* d = (const DenseCtx*)ctx;
+  const zc* xv = (const zc*)x;
+  zc* yv = (zc*)y;
+  for (size_t i = 0; i < d->n; i++) {
+    zc s = 0;
+    for (size_t j = 0; j < d->n; j++) s += d->a[i * d->n + j] * xv[j];
+    yv[i] = s;
+  }
+  return 1;
+}
+static int dense_diag(void* ctx, const double _Complex* x, double _Complex* y) {
+  const DenseCtx* d = (const DenseCtx*)ctx;
+  const zc* xv = (const zc*)x;
+  zc* yv = (zc*)y;
+  for (size_t i = 0; i < d->n; i++) {
+    zc dg = d->a[i * d->n + i];
+    if (std::abs(dg) == 0.0) return 0;
+    yv[i] = xv[i] / dg;
+  }
+  return 1;
+}
+extern "C" int vie_dense_gmres(const double _Complex* a, size_t n, const double _Complex* b, double _Complex* x,
+                               const vie_gmres_options* options, vie_gmres_report* report) {
+  if (!a || n == 0 || n > SIZE_MAX / n || n * n > SIZE_MAX / sizeof(zc)) {
+    if (report) report_init(report);
+    return 0;
+  }
+  DenseCtx ctx = {(const zc*)a, n};
+  vie_linear_operator op = {n, dense_apply, &ctx, dense_diag, &ctx};
+  return vie_gmres(&op, b, x, options, report);
+}
+
+// ------------------------------------------------------------------ measurement hooks
+extern "C" int dielectric_fft_time_apply(dielectric_fft_system* s, vie_hip_precision prec, int repeats,
+                                         double* ms_per_apply, double* pass_ms) {
+  if (!s || repeats < 1) return 0;
+  size_t n = s->nsys();
+  if (!ensure_vectors(s) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s, 1))
+    return 0;
+  vie::ReduceScratch rs;
+  rs.partial = s->sw.partial.as<c64>();
+  rs.counter = s->sw.counter.as<unsigned>();
+  const bool mixed = prec == VIE_HIP_MIXED;
+  // resident input: a fixed nonzero vector in the system layout
+  DevBuf in, out;
+  size_t es = mixed ? 8 : 16;
+  if (!in.alloc(n * es) || !out.alloc(n * es)) return 0;
+  {
+    std::vector<c64> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = c64{sin(0.7 * (double)i), cos(0.3 * (double)i)};
+    HIP_OK(hipMemcpyAsync(s->sw.r.p, h.data(), n * 16, hipMemcpyHostToDevice, s->stream));
+    if (mixed) vie::vec_cast<float, double>(in.as<c32>(), s->sw.r.as<c64>(), n, s->stream);
+    else vie::vec_cast<double, double>(in.as<c64>(), s->sw.r.as<c64>(), n, s->stream);
+    HIP_OK(hipStreamSynchronize(s->stream));
+  }
+  auto run = [&](hipEvent_t* ev) -> int {
+    return mixed ? sys_apply<float>(s, in.as<c32>(), out.as<c32>(), 1, rs, ev)
+                 : sys_apply<double>(s, in.as<c64>(), out.as<c64>(), 1, rs, ev);
+  };
+  hipEvent_t ev[7], t0, t1;
+  for (int i = 0; i < 7; i++) HIP_OK(hipEventCreate(&ev[i]));
+  HIP_OK(hipEventCreate(&t0));
+  HIP_OK(hipEventCreate(&t1));
+  int ok = run(nullptr) && run(nullptr);  // warm
+  HIP_OK(hipStreamSynchronize(s->stream));
+  HIP_OK(hipEventRecord(t0, s->stream));
+  for (int r = 0; r < repeats && ok; r++) ok = run(nullptr);
+  HIP_OK(hipEventRecord(t1, s->stream));
+  HIP_OK(hipEventSynchronize(t1));
+  float total = 0;
+  HIP_OK(hipEventElapsedTime(&total, t0, t1));
+  if (ms_per_apply) *ms_per_apply = total / repeats;
+  if (pass_ms) {
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < repeats && ok; r++) {
+      if (!s->plan) break;
+      ok = run(ev);
+      HIP_OK(hipStreamSynchronize(s->stream));
+      for (int q = 0; q < 5; q++) {
+        float t;
+        HIP_OK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+        acc[q] += t;
+      }
+      if (s->c) {
+        float t;
+        HIP_OK(hipEventElapsedTime(&t, ev[5], ev[6]));
+        acc[5] += t;
+      }
+    }
+    for (int q = 0; q < 6; q++) pass_ms[q] = acc[q] / repeats;
+  }
+  for (int i = 0; i < 7; i++) (void)hipEventDestroy(ev[i]);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return ok;
+}
+
+// Algorithmic bytes per fused apply (pruned five-pass pipeline):
+//   Z fwd : 3 N x in (+ N eps for the fused Jacobi), 3 nx ny Pz out
+//   Y fwd : 3 nx ny Pz in, 3 nx Py Pz out
+//   X     : 3 nx Py Pz in, 6 (Px/2+1)(Py/2+1)(Pz/2+1) spectrum, 3 nx Py Pz out
+//   Y inv : 3 nx Py Pz in, 3 nx ny Pz out
+//   Z inv : 3 nx ny Pz in, 3 N x + N eps re-read, 3 N out
+//   cross : c^2 + 2 c (3 N) block values + the 3 N volume vector read for B z_v
+//           and the 3 N read-modify-write of y_v for D z_c
+// with N the unknown-box cell count and 8 (FP32) or 16 (FP64) bytes a value.
+extern "C" int dielectric_fft_apply_bytes(const dielectric_fft_system* s, vie_hip_precision prec, double* pass,
+                                          double* total) {
+  if (!s) return 0;
+  double S = prec == VIE_HIP_MIXED ? 8.0 : 16.0;
+  double p[6] = {0, 0, 0, 0, 0, 0};
+  if (s->plan) {
+    const vie::Geom& g = s->plan->g;
+    double N = (double)g.nbox(), a = 3.0 * g.nx * g.ny * g.Pz, b = 3.0 * g.nx * g.Py * g.Pz;
+    double oct = 6.0 * (g.Px / 2 + 1) * (g.Py / 2 + 1) * (g.Pz / 2 + 1);
+    p[0] = (3 * N + N + a) * S;
+    p[1] = (a + b) * S;
+    p[2] = (2 * b + oct) * S;
+    p[3] = (b + a) * S;
+    p[4] = (a + 3 * N + N + 3 * N) * S;
+  }
+  double c = (double)s->c, nv = s->plan ? 3.0 * s->plan->g.nbox() : 0.0;
+  if (s->c) p[5] = (c * c + 2 * c * nv + (nv ? 3 * nv : 0)) * S;
+  double t = 0;
+  for (int q = 0; q < 6; q++) {
+    if (pass) pass[q] = p[q];
+    t += p[q];
+  }
+  if (total) *total = t;
+  return 1;
+}

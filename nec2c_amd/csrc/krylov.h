@@ -1,0 +1,82 @@
+// Device-resident GMRES building blocks (krylov.hip).
+#pragma once
+#include "vie_common.h"
+
+namespace vie {
+
+constexpr int kMaxRestart = 128;           // largest Krylov cycle the device solver keeps
+constexpr int kMaxConductorsShared = 256;  // conductor coefficients staged in LDS
+
+// Small per-solve state kept in device memory.  The host reads the leading
+// status words after each Arnoldi step.
+struct ArnoldiState {
+  // --- status block read by the host
+  double gabs;      // |g_{j+1}|, the residual estimate after column j
+  double beta;      // ||r|| at the start of the cycle
+  int breakdown;    // ||w|| <= eps_b ||A z||
+  int nonfinite;
+  int singular;     // zero pivot in back substitution
+  int used;         // columns in the current cycle
+  // --- working data
+  int j, m;
+  double inv_beta, inv_next;
+  c64 g[kMaxRestart + 1];
+  double cs[kMaxRestart];
+  c64 sn[kMaxRestart];
+  c64 y[kMaxRestart];
+  c64 H[(kMaxRestart + 1) * kMaxRestart];
+};
+
+struct ReduceScratch {
+  c64* partial = nullptr;      // (kMaxRestart + 2) * blocks partial sums
+  unsigned* counter = nullptr; // last-block counter, zero between launches
+};
+
+struct JacobiView {
+  size_t c, nbox;
+  const c64* cdiag;  // conductor diagonal C_ii
+  const c64* eps;    // [nbox]
+  c64 self[3];
+};
+
+template <typename T> struct CouplingView {
+  int c;
+  size_t nv, nbox;
+  const cx<T>* C;  // c x c row-major
+  const cx<T>* B;  // c rows of nv (volume -> conductor)
+  const cx<T>* D;  // c columns of nv (conductor -> volume)
+  cx<T>* zc;       // c scratch
+  c64* bz;         // c scratch (B z_v)
+  const cx<T>* eps;
+  cx<T> self[3];
+};
+
+size_t red_blocks(size_t n);
+
+template <typename T>
+void multidot(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const cx<T>* w, size_t n, int self_norm,
+              ReduceScratch& rs, c64* dots, hipStream_t s);
+template <typename T>
+void update(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const c64* h, cx<T>* w, size_t n,
+            int want_norm, ReduceScratch& rs, c64* norm_out, hipStream_t s);
+template <typename T> void norm2(const cx<T>* x, size_t n, ReduceScratch& rs, c64* out, hipStream_t s);
+void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, c64* out, hipStream_t s);
+void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, const c64* next2,
+                  double breakdown_eps, hipStream_t s);
+void backsolve(ArnoldiState* st, hipStream_t s);
+void cycle_init(ArnoldiState* st, const c64* beta2, hipStream_t s);
+template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s);
+template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st, hipStream_t s);
+template <typename T>
+void combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const ArnoldiState* st, const JacobiView& J, int precond,
+             hipStream_t s);
+template <typename T>
+void to_box(const c64* user, size_t c, size_t cells, const size_t* map, size_t nbox, cx<T>* sys, hipStream_t s);
+template <typename T>
+void from_box(const cx<T>* sys, size_t c, size_t cells, const size_t* map, size_t nbox, c64* user, hipStream_t s);
+void precondition_apply(const c64* x, c64* y, size_t n, const JacobiView& J, hipStream_t s);
+template <typename T>
+void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv, cx<T>* yc, cx<T>* yv, int precond,
+                    ReduceScratch& rs, hipStream_t s);
+
+}  // namespace vie

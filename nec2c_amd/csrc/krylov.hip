@@ -1,0 +1,556 @@
+// Device-resident vector kernels for the restarted GMRES and the conductor
+// coupling blocks.
+//
+// Reductions are deterministic: every block writes its partial sums, and the
+// last block to finish (counted with one device atomic, reset by that same
+// block so no memset is ever needed between launches) adds them in block
+// order.  Dot products and norms accumulate in FP64 whatever the storage
+// precision of the Krylov basis.
+#include "krylov.h"
+#include "vie_kernels.h"
+
+namespace vie {
+
+constexpr int KWG = 256;  // threads per block
+constexpr int KEPT = 4;   // elements per thread per vector
+constexpr int KCH = KWG * KEPT;
+
+size_t red_blocks(size_t n) { return (n + KCH - 1) / KCH; }
+
+__device__ inline c64 wave_reduce(c64 v) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    v.re += __shfl_xor(v.re, m, 64);
+    v.im += __shfl_xor(v.im, m, 64);
+  }
+  return v;
+}
+
+// Reduces per-wave values through LDS; returns the block total on thread 0.
+__device__ inline c64 block_reduce(c64 v, c64* lds) {
+  v = wave_reduce(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  c64 t = {0, 0};
+  if (threadIdx.x == 0)
+    for (int q = 0; q < KWG / 64; q++) t = t + lds[q];
+  return t;
+}
+
+// Last-block handshake: returns true in the one block that must finish.
+__device__ inline bool last_block(unsigned* counter, unsigned nblk) {
+  __shared__ bool is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned old = atomicAdd(counter, 1u);
+    is_last = (old == nblk - 1);
+    if (is_last) *counter = 0;
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last;
+}
+
+// Final stage: out[k] = sum_b partial[k * nblk + b] for k < nk (block order).
+__device__ inline void finish_sums(const c64* partial, int nk, unsigned nblk, c64* out, c64* lds) {
+  for (int k = 0; k < nk; k++) {
+    c64 s = {0, 0};
+    for (unsigned b = threadIdx.x; b < nblk; b += KWG) s = s + partial[(size_t)k * nblk + b];
+    s = block_reduce(s, lds);
+    if (threadIdx.x == 0) out[k] = s;
+  }
+}
+
+// ---------------------------------------------------------------- multidot
+// dots[k] = sum_i conj(V[k][i]) w[i]  (k < nk), plus dots[nk] = ||w||^2 when
+// self_norm.  V rows are ldv apart.
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_multidot(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host,
+                                                  const cx<T>* w, size_t n, int self_norm, c64* partial,
+                                                  unsigned* counter, c64* dots) {
+  __shared__ c64 lds[KWG / 64];
+  const int nk = nk_dev ? *nk_dev : nk_host;
+  const unsigned nblk = gridDim.x;
+  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
+  cx<T> wl[KEPT];
+#pragma unroll
+  for (int e = 0; e < KEPT; e++) {
+    size_t i = base + (size_t)e * KWG;
+    wl[e] = i < n ? w[i] : cx<T>{0, 0};
+  }
+  const int tot = nk + (self_norm ? 1 : 0);
+  for (int k = 0; k < tot; k++) {
+    c64 s = {0, 0};
+    if (k < nk) {
+      const cx<T>* vk = V + (size_t)k * ldv;
+#pragma unroll
+      for (int e = 0; e < KEPT; e++) {
+        size_t i = base + (size_t)e * KWG;
+        if (i < n) {
+          cx<T> a = vk[i];
+          double ar = a.re, ai = a.im, br = wl[e].re, bi = wl[e].im;
+          s.re += ar * br + ai * bi;
+          s.im += ar * bi - ai * br;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < KEPT; e++) {
+        double br = wl[e].re, bi = wl[e].im;
+        s.re += br * br + bi * bi;
+      }
+    }
+    s = block_reduce(s, lds);
+    if (threadIdx.x == 0) partial[(size_t)k * nblk + blockIdx.x] = s;
+  }
+  if (last_block(counter, nblk)) finish_sums(partial, tot, nblk, dots, lds);
+}
+
+// ---------------------------------------------------------------- update
+// w -= sum_k h[k] V[k] (h from device memory, FP64), optionally ||w||^2.
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_update(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host,
+                                                const c64* h, cx<T>* w, size_t n, int want_norm, c64* partial,
+                                                unsigned* counter, c64* norm_out) {
+  __shared__ c64 lds[KWG / 64];
+  __shared__ c64 hs[kMaxRestart + 1];
+  const int nk = nk_dev ? *nk_dev : nk_host;
+  for (int k = threadIdx.x; k < nk; k += KWG) hs[k] = h[k];
+  __syncthreads();
+  const unsigned nblk = gridDim.x;
+  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
+  double acc_re[KEPT], acc_im[KEPT];
+#pragma unroll
+  for (int e = 0; e < KEPT; e++) {
+    size_t i = base + (size_t)e * KWG;
+    cx<T> v = i < n ? w[i] : cx<T>{0, 0};
+    acc_re[e] = v.re;
+    acc_im[e] = v.im;
+  }
+  for (int k = 0; k < nk; k++) {
+    const cx<T>* vk = V + (size_t)k * ldv;
+    const double hr = hs[k].re, hi = hs[k].im;
+#pragma unroll
+    for (int e = 0; e < KEPT; e++) {
+      size_t i = base + (size_t)e * KWG;
+      if (i < n) {
+        cx<T> a = vk[i];
+        acc_re[e] -= hr * a.re - hi * a.im;
+        acc_im[e] -= hr * a.im + hi * a.re;
+      }
+    }
+  }
+  c64 s = {0, 0};
+#pragma unroll
+  for (int e = 0; e < KEPT; e++) {
+    size_t i = base + (size_t)e * KWG;
+    if (i < n) {
+      cx<T> r = {(T)acc_re[e], (T)acc_im[e]};
+      w[i] = r;
+      double rr = r.re, ri = r.im;
+      s.re += rr * rr + ri * ri;
+    }
+  }
+  if (!want_norm) return;
+  s = block_reduce(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (last_block(counter, nblk)) finish_sums(partial, 1, nblk, norm_out, lds);
+}
+
+// ---------------------------------------------------------------- Arnoldi tail
+// Finishes column j: H(:,j) = h1 + h2, H(j+1,j) = ||w||, applies the previous
+// Givens rotations, forms rotation j, updates g (vie_iterative.c:160-177).
+__device__ inline void givens(c64 a, c64 b, double* c, c64* s) {
+  double aa = hypot(a.re, a.im), bb = hypot(b.re, b.im);
+  if (bb == 0.0) {
+    *c = 1.0;
+    *s = c64{0, 0};
+  } else if (aa == 0.0) {
+    *c = 0.0;
+    *s = c64{b.re / bb, -b.im / bb};
+  } else {
+    double len = hypot(aa, bb);
+    *c = aa / len;
+    c64 u = {a.re / aa, a.im / aa}, v = {b.re / len, -b.im / len};
+    *s = u * v;
+  }
+}
+
+__global__ void k_arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2,
+                               const c64* next2, double breakdown_eps) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int j = st->j, m = st->m;
+  c64* H = st->H;  // row-major (m+1) x m
+  for (int k = 0; k <= j; k++) H[k * m + j] = h1[k] + (h2 ? h2[k] : c64{0, 0});
+  double before = sqrt(before2->re), next = sqrt(next2->re);
+  st->nonfinite = !(isfinite(before) && isfinite(next));
+  H[(j + 1) * m + j] = c64{next, 0};
+  st->breakdown = next <= breakdown_eps * before;
+  st->inv_next = st->breakdown || next == 0 ? 0.0 : 1.0 / next;
+  for (int k = 0; k < j; k++) {
+    c64 f = H[k * m + j], s = H[(k + 1) * m + j];
+    double c = st->cs[k];
+    c64 sn = st->sn[k];
+    H[k * m + j] = c * f + sn * s;
+    H[(k + 1) * m + j] = c * s - conj(sn) * f;
+  }
+  double c;
+  c64 sn;
+  givens(H[j * m + j], H[(j + 1) * m + j], &c, &sn);
+  st->cs[j] = c;
+  st->sn[j] = sn;
+  H[j * m + j] = c * H[j * m + j] + sn * H[(j + 1) * m + j];
+  H[(j + 1) * m + j] = c64{0, 0};
+  c64 gj = st->g[j];
+  st->g[j + 1] = c64{0, 0} - conj(sn) * gj;
+  st->g[j] = c * gj;
+  st->gabs = hypot(st->g[j + 1].re, st->g[j + 1].im);
+  st->used = j + 1;
+  st->j = j + 1;
+}
+
+// y = H(0:used,0:used)^{-1} g by back substitution (vie_iterative.c:179-186).
+__global__ void k_backsolve(ArnoldiState* st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int used = st->used, m = st->m;
+  const c64* H = st->H;
+  st->singular = 0;
+  for (int i = used - 1; i >= 0; i--) {
+    c64 v = st->g[i];
+    for (int k = i + 1; k < used; k++) v = v - H[i * m + k] * st->y[k];
+    c64 d = H[i * m + i];
+    if (d.re == 0.0 && d.im == 0.0) {
+      st->singular = 1;
+      return;
+    }
+    st->y[i] = cdiv(v, d);
+  }
+}
+
+// Starts a cycle: g = [beta, 0, ...], H = 0, j = 0.
+__global__ void k_cycle_init(ArnoldiState* st, const c64* beta2) {
+  const int m = st->m;
+  for (int i = threadIdx.x; i < (m + 1) * m; i += blockDim.x) st->H[i] = c64{0, 0};
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) st->g[i] = c64{0, 0};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double beta = sqrt(beta2->re);
+    st->g[0] = c64{beta, 0};
+    st->beta = beta;
+    st->inv_beta = beta > 0 ? 1.0 / beta : 0.0;
+    st->j = 0;
+    st->used = 0;
+    st->breakdown = 0;
+    st->nonfinite = !isfinite(beta);
+  }
+}
+
+// ---------------------------------------------------------------- elementwise
+template <typename T> __global__ void k_zero(cx<T>* x, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = cx<T>{0, 0};
+}
+
+template <typename To, typename From> __global__ void k_cast(cx<To>* y, const cx<From>* x, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = cast<To>(x[i]);
+}
+
+// V0 = r * (1/beta) in the basis precision.
+template <typename T> __global__ void k_scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double s = st->inv_beta;
+    v[i] = cx<T>{(T)(r[i].re * s), (T)(r[i].im * s)};
+  }
+}
+
+// v *= 1/||w|| in place (skipped on breakdown, where inv_next == 0 leaves the
+// slot unused by later columns).
+template <typename T> __global__ void k_scale_next(cx<T>* v, size_t n, const ArnoldiState* st) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s = st->inv_next;
+  if (i < n && s != 0.0) v[i] = cx<T>{(T)(v[i].re * s), (T)(v[i].im * s)};
+}
+
+// Jacobi inverse of element i of a system vector [c | 3 nbox].
+__device__ inline c64 jacobi_inv(const JacobiView& J, size_t i, c64 v) {
+  if (i < J.c) return cdiv(v, J.cdiag[i]);
+  size_t q = i - J.c;
+  size_t comp = q / J.nbox, cell = q - comp * J.nbox;
+  c64 e = J.eps[cell], one = {1, 0};
+  return cdiv(v, e - (e - one) * J.self[comp]);
+}
+
+// x += M^{-1} (sum_k y_k V[k])   (vie_iterative.c:187-188 with Z = M^{-1} V)
+template <typename T>
+__global__ void k_combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const ArnoldiState* st, JacobiView J,
+                          int precond) {
+  __shared__ c64 ys[kMaxRestart];
+  const int used = st->used;
+  for (int k = threadIdx.x; k < used; k += blockDim.x) ys[k] = st->y[k];
+  __syncthreads();
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  c64 s = {0, 0};
+  for (int k = 0; k < used; k++) {
+    cx<T> a = V[(size_t)k * ldv + i];
+    s = s + ys[k] * c64{(double)a.re, (double)a.im};
+  }
+  if (precond) s = jacobi_inv(J, i, s);
+  x[i] = x[i] + s;
+}
+
+// r = b - Ax and ||r||^2.
+__global__ __launch_bounds__(KWG) void k_residual(const c64* b, const c64* ax, c64* r, size_t n, c64* partial,
+                                                  unsigned* counter, c64* norm_out) {
+  __shared__ c64 lds[KWG / 64];
+  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
+  c64 s = {0, 0};
+#pragma unroll
+  for (int e = 0; e < KEPT; e++) {
+    size_t i = base + (size_t)e * KWG;
+    if (i < n) {
+      c64 v = b[i] - ax[i];
+      r[i] = v;
+      s.re += v.re * v.re + v.im * v.im;
+    }
+  }
+  s = block_reduce(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (last_block(counter, gridDim.x)) finish_sums(partial, 1, gridDim.x, norm_out, lds);
+}
+
+// ||x||^2 with a nonfinite flag folded in as NaN.
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_norm2(const cx<T>* x, size_t n, c64* partial, unsigned* counter,
+                                               c64* norm_out) {
+  __shared__ c64 lds[KWG / 64];
+  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
+  c64 s = {0, 0};
+#pragma unroll
+  for (int e = 0; e < KEPT; e++) {
+    size_t i = base + (size_t)e * KWG;
+    if (i < n) {
+      double a = x[i].re, b = x[i].im;
+      s.re += a * a + b * b;
+    }
+  }
+  s = block_reduce(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (last_block(counter, gridDim.x)) finish_sums(partial, 1, gridDim.x, norm_out, lds);
+}
+
+// ---------------------------------------------------------------- conductor coupling
+// z_c = v_c / C_ii (Jacobi) or v_c   (dielectric_fft.c:77-81)
+template <typename T> __global__ void k_cond_prep(const cx<T>* vc, const cx<T>* Cm, int c, int precond, cx<T>* zc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c) zc[i] = precond ? cdiv(vc[i], Cm[(size_t)i * c + i]) : vc[i];
+}
+
+// partial sums of B[r] . z_v (no conjugate), z_v = D^{-1} v_v on the fly.
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_bdot(const cx<T>* B, int c, const cx<T>* vv, size_t nv, size_t nbox,
+                                              const cx<T>* eps, cx<T> s0, cx<T> s1, cx<T> s2, int precond,
+                                              c64* partial, unsigned* counter, c64* out) {
+  __shared__ c64 lds[KWG / 64];
+  const unsigned nblk = gridDim.x;
+  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
+  const cx<T> one = {(T)1, (T)0};
+  cx<T> zl[KEPT];
+#pragma unroll
+  for (int e = 0; e < KEPT; e++) {
+    size_t i = base + (size_t)e * KWG;
+    cx<T> v = {0, 0};
+    if (i < nv) {
+      v = vv[i];
+      if (precond) {
+        size_t comp = i / nbox, cell = i - comp * nbox;
+        cx<T> ee = eps[cell], sf = comp == 0 ? s0 : comp == 1 ? s1 : s2;
+        v = cdiv(v, ee - (ee - one) * sf);
+      }
+    }
+    zl[e] = v;
+  }
+  for (int r = 0; r < c; r++) {
+    const cx<T>* br = B + (size_t)r * nv;
+    c64 s = {0, 0};
+#pragma unroll
+    for (int e = 0; e < KEPT; e++) {
+      size_t i = base + (size_t)e * KWG;
+      if (i < nv) {
+        cx<T> a = br[i];
+        double ar = a.re, ai = a.im, zr = zl[e].re, zi = zl[e].im;
+        s.re += ar * zr - ai * zi;
+        s.im += ar * zi + ai * zr;
+      }
+    }
+    s = block_reduce(s, lds);
+    if (threadIdx.x == 0) partial[(size_t)r * nblk + blockIdx.x] = s;
+  }
+  if (last_block(counter, nblk)) finish_sums(partial, c, nblk, out, lds);
+}
+
+// y_c = C z_c + (B z_v)   (dielectric_fft.c:56-60)
+template <typename T> __global__ void k_cond_finish(const cx<T>* Cm, const cx<T>* zc, const c64* bz, int c, cx<T>* yc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c) return;
+  c64 s = bz[i];
+  for (int j = 0; j < c; j++) {
+    cx<T> a = Cm[(size_t)i * c + j], z = zc[j];
+    s = s + c64{a.re, a.im} * c64{z.re, z.im};
+  }
+  yc[i] = cx<T>{(T)s.re, (T)s.im};
+}
+
+// y_v += sum_j D[j] z_c[j]   (dielectric_fft.c:63-64), D stored column-wise.
+template <typename T>
+__global__ void k_dadd(const cx<T>* D, int c, const cx<T>* zc, cx<T>* yv, size_t nv) {
+  __shared__ cx<T> zs[kMaxConductorsShared];
+  for (int j = threadIdx.x; j < c && j < kMaxConductorsShared; j += blockDim.x) zs[j] = zc[j];
+  __syncthreads();
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  c64 s = {(double)yv[i].re, (double)yv[i].im};
+  for (int j = 0; j < c; j++) {
+    cx<T> a = D[(size_t)j * nv + i], z = j < kMaxConductorsShared ? zs[j] : zc[j];
+    s = s + c64{a.re, a.im} * c64{z.re, z.im};
+  }
+  yv[i] = cx<T>{(T)s.re, (T)s.im};
+}
+
+// ---------------------------------------------------------------- layout
+// Scatter a host-order vector [c | 3 cells interleaved] into [c | 3][nbox].
+template <typename T>
+__global__ void k_to_box(const c64* user, size_t c, size_t cells, const size_t* map, size_t nbox, cx<T>* sys) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c) sys[i] = cast<T>(user[i]);
+  else if (i < c + 3 * cells) {
+    size_t q = i - c, cell = q / 3, comp = q - 3 * cell;
+    sys[c + comp * nbox + map[cell]] = cast<T>(user[i]);
+  }
+}
+
+template <typename T>
+__global__ void k_from_box(const cx<T>* sys, size_t c, size_t cells, const size_t* map, size_t nbox, c64* user) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c) user[i] = cast<double>(sys[i]);
+  else if (i < c + 3 * cells) {
+    size_t q = i - c, cell = q / 3, comp = q - 3 * cell;
+    user[i] = cast<double>(sys[c + comp * nbox + map[cell]]);
+  }
+}
+
+// Jacobi preconditioner application on a system vector (compatibility path).
+__global__ void k_precondition(const c64* x, c64* y, size_t n, JacobiView J) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = jacobi_inv(J, i, x[i]);
+}
+
+// ================================================================ host side
+static inline unsigned gblocks(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+template <typename T> void vec_zero(cx<T>* x, size_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL((k_zero<T>), dim3(gblocks(n)), dim3(256), 0, s, x, n);
+}
+template <typename To, typename From> void vec_cast(cx<To>* y, const cx<From>* x, size_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL((k_cast<To, From>), dim3(gblocks(n)), dim3(256), 0, s, y, x, n);
+}
+template void vec_zero<float>(c32*, size_t, hipStream_t);
+template void vec_zero<double>(c64*, size_t, hipStream_t);
+template void vec_cast<float, double>(c32*, const c64*, size_t, hipStream_t);
+template void vec_cast<double, float>(c64*, const c32*, size_t, hipStream_t);
+template void vec_cast<double, double>(c64*, const c64*, size_t, hipStream_t);
+
+template <typename T>
+void multidot(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const cx<T>* w, size_t n, int self_norm,
+              ReduceScratch& rs, c64* dots, hipStream_t s) {
+  unsigned nb = (unsigned)red_blocks(n);
+  hipLaunchKernelGGL((k_multidot<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk_dev, nk_host, w, n, self_norm,
+                     rs.partial, rs.counter, dots);
+}
+template <typename T>
+void update(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const c64* h, cx<T>* w, size_t n,
+            int want_norm, ReduceScratch& rs, c64* norm_out, hipStream_t s) {
+  unsigned nb = (unsigned)red_blocks(n);
+  hipLaunchKernelGGL((k_update<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk_dev, nk_host, h, w, n, want_norm,
+                     rs.partial, rs.counter, norm_out);
+}
+template <typename T> void norm2(const cx<T>* x, size_t n, ReduceScratch& rs, c64* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_norm2<T>), dim3((unsigned)red_blocks(n)), dim3(KWG), 0, s, x, n, rs.partial, rs.counter,
+                     out);
+}
+void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, c64* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual, dim3((unsigned)red_blocks(n)), dim3(KWG), 0, s, b, ax, r, n, rs.partial, rs.counter,
+                     out);
+}
+void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, const c64* next2,
+                  double breakdown_eps, hipStream_t s) {
+  hipLaunchKernelGGL(k_arnoldi_tail, dim3(1), dim3(64), 0, s, st, h1, h2, before2, next2, breakdown_eps);
+}
+void backsolve(ArnoldiState* st, hipStream_t s) { hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, s, st); }
+void cycle_init(ArnoldiState* st, const c64* beta2, hipStream_t s) {
+  hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2);
+}
+template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_scale_from<T>), dim3(gblocks(n)), dim3(256), 0, s, v, r, n, st);
+}
+template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_scale_next<T>), dim3(gblocks(n)), dim3(256), 0, s, v, n, st);
+}
+template <typename T>
+void combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const ArnoldiState* st, const JacobiView& J, int precond,
+             hipStream_t s) {
+  hipLaunchKernelGGL((k_combine<T>), dim3(gblocks(n)), dim3(256), 0, s, x, V, ldv, n, st, J, precond);
+}
+template <typename T>
+void to_box(const c64* user, size_t c, size_t cells, const size_t* map, size_t nbox, cx<T>* sys, hipStream_t s) {
+  size_t n = c + 3 * cells;
+  if (n) hipLaunchKernelGGL((k_to_box<T>), dim3(gblocks(n)), dim3(256), 0, s, user, c, cells, map, nbox, sys);
+}
+template <typename T>
+void from_box(const cx<T>* sys, size_t c, size_t cells, const size_t* map, size_t nbox, c64* user, hipStream_t s) {
+  size_t n = c + 3 * cells;
+  if (n) hipLaunchKernelGGL((k_from_box<T>), dim3(gblocks(n)), dim3(256), 0, s, sys, c, cells, map, nbox, user);
+}
+void precondition_apply(const c64* x, c64* y, size_t n, const JacobiView& J, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_precondition, dim3(gblocks(n)), dim3(256), 0, s, x, y, n, J);
+}
+
+template <typename T>
+void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv, cx<T>* yc, cx<T>* yv, int precond,
+                    ReduceScratch& rs, hipStream_t s) {
+  const int c = cv.c;
+  if (c == 0) return;
+  hipLaunchKernelGGL((k_cond_prep<T>), dim3(gblocks(c)), dim3(256), 0, s, vc, cv.C, c, precond, cv.zc);
+  if (cv.nv) {
+    unsigned nb = (unsigned)red_blocks(cv.nv);
+    hipLaunchKernelGGL((k_bdot<T>), dim3(nb), dim3(KWG), 0, s, cv.B, c, vv, cv.nv, cv.nbox, cv.eps, cv.self[0],
+                       cv.self[1], cv.self[2], precond, rs.partial, rs.counter, cv.bz);
+  } else {
+    hipLaunchKernelGGL((k_zero<double>), dim3(gblocks(c)), dim3(256), 0, s, cv.bz, (size_t)c);
+  }
+  hipLaunchKernelGGL((k_cond_finish<T>), dim3(gblocks(c)), dim3(256), 0, s, cv.C, cv.zc, cv.bz, c, yc);
+  if (cv.nv) hipLaunchKernelGGL((k_dadd<T>), dim3(gblocks(cv.nv)), dim3(256), 0, s, cv.D, c, cv.zc, yv, cv.nv);
+}
+
+#define VIE_INST(T)                                                                                              \
+  template void multidot<T>(const cx<T>*, size_t, const int*, int, const cx<T>*, size_t, int, ReduceScratch&, c64*, \
+                            hipStream_t);                                                                        \
+  template void update<T>(const cx<T>*, size_t, const int*, int, const c64*, cx<T>*, size_t, int, ReduceScratch&,  \
+                          c64*, hipStream_t);                                                                    \
+  template void norm2<T>(const cx<T>*, size_t, ReduceScratch&, c64*, hipStream_t);                               \
+  template void scale_from<T>(cx<T>*, const c64*, size_t, const ArnoldiState*, hipStream_t);                     \
+  template void scale_next<T>(cx<T>*, size_t, const ArnoldiState*, hipStream_t);                                 \
+  template void combine<T>(c64*, const cx<T>*, size_t, size_t, const ArnoldiState*, const JacobiView&, int,       \
+                           hipStream_t);                                                                         \
+  template void to_box<T>(const c64*, size_t, size_t, const size_t*, size_t, cx<T>*, hipStream_t);                \
+  template void from_box<T>(const cx<T>*, size_t, size_t, const size_t*, size_t, c64*, hipStream_t);              \
+  template void coupling_apply<T>(const CouplingView<T>&, const cx<T>*, const cx<T>*, cx<T>*, cx<T>*, int,         \
+                                  ReduceScratch&, hipStream_t);
+VIE_INST(float)
+VIE_INST(double)
+
+}  // namespace vie

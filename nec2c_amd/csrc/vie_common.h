@@ -1,0 +1,58 @@
+// Shared device/host definitions for the MI355X (gfx950) VIE FFT-GMRES library.
+//
+// Complex values are interleaved (re, im) pairs of float or double, the same
+// memory layout as C99 `complex float` / `complex double`, so the C-ABI can
+// hand host `complex double` arrays straight to hipMemcpy.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace vie {
+
+template <typename T> struct alignas(2 * sizeof(T)) cx { T re, im; };
+
+using c32 = cx<float>;
+using c64 = cx<double>;
+
+template <typename T> __host__ __device__ inline cx<T> mk(T r, T i) { return cx<T>{r, i}; }
+template <typename T> __host__ __device__ inline cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T> __host__ __device__ inline cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T> __host__ __device__ inline cx<T> operator*(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T> __host__ __device__ inline cx<T> operator*(T s, cx<T> a) { return {s * a.re, s * a.im}; }
+template <typename T> __host__ __device__ inline cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
+// multiply by -i
+template <typename T> __host__ __device__ inline cx<T> mul_mi(cx<T> a) { return {a.im, -a.re}; }
+template <typename T> __host__ __device__ inline cx<T> cdiv(cx<T> a, cx<T> b) {
+  // Smith's algorithm keeps the quotient finite for widely scaled operands.
+  if ((b.re < 0 ? -b.re : b.re) >= (b.im < 0 ? -b.im : b.im)) {
+    T r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  } else {
+    T r = b.re / b.im, d = b.im + b.re * r;
+    return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+  }
+}
+template <typename To, typename From> __host__ __device__ inline cx<To> cast(cx<From> a) {
+  return {(To)a.re, (To)a.im};
+}
+
+// Elements staged in LDS per workgroup by the pencil-FFT passes, and the
+// workgroup size.  FP32: 4096 x 8 B = 32 KiB; FP64: 2048 x 16 B = 32 KiB.
+template <typename T> struct PassShape;
+template <> struct PassShape<float> { static constexpr int E = 4096; static constexpr int WG = 256; };
+template <> struct PassShape<double> { static constexpr int E = 2048; static constexpr int WG = 256; };
+
+// Geometry of one FFT operator instance (one lattice).
+struct Geom {
+  int nx, ny, nz;        // unknown box (physical cells; holes carry eps = 1, x = 0)
+  int Px, Py, Pz;        // padded FFT extents, powers of two >= 2 n_kernel - 1
+  int kz;                // kernel z extent (nz, or 2 nz + gap for the PEC image)
+  int pec;               // 1: image cells mirrored below the physical box
+  int zoff;              // z index of the first physical plane inside the FFT grid
+  __host__ __device__ size_t nbox() const { return (size_t)nx * ny * nz; }
+};
+
+}  // namespace vie

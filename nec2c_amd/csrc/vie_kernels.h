@@ -1,0 +1,42 @@
+// Internal interfaces between the C-ABI layer and the HIP kernels.
+#pragma once
+#include <vector>
+#include "vie_common.h"
+
+namespace vie {
+
+// Device views of one FFT volume operator at precision T.
+template <typename T> struct VolumeView {
+  Geom g;
+  cx<T>* A;           // [3][nx][ny][Pz] workspace
+  cx<T>* B;           // [3][nx][Py][Pz] workspace
+  const cx<T>* spec;  // [6][Px/2+1][Py/2+1][Pz/2+1], scaled by 1/(Px Py Pz)
+  const cx<T>* eps;   // [nbox]
+  const cx<T>* tw;    // twiddle tables, x then y then z
+  cx<T> self[3];
+};
+
+bool max_fft_extent_ok(int P);
+
+// y = eps z - (eps-1) conv(z), z = precond ? D^{-1} xin : xin.  Volume part only.
+// ev (optional, 6 events): recorded before each of the five passes and after the last.
+template <typename T>
+int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, hipStream_t s,
+                 hipEvent_t* ev = nullptr);
+
+// ---- kernel spectrum (FP64) ----
+struct SpectrumResult {
+  c64 self[3];
+  int near_offsets = 0, far_offsets = 0, fallback_offsets = 0, max_near_order = 0;
+};
+// Builds the octant spectrum [6][Px/2+1][Py/2+1][Pz/2+1] (FP64, scaled by
+// 1/G) into spec64 and its FP32 rounding into spec32 (either may be null).
+// kshape: lattice extent of the kernel (z includes the PEC image).
+int build_spectrum(const int kshape[3], const int padded[3], double h, double k, c64* spec64,
+                   c32* spec32, SpectrumResult* out, hipStream_t s);
+
+// ---- vector kernels (krylov.hip) ----
+template <typename T> void vec_zero(cx<T>* x, size_t n, hipStream_t s);
+template <typename To, typename From> void vec_cast(cx<To>* y, const cx<From>* x, size_t n, hipStream_t s);
+
+}  // namespace vie

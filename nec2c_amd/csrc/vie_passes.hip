@@ -1,0 +1,328 @@
+// Fused five-pass FFT volume operator for gfx950.
+//
+// Computes, for the unknown box [3][nx][ny][nz] (component-major, z fastest),
+//   y = eps z - (eps - 1) IFFT( T^ . FFT( pad(z) ) ),   z = D^{-1} v or v,
+// which is the reference's apply (vie_fft.c:239-264) with its Jacobi
+// preconditioner (vie_fft.c:265-273) optionally fused on the input.  Passes:
+//   1. Z forward : scatter + zero-pad (+ PEC image) + Jacobi, FFT along z     -> A[3][nx][ny][Pz]
+//   2. Y forward : FFT along y of the ny nonzero rows                         -> B[3][nx][Py][Pz]
+//   3. X         : FFT along x, 3x3 symmetric spectral multiply, inverse FFT,
+//                  keep the nx physical rows                                  -> B (in place)
+//   4. Y inverse : inverse FFT along y, keep ny rows                          -> A
+//   5. Z inverse : inverse FFT along z, gather physical z, contrast           -> y
+// Only the region that can hold nonzeros is ever read or written, so each
+// pass touches the minimum the pruned pencil pipeline allows.  The 1/(PxPyPz)
+// inverse scaling lives in the spectrum.
+#include "fft_core.h"
+#include "vie_kernels.h"
+
+namespace vie {
+
+template <typename T> struct OpArgs {
+  Geom g;
+  const cx<T>* xin;   // [3][nbox] input (v when precond, else z)
+  cx<T>* yout;        // [3][nbox] output
+  cx<T>* A;           // [3][nx][ny][Pz]
+  cx<T>* B;           // [3][nx][Py][Pz]
+  const cx<T>* spec;  // [6][Px/2+1][Py/2+1][Pz/2+1] octant spectrum / G
+  const cx<T>* eps;   // [nbox] relative permittivity (1 in holes)
+  const cx<T>* tw;    // twiddles: x table (Px), then y (Py), then z (Pz)
+  cx<T> self[3];      // self-term diagonal N_ii(0)
+  int precond;
+};
+
+template <typename T> __device__ __forceinline__ cx<T> jacobi_scale(cx<T> v, cx<T> e, cx<T> s) {
+  // v / (eps - (eps - 1) self)   (vie_fft.c:271)
+  cx<T> one = {(T)1, (T)0};
+  return cdiv(v, e - (e - one) * s);
+}
+
+template <typename T> __device__ __forceinline__ cx<T> ld(const cx<T>* p) { return *p; }
+
+// ---------------------------------------------------------------- pass 1
+template <typename T, int PZ, int LW>
+__global__ __launch_bounds__(256) void k_zfwd(OpArgs<T> a) {
+  constexpr int WG = 256, E = LW * PZ;
+  __shared__ cx<T> lds[E];
+  __shared__ cx<T> tw[PZ];
+  const int tid = threadIdx.x;
+  const Geom& g = a.g;
+  const long lines = (long)g.nx * g.ny;  // (x, y) pencils
+  const long line0 = (long)blockIdx.x * LW;
+  const cx<T>* twz = a.tw + g.Px + g.Py;
+  for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
+  const size_t nbox = g.nbox();
+  for (int comp = 0; comp < 3; comp++) {
+    const cx<T> sgn = {(T)(comp == 2 ? 1 : -1), (T)0};
+    for (int f = tid; f < E; f += WG) {
+      int l = f / PZ, e = f % PZ;
+      long line = line0 + l;
+      cx<T> v = {(T)0, (T)0};
+      if (line < lines) {
+        int z = -1;
+        bool image = false;
+        if (e >= g.zoff && e - g.zoff < g.nz) z = e - g.zoff;
+        else if (g.pec && e < g.nz) { z = g.nz - 1 - e; image = true; }
+        if (z >= 0) {
+          size_t cell = (size_t)line * g.nz + z;
+          v = a.xin[comp * nbox + cell];
+          if (a.precond) v = jacobi_scale(v, a.eps[cell], a.self[comp]);
+          if (image) v = sgn * v;
+        }
+      }
+      lds[l * PZ + e] = v;
+    }
+    __syncthreads();
+    fft_lines<T, PZ, LW, WG, 1, PZ, false>(lds, tw, tid);
+    for (int f = tid; f < E; f += WG) {
+      int l = f / PZ, e = f % PZ;
+      long line = line0 + l;
+      if (line < lines) a.A[((size_t)comp * lines + line) * PZ + e] = lds[l * PZ + e];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- pass 2 / 4
+// Lines are (comp, x, kz) with kz fastest; a workgroup takes LW consecutive
+// lines so each y row it loads is LW contiguous values.
+template <typename T, int PY, int LW, bool INV>
+__global__ __launch_bounds__(256) void k_ypass(OpArgs<T> a) {
+  constexpr int WG = 256, E = LW * PY;
+  __shared__ cx<T> lds[E];
+  __shared__ cx<T> tw[PY];
+  const int tid = threadIdx.x;
+  const Geom& g = a.g;
+  const long lines = 3L * g.nx * g.Pz;
+  const long line0 = (long)blockIdx.x * LW;
+  const cx<T>* twy = a.tw + g.Px;
+  for (int i = tid; i < PY; i += WG) tw[i] = twy[i];
+  const int nin = INV ? PY : g.ny;      // rows read
+  const cx<T>* src = INV ? a.B : a.A;
+  const int srows = INV ? g.Py : g.ny;  // rows per (comp, x) in the source
+  for (int f = tid; f < E; f += WG) {
+    int e = f / LW, l = f % LW;
+    long line = line0 + l;
+    cx<T> v = {(T)0, (T)0};
+    if (line < lines && e < nin) {
+      long cxi = line / g.Pz, kz = line % g.Pz;
+      v = src[((size_t)cxi * srows + e) * g.Pz + kz];
+      if (INV) v = conj(v);
+    }
+    lds[e * LW + l] = v;
+  }
+  __syncthreads();
+  fft_lines<T, PY, LW, WG, LW, 1, true>(lds, tw, tid);
+  cx<T>* dst = INV ? a.A : a.B;
+  const int nout = INV ? g.ny : PY;
+  const int drows = INV ? g.ny : g.Py;
+  for (int f = tid; f < E; f += WG) {
+    int e = f / LW, l = f % LW;
+    long line = line0 + l;
+    if (line < lines && e < nout) {
+      long cxi = line / g.Pz, kz = line % g.Pz;
+      cx<T> v = lds[e * LW + l];
+      dst[((size_t)cxi * drows + e) * g.Pz + kz] = INV ? conj(v) : v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- pass 3
+// Lines are (ky, kz) with kz fastest.  Forward x FFT of each component is
+// staged through LDS and parked in registers; the 3x3 multiply runs in
+// registers; each product component goes back through LDS for the inverse.
+template <typename T, int PX, int LW>
+__global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
+  constexpr int WG = 256, E = LW * PX, EPT = (E + WG - 1) / WG;
+  __shared__ cx<T> lds[E];
+  __shared__ cx<T> tw[PX];
+  const int tid = threadIdx.x;
+  const Geom& g = a.g;
+  const long lines = (long)g.Py * g.Pz;
+  const long line0 = (long)blockIdx.x * LW;
+  const size_t plane = (size_t)g.Py * g.Pz;
+  for (int i = tid; i < PX; i += WG) tw[i] = a.tw[i];
+  cx<T> hold[3][EPT];
+  for (int comp = 0; comp < 3; comp++) {
+    for (int f = tid; f < E; f += WG) {
+      int e = f / LW, l = f % LW;
+      long line = line0 + l;
+      cx<T> v = {(T)0, (T)0};
+      if (line < lines && e < g.nx) v = a.B[((size_t)comp * g.nx + e) * plane + line];
+      lds[e * LW + l] = v;
+    }
+    __syncthreads();
+    fft_lines<T, PX, LW, WG, LW, 1, true>(lds, tw, tid);
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      int f = tid + i * WG;
+      if (E % WG == 0 || f < E) hold[comp][i] = lds[f];
+    }
+    __syncthreads();
+  }
+  // spectral multiply: dst_i = sum_j T^_ij src_j   (vie_fft.c:253-256)
+  const int Hx = g.Px / 2 + 1, Hy = g.Py / 2 + 1, Hz = g.Pz / 2 + 1;
+  const size_t oct = (size_t)Hx * Hy * Hz;
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    int f = tid + i * WG;
+    if (E % WG == 0 || f < E) {
+      int kx = f / LW, l = f % LW;
+      long line = line0 + l;
+      if (line < lines) {
+        int ky = (int)(line / g.Pz), kz = (int)(line % g.Pz);
+        int ox = kx <= g.Px / 2 ? kx : g.Px - kx;
+        int oy = ky <= g.Py / 2 ? ky : g.Py - ky;
+        int oz = kz <= g.Pz / 2 ? kz : g.Pz - kz;
+        T sx = kx <= g.Px / 2 ? (T)1 : (T)-1;
+        T sy = ky <= g.Py / 2 ? (T)1 : (T)-1;
+        T sz = kz <= g.Pz / 2 ? (T)1 : (T)-1;
+        size_t o = ((size_t)ox * Hy + oy) * Hz + oz;
+        cx<T> txx = a.spec[o], tyy = a.spec[oct + o], tzz = a.spec[2 * oct + o];
+        cx<T> txy = (sx * sy) * a.spec[3 * oct + o];
+        cx<T> txz = (sx * sz) * a.spec[4 * oct + o];
+        cx<T> tyz = (sy * sz) * a.spec[5 * oct + o];
+        cx<T> s0 = hold[0][i], s1 = hold[1][i], s2 = hold[2][i];
+        hold[0][i] = txx * s0 + txy * s1 + txz * s2;
+        hold[1][i] = txy * s0 + tyy * s1 + tyz * s2;
+        hold[2][i] = txz * s0 + tyz * s1 + tzz * s2;
+      }
+    }
+  }
+  for (int comp = 0; comp < 3; comp++) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      int f = tid + i * WG;
+      if (E % WG == 0 || f < E) lds[f] = conj(hold[comp][i]);
+    }
+    __syncthreads();
+    fft_lines<T, PX, LW, WG, LW, 1, true>(lds, tw, tid);
+    for (int f = tid; f < E; f += WG) {
+      int e = f / LW, l = f % LW;
+      long line = line0 + l;
+      if (line < lines && e < g.nx) a.B[((size_t)comp * g.nx + e) * plane + line] = conj(lds[e * LW + l]);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- pass 5
+template <typename T, int PZ, int LW>
+__global__ __launch_bounds__(256) void k_zinv(OpArgs<T> a) {
+  constexpr int WG = 256, E = LW * PZ;
+  __shared__ cx<T> lds[E];
+  __shared__ cx<T> tw[PZ];
+  const int tid = threadIdx.x;
+  const Geom& g = a.g;
+  const long lines = (long)g.nx * g.ny;
+  const long line0 = (long)blockIdx.x * LW;
+  const cx<T>* twz = a.tw + g.Px + g.Py;
+  for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
+  const size_t nbox = g.nbox();
+  const cx<T> one = {(T)1, (T)0};
+  for (int comp = 0; comp < 3; comp++) {
+    for (int f = tid; f < E; f += WG) {
+      int l = f / PZ, e = f % PZ;
+      long line = line0 + l;
+      cx<T> v = {(T)0, (T)0};
+      if (line < lines) v = conj(a.A[((size_t)comp * lines + line) * PZ + e]);
+      lds[l * PZ + e] = v;
+    }
+    __syncthreads();
+    fft_lines<T, PZ, LW, WG, 1, PZ, false>(lds, tw, tid);
+    // gather + contrast: y = eps z - (eps - 1) dst   (vie_fft.c:261-262)
+    for (int f = tid; f < LW * g.nz; f += WG) {
+      int l = f / g.nz, z = f % g.nz;
+      long line = line0 + l;
+      if (line < lines) {
+        size_t cell = (size_t)line * g.nz + z;
+        cx<T> e = a.eps[cell];
+        cx<T> v = a.xin[comp * nbox + cell];
+        if (a.precond) v = jacobi_scale(v, e, a.self[comp]);
+        cx<T> d = conj(lds[l * PZ + g.zoff + z]);
+        a.yout[comp * nbox + cell] = e * v - (e - one) * d;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- launch
+template <typename T, int P> struct PassLauncher {
+  static constexpr int E = PassShape<T>::E;
+  static constexpr int LWZ = (E / P) < 16 ? (E / P) : 16;
+  static constexpr int LWY = (E / P) < 16 ? (E / P) : 16;
+  static void zfwd(const OpArgs<T>& a, hipStream_t s) {
+    long lines = (long)a.g.nx * a.g.ny;
+    hipLaunchKernelGGL((k_zfwd<T, P, LWZ>), dim3((lines + LWZ - 1) / LWZ), dim3(256), 0, s, a);
+  }
+  static void zinv(const OpArgs<T>& a, hipStream_t s) {
+    long lines = (long)a.g.nx * a.g.ny;
+    hipLaunchKernelGGL((k_zinv<T, P, LWZ>), dim3((lines + LWZ - 1) / LWZ), dim3(256), 0, s, a);
+  }
+  static void ypass(const OpArgs<T>& a, bool inv, hipStream_t s) {
+    long lines = 3L * a.g.nx * a.g.Pz;
+    dim3 grid((lines + LWY - 1) / LWY);
+    if (inv) hipLaunchKernelGGL((k_ypass<T, P, LWY, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_ypass<T, P, LWY, false>), grid, dim3(256), 0, s, a);
+  }
+  static void xpass(const OpArgs<T>& a, hipStream_t s) {
+    long lines = (long)a.g.Py * a.g.Pz;
+    hipLaunchKernelGGL((k_xpass<T, P, LWY>), dim3((lines + LWY - 1) / LWY), dim3(256), 0, s, a);
+  }
+};
+
+template <typename T, typename F> static bool dispatch_p(int P, F&& f) {
+  switch (P) {
+    case 1: f(PassLauncher<T, 1>()); return true;
+    case 2: f(PassLauncher<T, 2>()); return true;
+    case 4: f(PassLauncher<T, 4>()); return true;
+    case 8: f(PassLauncher<T, 8>()); return true;
+    case 16: f(PassLauncher<T, 16>()); return true;
+    case 32: f(PassLauncher<T, 32>()); return true;
+    case 64: f(PassLauncher<T, 64>()); return true;
+    case 128: f(PassLauncher<T, 128>()); return true;
+    case 256: f(PassLauncher<T, 256>()); return true;
+    case 512: f(PassLauncher<T, 512>()); return true;
+    case 1024: f(PassLauncher<T, 1024>()); return true;
+    default: return false;
+  }
+}
+
+bool max_fft_extent_ok(int P) { return P >= 1 && P <= 1024 && (P & (P - 1)) == 0; }
+
+template <typename T>
+int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, hipStream_t s, hipEvent_t* ev) {
+  OpArgs<T> a;
+  a.g = v.g;
+  a.xin = xin;
+  a.yout = yout;
+  a.A = v.A;
+  a.B = v.B;
+  a.spec = v.spec;
+  a.eps = v.eps;
+  a.tw = v.tw;
+  for (int i = 0; i < 3; i++) a.self[i] = v.self[i];
+  a.precond = precond;
+  // A unit axis pads to P = 1, whose DFT is the identity (zero stages).
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
+  mark(0);
+  bool ok = dispatch_p<T>(v.g.Pz, [&](auto L) { L.zfwd(a, s); });
+  mark(1);
+  ok = ok && dispatch_p<T>(v.g.Py, [&](auto L) { L.ypass(a, false, s); });
+  mark(2);
+  ok = ok && dispatch_p<T>(v.g.Px, [&](auto L) { L.xpass(a, s); });
+  mark(3);
+  ok = ok && dispatch_p<T>(v.g.Py, [&](auto L) { L.ypass(a, true, s); });
+  mark(4);
+  ok = ok && dispatch_p<T>(v.g.Pz, [&](auto L) { L.zinv(a, s); });
+  mark(5);
+  return ok && hipGetLastError() == hipSuccess;
+}
+
+template int volume_apply<float>(const VolumeView<float>&, const c32*, c32*, int, hipStream_t, hipEvent_t*);
+template int volume_apply<double>(const VolumeView<double>&, const c64*, c64*, int, hipStream_t, hipEvent_t*);
+
+}  // namespace vie

@@ -1,0 +1,500 @@
+"""Python mirror of the reference's VIE FFT-GMRES interface over libvie_hip.so.
+
+The names, argument meaning and error behaviour follow the reference's C API
+(vie_fft.h, vie_iterative.h, dielectric_fft.h); every call goes through the
+C-ABI of nec2c_amd/lib/libvie_hip.so, whose kernels run on the GPU.  There is
+no CPU compute path here: if the library is missing, ``lib()`` raises.
+
+Vectors are numpy complex128 arrays in the reference's layout: conductor
+unknowns first, then 3 interleaved components per cell in the caller's cell
+order.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libvie_hip.so")
+
+FP64, MIXED = 0, 1
+STATUS = ("converged", "iteration limit", "Krylov breakdown",
+          "operator/preconditioner failure", "invalid input",
+          "workspace allocation/size failure", "nonfinite arithmetic")
+
+
+class CDouble(ctypes.Structure):
+    """C99 ``double _Complex`` passed by value (two SSE doubles on x86-64)."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+    @classmethod
+    def of(cls, z):
+        z = complex(z)
+        return cls(z.real, z.imag)
+
+
+VectorMap = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+
+
+class LinearOperator(ctypes.Structure):        # vie_iterative.h:11-18
+    _fields_ = [("order", ctypes.c_size_t), ("apply", ctypes.c_void_p), ("context", ctypes.c_void_p),
+                ("precondition", ctypes.c_void_p), ("precondition_context", ctypes.c_void_p)]
+
+
+class GmresOptions(ctypes.Structure):          # vie_iterative.h:20-26
+    _fields_ = [("restart", ctypes.c_size_t), ("maximum_iterations", ctypes.c_size_t),
+                ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double)]
+
+
+class GmresReport(ctypes.Structure):           # vie_iterative.h:39-47
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_size_t),
+                ("operator_applications", ctypes.c_size_t), ("restarts", ctypes.c_size_t),
+                ("residual_norm", ctypes.c_double), ("relative_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SolveOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("inner_reduction", ctypes.c_double)]
+
+
+class Material(ctypes.Structure):              # dielectric.h:26-35
+    _fields_ = [("id", ctypes.c_int), ("epsilon_r", ctypes.c_double), ("source_line", ctypes.c_size_t),
+                ("epsilon_loss", ctypes.c_double), ("conductivity", ctypes.c_double),
+                ("tabulated", ctypes.c_int), ("samples", ctypes.c_void_p), ("sample_count", ctypes.c_size_t)]
+
+
+class Model(ctypes.Structure):                 # dielectric.h:88-107
+    _fields_ = [("loaded", ctypes.c_int), ("format_version", ctypes.c_int),
+                ("path_is_absolute", ctypes.c_int), ("mesh_record_present", ctypes.c_int),
+                ("resource_path", ctypes.c_char_p),
+                ("materials", ctypes.POINTER(Material)), ("material_count", ctypes.c_size_t),
+                ("samples", ctypes.c_void_p), ("sample_count", ctypes.c_size_t),
+                ("regions", ctypes.c_void_p), ("region_count", ctypes.c_size_t),
+                ("boxes", ctypes.c_void_p), ("box_count", ctypes.c_size_t),
+                ("max_kh", ctypes.c_double), ("max_cells", ctypes.c_size_t), ("max_aspect", ctypes.c_double)]
+
+
+class Diagnostic(ctypes.Structure):            # dielectric.h:109-116
+    _fields_ = [("resource", ctypes.c_char * (255 * 4 + 1)), ("record", ctypes.c_char * 32),
+                ("value", ctypes.c_char * 128), ("message", ctypes.c_char * 256), ("line", ctypes.c_size_t)]
+
+
+class MeshCell(ctypes.Structure):              # dielectric.h:118-127
+    _fields_ = [("box_id", ctypes.c_size_t), ("region_id", ctypes.c_int), ("material_id", ctypes.c_int),
+                ("center", ctypes.c_double * 3), ("size", ctypes.c_double * 3),
+                ("q", ctypes.c_double), ("aspect", ctypes.c_double)]
+
+
+class MeshResult(ctypes.Structure):            # dielectric.h:147-178
+    _fields_ = [("built", ctypes.c_int), ("accuracy_warning", ctypes.c_int), ("frequency_mhz", ctypes.c_double),
+                ("regions", ctypes.c_void_p), ("region_count", ctypes.c_size_t),
+                ("boxes", ctypes.c_void_p), ("box_count", ctypes.c_size_t),
+                ("cells", ctypes.POINTER(MeshCell)), ("cell_count", ctypes.c_size_t)] + \
+               [(f, ctypes.c_size_t) for f in ("vacuum_box_count", "feature_warning_count")] + \
+               [(f, ctypes.c_double) for f in ("declared_volume", "active_volume", "vacuum_volume",
+                                               "numerical_volume", "volume_relative_error", "q_min",
+                                               "q_median", "q_p95", "q_max", "aspect_max")] + \
+               [("has_conductors", ctypes.c_int), ("minimum_conductor_gap", ctypes.c_double),
+                ("required_conductor_gap", ctypes.c_double)] + \
+               [(f, ctypes.c_size_t) for f in ("conductor_unknowns", "volume_unknowns", "total_unknowns",
+                                               "matrix_bytes", "workspace_bytes")]
+
+
+# every symbol the public headers declare (include/*.h)
+EXPORTS = (
+    "vie_gmres_default_options", "vie_iterative_status_name", "vie_gmres", "vie_dense_gmres",
+    "vie_fft_backend_set", "vie_fft_fftw_available", "vie_fft_backend_requested_name",
+    "vie_fft_backend_name", "vie_fft_backend_last_used_name", "vie_fft_backend_clear_last_used",
+    "vie_fft_create", "vie_fft_set_permittivity", "vie_fft_destroy", "vie_fft_bytes",
+    "vie_fft_threads", "vie_fft_parallel_available", "vie_fft_operator",
+    "dielectric_gmres_settings", "dielectric_fft_prepare", "dielectric_fft_prepare_pec",
+    "dielectric_fft_free", "dielectric_fft_entry", "dielectric_fft_get_operator",
+    "dielectric_fft_storage", "dielectric_fft_threads", "dielectric_fft_backend",
+    "dielectric_fft_solve", "vie_fft_solve", "vie_hip_set_solve_options", "vie_hip_get_solve_options",
+    "dielectric_fft_budget", "dielectric_fft_set_budget", "dielectric_fft_blocks_changed",
+    "dielectric_fft_plan", "vie_hip_last_error", "vie_hip_device_count",
+    "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
+    "dielectric_fft_time_apply", "dielectric_fft_apply_bytes",
+)
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libvie_hip.so; raise loudly when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m nec2c_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, d, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    sig = {
+        "vie_gmres_default_options": (None, [ctypes.POINTER(GmresOptions)]),
+        "vie_iterative_status_name": (ctypes.c_char_p, [i]),
+        "vie_gmres": (i, [ctypes.POINTER(LinearOperator), vp, vp, ctypes.POINTER(GmresOptions),
+                          ctypes.POINTER(GmresReport)]),
+        "vie_dense_gmres": (i, [vp, sz, vp, vp, ctypes.POINTER(GmresOptions), ctypes.POINTER(GmresReport)]),
+        "vie_fft_backend_set": (i, [ctypes.c_char_p]),
+        "vie_fft_fftw_available": (i, []),
+        "vie_fft_backend_requested_name": (ctypes.c_char_p, []),
+        "vie_fft_backend_name": (ctypes.c_char_p, [vp]),
+        "vie_fft_backend_last_used_name": (ctypes.c_char_p, []),
+        "vie_fft_backend_clear_last_used": (None, []),
+        "vie_fft_create": (i, [ctypes.POINTER(vp), szp, sz, szp, d, d, CDouble, sz]),
+        "vie_fft_set_permittivity": (i, [vp, sz, CDouble]),
+        "vie_fft_destroy": (None, [vp]),
+        "vie_fft_bytes": (sz, [vp]),
+        "vie_fft_threads": (sz, [vp]),
+        "vie_fft_parallel_available": (i, []),
+        "vie_fft_operator": (None, [vp, ctypes.POINTER(LinearOperator)]),
+        "dielectric_fft_prepare": (i, [ctypes.POINTER(vp), ctypes.POINTER(Model), ctypes.POINTER(MeshResult),
+                                       sz, d, sz, ctypes.POINTER(Diagnostic)]),
+        "dielectric_fft_prepare_pec": (i, [ctypes.POINTER(vp), ctypes.POINTER(Model), ctypes.POINTER(MeshResult),
+                                           sz, d, sz, ctypes.POINTER(Diagnostic)]),
+        "dielectric_fft_free": (None, [vp]),
+        "dielectric_fft_entry": (vp, [vp, sz, sz]),
+        "dielectric_fft_get_operator": (None, [vp, ctypes.POINTER(LinearOperator)]),
+        "dielectric_fft_storage": (sz, [vp]),
+        "dielectric_fft_threads": (sz, [vp]),
+        "dielectric_fft_backend": (ctypes.c_char_p, [vp]),
+        "dielectric_fft_solve": (i, [vp, vp, vp, ctypes.POINTER(GmresOptions), ctypes.POINTER(GmresReport)]),
+        "vie_fft_solve": (i, [vp, vp, vp, ctypes.POINTER(GmresOptions), ctypes.POINTER(GmresReport)]),
+        "vie_hip_set_solve_options": (None, [ctypes.POINTER(SolveOptions)]),
+        "vie_hip_get_solve_options": (None, [ctypes.POINTER(SolveOptions)]),
+        "dielectric_fft_budget": (sz, []),
+        "dielectric_fft_set_budget": (None, [sz]),
+        "dielectric_fft_blocks_changed": (None, [vp]),
+        "dielectric_fft_plan": (vp, [vp]),
+        "vie_hip_last_error": (ctypes.c_char_p, []),
+        "vie_hip_device_count": (i, []),
+        "dielectric_fft_upload_rhs": (i, [vp, vp]),
+        "dielectric_fft_solve_resident": (i, [vp, ctypes.POINTER(GmresOptions), ctypes.POINTER(SolveOptions),
+                                              ctypes.POINTER(GmresReport)]),
+        "dielectric_fft_download_solution": (i, [vp, vp]),
+        "dielectric_fft_time_apply": (i, [vp, i, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
+        "dielectric_fft_apply_bytes": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    return lib().vie_hip_last_error().decode()
+
+
+def device_count() -> int:
+    return lib().vie_hip_device_count()
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _options(restart=30, maximum_iterations=300, relative_tolerance=1e-12, absolute_tolerance=0.0):
+    return GmresOptions(restart, maximum_iterations, relative_tolerance, absolute_tolerance)
+
+
+def gmres_settings() -> GmresOptions:
+    """dielectric_gmres_settings (dielectric_fft.c:8)."""
+    return GmresOptions.in_dll(lib(), "dielectric_gmres_settings")
+
+
+def backend_set(name) -> bool:
+    return bool(lib().vie_fft_backend_set(None if name is None else name.encode()))
+
+
+class FFTPlan:
+    """vie_fft_plan (vie_fft.h) on the GPU."""
+
+    def __init__(self, shape, xyz, pitch, k0, epsilon_r, byte_limit=1 << 62):
+        L = lib()
+        self.shape = np.ascontiguousarray(shape, dtype=np.uintp)
+        self.xyz = np.ascontiguousarray(np.asarray(xyz).reshape(-1, 3), dtype=np.uintp)
+        self.cells = len(self.xyz)
+        p = ctypes.c_void_p()
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        ok = L.vie_fft_create(ctypes.byref(p), self.shape.ctypes.data_as(szp), self.cells,
+                              self.xyz.ctypes.data_as(szp), float(pitch), float(k0), CDouble.of(epsilon_r),
+                              int(byte_limit))
+        if not ok:
+            raise ValueError(f"vie_fft_create rejected the plan: {last_error()}")
+        self.p = p
+        self.op = LinearOperator()
+        L.vie_fft_operator(self.p, ctypes.byref(self.op))
+
+    @property
+    def order(self):
+        return 3 * self.cells
+
+    def set_permittivity(self, cell, er) -> bool:
+        return bool(lib().vie_fft_set_permittivity(self.p, int(cell), CDouble.of(er)))
+
+    def bytes(self):
+        return lib().vie_fft_bytes(self.p)
+
+    def backend(self):
+        return lib().vie_fft_backend_name(self.p).decode()
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.order, dtype=np.complex128)
+        f = ctypes.cast(self.op.apply, VectorMap)
+        if not f(self.op.context, x.ctypes.data, y.ctypes.data):
+            raise RuntimeError(f"apply failed: {last_error()}")
+        return y
+
+    def precondition(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.order, dtype=np.complex128)
+        f = ctypes.cast(self.op.precondition, VectorMap)
+        if not f(self.op.precondition_context, x.ctypes.data, y.ctypes.data):
+            raise RuntimeError(f"precondition failed: {last_error()}")
+        return y
+
+    def solve(self, b, x=None, **opts):
+        """vie_fft_solve: device-resident GMRES; returns (ok, x, report)."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros_like(b) if x is None else np.array(x, dtype=np.complex128)
+        o = _options(**opts)
+        rep = GmresReport()
+        ok = lib().vie_fft_solve(self.p, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(rep))
+        return bool(ok), x, rep
+
+    def gmres(self, b, x=None, **opts):
+        """vie_gmres on this plan's operator (dispatches to the device solve)."""
+        return gmres(self.op, b, x, **opts)
+
+    def close(self):
+        if getattr(self, "p", None):
+            lib().vie_fft_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+
+def gmres(op: LinearOperator, b, x=None, **opts):
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    x = np.zeros_like(b) if x is None else np.array(x, dtype=np.complex128)
+    o = _options(**opts)
+    rep = GmresReport()
+    ok = lib().vie_gmres(ctypes.byref(op), _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(rep))
+    return bool(ok), x, rep
+
+
+class CallbackOperator:
+    """A vie_linear_operator whose callbacks are Python functions (tests)."""
+
+    def __init__(self, order, apply, precondition=None):
+        def wrap(fn):
+            def cb(ctx, xin, yout):
+                xi = np.ctypeslib.as_array((ctypes.c_double * (2 * order)).from_address(xin)).view(np.complex128)
+                yo = np.ctypeslib.as_array((ctypes.c_double * (2 * order)).from_address(yout)).view(np.complex128)
+                r = fn(xi.copy())
+                if r is None:
+                    return 0
+                yo[:] = r
+                return 1
+            return VectorMap(cb)
+        self._apply = wrap(apply)
+        self._prec = wrap(precondition) if precondition else None
+        self.op = LinearOperator(order, ctypes.cast(self._apply, ctypes.c_void_p), None,
+                                 ctypes.cast(self._prec, ctypes.c_void_p) if self._prec else None, None)
+
+
+@dataclass
+class Cell:
+    center: tuple
+    size: tuple
+    material_id: int
+
+
+@dataclass
+class MaterialSpec:
+    id: int
+    epsilon_r: float
+    epsilon_loss: float = 0.0
+    conductivity: float = 0.0
+
+
+class System:
+    """dielectric_fft_system (dielectric_fft.h) on the GPU."""
+
+    def __init__(self, cells, materials, conductors, k0, pec=False, extra_bytes=0):
+        L = lib()
+        self._mats = (Material * max(1, len(materials)))()
+        for i, m in enumerate(materials):
+            self._mats[i].id = m.id
+            self._mats[i].epsilon_r = m.epsilon_r
+            self._mats[i].epsilon_loss = m.epsilon_loss
+            self._mats[i].conductivity = m.conductivity
+        self._cells = (MeshCell * max(1, len(cells)))()
+        for i, c in enumerate(cells):
+            self._cells[i].center[:] = list(c.center)
+            self._cells[i].size[:] = list(c.size)
+            self._cells[i].material_id = c.material_id
+        self.model = Model()
+        self.model.materials = self._mats
+        self.model.material_count = len(materials)
+        self.mesh = MeshResult()
+        self.mesh.cells = self._cells
+        self.mesh.cell_count = len(cells)
+        self.diag = Diagnostic()
+        s = ctypes.c_void_p()
+        fn = L.dielectric_fft_prepare_pec if pec else L.dielectric_fft_prepare
+        ok = fn(ctypes.byref(s), ctypes.byref(self.model), ctypes.byref(self.mesh), int(conductors),
+                float(k0), int(extra_bytes), ctypes.byref(self.diag))
+        self.message = self.diag.message.decode()
+        if not ok:
+            self.s = None
+            raise ValueError(self.message)
+        self.s = s
+        self.c = int(conductors)
+        self.v = 3 * len(cells)
+        self.op = LinearOperator()
+
+    @property
+    def order(self):
+        return self.c + self.v
+
+    def set(self, row, col, value) -> bool:
+        p = lib().dielectric_fft_entry(self.s, int(row), int(col))
+        if not p:
+            return False
+        arr = (ctypes.c_double * 2).from_address(p)
+        arr[0], arr[1] = complex(value).real, complex(value).imag
+        return True
+
+    def set_blocks(self, C=None, B=None, D=None):
+        """Write conductor blocks: C (c x c), B (c x v), D (v x c)."""
+        c, v = self.c, self.v
+        for r in range(c):
+            if C is not None:
+                for q in range(c):
+                    self.set(r, q, C[r, q])
+        # bulk writes straight into the entry storage
+        if B is not None and c:
+            base = lib().dielectric_fft_entry(self.s, 0, 0)
+            rows = np.ctypeslib.as_array((ctypes.c_double * (2 * c * (c + v))).from_address(base))
+            rows = rows.view(np.complex128).reshape(c, c + v)
+            rows[:, c:] = B
+        if D is not None and c and v:
+            base = lib().dielectric_fft_entry(self.s, c, 0)
+            dd = np.ctypeslib.as_array((ctypes.c_double * (2 * v * c)).from_address(base))
+            dd.view(np.complex128).reshape(v, c)[:] = D
+        lib().dielectric_fft_blocks_changed(self.s)
+
+    def operator(self) -> LinearOperator:
+        lib().dielectric_fft_get_operator(self.s, ctypes.byref(self.op))
+        return self.op
+
+    def storage(self):
+        return lib().dielectric_fft_storage(self.s)
+
+    def backend(self):
+        return lib().dielectric_fft_backend(self.s).decode()
+
+    def apply(self, x):
+        op = self.operator()
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.order, dtype=np.complex128)
+        if not ctypes.cast(op.apply, VectorMap)(op.context, x.ctypes.data, y.ctypes.data):
+            raise RuntimeError(f"apply failed: {last_error()}")
+        return y
+
+    def precondition(self, x):
+        op = self.operator()
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.order, dtype=np.complex128)
+        if not ctypes.cast(op.precondition, VectorMap)(op.precondition_context, x.ctypes.data, y.ctypes.data):
+            raise RuntimeError(f"precondition failed: {last_error()}")
+        return y
+
+    def solve(self, b, x=None, precision=None, inner_reduction=0.0, **opts):
+        """dielectric_fft_solve; precision None keeps the library default."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros_like(b) if x is None else np.array(x, dtype=np.complex128)
+        o = _options(**opts)
+        rep = GmresReport()
+        L = lib()
+        if precision is not None:
+            old = SolveOptions()
+            L.vie_hip_get_solve_options(ctypes.byref(old))
+            L.vie_hip_set_solve_options(ctypes.byref(SolveOptions(precision, inner_reduction)))
+        try:
+            ok = L.dielectric_fft_solve(self.s, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(rep))
+        finally:
+            if precision is not None:
+                L.vie_hip_set_solve_options(ctypes.byref(old))
+        return bool(ok), x, rep
+
+    def gmres(self, b, x=None, **opts):
+        return gmres(self.operator(), b, x, **opts)
+
+    # resident work
+    def upload_rhs(self, b):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        if not lib().dielectric_fft_upload_rhs(self.s, _ptr(b)):
+            raise RuntimeError(last_error())
+
+    def solve_resident(self, precision=FP64, inner_reduction=0.0, **opts):
+        o = _options(**opts)
+        so = SolveOptions(precision, inner_reduction)
+        rep = GmresReport()
+        ok = lib().dielectric_fft_solve_resident(self.s, ctypes.byref(o), ctypes.byref(so), ctypes.byref(rep))
+        return bool(ok), rep
+
+    def download_solution(self):
+        x = np.zeros(self.order, dtype=np.complex128)
+        if not lib().dielectric_fft_download_solution(self.s, _ptr(x)):
+            raise RuntimeError(last_error())
+        return x
+
+    def time_apply(self, precision=FP64, repeats=20):
+        ms = ctypes.c_double()
+        passes = (ctypes.c_double * 6)()
+        if not lib().dielectric_fft_time_apply(self.s, precision, repeats, ctypes.byref(ms), passes):
+            raise RuntimeError(last_error())
+        return ms.value, list(passes)
+
+    def apply_bytes(self, precision=FP64):
+        passes = (ctypes.c_double * 6)()
+        tot = ctypes.c_double()
+        lib().dielectric_fft_apply_bytes(self.s, precision, passes, ctypes.byref(tot))
+        return tot.value, list(passes)
+
+    def close(self):
+        if getattr(self, "s", None):
+            lib().dielectric_fft_free(self.s)
+            self.s = None
+
+    def __del__(self):
+        self.close()
+
+
+def lattice_cells(shape, pitch, origin=(0.0, 0.0, 0.0), material_ids=None, mask=None):
+    """Cells of a box lattice (x slowest, z fastest), optionally masked."""
+    nx, ny, nz = shape
+    out = []
+    idx = 0
+    for x in range(nx):
+        for y in range(ny):
+            for z in range(nz):
+                if mask is not None and not mask[x, y, z]:
+                    continue
+                mid = 1 if material_ids is None else int(material_ids[x, y, z])
+                out.append(Cell((origin[0] + pitch * x, origin[1] + pitch * y, origin[2] + pitch * z),
+                                (pitch, pitch, pitch), mid))
+                idx += 1
+    return out
