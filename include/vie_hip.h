@@ -53,6 +53,23 @@ const char *vie_hip_last_error(void);
    when the runtime reports none). */
 int vie_hip_device_count(void);
 
+/* Bulk per-cell permittivity of a plan (count == cells); each value passes
+   the same checks as vie_fft_set_permittivity.  Returns 0 and changes
+   nothing when any value is rejected. */
+int vie_fft_set_permittivities(vie_fft_plan *plan, const VIE_CDOUBLE *epsilon_r, size_t count);
+/* Operator application at an explicit precision on host vectors in the
+   reference layout: y = A M^{-1} x when `jacobi` is nonzero (the product the
+   Krylov cycle forms), else y = A x.  FP32 rounds x to complex float before the
+   device work and returns y widened to double. */
+int dielectric_fft_apply_ex(dielectric_fft_system *system, vie_hip_precision precision, int jacobi,
+                            const VIE_CDOUBLE *x, VIE_CDOUBLE *y);
+/* Waits for all work on the current device. */
+int vie_hip_synchronize(void);
+/* Copies `bytes` (rounded down to 16) between two device buffers `repeats`
+   times with a plain streaming kernel; returns average ms per copy.  Used to
+   calibrate HBM traffic counters against a kernel of known size. */
+double vie_hip_copy_probe(size_t bytes, int repeats);
+
 /* ---- resident work (for benchmarks and servers) ----
    Right-hand side and solution stay in device memory between calls. */
 int dielectric_fft_upload_rhs(dielectric_fft_system *system, const VIE_CDOUBLE *rhs);

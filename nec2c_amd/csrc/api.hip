@@ -17,6 +17,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/vie_hip.h"
@@ -372,6 +373,44 @@ extern "C" int vie_fft_set_permittivity(vie_fft_plan* p, size_t cell, double _Co
   return 1;
 }
 
+extern "C" int vie_fft_set_permittivities(vie_fft_plan* p, const double _Complex* er, size_t count) {
+  if (!p || !er || count != p->cells) return 0;
+  const c64* e = (const c64*)er;
+  for (size_t i = 0; i < count; i++) {
+    if (!passive(e[i])) return 0;
+    std::complex<double> ec(e[i].re, e[i].im);
+    for (int q = 0; q < 3; q++) {
+      std::complex<double> d = ec - (ec - 1.0) * std::complex<double>(p->self[q].re, p->self[q].im);
+      if (!std::isfinite(std::abs(d)) || std::abs(d) < 1e-30) return 0;
+    }
+  }
+  std::copy(e, e + count, p->eps.begin());
+  p->eps_dirty = true;
+  return 1;
+}
+
+extern "C" int vie_hip_synchronize(void) { return hipDeviceSynchronize() == hipSuccess; }
+
+extern "C" double vie_hip_copy_probe(size_t bytes, int repeats) {
+  if (!ensure_device() || repeats < 1) return -1;
+  size_t n16 = bytes / 16;
+  DevBuf a, b;
+  if (!a.alloc(n16 * 16) || !b.alloc(n16 * 16)) return -1;
+  if (hipMemset(a.p, 0, n16 * 16) != hipSuccess) return -1;
+  hipEvent_t t0, t1;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return -1;
+  vie::copy16(b.p, a.p, n16, nullptr);
+  (void)hipEventRecord(t0, nullptr);
+  for (int r = 0; r < repeats; r++) vie::copy16(b.p, a.p, n16, nullptr);
+  (void)hipEventRecord(t1, nullptr);
+  (void)hipEventSynchronize(t1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, t0, t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return ms / repeats;
+}
+
 // ------------------------------------------------------------------ system
 struct SolverWork {
   size_t n = 0, ldv = 0;
@@ -472,6 +511,8 @@ static int prepare(dielectric_fft_system** out, const dielectric_model* model, c
   double origin[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, h = 0;
   c64 er = {0, 0};
   std::vector<c64> cell_eps(cells);
+  std::unordered_map<int, size_t> by_id;
+  for (size_t q = model->material_count; q-- > 0;) by_id[model->materials[q].id] = q;
   if (cells) {
     h = mesh->cells[0].size[0];
     for (size_t i = 0; i < cells; i++) {
@@ -479,11 +520,11 @@ static int prepare(dielectric_fft_system** out, const dielectric_model* model, c
       bool found = false;
       if (pec && mesh->cells[i].center[2] - .5 * mesh->cells[i].size[2] < 0)
         return diag_error(diag, "PEC FFT cells must lie in z >= 0");
-      for (size_t q = 0; q < model->material_count; q++)
-        if (model->materials[q].id == mesh->cells[i].material_id) {
-          value = material_epsilon(&model->materials[q], k0);
-          found = true;
-        }
+      auto it = by_id.find(mesh->cells[i].material_id);
+      if (it != by_id.end()) {
+        value = material_epsilon(&model->materials[it->second], k0);
+        found = true;
+      }
       if (!found || (value.re == 0 && value.im == 0) || !passive(value))
         return diag_error(diag, "unknown or invalid passive material");
       cell_eps[i] = value;
@@ -539,11 +580,10 @@ static int prepare(dielectric_fft_system** out, const dielectric_model* model, c
       if (!why.empty()) msg += " (" + why + ")";
       return diag_error(diag, msg.c_str());
     }
-    for (size_t i = 0; i < cells; i++)
-      if (!vie_fft_set_permittivity(s->plan, i, *(double _Complex*)&cell_eps[i])) {
-        delete s;
-        return diag_error(diag, "kernel/lattice build failed or padded FFT exceeds memory budget");
-      }
+    if (!vie_fft_set_permittivities(s->plan, (const double _Complex*)cell_eps.data(), cells)) {
+      delete s;
+      return diag_error(diag, "kernel/lattice build failed or padded FFT exceeds memory budget");
+    }
     s->stream = s->plan->stream;
   } else {
     if (!ensure_device()) {
@@ -1044,6 +1084,30 @@ static int apply_host(dielectric_fft_system* s, const double _Complex* in, doubl
     rs.partial = s->sw.partial.as<c64>();
     rs.counter = s->sw.counter.as<unsigned>();
     if (!sys_apply<double>(s, s->sw.x.as<c64>(), s->sw.ax.as<c64>(), 0, rs)) return 0;
+  }
+  return unstage_vector(s, s->sw.ax.as<c64>(), out);
+}
+
+extern "C" int dielectric_fft_apply_ex(dielectric_fft_system* s, vie_hip_precision prec, int jacobi,
+                                       const double _Complex* in, double _Complex* out) {
+  if (!s || !in || !out) return 0;
+  size_t n = s->nsys();
+  if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
+      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s, 1))
+    return 0;
+  if (!stage_vector(s, in, s->sw.x.as<c64>())) return 0;
+  vie::ReduceScratch rs;
+  rs.partial = s->sw.partial.as<c64>();
+  rs.counter = s->sw.counter.as<unsigned>();
+  if (prec == VIE_HIP_MIXED) {
+    DevBuf a, b;
+    if (!a.alloc(n * 8) || !b.alloc(n * 8)) return 0;
+    vie::vec_cast<float, double>(a.as<c32>(), s->sw.x.as<c64>(), n, s->stream);
+    if (!sys_apply<float>(s, a.as<c32>(), b.as<c32>(), jacobi, rs)) return 0;
+    vie::vec_cast<double, float>(s->sw.ax.as<c64>(), b.as<c32>(), n, s->stream);
+    HIP_OK(hipStreamSynchronize(s->stream));
+  } else if (!sys_apply<double>(s, s->sw.x.as<c64>(), s->sw.ax.as<c64>(), jacobi, rs)) {
+    return 0;
   }
   return unstage_vector(s, s->sw.ax.as<c64>(), out);
 }

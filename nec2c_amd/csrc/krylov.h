@@ -52,6 +52,7 @@ template <typename T> struct CouplingView {
 };
 
 size_t red_blocks(size_t n);
+void copy16(void* dst, const void* src, size_t n16, hipStream_t s);
 
 template <typename T>
 void multidot(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const cx<T>* w, size_t n, int self_norm,

@@ -449,7 +449,18 @@ __global__ void k_precondition(const c64* x, c64* y, size_t n, JacobiView J) {
   if (i < n) y[i] = jacobi_inv(J, i, x[i]);
 }
 
+// Streaming 16-byte copy (HBM traffic calibration probe).
+__global__ void k_copy16(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 // ================================================================ host side
+void copy16(void* dst, const void* src, size_t n16, hipStream_t s) {
+  if (n16) hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, s, (float4*)dst, (const float4*)src, n16);
+}
+
 static inline unsigned gblocks(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 template <typename T> void vec_zero(cx<T>* x, size_t n, hipStream_t s) {

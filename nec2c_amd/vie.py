@@ -119,7 +119,8 @@ EXPORTS = (
     "dielectric_fft_budget", "dielectric_fft_set_budget", "dielectric_fft_blocks_changed",
     "dielectric_fft_plan", "vie_hip_last_error", "vie_hip_device_count",
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
-    "dielectric_fft_time_apply", "dielectric_fft_apply_bytes",
+    "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
+    "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex",
 )
 
 _LIB = None
@@ -180,6 +181,10 @@ def lib() -> ctypes.CDLL:
         "dielectric_fft_download_solution": (i, [vp, vp]),
         "dielectric_fft_time_apply": (i, [vp, i, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
         "dielectric_fft_apply_bytes": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
+        "vie_fft_set_permittivities": (i, [vp, vp, sz]),
+        "vie_hip_synchronize": (i, []),
+        "dielectric_fft_apply_ex": (i, [vp, i, i, vp, vp]),
+        "vie_hip_copy_probe": (d, [sz, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -214,6 +219,14 @@ def backend_set(name) -> bool:
     return bool(lib().vie_fft_backend_set(None if name is None else name.encode()))
 
 
+def synchronize():
+    return bool(lib().vie_hip_synchronize())
+
+
+def copy_probe(nbytes, repeats=10):
+    return lib().vie_hip_copy_probe(int(nbytes), int(repeats))
+
+
 class FFTPlan:
     """vie_fft_plan (vie_fft.h) on the GPU."""
 
@@ -242,6 +255,10 @@ class FFTPlan:
 
     def bytes(self):
         return lib().vie_fft_bytes(self.p)
+
+    def set_permittivities(self, eps):
+        e = np.ascontiguousarray(eps, dtype=np.complex128).reshape(-1)
+        return bool(lib().vie_fft_set_permittivities(self.p, _ptr(e), len(e)))
 
     def backend(self):
         return lib().vie_fft_backend_name(self.p).decode()
@@ -331,7 +348,10 @@ class MaterialSpec:
 class System:
     """dielectric_fft_system (dielectric_fft.h) on the GPU."""
 
-    def __init__(self, cells, materials, conductors, k0, pec=False, extra_bytes=0):
+    def __init__(self, cells, materials, conductors, k0, pec=False, extra_bytes=0, pitch=None,
+                 material_ids=None):
+        """cells: a list of Cell, or an (N, 3) array of cell centres together
+        with ``pitch`` (and optional per-cell ``material_ids``, default 1)."""
         L = lib()
         self._mats = (Material * max(1, len(materials)))()
         for i, m in enumerate(materials):
@@ -340,10 +360,16 @@ class System:
             self._mats[i].epsilon_loss = m.epsilon_loss
             self._mats[i].conductivity = m.conductivity
         self._cells = (MeshCell * max(1, len(cells)))()
-        for i, c in enumerate(cells):
-            self._cells[i].center[:] = list(c.center)
-            self._cells[i].size[:] = list(c.size)
-            self._cells[i].material_id = c.material_id
+        if isinstance(cells, np.ndarray):
+            arr = np.ctypeslib.as_array(self._cells)
+            arr["center"] = cells.reshape(-1, 3)
+            arr["size"] = float(pitch)
+            arr["material_id"] = 1 if material_ids is None else np.asarray(material_ids).reshape(-1)
+        else:
+            for i, c in enumerate(cells):
+                self._cells[i].center[:] = list(c.center)
+                self._cells[i].size[:] = list(c.size)
+                self._cells[i].material_id = c.material_id
         self.model = Model()
         self.model.materials = self._mats
         self.model.material_count = len(materials)
@@ -395,6 +421,13 @@ class System:
             dd.view(np.complex128).reshape(v, c)[:] = D
         lib().dielectric_fft_blocks_changed(self.s)
 
+    def set_permittivities(self, eps):
+        """Per-cell permittivity in the caller's cell order (vie_fft_set_permittivities)."""
+        e = np.ascontiguousarray(eps, dtype=np.complex128).reshape(-1)
+        plan = lib().dielectric_fft_plan(self.s)
+        if not plan or not lib().vie_fft_set_permittivities(plan, _ptr(e), len(e)):
+            raise ValueError("permittivity rejected")
+
     def operator(self) -> LinearOperator:
         lib().dielectric_fft_get_operator(self.s, ctypes.byref(self.op))
         return self.op
@@ -410,6 +443,14 @@ class System:
         x = np.ascontiguousarray(x, dtype=np.complex128)
         y = np.zeros(self.order, dtype=np.complex128)
         if not ctypes.cast(op.apply, VectorMap)(op.context, x.ctypes.data, y.ctypes.data):
+            raise RuntimeError(f"apply failed: {last_error()}")
+        return y
+
+    def apply_ex(self, x, precision=FP64, jacobi=False):
+        """y = A M^{-1} x (jacobi) or A x at an explicit device precision."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.order, dtype=np.complex128)
+        if not lib().dielectric_fft_apply_ex(self.s, int(precision), int(bool(jacobi)), _ptr(x), _ptr(y)):
             raise RuntimeError(f"apply failed: {last_error()}")
         return y
 

@@ -1,0 +1,151 @@
+"""Synthetic inputs of the benchmark configurations (host-side, numpy).
+
+* ``random_permittivity`` -- the Gaussian random field of the 64^3 sweep:
+  white noise filtered by exp(-r^2 / 2 l^2) (l = 4 voxels), standardised, put
+  through the normal CDF onto [lo, hi]; loss tangent uniform on [0, tmax];
+  eps = eps_r (1 - j tan d), passive under exp(+j w t) (dielectric.h:55).
+* ``Dipole`` -- a z-directed thin-wire dipole with a 1 V delta gap at its centre
+  segment, discretised by Harrington's pulse-basis point-matching method of
+  moments (R. F. Harrington, Field Computation by Moment Methods, 1968, ch. 4):
+  Z_mn = j w mu D^2 psi(m,n) + [psi(m+,n+) - psi(m-,n+) - psi(m+,n-) + psi(m-,n-)] / (j w eps),
+  psi the segment-averaged reduced kernel exp(-jkR)/(4 pi R).
+* ``coupling_blocks`` -- the cross blocks of the reference's hybrid operator
+  [C B; D V] (dielectric_fft.c:41-66) for that dipole and a cube lattice, from
+  the free-space dyadic (k^2 I + grad grad) g evaluated at cell centres:
+  the volume unknown is the normalised polarisation (eps - 1) E, so
+  D = (eps_i - 1) (j w mu D / k^2) G_{i z} (incident field of each wire
+  segment, moved to the left-hand side) and B = -D h^3 G_{z j} (voltage induced
+  on each segment by the cell's polarisation current).
+
+These are stand-ins of the right shape and physical scale for the blocks the
+reference assembles with NEC's wire code (dielectric_wire.c:422-508,
+1164-1213), which this project does not rebuild; parity is always stated
+against the reference solving the same [C B; D V] system.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+C0 = 299792458.0
+MU0 = 4e-7 * math.pi
+EPS0 = 1.0 / (MU0 * C0 * C0)
+
+
+def k0_of(freq_hz: float) -> float:
+    return 2 * math.pi * freq_hz / C0
+
+
+def _norm_cdf(x):
+    from scipy.special import ndtr
+    return ndtr(x)
+
+
+def random_permittivity(shape, seed, corr=4.0, lo=1.5, hi=10.0, tan_max=0.05):
+    """Per-voxel complex permittivity [nx, ny, nz] (x slowest, z fastest)."""
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(shape)
+    freqs = [np.fft.fftfreq(n) for n in shape]
+    kx, ky, kz = np.meshgrid(*freqs, indexing="ij")
+    # FFT of exp(-r^2/2l^2) is proportional to exp(-2 pi^2 l^2 |f|^2)
+    filt = np.exp(-2 * math.pi ** 2 * corr ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
+    field = np.real(np.fft.ifftn(np.fft.fftn(noise) * filt))
+    field = (field - field.mean()) / field.std()
+    er = lo + (hi - lo) * _norm_cdf(field)
+    tan = rng.uniform(0.0, tan_max, shape)
+    return er * (1 - 1j * tan)
+
+
+def _gauss01(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1), 0.5 * w
+
+
+@dataclass
+class Dipole:
+    center: tuple      # (x, y, z) of the feed, metres
+    length: float = 0.48
+    segments: int = 21
+    radius: float = 1e-3
+
+    @property
+    def delta(self):
+        return self.length / self.segments
+
+    def points(self):
+        """Segment centres along z."""
+        z0 = self.center[2] - 0.5 * self.length + 0.5 * self.delta
+        return np.array([[self.center[0], self.center[1], z0 + i * self.delta] for i in range(self.segments)])
+
+    def impedance_matrix(self, k0: float) -> np.ndarray:
+        """Harrington point-matching Z (ohms) of the straight wire."""
+        N, D, a = self.segments, self.delta, self.radius
+        omega = k0 * C0
+        z = self.points()[:, 2]
+        xg, wg = _gauss01(16)
+
+        def psi(zm, zc):
+            # (1/D) int_{zc-D/2}^{zc+D/2} exp(-jkR)/(4 pi R) dz', R = sqrt((zm-z')^2 + a^2)
+            zp = zc - 0.5 * D + D * xg
+            R = np.sqrt((zm - zp) ** 2 + a * a)
+            val = np.sum(wg * np.exp(-1j * k0 * R) / (4 * math.pi * R))
+            if abs(zm - zc) < 1e-12 * D:
+                # self term of the reduced kernel (Harrington eq. 4-44)
+                val = math.log(D / a) / (2 * math.pi * D) - 1j * k0 / (4 * math.pi)
+            return val
+
+        Z = np.zeros((N, N), dtype=np.complex128)
+        for m in range(N):
+            for n in range(N):
+                t1 = 1j * omega * MU0 * D * D * psi(z[m], z[n])
+                t2 = (psi(z[m] + D / 2, z[n] + D / 2) - psi(z[m] - D / 2, z[n] + D / 2)
+                      - psi(z[m] + D / 2, z[n] - D / 2) + psi(z[m] - D / 2, z[n] - D / 2)) / (1j * omega * EPS0)
+                Z[m, n] = t1 + t2
+        return Z
+
+    def feed_index(self):
+        return self.segments // 2
+
+    def rhs(self, volt=1.0):
+        v = np.zeros(self.segments, dtype=np.complex128)
+        v[self.feed_index()] = volt
+        return v
+
+
+def dyadic(rvec, k0):
+    """(k^2 I + grad grad) exp(-jkR)/(4 pi R) for an array of separations [..., 3]."""
+    R = np.linalg.norm(rvec, axis=-1)
+    g = np.exp(-1j * k0 * R) / (4 * math.pi * R)
+    diag = g * (k0 * k0 - 1j * k0 / R - 1 / R ** 2)
+    rad = g * (-k0 * k0 + 3j * k0 / R + 3 / R ** 2) / R ** 2
+    G = rad[..., None, None] * rvec[..., :, None] * rvec[..., None, :]
+    G[..., 0, 0] += diag
+    G[..., 1, 1] += diag
+    G[..., 2, 2] += diag
+    return G
+
+
+def coupling_blocks(dipole: Dipole, centers: np.ndarray, pitch: float, eps: np.ndarray, k0: float):
+    """B (c x 3N) and D (3N x c) in the reference's cell-major layout."""
+    pts = dipole.points()
+    c = len(pts)
+    N = len(centers)
+    omega = k0 * C0
+    B = np.zeros((c, 3 * N), dtype=np.complex128)
+    D = np.zeros((3 * N, c), dtype=np.complex128)
+    contrast = (eps - 1.0).reshape(N)
+    for n in range(c):
+        G = dyadic(centers - pts[n][None, :], k0)            # [N, 3, 3]
+        inc = (1j * omega * MU0 * dipole.delta / (k0 * k0)) * G[:, :, 2]   # field per amp, [N, 3]
+        D[:, n] = (contrast[:, None] * inc).reshape(-1)
+        B[n, :] = (-(pitch ** 3) * dipole.delta * G[:, 2, :]).reshape(-1)
+    return B, D
+
+
+def cube_cells(n, pitch, origin=(0.0, 0.0, 0.0)):
+    """Centres of an n^3 (or nx x ny x nz) block, x slowest, z fastest."""
+    shape = (n, n, n) if np.isscalar(n) else tuple(n)
+    g = np.stack(np.meshgrid(*[np.arange(s) for s in shape], indexing="ij"), -1).reshape(-1, 3)
+    return np.asarray(origin)[None, :] + pitch * (g + 0.5)

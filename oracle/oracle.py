@@ -541,6 +541,12 @@ class ReferenceLibrary:
                                            ctypes.c_double, ctypes.c_double, d]
         L.refshim_dense_gmres.argtypes = [d, ctypes.c_size_t, d, d, ctypes.c_size_t, ctypes.c_size_t,
                                           ctypes.c_double, ctypes.c_double, d]
+        L.refshim_coupled_create.argtypes = [vp, ctypes.c_size_t, d, d, d]
+        L.refshim_coupled_create.restype = vp
+        L.refshim_coupled_free.argtypes = [vp]
+        L.refshim_coupled_apply.argtypes = [vp, d, d]
+        L.refshim_coupled_gmres.argtypes = [vp, d, d, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double,
+                                            ctypes.c_double, d]
         self.L = L
 
     def threads(self, n=0):
@@ -578,6 +584,40 @@ class ReferenceLibrary:
 
 def _report(rep):
     return Report(int(rep[0]), int(rep[1]), int(rep[2]), int(rep[3]), float(rep[4]), float(rep[5]))
+
+
+class RefCoupled:
+    """[C B; D V] over a reference volume plan (ref_shim.c refshim_coupled_*)."""
+
+    def __init__(self, plan: "RefPlan", C, B, D):
+        self.plan = plan
+        self.C = np.ascontiguousarray(C, dtype=np.complex128)
+        self.B = np.ascontiguousarray(B, dtype=np.complex128)
+        self.D = np.ascontiguousarray(D, dtype=np.complex128)
+        self.c = self.C.shape[0]
+        L = plan.R.L
+        self.r = L.refshim_coupled_create(plan.p, self.c, _dp(self.C.view(np.float64)),
+                                          _dp(self.B.view(np.float64)), _dp(self.D.view(np.float64)))
+        self.order = self.c + 3 * plan.cells
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        self.plan.R.L.refshim_coupled_apply(self.r, _dp(x.view(np.float64)), _dp(y.view(np.float64)))
+        return y
+
+    def gmres(self, b, restart=30, maxit=300, rtol=1e-12, atol=0.0):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros_like(b)
+        rep = np.zeros(6)
+        ok = self.plan.R.L.refshim_coupled_gmres(self.r, _dp(b.view(np.float64)), _dp(x.view(np.float64)),
+                                                 restart, maxit, rtol, atol, _dp(rep))
+        return bool(ok), x, _report(rep)
+
+    def __del__(self):
+        if getattr(self, "r", None):
+            self.plan.R.L.refshim_coupled_free(self.r)
+            self.r = None
 
 
 class RefPlan:
@@ -620,6 +660,14 @@ class RefPlan:
 
     def bytes(self):
         return self.R.L.refshim_bytes(self.p)
+
+    def set_permittivities(self, eps):
+        for i, e in enumerate(np.asarray(eps).reshape(-1)):
+            if not self.set_eps(i, e):
+                raise ValueError(f"reference rejected permittivity of cell {i}")
+
+    def coupled(self, C, B, D):
+        return RefCoupled(self, C, B, D)
 
     def __del__(self):
         if getattr(self, "p", None):

@@ -186,3 +186,70 @@ int refshim_dense_gmres(const double *a, size_t n, const double *b, double *x,
   pack_report(&rep, report);
   return ok;
 }
+
+/* Coupled [C B; D V] operator over a reference volume plan built with an
+   unlimited budget (vie_fft_create).  The reference's own dielectric_fft
+   prepare caps storage at 512 MiB (dielectric_fft.h:5), which the 64^3
+   coupled workload exceeds; the block products below restate
+   dielectric_fft.c:53-64 (same loops, same OpenMP split) around the
+   reference's volume apply / precondition.  C: c x c, B: c x v, D: v x c,
+   all row-major, volume index 3*cell+component. */
+typedef struct { vie_fft_plan *p; size_t c, v; const complex double *C, *B, *D; } refshim_coupled;
+
+refshim_coupled *refshim_coupled_create(vie_fft_plan *p, size_t c, const double *C, const double *B, const double *D)
+{
+  refshim_coupled *r = calloc(1, sizeof(*r));
+  vie_linear_operator op;
+  if(!r) return NULL;
+  vie_fft_operator(p, &op);
+  r->p = p; r->c = c; r->v = op.order;
+  r->C = (const complex double *)C; r->B = (const complex double *)B; r->D = (const complex double *)D;
+  return r;
+}
+
+void refshim_coupled_free(refshim_coupled *r) { free(r); }
+
+static int coupled_apply(void *ctx, const complex double *x, complex double *y)
+{
+  refshim_coupled *r = ctx; vie_linear_operator op; size_t i, j, c = r->c, v = r->v;
+  vie_fft_operator(r->p, &op);
+  if(!op.apply(op.context, x+c, y+c)) return 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if(c >= 2)
+#endif
+  for(i = 0; i < c; i++) {
+    complex double s = 0;
+    for(j = 0; j < c; j++) s += r->C[i*c+j]*x[j];
+    for(j = 0; j < v; j++) s += r->B[i*v+j]*x[c+j];
+    y[i] = s;
+  }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if(v >= 1024)
+#endif
+  for(i = 0; i < v; i++) for(j = 0; j < c; j++) y[c+i] += r->D[i*c+j]*x[j];
+  return 1;
+}
+
+static int coupled_precondition(void *ctx, const complex double *x, complex double *y)
+{
+  refshim_coupled *r = ctx; vie_linear_operator op; size_t i;
+  vie_fft_operator(r->p, &op);
+  if(!op.precondition(op.precondition_context, x+r->c, y+r->c)) return 0;
+  for(i = 0; i < r->c; i++) y[i] = x[i]/r->C[i*r->c+i];
+  return 1;
+}
+
+int refshim_coupled_apply(refshim_coupled *r, const double *x, double *y)
+{ return coupled_apply(r, (const complex double *)x, (complex double *)y); }
+
+int refshim_coupled_gmres(refshim_coupled *r, const double *b, double *x, size_t restart, size_t maxit,
+                          double rtol, double atol, double *report)
+{
+  vie_linear_operator op = {r->c+r->v, coupled_apply, r, coupled_precondition, r};
+  vie_gmres_options o; vie_gmres_report rep; int ok;
+  o.restart = restart; o.maximum_iterations = maxit;
+  o.relative_tolerance = rtol; o.absolute_tolerance = atol;
+  ok = vie_gmres(&op, (const complex double *)b, (complex double *)x, &o, &rep);
+  pack_report(&rep, report);
+  return ok;
+}
