@@ -73,6 +73,10 @@ double vie_hip_copy_probe(size_t bytes, int repeats);
 /* ---- resident work (for benchmarks and servers) ----
    Right-hand side and solution stay in device memory between calls. */
 int dielectric_fft_upload_rhs(dielectric_fft_system *system, const VIE_CDOUBLE *rhs);
+/* Makes everything a later solve with these options needs resident: uploads
+   C/B/D and the permittivity, allocates the Krylov basis for either
+   precision and the solver vectors.  Solves then allocate and copy nothing. */
+int dielectric_fft_reserve(dielectric_fft_system *system, const vie_gmres_options *options);
 int dielectric_fft_solve_resident(dielectric_fft_system *system, const vie_gmres_options *options,
                                   const vie_hip_solve_options *solve, vie_gmres_report *report);
 int dielectric_fft_download_solution(dielectric_fft_system *system, VIE_CDOUBLE *x);

@@ -412,10 +412,22 @@ extern "C" double vie_hip_copy_probe(size_t bytes, int repeats) {
 }
 
 // ------------------------------------------------------------------ system
+struct StatusBlock {
+  double gabs, beta;
+  int breakdown, nonfinite, singular, used, done, pad_;
+};
+
 struct SolverWork {
   size_t n = 0, ldv = 0;
   int m = 0, prec = -1;
-  DevBuf V, x, b, r, ax, state, dots, partial, counter, scal;
+  DevBuf V, x, b, r, ax, state, dots, partial, scal;
+  void* status_host = nullptr;      // pinned ring of status blocks
+  hipEvent_t status_ev[2] = {nullptr, nullptr};
+  ~SolverWork() {
+    if (status_host) (void)hipHostFree(status_host);
+    for (auto& e : status_ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 struct dielectric_fft_system {
@@ -739,14 +751,14 @@ static vie::JacobiView jacobi_view(dielectric_fft_system* s) {
 static int ensure_scratch(dielectric_fft_system* s, int m) {
   SolverWork& w = s->sw;
   size_t n = s->nsys();
-  size_t blocks = vie::red_blocks(std::max<size_t>(n, 1));
-  size_t need = (size_t)std::max((size_t)m + 2, s->c) * blocks * sizeof(c64);
-  if (!w.partial.alloc(need)) return 0;
-  if (!w.counter.p) {
-    if (!w.counter.alloc(64)) return 0;
-    HIP_OK(hipMemsetAsync(w.counter.p, 0, 64, s->stream));
-  }
+  (void)m;
+  (void)n;
+  if (!w.partial.alloc(vie::kReduceScratchBytes)) return 0;
   if (!w.scal.alloc(16 * sizeof(c64))) return 0;
+  if (!w.status_host) {
+    HIP_OK(hipHostMalloc(&w.status_host, 2 * sizeof(StatusBlock), hipHostMallocDefault));
+    for (auto& e : w.status_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   return 1;
 }
 
@@ -756,6 +768,7 @@ static int ensure_solver(dielectric_fft_system* s, int m, int prec) {
   size_t ldv = (n + 63) / 64 * 64;
   size_t es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
   if (!ensure_scratch(s, m)) return 0;
+  // a larger basis from an earlier reserve or FP64 solve is reused as is
   if (!(w.V.alloc((size_t)(m + 1) * ldv * es) && w.x.alloc(n * 16) && w.b.alloc(n * 16) && w.r.alloc(n * 16) &&
         w.ax.alloc(n * 16) && w.state.alloc(sizeof(vie::ArnoldiState)) &&
         w.dots.alloc(2 * (vie::kMaxRestart + 2) * sizeof(c64))))
@@ -775,10 +788,6 @@ static int ensure_solver(dielectric_fft_system* s, int m, int prec) {
 // residual acceptance).  MIXED: the same outer loop with FP32 cycles that stop
 // once |g| <= max(target, inner_reduction * beta); each restart's FP64 true
 // residual refines the solution.
-struct StatusBlock {
-  double gabs, beta;
-  int breakdown, nonfinite, singular, used;
-};
 
 static int read_status(dielectric_fft_system* s, StatusBlock* sb) {
   HIP_OK(hipMemcpyAsync(sb, s->sw.state.p, sizeof(StatusBlock), hipMemcpyDeviceToHost, s->stream));
@@ -803,7 +812,6 @@ static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, co
   hipStream_t st = s->stream;
   vie::ReduceScratch rs;
   rs.partial = w.partial.as<c64>();
-  rs.counter = w.counter.as<unsigned>();
   c64* x = w.x.as<c64>();
   c64* b = w.b.as<c64>();
   c64* r = w.r.as<c64>();
@@ -812,7 +820,7 @@ static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, co
   vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
   c64* dots1 = w.dots.as<c64>();
   c64* dots2 = dots1 + vie::kMaxRestart + 2;
-  c64* scal = w.scal.as<c64>();  // [0] ||b||^2 [1] ||x||^2 [2] ||r||^2 [3] next^2
+  c64* scal = w.scal.as<c64>();  // [0] ||b||^2 [1] ||x||^2 [2] ||r||^2
   const vie::JacobiView J = jacobi_view(s);
   const double reduction = so && so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
   const bool mixed = sizeof(T) == sizeof(float);
@@ -864,47 +872,73 @@ static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, co
       return 0;
     }
     const double inner_target = mixed ? std::max(target, reduction * beta) : target;
-    vie::cycle_init(sd, scal + 2, st);
+    vie::cycle_init(sd, scal + 2, inner_target, st);
     vie::scale_from<T>(V, r, n, sd, st);
-    int used = 0;
-    bool breakdown = false;
-    StatusBlock sb;
-    for (int j = 0; j < m && rep->iterations < o->maximum_iterations; j++) {
+    // Arnoldi steps with one step of lookahead: step j is queued before the
+    // status of step j-1 is read, so the GPU never waits for the host.  The
+    // device tail marks the cycle done; a step queued after that is a no-op
+    // for the state (its vector work lands in an unused basis slot).
+    StatusBlock* ring = (StatusBlock*)w.status_host;
+    const size_t it0 = rep->iterations;
+    int issued = 0, checked = 0;
+    bool done = false;
+    auto issue = [&](int j) -> int {
       vie::cx<T>* vj = V + (size_t)j * ldv;
       vie::cx<T>* wv = V + (size_t)(j + 1) * ldv;
-      rep->operator_applications++;
-      if (!sys_apply<T>(s, vj, wv, 1, rs)) {
-        rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-        return 0;
-      }
+      if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
       // two classical Gram-Schmidt passes (same role as the reference's two
       // MGS passes, vie_iterative.c:151-159); ||w|| before and after
-      vie::multidot<T>(V, ldv, nullptr, j + 1, wv, n, 1, rs, dots1, st);
-      vie::update<T>(V, ldv, nullptr, j + 1, dots1, wv, n, 0, rs, nullptr, st);
-      vie::multidot<T>(V, ldv, nullptr, j + 1, wv, n, 0, rs, dots2, st);
-      vie::update<T>(V, ldv, nullptr, j + 1, dots2, wv, n, 1, rs, scal + 3, st);
-      vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, scal + 3, Prec<T>::eps_b, st);
+      vie::multidot<T>(V, ldv, j + 1, wv, n, 1, rs, dots1, st);
+      vie::update<T>(V, ldv, j + 1, dots1, wv, n, 0, rs, st);
+      vie::multidot<T>(V, ldv, j + 1, wv, n, 0, rs, dots2, st);
+      unsigned nb = vie::update<T>(V, ldv, j + 1, dots2, wv, n, 1, rs, st);
+      vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, rs, nb, Prec<T>::eps_b, st);
       vie::scale_next<T>(wv, n, sd, st);
-      if (!read_status(s, &sb)) {
-        rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-        return 0;
-      }
+      HIP_OK(hipMemcpyAsync(&ring[j & 1], sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipEventRecord(w.status_ev[j & 1], st));
+      return 1;
+    };
+    auto check = [&](int j) -> int {
+      HIP_OK(hipEventSynchronize(w.status_ev[j & 1]));
+      const StatusBlock& sb = ring[j & 1];
       if (sb.nonfinite) {
         rep->status = VIE_ITERATIVE_NONFINITE;
         return 0;
       }
-      rep->iterations++;
-      used = j + 1;
-      breakdown = sb.breakdown != 0;
-      if (breakdown || sb.gabs <= inner_target) break;
+      done = sb.done != 0;
+      return 1;
+    };
+    while (!done) {
+      bool can_issue = issued < m && it0 + issued < o->maximum_iterations;
+      if (can_issue) {
+        if (!issue(issued)) {
+          rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+          return 0;
+        }
+        issued++;
+        rep->operator_applications++;
+      }
+      if (checked < issued && (issued - checked > 1 || !can_issue)) {
+        if (!check(checked)) return 0;
+        checked++;
+      } else if (!can_issue) {
+        break;
+      }
     }
+    StatusBlock sb;
+    if (!read_status(s, &sb)) return 0;
+    if (sb.nonfinite) {
+      rep->status = VIE_ITERATIVE_NONFINITE;
+      return 0;
+    }
+    rep->iterations = it0 + (size_t)sb.used;
+    const bool breakdown = sb.breakdown != 0;
     vie::backsolve(sd, st);
     if (!read_status(s, &sb)) return 0;
     if (sb.singular) {
       rep->status = VIE_ITERATIVE_BREAKDOWN;
       return 0;
     }
-    (void)used;
     vie::combine<T>(x, V, ldv, n, sd, J, 1, st);
     if (!true_residual(&beta)) return 0;
     rep->relative_residual = bnorm > 0 ? beta / bnorm : beta;
@@ -1051,6 +1085,22 @@ extern "C" int dielectric_fft_upload_rhs(dielectric_fft_system* s, const double 
   return 1;
 }
 
+extern "C" int dielectric_fft_reserve(dielectric_fft_system* s, const vie_gmres_options* options) {
+  vie_gmres_options defaults = {30, 300, 1e-12, 0};
+  if (!s) return 0;
+  if (!options) options = &defaults;
+  if (!validate_options(options)) return 0;
+  size_t n = s->c + s->v, m = options->restart;
+  if (m > n) m = n;
+  if (m > options->maximum_iterations) m = options->maximum_iterations;
+  if (m > (size_t)vie::kMaxRestart) return 0;
+  // FP64-sized basis serves both precisions
+  if (!ensure_solver(s, (int)m, VIE_HIP_FP64) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)))
+    return 0;
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return 1;
+}
+
 extern "C" int dielectric_fft_solve_resident(dielectric_fft_system* s, const vie_gmres_options* options,
                                              const vie_hip_solve_options* so, vie_gmres_report* report) {
   vie_gmres_report local;
@@ -1082,7 +1132,6 @@ static int apply_host(dielectric_fft_system* s, const double _Complex* in, doubl
   } else {
     vie::ReduceScratch rs;
     rs.partial = s->sw.partial.as<c64>();
-    rs.counter = s->sw.counter.as<unsigned>();
     if (!sys_apply<double>(s, s->sw.x.as<c64>(), s->sw.ax.as<c64>(), 0, rs)) return 0;
   }
   return unstage_vector(s, s->sw.ax.as<c64>(), out);
@@ -1098,7 +1147,6 @@ extern "C" int dielectric_fft_apply_ex(dielectric_fft_system* s, vie_hip_precisi
   if (!stage_vector(s, in, s->sw.x.as<c64>())) return 0;
   vie::ReduceScratch rs;
   rs.partial = s->sw.partial.as<c64>();
-  rs.counter = s->sw.counter.as<unsigned>();
   if (prec == VIE_HIP_MIXED) {
     DevBuf a, b;
     if (!a.alloc(n * 8) || !b.alloc(n * 8)) return 0;
@@ -1407,7 +1455,6 @@ extern "C" int dielectric_fft_time_apply(dielectric_fft_system* s, vie_hip_preci
     return 0;
   vie::ReduceScratch rs;
   rs.partial = s->sw.partial.as<c64>();
-  rs.counter = s->sw.counter.as<unsigned>();
   const bool mixed = prec == VIE_HIP_MIXED;
   // resident input: a fixed nonzero vector in the system layout
   DevBuf in, out;

@@ -17,9 +17,12 @@ struct ArnoldiState {
   int nonfinite;
   int singular;     // zero pivot in back substitution
   int used;         // columns in the current cycle
+  int done;         // cycle finished: later Arnoldi tails leave the state alone
+  int pad_;
   // --- working data
   int j, m;
   double inv_beta, inv_next;
+  double inner_target;  // cycle stops once |g_{j+1}| <= inner_target
   c64 g[kMaxRestart + 1];
   double cs[kMaxRestart];
   c64 sn[kMaxRestart];
@@ -27,9 +30,24 @@ struct ArnoldiState {
   c64 H[(kMaxRestart + 1) * kMaxRestart];
 };
 
+constexpr int kMaxKeys = 256;  // sums per reduction launch (>= kMaxRestart + 2)
+
 struct ReduceScratch {
-  c64* partial = nullptr;      // (kMaxRestart + 2) * blocks partial sums
-  unsigned* counter = nullptr; // last-block counter, zero between launches
+  c64* partial = nullptr;  // kMaxDotBlocksHost x kMaxKeys partial sums
+};
+constexpr size_t kReduceScratchBytes = (size_t)1024 * kMaxKeys * sizeof(c64);
+
+// Jacobi scaling of the volume part of a system vector, for kernels that
+// read v and need z = D^{-1} v (index i counts from the first volume entry).
+template <typename T> struct JacobiLite {
+  size_t nbox;
+  const cx<T>* eps;
+  cx<T> self[3];
+  __device__ inline cx<T> apply(size_t i, cx<T> v) const {
+    size_t comp = i / nbox, cell = i - comp * nbox;
+    cx<T> e = eps[cell], one = {(T)1, (T)0};
+    return cdiv(v, e - (e - one) * self[comp]);
+  }
 };
 
 struct JacobiView {
@@ -54,18 +72,21 @@ template <typename T> struct CouplingView {
 size_t red_blocks(size_t n);
 void copy16(void* dst, const void* src, size_t n16, hipStream_t s);
 
+// dots[k] = V[k]^H w (k < nk), dots[nk] = ||w||^2 when self_norm.
 template <typename T>
-void multidot(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const cx<T>* w, size_t n, int self_norm,
-              ReduceScratch& rs, c64* dots, hipStream_t s);
+void multidot(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n, int self_norm, ReduceScratch& rs,
+              c64* dots, hipStream_t s);
+// w -= sum_k h[k] V[k]; with want_norm the ||w||^2 partials stay in rs for
+// arnoldi_tail.  Returns the number of partials.
 template <typename T>
-void update(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const c64* h, cx<T>* w, size_t n,
-            int want_norm, ReduceScratch& rs, c64* norm_out, hipStream_t s);
+unsigned update(const cx<T>* V, size_t ldv, int nk, const c64* h, cx<T>* w, size_t n, int want_norm,
+                ReduceScratch& rs, hipStream_t s);
 template <typename T> void norm2(const cx<T>* x, size_t n, ReduceScratch& rs, c64* out, hipStream_t s);
 void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, c64* out, hipStream_t s);
-void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, const c64* next2,
-                  double breakdown_eps, hipStream_t s);
+void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, ReduceScratch& rs,
+                  unsigned nblk, double breakdown_eps, hipStream_t s);
 void backsolve(ArnoldiState* st, hipStream_t s);
-void cycle_init(ArnoldiState* st, const c64* beta2, hipStream_t s);
+void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s);
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s);
 template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st, hipStream_t s);
 template <typename T>

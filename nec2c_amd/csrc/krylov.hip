@@ -12,10 +12,23 @@
 namespace vie {
 
 constexpr int KWG = 256;  // threads per block
-constexpr int KEPT = 4;   // elements per thread per vector
+constexpr int KEPT = 4;   // elements per thread (elementwise reductions)
 constexpr int KCH = KWG * KEPT;
+constexpr int kMaxDotBlocks = 1024;  // grid-stride cap: partials per key
 
 size_t red_blocks(size_t n) { return (n + KCH - 1) / KCH; }
+
+template <typename T> struct DotShape;
+template <typename T> static unsigned dot_blocks(size_t n) {
+  size_t ch = (size_t)64 * DotShape<T>::CPL;
+  size_t b = (n + ch - 1) / ch;
+  return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
+}
+static unsigned update_blocks(size_t n) {
+  size_t ch = (size_t)KWG * 2;
+  size_t b = (n + ch - 1) / ch;
+  return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
+}
 
 __device__ inline c64 wave_reduce(c64 v) {
   for (int m = 32; m >= 1; m >>= 1) {
@@ -38,125 +51,155 @@ __device__ inline c64 block_reduce(c64 v, c64* lds) {
   return t;
 }
 
-// Last-block handshake: returns true in the one block that must finish.
-__device__ inline bool last_block(unsigned* counter, unsigned nblk) {
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned old = atomicAdd(counter, 1u);
-    is_last = (old == nblk - 1);
-    if (is_last) *counter = 0;
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
-}
+// ---------------------------------------------------------------- dots
+// partial[blk * ldp + k] = block's share of sum_i op(V[k][i]) z[i], where
+// op is conj (Krylov projections) or identity (the conductor rows B z), and
+// z = w, or D^{-1} w (Jacobi on the fly) with JAC.  With self_norm, key nk
+// receives ||w||^2.  The vector is cut into chunks of 64 * CPL elements;
+// every wave of a block holds the chunk of z in registers and takes whole
+// keys (k = wave, wave + 4, ...), so one shuffle reduction covers CPL loads
+// per lane.  Grid-stride over chunks; LDS holds the block's per-key sums.
+template <typename T> struct DotShape { static constexpr int CPL = 128 / (int)sizeof(cx<T>); };
 
-// Final stage: out[k] = sum_b partial[k * nblk + b] for k < nk (block order).
-__device__ inline void finish_sums(const c64* partial, int nk, unsigned nblk, c64* out, c64* lds) {
-  for (int k = 0; k < nk; k++) {
-    c64 s = {0, 0};
-    for (unsigned b = threadIdx.x; b < nblk; b += KWG) s = s + partial[(size_t)k * nblk + b];
-    s = block_reduce(s, lds);
-    if (threadIdx.x == 0) out[k] = s;
-  }
-}
-
-// ---------------------------------------------------------------- multidot
-// dots[k] = sum_i conj(V[k][i]) w[i]  (k < nk), plus dots[nk] = ||w||^2 when
-// self_norm.  V rows are ldv apart.
-template <typename T>
-__global__ __launch_bounds__(KWG) void k_multidot(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host,
-                                                  const cx<T>* w, size_t n, int self_norm, c64* partial,
-                                                  unsigned* counter, c64* dots) {
-  __shared__ c64 lds[KWG / 64];
-  const int nk = nk_dev ? *nk_dev : nk_host;
-  const unsigned nblk = gridDim.x;
-  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
-  cx<T> wl[KEPT];
-#pragma unroll
-  for (int e = 0; e < KEPT; e++) {
-    size_t i = base + (size_t)e * KWG;
-    wl[e] = i < n ? w[i] : cx<T>{0, 0};
-  }
+template <typename T, bool CONJ, bool JAC>
+__global__ __launch_bounds__(KWG) void k_dots(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n,
+                                              int self_norm, c64* partial, int ldp, JacobiLite<T> J) {
+  constexpr int CPL = DotShape<T>::CPL;
+  constexpr int WAVES = KWG / 64;
+  __shared__ c64 lds[kMaxKeys];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tot = nk + (self_norm ? 1 : 0);
-  for (int k = 0; k < tot; k++) {
-    c64 s = {0, 0};
-    if (k < nk) {
-      const cx<T>* vk = V + (size_t)k * ldv;
+  for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
+  __syncthreads();
+  const size_t chunk = (size_t)64 * CPL;
+  const size_t nchunks = (n + chunk - 1) / chunk;
+  for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const size_t base = ch * chunk + lane;
+    double zr[CPL], zi[CPL];
 #pragma unroll
-      for (int e = 0; e < KEPT; e++) {
-        size_t i = base + (size_t)e * KWG;
-        if (i < n) {
-          cx<T> a = vk[i];
-          double ar = a.re, ai = a.im, br = wl[e].re, bi = wl[e].im;
-          s.re += ar * br + ai * bi;
-          s.im += ar * bi - ai * br;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < KEPT; e++) {
-        double br = wl[e].re, bi = wl[e].im;
-        s.re += br * br + bi * bi;
-      }
+    for (int e = 0; e < CPL; e++) {
+      size_t i = base + (size_t)e * 64;
+      cx<T> v = i < n ? w[i] : cx<T>{0, 0};
+      if (JAC && i < n) v = J.apply(i, v);
+      zr[e] = v.re;
+      zi[e] = v.im;
     }
-    s = block_reduce(s, lds);
-    if (threadIdx.x == 0) partial[(size_t)k * nblk + blockIdx.x] = s;
+    for (int k = wave; k < tot; k += WAVES) {
+      c64 s = {0, 0};
+      if (k < nk) {
+        const cx<T>* vk = V + (size_t)k * ldv;
+        cx<T> a[CPL];
+#pragma unroll
+        for (int e = 0; e < CPL; e++) {
+          size_t i = base + (size_t)e * 64;
+          a[e] = i < n ? vk[i] : cx<T>{0, 0};
+        }
+#pragma unroll
+        for (int e = 0; e < CPL; e++) {
+          double ar = a[e].re, ai = CONJ ? -(double)a[e].im : (double)a[e].im;
+          s.re += ar * zr[e] - ai * zi[e];
+          s.im += ar * zi[e] + ai * zr[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; e++) s.re += zr[e] * zr[e] + zi[e] * zi[e];
+      }
+      s = wave_reduce(s);
+      if (lane == 0) lds[k] = lds[k] + s;  // keys are owned by one wave
+    }
   }
-  if (last_block(counter, nblk)) finish_sums(partial, tot, nblk, dots, lds);
+  __syncthreads();
+  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)blockIdx.x * ldp + k] = lds[k];
+}
+
+// out[k] = sum_b partial[b * ldp + k] (+ add[k] when given), 32 lanes per key,
+// fixed summation order.
+__global__ __launch_bounds__(KWG) void k_finish(const c64* partial, int ldp, unsigned nblk, int nk, c64* out) {
+  const int key = blockIdx.x * (KWG / 32) + (threadIdx.x >> 5);
+  const int sub = threadIdx.x & 31;
+  c64 s = {0, 0};
+  if (key < nk)
+    for (unsigned b = sub; b < nblk; b += 32) s = s + partial[(size_t)b * ldp + key];
+  for (int m = 16; m >= 1; m >>= 1) {
+    s.re += __shfl_xor(s.re, m, 32);
+    s.im += __shfl_xor(s.im, m, 32);
+  }
+  if (key < nk && sub == 0) out[key] = s;
 }
 
 // ---------------------------------------------------------------- update
-// w -= sum_k h[k] V[k] (h from device memory, FP64), optionally ||w||^2.
+// w -= sum_k h[k] V[k] (h from device memory, FP64 accumulation), with the
+// block's share of ||w||^2 in partial[blk * ldp] when want_norm.  Two
+// elements per thread and four basis vectors per loop step keep eight loads
+// in flight per lane.
+constexpr int UEPT = 2;
+
 template <typename T>
-__global__ __launch_bounds__(KWG) void k_update(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host,
-                                                const c64* h, cx<T>* w, size_t n, int want_norm, c64* partial,
-                                                unsigned* counter, c64* norm_out) {
+__global__ __launch_bounds__(KWG) void k_update(const cx<T>* V, size_t ldv, int nk, const c64* h, cx<T>* w,
+                                                size_t n, int want_norm, c64* partial, int ldp) {
+  __shared__ c64 hs[kMaxKeys];
   __shared__ c64 lds[KWG / 64];
-  __shared__ c64 hs[kMaxRestart + 1];
-  const int nk = nk_dev ? *nk_dev : nk_host;
   for (int k = threadIdx.x; k < nk; k += KWG) hs[k] = h[k];
   __syncthreads();
-  const unsigned nblk = gridDim.x;
-  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
-  double acc_re[KEPT], acc_im[KEPT];
+  c64 s = {0, 0};
+  const size_t step = (size_t)gridDim.x * KWG * UEPT;
+  for (size_t base = (size_t)blockIdx.x * KWG * UEPT + threadIdx.x; base < n; base += step) {
+    double ar[UEPT], ai[UEPT];
+    bool in[UEPT];
 #pragma unroll
-  for (int e = 0; e < KEPT; e++) {
-    size_t i = base + (size_t)e * KWG;
-    cx<T> v = i < n ? w[i] : cx<T>{0, 0};
-    acc_re[e] = v.re;
-    acc_im[e] = v.im;
-  }
-  for (int k = 0; k < nk; k++) {
-    const cx<T>* vk = V + (size_t)k * ldv;
-    const double hr = hs[k].re, hi = hs[k].im;
-#pragma unroll
-    for (int e = 0; e < KEPT; e++) {
+    for (int e = 0; e < UEPT; e++) {
       size_t i = base + (size_t)e * KWG;
-      if (i < n) {
-        cx<T> a = vk[i];
-        acc_re[e] -= hr * a.re - hi * a.im;
-        acc_im[e] -= hr * a.im + hi * a.re;
+      in[e] = i < n;
+      cx<T> v = in[e] ? w[i] : cx<T>{0, 0};
+      ar[e] = v.re;
+      ai[e] = v.im;
+    }
+    int k = 0;
+    for (; k + 4 <= nk; k += 4) {
+      cx<T> a[4][UEPT];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int e = 0; e < UEPT; e++) {
+          size_t i = base + (size_t)e * KWG;
+          a[q][e] = in[e] ? V[(size_t)(k + q) * ldv + i] : cx<T>{0, 0};
+        }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const double hr = hs[k + q].re, hi = hs[k + q].im;
+#pragma unroll
+        for (int e = 0; e < UEPT; e++) {
+          ar[e] -= hr * a[q][e].re - hi * a[q][e].im;
+          ai[e] -= hr * a[q][e].im + hi * a[q][e].re;
+        }
       }
     }
-  }
-  c64 s = {0, 0};
+    for (; k < nk; k++) {
+      const double hr = hs[k].re, hi = hs[k].im;
 #pragma unroll
-  for (int e = 0; e < KEPT; e++) {
-    size_t i = base + (size_t)e * KWG;
-    if (i < n) {
-      cx<T> r = {(T)acc_re[e], (T)acc_im[e]};
-      w[i] = r;
-      double rr = r.re, ri = r.im;
-      s.re += rr * rr + ri * ri;
+      for (int e = 0; e < UEPT; e++) {
+        size_t i = base + (size_t)e * KWG;
+        if (in[e]) {
+          cx<T> a = V[(size_t)k * ldv + i];
+          ar[e] -= hr * a.re - hi * a.im;
+          ai[e] -= hr * a.im + hi * a.re;
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < UEPT; e++) {
+      size_t i = base + (size_t)e * KWG;
+      if (in[e]) {
+        cx<T> r = {(T)ar[e], (T)ai[e]};
+        w[i] = r;
+        double rr = r.re, ri = r.im;
+        s.re += rr * rr + ri * ri;
+      }
     }
   }
   if (!want_norm) return;
   s = block_reduce(s, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (last_block(counter, nblk)) finish_sums(partial, 1, nblk, norm_out, lds);
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ldp] = s;
 }
 
 // ---------------------------------------------------------------- Arnoldi tail
@@ -178,59 +221,79 @@ __device__ inline void givens(c64 a, c64 b, double* c, c64* s) {
   }
 }
 
-__global__ void k_arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2,
-                               const c64* next2, double breakdown_eps) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(KWG) void k_arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2,
+                                                      const c64* before2, const c64* norm_partial, int ldp,
+                                                      unsigned nblk, double breakdown_eps) {
+  __shared__ c64 col[kMaxRestart + 2];
+  __shared__ c64 sn[kMaxRestart];
+  __shared__ double cs[kMaxRestart];
+  __shared__ c64 lds[KWG / 64];
+  if (st->done) return;  // a step issued after the cycle ended
   const int j = st->j, m = st->m;
+  for (int k = threadIdx.x; k <= j; k += KWG) col[k] = h1[k] + (h2 ? h2[k] : c64{0, 0});
+  for (int k = threadIdx.x; k < j; k += KWG) {
+    sn[k] = st->sn[k];
+    cs[k] = st->cs[k];
+  }
+  c64 s = {0, 0};
+  for (unsigned b = threadIdx.x; b < nblk; b += KWG) s = s + norm_partial[(size_t)b * ldp];
+  s = block_reduce(s, lds);  // contains the barrier that publishes col / sn / cs
+  if (threadIdx.x != 0) return;
   c64* H = st->H;  // row-major (m+1) x m
-  for (int k = 0; k <= j; k++) H[k * m + j] = h1[k] + (h2 ? h2[k] : c64{0, 0});
-  double before = sqrt(before2->re), next = sqrt(next2->re);
+  double before = sqrt(before2->re), next = sqrt(s.re);
   st->nonfinite = !(isfinite(before) && isfinite(next));
-  H[(j + 1) * m + j] = c64{next, 0};
   st->breakdown = next <= breakdown_eps * before;
   st->inv_next = st->breakdown || next == 0 ? 0.0 : 1.0 / next;
+  col[j + 1] = c64{next, 0};
   for (int k = 0; k < j; k++) {
-    c64 f = H[k * m + j], s = H[(k + 1) * m + j];
-    double c = st->cs[k];
-    c64 sn = st->sn[k];
-    H[k * m + j] = c * f + sn * s;
-    H[(k + 1) * m + j] = c * s - conj(sn) * f;
+    c64 f = col[k], g = col[k + 1];
+    col[k] = cs[k] * f + sn[k] * g;
+    col[k + 1] = cs[k] * g - conj(sn[k]) * f;
   }
   double c;
-  c64 sn;
-  givens(H[j * m + j], H[(j + 1) * m + j], &c, &sn);
+  c64 z;
+  givens(col[j], col[j + 1], &c, &z);
   st->cs[j] = c;
-  st->sn[j] = sn;
-  H[j * m + j] = c * H[j * m + j] + sn * H[(j + 1) * m + j];
-  H[(j + 1) * m + j] = c64{0, 0};
+  st->sn[j] = z;
+  col[j] = c * col[j] + z * col[j + 1];
+  col[j + 1] = c64{0, 0};
+  for (int k = 0; k <= j + 1; k++) H[k * m + j] = col[k];
   c64 gj = st->g[j];
-  st->g[j + 1] = c64{0, 0} - conj(sn) * gj;
+  st->g[j + 1] = c64{0, 0} - conj(z) * gj;
   st->g[j] = c * gj;
   st->gabs = hypot(st->g[j + 1].re, st->g[j + 1].im);
   st->used = j + 1;
   st->j = j + 1;
+  st->done = st->breakdown || st->nonfinite || st->gabs <= st->inner_target || j + 1 >= m;
 }
 
-// y = H(0:used,0:used)^{-1} g by back substitution (vie_iterative.c:179-186).
-__global__ void k_backsolve(ArnoldiState* st) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// y = H(0:used,0:used)^{-1} g by back substitution (vie_iterative.c:179-186):
+// one row dot product per step, spread over the block.
+__global__ __launch_bounds__(KWG) void k_backsolve(ArnoldiState* st) {
+  __shared__ c64 y[kMaxRestart];
+  __shared__ c64 lds[KWG / 64];
+  __shared__ int bad;
   const int used = st->used, m = st->m;
   const c64* H = st->H;
-  st->singular = 0;
+  if (threadIdx.x == 0) bad = 0;
   for (int i = used - 1; i >= 0; i--) {
-    c64 v = st->g[i];
-    for (int k = i + 1; k < used; k++) v = v - H[i * m + k] * st->y[k];
-    c64 d = H[i * m + i];
-    if (d.re == 0.0 && d.im == 0.0) {
-      st->singular = 1;
-      return;
+    c64 part = {0, 0};
+    for (int k = i + 1 + threadIdx.x; k < used; k += KWG) part = part + H[i * m + k] * y[k];
+    part = block_reduce(part, lds);
+    if (threadIdx.x == 0) {
+      c64 d = H[i * m + i];
+      if (d.re == 0.0 && d.im == 0.0) bad = 1;
+      else y[i] = cdiv(st->g[i] - part, d);
     }
-    st->y[i] = cdiv(v, d);
+    __syncthreads();
+    if (bad) break;
   }
+  if (threadIdx.x == 0) st->singular = bad;
+  for (int k = threadIdx.x; k < used && !bad; k += KWG) st->y[k] = y[k];
 }
 
 // Starts a cycle: g = [beta, 0, ...], H = 0, j = 0.
-__global__ void k_cycle_init(ArnoldiState* st, const c64* beta2) {
+__global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_target) {
   const int m = st->m;
   for (int i = threadIdx.x; i < (m + 1) * m; i += blockDim.x) st->H[i] = c64{0, 0};
   for (int i = threadIdx.x; i <= m; i += blockDim.x) st->g[i] = c64{0, 0};
@@ -244,6 +307,8 @@ __global__ void k_cycle_init(ArnoldiState* st, const c64* beta2) {
     st->used = 0;
     st->breakdown = 0;
     st->nonfinite = !isfinite(beta);
+    st->done = 0;
+    st->inner_target = inner_target;
   }
 }
 
@@ -303,44 +368,32 @@ __global__ void k_combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const Ar
   x[i] = x[i] + s;
 }
 
-// r = b - Ax and ||r||^2.
+// r = b - Ax and the block's share of ||r||^2 (partial[blk * ldp]).
 __global__ __launch_bounds__(KWG) void k_residual(const c64* b, const c64* ax, c64* r, size_t n, c64* partial,
-                                                  unsigned* counter, c64* norm_out) {
+                                                  int ldp) {
   __shared__ c64 lds[KWG / 64];
-  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
   c64 s = {0, 0};
-#pragma unroll
-  for (int e = 0; e < KEPT; e++) {
-    size_t i = base + (size_t)e * KWG;
-    if (i < n) {
-      c64 v = b[i] - ax[i];
-      r[i] = v;
-      s.re += v.re * v.re + v.im * v.im;
-    }
+  const size_t step = (size_t)gridDim.x * KWG;
+  for (size_t i = (size_t)blockIdx.x * KWG + threadIdx.x; i < n; i += step) {
+    c64 v = b[i] - ax[i];
+    r[i] = v;
+    s.re += v.re * v.re + v.im * v.im;
   }
   s = block_reduce(s, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (last_block(counter, gridDim.x)) finish_sums(partial, 1, gridDim.x, norm_out, lds);
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ldp] = s;
 }
 
-// ||x||^2 with a nonfinite flag folded in as NaN.
 template <typename T>
-__global__ __launch_bounds__(KWG) void k_norm2(const cx<T>* x, size_t n, c64* partial, unsigned* counter,
-                                               c64* norm_out) {
+__global__ __launch_bounds__(KWG) void k_norm2(const cx<T>* x, size_t n, c64* partial, int ldp) {
   __shared__ c64 lds[KWG / 64];
-  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
   c64 s = {0, 0};
-#pragma unroll
-  for (int e = 0; e < KEPT; e++) {
-    size_t i = base + (size_t)e * KWG;
-    if (i < n) {
-      double a = x[i].re, b = x[i].im;
-      s.re += a * a + b * b;
-    }
+  const size_t step = (size_t)gridDim.x * KWG;
+  for (size_t i = (size_t)blockIdx.x * KWG + threadIdx.x; i < n; i += step) {
+    double a = x[i].re, b = x[i].im;
+    s.re += a * a + b * b;
   }
   s = block_reduce(s, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (last_block(counter, gridDim.x)) finish_sums(partial, 1, gridDim.x, norm_out, lds);
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ldp] = s;
 }
 
 // ---------------------------------------------------------------- conductor coupling
@@ -348,49 +401,6 @@ __global__ __launch_bounds__(KWG) void k_norm2(const cx<T>* x, size_t n, c64* pa
 template <typename T> __global__ void k_cond_prep(const cx<T>* vc, const cx<T>* Cm, int c, int precond, cx<T>* zc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < c) zc[i] = precond ? cdiv(vc[i], Cm[(size_t)i * c + i]) : vc[i];
-}
-
-// partial sums of B[r] . z_v (no conjugate), z_v = D^{-1} v_v on the fly.
-template <typename T>
-__global__ __launch_bounds__(KWG) void k_bdot(const cx<T>* B, int c, const cx<T>* vv, size_t nv, size_t nbox,
-                                              const cx<T>* eps, cx<T> s0, cx<T> s1, cx<T> s2, int precond,
-                                              c64* partial, unsigned* counter, c64* out) {
-  __shared__ c64 lds[KWG / 64];
-  const unsigned nblk = gridDim.x;
-  const size_t base = (size_t)blockIdx.x * KCH + threadIdx.x;
-  const cx<T> one = {(T)1, (T)0};
-  cx<T> zl[KEPT];
-#pragma unroll
-  for (int e = 0; e < KEPT; e++) {
-    size_t i = base + (size_t)e * KWG;
-    cx<T> v = {0, 0};
-    if (i < nv) {
-      v = vv[i];
-      if (precond) {
-        size_t comp = i / nbox, cell = i - comp * nbox;
-        cx<T> ee = eps[cell], sf = comp == 0 ? s0 : comp == 1 ? s1 : s2;
-        v = cdiv(v, ee - (ee - one) * sf);
-      }
-    }
-    zl[e] = v;
-  }
-  for (int r = 0; r < c; r++) {
-    const cx<T>* br = B + (size_t)r * nv;
-    c64 s = {0, 0};
-#pragma unroll
-    for (int e = 0; e < KEPT; e++) {
-      size_t i = base + (size_t)e * KWG;
-      if (i < nv) {
-        cx<T> a = br[i];
-        double ar = a.re, ai = a.im, zr = zl[e].re, zi = zl[e].im;
-        s.re += ar * zr - ai * zi;
-        s.im += ar * zi + ai * zr;
-      }
-    }
-    s = block_reduce(s, lds);
-    if (threadIdx.x == 0) partial[(size_t)r * nblk + blockIdx.x] = s;
-  }
-  if (last_block(counter, nblk)) finish_sums(partial, c, nblk, out, lds);
 }
 
 // y_c = C z_c + (B z_v)   (dielectric_fft.c:56-60)
@@ -476,34 +486,44 @@ template void vec_cast<double, float>(c64*, const c32*, size_t, hipStream_t);
 template void vec_cast<double, double>(c64*, const c64*, size_t, hipStream_t);
 
 template <typename T>
-void multidot(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const cx<T>* w, size_t n, int self_norm,
-              ReduceScratch& rs, c64* dots, hipStream_t s) {
-  unsigned nb = (unsigned)red_blocks(n);
-  hipLaunchKernelGGL((k_multidot<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk_dev, nk_host, w, n, self_norm,
-                     rs.partial, rs.counter, dots);
+void multidot(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n, int self_norm, ReduceScratch& rs,
+              c64* dots, hipStream_t s) {
+  unsigned nb = dot_blocks<T>(n);
+  JacobiLite<T> none{};
+  hipLaunchKernelGGL((k_dots<T, true, false>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, w, n, self_norm, rs.partial,
+                     kMaxKeys, none);
+  int tot = nk + (self_norm ? 1 : 0);
+  hipLaunchKernelGGL(k_finish, dim3((tot + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, tot, dots);
 }
 template <typename T>
-void update(const cx<T>* V, size_t ldv, const int* nk_dev, int nk_host, const c64* h, cx<T>* w, size_t n,
-            int want_norm, ReduceScratch& rs, c64* norm_out, hipStream_t s) {
-  unsigned nb = (unsigned)red_blocks(n);
-  hipLaunchKernelGGL((k_update<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk_dev, nk_host, h, w, n, want_norm,
-                     rs.partial, rs.counter, norm_out);
+unsigned update(const cx<T>* V, size_t ldv, int nk, const c64* h, cx<T>* w, size_t n, int want_norm,
+                ReduceScratch& rs, hipStream_t s) {
+  unsigned nb = update_blocks(n);
+  hipLaunchKernelGGL((k_update<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, h, w, n, want_norm, rs.partial, kMaxKeys);
+  return nb;
+}
+static unsigned elem_blocks(size_t n) {
+  size_t b = (n + KWG - 1) / KWG;
+  return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
 }
 template <typename T> void norm2(const cx<T>* x, size_t n, ReduceScratch& rs, c64* out, hipStream_t s) {
-  hipLaunchKernelGGL((k_norm2<T>), dim3((unsigned)red_blocks(n)), dim3(KWG), 0, s, x, n, rs.partial, rs.counter,
-                     out);
+  unsigned nb = elem_blocks(n);
+  hipLaunchKernelGGL((k_norm2<T>), dim3(nb), dim3(KWG), 0, s, x, n, rs.partial, kMaxKeys);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, 1, out);
 }
 void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, c64* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_residual, dim3((unsigned)red_blocks(n)), dim3(KWG), 0, s, b, ax, r, n, rs.partial, rs.counter,
-                     out);
+  unsigned nb = elem_blocks(n);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(KWG), 0, s, b, ax, r, n, rs.partial, kMaxKeys);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, 1, out);
 }
-void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, const c64* next2,
-                  double breakdown_eps, hipStream_t s) {
-  hipLaunchKernelGGL(k_arnoldi_tail, dim3(1), dim3(64), 0, s, st, h1, h2, before2, next2, breakdown_eps);
+void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, ReduceScratch& rs,
+                  unsigned nblk, double breakdown_eps, hipStream_t s) {
+  hipLaunchKernelGGL(k_arnoldi_tail, dim3(1), dim3(KWG), 0, s, st, h1, h2, before2, rs.partial, kMaxKeys, nblk,
+                     breakdown_eps);
 }
-void backsolve(ArnoldiState* st, hipStream_t s) { hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, s, st); }
-void cycle_init(ArnoldiState* st, const c64* beta2, hipStream_t s) {
-  hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2);
+void backsolve(ArnoldiState* st, hipStream_t s) { hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(KWG), 0, s, st); }
+void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s) {
+  hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2, inner_target);
 }
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s) {
   hipLaunchKernelGGL((k_scale_from<T>), dim3(gblocks(n)), dim3(256), 0, s, v, r, n, st);
@@ -537,9 +557,21 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
   if (c == 0) return;
   hipLaunchKernelGGL((k_cond_prep<T>), dim3(gblocks(c)), dim3(256), 0, s, vc, cv.C, c, precond, cv.zc);
   if (cv.nv) {
-    unsigned nb = (unsigned)red_blocks(cv.nv);
-    hipLaunchKernelGGL((k_bdot<T>), dim3(nb), dim3(KWG), 0, s, cv.B, c, vv, cv.nv, cv.nbox, cv.eps, cv.self[0],
-                       cv.self[1], cv.self[2], precond, rs.partial, rs.counter, cv.bz);
+    // B z_v in row chunks of kMaxKeys, Jacobi applied to v_v on the fly
+    JacobiLite<T> J{cv.nbox, cv.eps, {cv.self[0], cv.self[1], cv.self[2]}};
+    unsigned nb = dot_blocks<T>(cv.nv);
+    for (int r0 = 0; r0 < c; r0 += kMaxKeys) {
+      int rows = c - r0 < kMaxKeys ? c - r0 : kMaxKeys;
+      const cx<T>* Br = cv.B + (size_t)r0 * cv.nv;
+      if (precond)
+        hipLaunchKernelGGL((k_dots<T, false, true>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
+                           rs.partial, kMaxKeys, J);
+      else
+        hipLaunchKernelGGL((k_dots<T, false, false>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
+                           rs.partial, kMaxKeys, J);
+      hipLaunchKernelGGL(k_finish, dim3((rows + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, rows,
+                         cv.bz + r0);
+    }
   } else {
     hipLaunchKernelGGL((k_zero<double>), dim3(gblocks(c)), dim3(256), 0, s, cv.bz, (size_t)c);
   }
@@ -548,10 +580,10 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
 }
 
 #define VIE_INST(T)                                                                                              \
-  template void multidot<T>(const cx<T>*, size_t, const int*, int, const cx<T>*, size_t, int, ReduceScratch&, c64*, \
+  template void multidot<T>(const cx<T>*, size_t, int, const cx<T>*, size_t, int, ReduceScratch&, c64*,            \
                             hipStream_t);                                                                        \
-  template void update<T>(const cx<T>*, size_t, const int*, int, const c64*, cx<T>*, size_t, int, ReduceScratch&,  \
-                          c64*, hipStream_t);                                                                    \
+  template unsigned update<T>(const cx<T>*, size_t, int, const c64*, cx<T>*, size_t, int, ReduceScratch&,          \
+                              hipStream_t);                                                                      \
   template void norm2<T>(const cx<T>*, size_t, ReduceScratch&, c64*, hipStream_t);                               \
   template void scale_from<T>(cx<T>*, const c64*, size_t, const ArnoldiState*, hipStream_t);                     \
   template void scale_next<T>(cx<T>*, size_t, const ArnoldiState*, hipStream_t);                                 \

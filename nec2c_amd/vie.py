@@ -120,7 +120,7 @@ EXPORTS = (
     "dielectric_fft_plan", "vie_hip_last_error", "vie_hip_device_count",
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
     "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
-    "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex",
+    "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_reserve",
 )
 
 _LIB = None
@@ -184,6 +184,7 @@ def lib() -> ctypes.CDLL:
         "vie_fft_set_permittivities": (i, [vp, vp, sz]),
         "vie_hip_synchronize": (i, []),
         "dielectric_fft_apply_ex": (i, [vp, i, i, vp, vp]),
+        "dielectric_fft_reserve": (i, [vp, ctypes.POINTER(GmresOptions)]),
         "vie_hip_copy_probe": (d, [sz, i]),
     }
     for name, (res, args) in sig.items():
@@ -487,6 +488,12 @@ class System:
     def upload_rhs(self, b):
         b = np.ascontiguousarray(b, dtype=np.complex128)
         if not lib().dielectric_fft_upload_rhs(self.s, _ptr(b)):
+            raise RuntimeError(last_error())
+
+    def reserve(self, **opts):
+        """dielectric_fft_reserve: make a later resident solve allocation-free."""
+        o = _options(**opts)
+        if not lib().dielectric_fft_reserve(self.s, ctypes.byref(o)):
             raise RuntimeError(last_error())
 
     def solve_resident(self, precision=FP64, inner_reduction=0.0, **opts):
