@@ -1,0 +1,232 @@
+"""CPU-side checks of libvie_hip.so: it loads, exports every symbol the
+public headers declare, keeps the reference's names/constants, rejects bad
+arguments before touching a device, and its host GMRES over callback
+operators reproduces the reference's behaviour (tests/vie_iterative_test.c).
+No device compute happens here."""
+import ctypes
+import glob
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from nec2c_amd import vie
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M):
+            line = text[text.rfind("\n", 0, m.start()) + 1:m.end()]
+            if line.lstrip().startswith(("typedef", "#", "return", "if")):
+                continue
+            names.add(m.group(1))
+        for m in re.finditer(r"extern\s+\w+\s+(\w+)\s*;", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = vie.lib()
+    names = declared_symbols()
+    assert "vie_gmres" in names and "dielectric_fft_prepare" in names and "dielectric_gmres_settings" in names
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(vie.EXPORTS) <= names | {"dielectric_gmres_settings"}
+
+
+def test_reference_constants_and_names():
+    L = vie.lib()
+    o = vie.gmres_settings()
+    assert (o.restart, o.maximum_iterations, o.relative_tolerance, o.absolute_tolerance) == (30, 300, 1e-12, 0.0)
+    d = vie.GmresOptions()
+    L.vie_gmres_default_options(ctypes.byref(d))
+    assert (d.restart, d.maximum_iterations, d.relative_tolerance, d.absolute_tolerance) == (30, 300, 1e-12, 0.0)
+    assert [L.vie_iterative_status_name(i).decode() for i in range(7)] == list(vie.STATUS)
+    assert L.vie_iterative_status_name(99) == b"unknown status"
+    assert L.dielectric_fft_backend(None) == b"none"
+    assert L.vie_fft_fftw_available() == 0
+
+
+def test_backend_selection():
+    assert not vie.backend_set("fftw") and not vie.backend_set("builtin") and not vie.backend_set("invalid")
+    assert vie.backend_set("hip")
+    assert vie.lib().vie_fft_backend_requested_name() == b"hip"
+    assert vie.backend_set("auto") and vie.backend_set(None)
+    assert vie.lib().vie_fft_backend_requested_name() == b"auto"
+    vie.lib().vie_fft_backend_clear_last_used()
+    assert vie.lib().vie_fft_backend_last_used_name() is None
+
+
+@pytest.mark.parametrize("args", [
+    ((5, 3, 2), [0, 0, 0, 4, 2, 1, 1, 0, 0, 0, 1, 0, 0, 0, 1, 1, 1, 1], .25, .2, 3, 1),     # budget
+    ((5, 3, 2), [0, 0, 0, 0, 0, 0], .25, .2, 3, 10 ** 9),                                   # duplicate
+    ((1, 1, 1), [0, 0, 0, 4, 2, 1], .25, .2, 3, 10 ** 9),                                   # outside
+    ((1, 1, 1), [0, 0, 0], .25, 5.0, 3, 10 ** 9),                                           # k h > 1
+    ((1, 1, 1), [0, 0, 0], .25, .2, 3 + .4j, 10 ** 9),                                      # active
+    ((1, 1, 1), [0, 0, 0], .25, .2, 0.5, 10 ** 9),                                          # Re < 1
+    ((1, 1, 1), [0, 0, 0], -.25, .2, 3, 10 ** 9),                                           # pitch
+    ((0, 2, 2), [0, 0, 0], .25, .2, 3, 10 ** 9),                                            # empty axis
+])
+def test_create_rejections_before_any_device_use(args):
+    # tests/vie_fft_test.c:50-54,69 -- all of these fail on argument checks
+    shape, xyz, h, k, er, limit = args
+    with pytest.raises(ValueError):
+        vie.FFTPlan(shape, xyz, h, k, er, byte_limit=limit)
+
+
+def test_create_overflow_shape_rejected():
+    L = vie.lib()
+    p = ctypes.c_void_p(1234)
+    shape = (ctypes.c_size_t * 3)(2 ** 64 - 1, 2, 2)
+    xyz = (ctypes.c_size_t * 3)(0, 0, 0)
+    assert not L.vie_fft_create(ctypes.byref(p), shape, 1, xyz, .25, .2, vie.CDouble.of(3), 2 ** 64 - 1)
+    assert not p.value
+
+
+def test_prepare_rejections_before_any_device_use():
+    M = [vie.MaterialSpec(1, 3.0), vie.MaterialSpec(2, 2.0)]
+    C = vie.Cell
+    cases = [
+        ([C((0, 0, 0), (.25,) * 3, 1), C((.3, 0, 0), (.25,) * 3, 1)], "common lattice"),
+        ([C((0, 0, 0), (.25,) * 3, 1), C((.25, 0, 0), (.25, .25, .125), 1)], "equal cubic"),
+        ([C((0, 0, 0), (.25,) * 3, 9)], "passive material"),
+        ([C((0, 0, 0), (.25,) * 3, 1), C((float("nan"), 0, 0), (.25,) * 3, 1)], "equal cubic"),
+    ]
+    for cells, msg in cases:
+        with pytest.raises(ValueError, match=msg):
+            vie.System(cells, M, 0, .5)
+    # PEC: cells below z = 0, and a gap off the half-cell lattice (dielectric_fft_test.c:127-129)
+    with pytest.raises(ValueError, match="z >= 0"):
+        vie.System([C((0, 0, 0), (.25,) * 3, 1)], M, 0, .5, pec=True)
+    with pytest.raises(ValueError, match="half-cell lattice"):
+        vie.System([C((0, 0, .155), (.25,) * 3, 1), C((.25, -.25, .405), (.25,) * 3, 2)], M, 0, .5, pec=True)
+    # budget (dielectric_fft_test.c:159)
+    L = vie.lib()
+    old = L.dielectric_fft_budget()
+    try:
+        L.dielectric_fft_set_budget(512 * 1024 * 1024)
+        with pytest.raises(ValueError, match="budget"):
+            vie.System([C((0, 0, 0), (.25,) * 3, 1)], M, 2, .5, extra_bytes=512 * 1024 * 1024)
+    finally:
+        L.dielectric_fft_set_budget(old)
+    # every diagnostic carries the reference's "FFT: " prefix (dielectric_fft.c:21-26)
+    try:
+        vie.System([C((0, 0, 0), (.25,) * 3, 9)], M, 0, .5)
+    except ValueError as e:
+        assert str(e).startswith("FFT: ")
+
+
+# ---- host GMRES over callback operators: tests/vie_iterative_test.c ----
+def stencil(x):
+    y = (3 + .2j) * x
+    y[1:] += (.3 - .1j) * x[:-1]
+    y[:-1] += (-.5 + .3j) * x[1:]
+    return y
+
+
+def test_matrix_free_stencil_restarts():
+    n = 40
+    exp = np.sin(np.arange(n)) + 1j * np.cos(.4 * np.arange(n))
+    op = vie.CallbackOperator(n, stencil)
+    ok, x, rep = vie.gmres(op.op, stencil(exp), restart=3)
+    assert ok and rep.restarts > 0 and rep.relative_residual <= 1e-12
+    assert np.max(np.abs(x - exp)) < 1e-10
+    ok, x2, rep2 = vie.gmres(op.op, stencil(exp), x, restart=3)
+    assert ok and rep2.iterations == 0
+    ok, x3, rep3 = vie.gmres(op.op, stencil(exp), restart=3, maximum_iterations=1)
+    assert not ok and rep3.status == 1 and rep3.iterations == 1 and rep3.relative_residual > 1e-12
+
+
+def test_dense_complex_jacobi(ref):
+    N = 9
+    a = np.zeros((N, N), complex)
+    for i in range(N):
+        a[i, i] = (i + 1) * (2 + 1j)
+        if i + 1 < N:
+            a[i, i + 1] = .6 - .8j
+        if i > 0:
+            a[i, i - 1] = -.4 + .1j
+    exp = np.arange(1, N + 1) - 1j * (np.arange(N) % 3)
+    b = a @ exp
+    x = np.zeros(N, complex)
+    rep = vie.GmresReport()
+    A = np.ascontiguousarray(a)
+    ok = vie.lib().vie_dense_gmres(A.ctypes.data, N, b.ctypes.data, x.ctypes.data, None, ctypes.byref(rep))
+    assert ok and np.max(np.abs(x - exp)) < 1e-10 and rep.relative_residual <= 1e-12
+    okr, xr, repr_ = ref.dense_gmres(a, b)
+    assert (rep.iterations, rep.restarts, rep.operator_applications) == \
+        (repr_.iterations, repr_.restarts, repr_.operator_applications)
+    assert np.max(np.abs(x - xr)) < 1e-13
+
+
+def test_edge_cases_and_status_codes():
+    n = 3
+    b = np.array([1 + 1j, 2 - 1j, -3 + 2j])
+    ident = vie.CallbackOperator(n, lambda v: v)
+    ok, x, rep = vie.gmres(ident.op, b)
+    assert ok and rep.iterations == 1                                   # happy breakdown
+    ok, x, rep = vie.gmres(ident.op, np.zeros(3, complex))
+    assert ok and rep.iterations == 0 and rep.relative_residual == 0.0  # zero RHS
+    x0 = np.array([1e-18, 0, 0], complex)
+    ok, x, rep = vie.gmres(ident.op, np.zeros(3, complex), x0, absolute_tolerance=1e-15)
+    assert ok and rep.iterations == 0
+    e0 = np.array([1, 0, 0], complex)
+    zero = vie.CallbackOperator(n, lambda v: np.zeros_like(v))
+    ok, x, rep = vie.gmres(zero.op, e0)
+    assert not ok and rep.status == 2                                   # breakdown, no claim
+    fail = vie.CallbackOperator(n, lambda v: None)
+    ok, x, rep = vie.gmres(fail.op, e0)
+    assert not ok and rep.status == 3
+    pfail = vie.CallbackOperator(n, lambda v: v, lambda v: None)
+    ok, x, rep = vie.gmres(pfail.op, e0)
+    assert not ok and rep.status == 3
+
+    def nan_map(v):
+        v = v.copy()
+        v[0] = np.nan
+        return v
+    pnan = vie.CallbackOperator(n, lambda v: v, nan_map)
+    ok, x, rep = vie.gmres(pnan.op, e0)
+    assert not ok and rep.status == 6
+    onan = vie.CallbackOperator(n, nan_map)
+    ok, x, rep = vie.gmres(onan.op, e0)
+    assert not ok and rep.status == 6
+    ok, x, rep = vie.gmres(ident.op, e0, restart=0)
+    assert not ok and rep.status == 4
+    # rhs aliasing x is rejected
+    L = vie.lib()
+    rep = vie.GmresReport()
+    bb = e0.copy()
+    assert not L.vie_gmres(ctypes.byref(ident.op), bb.ctypes.data, bb.ctypes.data, None, ctypes.byref(rep))
+    assert rep.status == 4
+    # workspace size overflow is rejected before reading the vectors
+    big = vie.LinearOperator(2 ** 64 - 1, ident.op.apply, None, None, None)
+    assert not L.vie_gmres(ctypes.byref(big), bb.ctypes.data, e0.copy().ctypes.data, None, ctypes.byref(rep))
+    assert rep.status == 5
+    assert not L.vie_dense_gmres(None, 0, bb.ctypes.data, e0.copy().ctypes.data, None, ctypes.byref(rep))
+
+
+def test_struct_layouts_match_c():
+    assert ctypes.sizeof(vie.MeshCell) == 80
+    assert ctypes.sizeof(vie.MeshResult) == 224
+    assert ctypes.sizeof(vie.Diagnostic) == 1448
+    assert ctypes.sizeof(vie.Model) == 112
+    assert ctypes.sizeof(vie.Material) == 64
+    assert ctypes.sizeof(vie.GmresReport) == 48
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
