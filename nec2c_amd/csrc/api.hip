@@ -823,6 +823,8 @@ static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, co
   c64* scal = w.scal.as<c64>();  // [0] ||b||^2 [1] ||x||^2 [2] ||r||^2
   const vie::JacobiView J = jacobi_view(s);
   const double reduction = so && so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
+  const char* ortho = getenv("VIE_HIP_ORTHO");
+  const bool dcgs2 = !(ortho && !strcmp(ortho, "cgs2"));
   const bool mixed = sizeof(T) == sizeof(float);
   // m must be set in the state once
   {
@@ -880,20 +882,36 @@ static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, co
     // for the state (its vector work lands in an unused basis slot).
     StatusBlock* ring = (StatusBlock*)w.status_host;
     const size_t it0 = rep->iterations;
+    // columns this cycle may add (the reference counts Arnoldi steps, vie_iterative.c:134)
+    const int m_eff = (int)std::min<size_t>((size_t)m, o->maximum_iterations - it0);
+    // Delayed CGS2 issues m_eff matvec steps plus one closing step that only
+    // finalises the last column; CGS2 issues m_eff steps.
+    const int steps = dcgs2 ? m_eff + 1 : m_eff;
     int issued = 0, checked = 0;
     bool done = false;
     auto issue = [&](int j) -> int {
       vie::cx<T>* vj = V + (size_t)j * ldv;
       vie::cx<T>* wv = V + (size_t)(j + 1) * ldv;
-      if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
-      // two classical Gram-Schmidt passes (same role as the reference's two
-      // MGS passes, vie_iterative.c:151-159); ||w|| before and after
-      vie::multidot<T>(V, ldv, j + 1, wv, n, 1, rs, dots1, st);
-      vie::update<T>(V, ldv, j + 1, dots1, wv, n, 0, rs, st);
-      vie::multidot<T>(V, ldv, j + 1, wv, n, 0, rs, dots2, st);
-      unsigned nb = vie::update<T>(V, ldv, j + 1, dots2, wv, n, 1, rs, st);
-      vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, rs, nb, Prec<T>::eps_b, st);
-      vie::scale_next<T>(wv, n, sd, st);
+      if (dcgs2) {
+        if (j < m_eff) {
+          if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
+          rep->operator_applications++;
+          vie::dcgs2_step<T>(V, ldv, j, wv, n, sd, rs, dots1, Prec<T>::eps_b, st);
+        } else {
+          vie::dcgs2_step<T>(V, ldv, j, (const vie::cx<T>*)nullptr, n, sd, rs, dots1, Prec<T>::eps_b, st);
+        }
+      } else {
+        if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
+        rep->operator_applications++;
+        // two classical Gram-Schmidt passes (same role as the reference's two
+        // MGS passes, vie_iterative.c:151-159); ||w|| before and after
+        vie::multidot<T>(V, ldv, j + 1, wv, n, 1, rs, dots1, st);
+        vie::update<T>(V, ldv, j + 1, dots1, wv, n, 0, rs, st);
+        vie::multidot<T>(V, ldv, j + 1, wv, n, 0, rs, dots2, st);
+        unsigned nb = vie::update<T>(V, ldv, j + 1, dots2, wv, n, 1, rs, st);
+        vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, rs, nb, Prec<T>::eps_b, st);
+        vie::scale_next<T>(wv, n, sd, st);
+      }
       HIP_OK(hipMemcpyAsync(&ring[j & 1], sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
       HIP_OK(hipEventRecord(w.status_ev[j & 1], st));
       return 1;
@@ -909,14 +927,13 @@ static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, co
       return 1;
     };
     while (!done) {
-      bool can_issue = issued < m && it0 + issued < o->maximum_iterations;
+      bool can_issue = issued < steps;
       if (can_issue) {
         if (!issue(issued)) {
           rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
           return 0;
         }
         issued++;
-        rep->operator_applications++;
       }
       if (checked < issued && (issued - checked > 1 || !can_issue)) {
         if (!check(checked)) return 0;

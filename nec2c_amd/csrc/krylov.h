@@ -23,6 +23,12 @@ struct ArnoldiState {
   int j, m;
   double inv_beta, inv_next;
   double inner_target;  // cycle stops once |g_{j+1}| <= inner_target
+  // delayed (DCGS2) orthogonalisation: coefficients of the fused update,
+  // the unrotated Hessenberg and the norm estimate of the last column
+  double dc_alpha, dc_f_re, dc_f_im, dc_before;
+  c64 dc_a[kMaxRestart];
+  c64 dc_e[kMaxRestart];
+  c64 Hraw[(kMaxRestart + 1) * kMaxRestart];
   c64 g[kMaxRestart + 1];
   double cs[kMaxRestart];
   c64 sn[kMaxRestart];
@@ -30,7 +36,7 @@ struct ArnoldiState {
   c64 H[(kMaxRestart + 1) * kMaxRestart];
 };
 
-constexpr int kMaxKeys = 256;  // sums per reduction launch (>= kMaxRestart + 2)
+constexpr int kMaxKeys = 320;  // sums per reduction launch (>= 2 kMaxRestart + 3)
 
 struct ReduceScratch {
   c64* partial = nullptr;  // kMaxDotBlocksHost x kMaxKeys partial sums
@@ -86,6 +92,12 @@ void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, 
 void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, ReduceScratch& rs,
                   unsigned nblk, double breakdown_eps, hipStream_t s);
 void backsolve(ArnoldiState* st, hipStream_t s);
+// Delayed CGS2 Arnoldi step j (slot j holds the once-projected candidate u,
+// slot j+1 holds w = A M^{-1} u, or nothing when w is null: the closing step
+// of a cycle).  One reduction pass, one small solve, one fused update pass.
+template <typename T>
+void dcgs2_step(cx<T>* Q, size_t ldv, int j, const cx<T>* w, size_t n, ArnoldiState* st, ReduceScratch& rs,
+                c64* dots, double breakdown_eps, hipStream_t s);
 void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s);
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s);
 template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st, hipStream_t s);

@@ -62,64 +62,70 @@ __device__ inline c64 block_reduce(c64 v, c64* lds) {
 template <typename T> struct DotShape { static constexpr int CPL = 128 / (int)sizeof(cx<T>); };
 
 template <typename T, bool CONJ, bool JAC>
-__global__ __launch_bounds__(KWG) void k_dots(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n,
+__global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n,
                                               int self_norm, c64* partial, int ldp, JacobiLite<T> J) {
   constexpr int CPL = DotShape<T>::CPL;
   constexpr int WAVES = KWG / 64;
+  constexpr int CH = 64 * CPL;
   __shared__ c64 lds[kMaxKeys];
+  __shared__ c64 zs[CH];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tot = nk + (self_norm ? 1 : 0);
   for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
-  __syncthreads();
-  const size_t chunk = (size_t)64 * CPL;
-  const size_t nchunks = (n + chunk - 1) / chunk;
+  const size_t nchunks = (n + CH - 1) / CH;
   for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const size_t base = ch * chunk + lane;
-    double zr[CPL], zi[CPL];
-#pragma unroll
-    for (int e = 0; e < CPL; e++) {
-      size_t i = base + (size_t)e * 64;
+    const size_t base = ch * CH;
+    __syncthreads();
+    for (int e = threadIdx.x; e < CH; e += KWG) {
+      size_t i = base + e;
       cx<T> v = i < n ? w[i] : cx<T>{0, 0};
       if (JAC && i < n) v = J.apply(i, v);
-      zr[e] = v.re;
-      zi[e] = v.im;
+      zs[e] = c64{(double)v.re, (double)v.im};
     }
+    __syncthreads();
+    cx<T> a[CPL], an[CPL];
+    auto load_key = [&](int k, cx<T>* dst) {
+      const cx<T>* vk = V + (size_t)k * ldv + base + lane;
+#pragma unroll
+      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? vk[e * 64] : cx<T>{0, 0};
+    };
+    if (wave < nk) load_key(wave, a);
     for (int k = wave; k < tot; k += WAVES) {
       c64 s = {0, 0};
       if (k < nk) {
-        const cx<T>* vk = V + (size_t)k * ldv;
-        cx<T> a[CPL];
+        if (k + WAVES < nk) load_key(k + WAVES, an);
 #pragma unroll
         for (int e = 0; e < CPL; e++) {
-          size_t i = base + (size_t)e * 64;
-          a[e] = i < n ? vk[i] : cx<T>{0, 0};
-        }
-#pragma unroll
-        for (int e = 0; e < CPL; e++) {
+          const c64 z = zs[lane + e * 64];
           double ar = a[e].re, ai = CONJ ? -(double)a[e].im : (double)a[e].im;
-          s.re += ar * zr[e] - ai * zi[e];
-          s.im += ar * zi[e] + ai * zr[e];
+          s.re += ar * z.re - ai * z.im;
+          s.im += ar * z.im + ai * z.re;
         }
+#pragma unroll
+        for (int e = 0; e < CPL; e++) a[e] = an[e];
       } else {
 #pragma unroll
-        for (int e = 0; e < CPL; e++) s.re += zr[e] * zr[e] + zi[e] * zi[e];
+        for (int e = 0; e < CPL; e++) {
+          const c64 z = zs[lane + e * 64];
+          s.re += z.re * z.re + z.im * z.im;
+        }
       }
       s = wave_reduce(s);
       if (lane == 0) lds[k] = lds[k] + s;  // keys are owned by one wave
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)blockIdx.x * ldp + k] = lds[k];
+  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + blockIdx.x] = lds[k];
 }
 
-// out[k] = sum_b partial[b * ldp + k] (+ add[k] when given), 32 lanes per key,
-// fixed summation order.
+// out[k] = sum_b partial[k * ldp + b] (key-major, contiguous per key),
+// 32 lanes per key, fixed summation order.
 __global__ __launch_bounds__(KWG) void k_finish(const c64* partial, int ldp, unsigned nblk, int nk, c64* out) {
   const int key = blockIdx.x * (KWG / 32) + (threadIdx.x >> 5);
   const int sub = threadIdx.x & 31;
   c64 s = {0, 0};
   if (key < nk)
-    for (unsigned b = sub; b < nblk; b += 32) s = s + partial[(size_t)b * ldp + key];
+    for (unsigned b = sub; b < nblk; b += 32) s = s + partial[(size_t)key * ldp + b];
   for (int m = 16; m >= 1; m >>= 1) {
     s.re += __shfl_xor(s.re, m, 32);
     s.im += __shfl_xor(s.im, m, 32);
@@ -199,7 +205,7 @@ __global__ __launch_bounds__(KWG) void k_update(const cx<T>* V, size_t ldv, int 
   }
   if (!want_norm) return;
   s = block_reduce(s, lds);
-  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ldp] = s;
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------- Arnoldi tail
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(KWG) void k_arnoldi_tail(ArnoldiState* st, const c6
     cs[k] = st->cs[k];
   }
   c64 s = {0, 0};
-  for (unsigned b = threadIdx.x; b < nblk; b += KWG) s = s + norm_partial[(size_t)b * ldp];
+  for (unsigned b = threadIdx.x; b < nblk; b += KWG) s = s + norm_partial[b];
   s = block_reduce(s, lds);  // contains the barrier that publishes col / sn / cs
   if (threadIdx.x != 0) return;
   c64* H = st->H;  // row-major (m+1) x m
@@ -312,6 +318,290 @@ __global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_ta
   }
 }
 
+// ---------------------------------------------------------------- delayed CGS2
+// Keys of the single reduction of step j:
+//   a_k = q_k^H u, b_k = q_k^H w (k < j), nu = u^H u, c = u^H w, d = w^H w.
+// Layout: [a_0..a_{j-1}, b_0..b_{j-1}, nu, c, d]; without w only [a, nu].
+template <typename T>
+__global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, int j, const cx<T>* u,
+                                                 const cx<T>* w, size_t n, c64* partial, int ldp) {
+  constexpr int CPL = 8;  // u, w and q in flight per lane
+  constexpr int WAVES = KWG / 64;
+  constexpr int CH = 64 * CPL;
+  __shared__ c64 lds[kMaxKeys];
+  __shared__ c64 us[CH], ws[CH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool hw = w != nullptr;
+  const int tot = hw ? 2 * j + 3 : j + 1;
+  for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
+  const size_t nchunks = (n + CH - 1) / CH;
+  for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const size_t base = ch * CH;
+    __syncthreads();
+    for (int e = threadIdx.x; e < CH; e += KWG) {
+      size_t i = base + e;
+      cx<T> a = i < n ? u[i] : cx<T>{0, 0}, b = (hw && i < n) ? w[i] : cx<T>{0, 0};
+      us[e] = c64{(double)a.re, (double)a.im};
+      ws[e] = c64{(double)b.re, (double)b.im};
+    }
+    __syncthreads();
+    // basis vectors: one load feeds both projections; the next key's load is
+    // issued before this key's reduction (software pipelining)
+    cx<T> q[CPL], qn[CPL];
+    auto load_key = [&](int k, cx<T>* dst) {
+      const cx<T>* qk = Q + (size_t)k * ldv + base + lane;
+#pragma unroll
+      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? qk[e * 64] : cx<T>{0, 0};
+    };
+    if (wave < j) load_key(wave, q);
+    for (int k = wave; k < j; k += WAVES) {
+      if (k + WAVES < j) load_key(k + WAVES, qn);
+      c64 sa = {0, 0}, sb = {0, 0};
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const c64 uu = us[lane + e * 64];
+        double qr = q[e].re, qi = q[e].im;
+        sa.re += qr * uu.re + qi * uu.im;
+        sa.im += qr * uu.im - qi * uu.re;
+        if (hw) {
+          const c64 ww = ws[lane + e * 64];
+          sb.re += qr * ww.re + qi * ww.im;
+          sb.im += qr * ww.im - qi * ww.re;
+        }
+      }
+      sa = wave_reduce(sa);
+      if (hw) sb = wave_reduce(sb);
+      if (lane == 0) {
+        lds[k] = lds[k] + sa;
+        if (hw) lds[j + k] = lds[j + k] + sb;
+      }
+#pragma unroll
+      for (int e = 0; e < CPL; e++) q[e] = qn[e];
+    }
+    // the scalars nu, c, d: one wave each
+    const int nscal = hw ? 3 : 1;
+    for (int q = wave; q < nscal; q += WAVES) {
+      c64 s = {0, 0};
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const c64 uu = us[lane + e * 64], ww = ws[lane + e * 64];
+        if (q == 0) s.re += uu.re * uu.re + uu.im * uu.im;
+        else if (q == 1) {
+          s.re += uu.re * ww.re + uu.im * ww.im;
+          s.im += uu.re * ww.im - uu.im * ww.re;
+        } else s.re += ww.re * ww.re + ww.im * ww.im;
+      }
+      s = wave_reduce(s);
+      int key = (hw ? 2 * j : j) + q;
+      if (lane == 0) lds[key] = lds[key] + s;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + blockIdx.x] = lds[k];
+}
+
+// Small solve of the delayed step (one block).  With u = alpha q_j + Q a:
+//  1. alpha^2 = nu - |a|^2 (Q orthonormal);
+//  2. column j-1 becomes final: Hraw(0:j, j-1) += a, Hraw(j, j-1) = alpha;
+//     it is rotated into H, the residual estimate |g_j| formed and the cycle
+//     marked done on convergence or breakdown (vie_iterative.c:160-177);
+//  3. A M^{-1} q_j = (w - Q_{j+1} t)/alpha with t = Hraw(:, 0:j-1) a, so its
+//     projections are h = [(b - t)/alpha ; ((c - a^H b)/alpha - t_j)/alpha];
+//  4. next candidate u' = (w - Q_j e - f u)/alpha with s = t + alpha h,
+//     e = s(0:j-1) - (s_j/alpha) a, f = s_j/alpha; q_j = (u - Q_j a)/alpha.
+__global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int j, int has_w, const c64* dots,
+                                                double breakdown_eps) {
+  __shared__ c64 a[kMaxRestart], b[kMaxRestart], t[kMaxRestart + 1], col[kMaxRestart + 1];
+  __shared__ c64 sn[kMaxRestart];
+  __shared__ double cs[kMaxRestart];
+  __shared__ c64 red[KWG / 64];
+  __shared__ double alpha_s;
+  __shared__ int stop;
+  if (st->done) return;
+  const int m = st->m;
+  c64* Hraw = st->Hraw;
+  for (int k = threadIdx.x; k < j; k += KWG) {
+    a[k] = dots[k];
+    if (has_w) b[k] = dots[j + k];
+    sn[k] = st->sn[k];
+    cs[k] = st->cs[k];
+  }
+  __syncthreads();
+  // |a|^2 and a^H b in parallel
+  c64 p2 = {0, 0}, pab = {0, 0};
+  for (int k = threadIdx.x; k < j; k += KWG) {
+    p2.re += a[k].re * a[k].re + a[k].im * a[k].im;
+    if (has_w) pab = pab + conj(a[k]) * b[k];
+  }
+  p2 = block_reduce(p2, red);
+  if (threadIdx.x == 0) {
+    const c64 nu = dots[has_w ? 2 * j : j];
+    double al2 = nu.re - p2.re;
+    alpha_s = al2 > 0 ? sqrt(al2) : 0.0;
+  }
+  __syncthreads();
+  pab = block_reduce(pab, red);  // thread 0 holds a^H b
+  const double alpha = alpha_s;
+  // column j-1 becomes final: Hraw(0:j, j-1) += a, Hraw(j, j-1) = alpha
+  if (j > 0) {
+    for (int k = threadIdx.x; k <= j; k += KWG) {
+      c64 v = k < j ? Hraw[k * m + j - 1] + a[k] : c64{alpha, 0};
+      Hraw[k * m + j - 1] = v;
+      col[k] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    stop = 0;
+    if (!isfinite(alpha)) {
+      st->nonfinite = 1;
+      stop = 1;
+    } else if (j == 0) {
+      // q_0 = u / alpha: the least-squares right-hand side is beta * alpha
+      st->g[0] = alpha * st->g[0];
+    } else {
+      const int c = j - 1;
+      st->breakdown = alpha <= breakdown_eps * st->dc_before;
+      for (int k = 0; k < c; k++) {
+        c64 f = col[k], gg = col[k + 1];
+        col[k] = cs[k] * f + sn[k] * gg;
+        col[k + 1] = cs[k] * gg - conj(sn[k]) * f;
+      }
+      double cr;
+      c64 z;
+      givens(col[c], col[c + 1], &cr, &z);
+      st->cs[c] = cr;
+      st->sn[c] = z;
+      col[c] = cr * col[c] + z * col[c + 1];
+      col[c + 1] = c64{0, 0};
+      c64 gc = st->g[c];
+      st->g[c + 1] = c64{0, 0} - conj(z) * gc;
+      st->g[c] = cr * gc;
+      st->gabs = hypot(st->g[c + 1].re, st->g[c + 1].im);
+      st->used = j;
+      st->j = j;
+      if (st->breakdown || st->gabs <= st->inner_target || j >= m) stop = 1;
+    }
+    if (!has_w) stop = 1;
+    if (stop) st->done = 1;
+  }
+  __syncthreads();
+  if (j > 0)
+    for (int k = threadIdx.x; k <= j; k += KWG) st->H[k * m + j - 1] = col[k];
+  if (stop) return;
+  // t = Hraw(0:j, 0:j-1) a  (Hessenberg: row k only has columns >= k-1)
+  for (int k = threadIdx.x; k <= j; k += KWG) {
+    c64 s = {0, 0};
+    for (int l = (k > 0 ? k - 1 : 0); l < j; l++) s = s + Hraw[k * m + l] * a[l];
+    t[k] = s;
+  }
+  __syncthreads();
+  const double ia = 1.0 / alpha;
+  __shared__ c64 f_s;
+  if (threadIdx.x == 0) {
+    const c64 cc = dots[2 * j + 1], dd = dots[2 * j + 2];
+    c64 hj = ia * (ia * (cc - pab) - t[j]);
+    c64 sj = t[j] + alpha * hj;  // f = s_j / alpha
+    f_s = ia * sj;
+    st->dc_alpha = alpha;
+    st->dc_f_re = f_s.re;
+    st->dc_f_im = f_s.im;
+    st->dc_before = sqrt(dd.re) * ia;
+    Hraw[j * m + j] = hj;
+    st->nonfinite = !(isfinite(hj.re) && isfinite(hj.im) && isfinite(st->dc_before));
+    if (st->nonfinite) st->done = 1;
+  }
+  __syncthreads();
+  const c64 f = f_s;
+  for (int k = threadIdx.x; k < j; k += KWG) {
+    c64 hk = ia * (b[k] - t[k]);
+    Hraw[k * m + j] = hk;
+    c64 sk = t[k] + alpha * hk;
+    st->dc_a[k] = a[k];
+    st->dc_e[k] = sk - f * a[k];
+  }
+  // rows below the new column stay zero
+  for (int k = j + 1 + threadIdx.x; k <= m; k += KWG) Hraw[k * m + j] = c64{0, 0};
+}
+
+// Fused update of step j: slot j <- (u - Q_j a)/alpha, slot j+1 <- (w - Q_j e - f u)/alpha.
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, int j, size_t n, const ArnoldiState* st) {
+  __shared__ c64 ca[kMaxRestart], ce[kMaxRestart];
+  if (st->done) return;
+  for (int k = threadIdx.x; k < j; k += KWG) {
+    ca[k] = st->dc_a[k];
+    ce[k] = st->dc_e[k];
+  }
+  __syncthreads();
+  const double ia = 1.0 / st->dc_alpha;
+  const double fr = st->dc_f_re, fi = st->dc_f_im;
+  cx<T>* u = Q + (size_t)j * ldv;
+  cx<T>* w = Q + (size_t)(j + 1) * ldv;
+  const size_t step = (size_t)gridDim.x * KWG * UEPT;
+  for (size_t base = (size_t)blockIdx.x * KWG * UEPT + threadIdx.x; base < n; base += step) {
+    double ur[UEPT], ui[UEPT], wr[UEPT], wi[UEPT];
+    bool in[UEPT];
+#pragma unroll
+    for (int e = 0; e < UEPT; e++) {
+      size_t i = base + (size_t)e * KWG;
+      in[e] = i < n;
+      cx<T> uu = in[e] ? u[i] : cx<T>{0, 0}, ww = in[e] ? w[i] : cx<T>{0, 0};
+      ur[e] = uu.re;
+      ui[e] = uu.im;
+      // w - f u uses the original u
+      wr[e] = ww.re - (fr * uu.re - fi * uu.im);
+      wi[e] = ww.im - (fr * uu.im + fi * uu.re);
+    }
+    int k = 0;
+    for (; k + 4 <= j; k += 4) {
+      cx<T> q[4][UEPT];
+#pragma unroll
+      for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int e = 0; e < UEPT; e++) {
+          size_t i = base + (size_t)e * KWG;
+          q[p][e] = in[e] ? Q[(size_t)(k + p) * ldv + i] : cx<T>{0, 0};
+        }
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        const double ar = ca[k + p].re, ai = ca[k + p].im, er = ce[k + p].re, ei = ce[k + p].im;
+#pragma unroll
+        for (int e = 0; e < UEPT; e++) {
+          double qr = q[p][e].re, qi = q[p][e].im;
+          ur[e] -= ar * qr - ai * qi;
+          ui[e] -= ar * qi + ai * qr;
+          wr[e] -= er * qr - ei * qi;
+          wi[e] -= er * qi + ei * qr;
+        }
+      }
+    }
+    for (; k < j; k++) {
+      const double ar = ca[k].re, ai = ca[k].im, er = ce[k].re, ei = ce[k].im;
+#pragma unroll
+      for (int e = 0; e < UEPT; e++) {
+        size_t i = base + (size_t)e * KWG;
+        if (in[e]) {
+          cx<T> qq = Q[(size_t)k * ldv + i];
+          double qr = qq.re, qi = qq.im;
+          ur[e] -= ar * qr - ai * qi;
+          ui[e] -= ar * qi + ai * qr;
+          wr[e] -= er * qr - ei * qi;
+          wi[e] -= er * qi + ei * qr;
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < UEPT; e++) {
+      size_t i = base + (size_t)e * KWG;
+      if (in[e]) {
+        u[i] = cx<T>{(T)(ur[e] * ia), (T)(ui[e] * ia)};
+        w[i] = cx<T>{(T)(wr[e] * ia), (T)(wi[e] * ia)};
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- elementwise
 template <typename T> __global__ void k_zero(cx<T>* x, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,7 +670,7 @@ __global__ __launch_bounds__(KWG) void k_residual(const c64* b, const c64* ax, c
     s.re += v.re * v.re + v.im * v.im;
   }
   s = block_reduce(s, lds);
-  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ldp] = s;
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 template <typename T>
@@ -393,7 +683,7 @@ __global__ __launch_bounds__(KWG) void k_norm2(const cx<T>* x, size_t n, c64* pa
     s.re += a * a + b * b;
   }
   s = block_reduce(s, lds);
-  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ldp] = s;
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------- conductor coupling
@@ -491,15 +781,15 @@ void multidot(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n, int 
   unsigned nb = dot_blocks<T>(n);
   JacobiLite<T> none{};
   hipLaunchKernelGGL((k_dots<T, true, false>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, w, n, self_norm, rs.partial,
-                     kMaxKeys, none);
+                     kMaxDotBlocks, none);
   int tot = nk + (self_norm ? 1 : 0);
-  hipLaunchKernelGGL(k_finish, dim3((tot + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, tot, dots);
+  hipLaunchKernelGGL(k_finish, dim3((tot + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, tot, dots);
 }
 template <typename T>
 unsigned update(const cx<T>* V, size_t ldv, int nk, const c64* h, cx<T>* w, size_t n, int want_norm,
                 ReduceScratch& rs, hipStream_t s) {
   unsigned nb = update_blocks(n);
-  hipLaunchKernelGGL((k_update<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, h, w, n, want_norm, rs.partial, kMaxKeys);
+  hipLaunchKernelGGL((k_update<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, h, w, n, want_norm, rs.partial, kMaxDotBlocks);
   return nb;
 }
 static unsigned elem_blocks(size_t n) {
@@ -508,20 +798,33 @@ static unsigned elem_blocks(size_t n) {
 }
 template <typename T> void norm2(const cx<T>* x, size_t n, ReduceScratch& rs, c64* out, hipStream_t s) {
   unsigned nb = elem_blocks(n);
-  hipLaunchKernelGGL((k_norm2<T>), dim3(nb), dim3(KWG), 0, s, x, n, rs.partial, kMaxKeys);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, 1, out);
+  hipLaunchKernelGGL((k_norm2<T>), dim3(nb), dim3(KWG), 0, s, x, n, rs.partial, kMaxDotBlocks);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, 1, out);
 }
 void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, c64* out, hipStream_t s) {
   unsigned nb = elem_blocks(n);
-  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(KWG), 0, s, b, ax, r, n, rs.partial, kMaxKeys);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, 1, out);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(KWG), 0, s, b, ax, r, n, rs.partial, kMaxDotBlocks);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, 1, out);
 }
 void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, ReduceScratch& rs,
                   unsigned nblk, double breakdown_eps, hipStream_t s) {
-  hipLaunchKernelGGL(k_arnoldi_tail, dim3(1), dim3(KWG), 0, s, st, h1, h2, before2, rs.partial, kMaxKeys, nblk,
+  hipLaunchKernelGGL(k_arnoldi_tail, dim3(1), dim3(KWG), 0, s, st, h1, h2, before2, rs.partial, kMaxDotBlocks, nblk,
                      breakdown_eps);
 }
 void backsolve(ArnoldiState* st, hipStream_t s) { hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(KWG), 0, s, st); }
+
+template <typename T>
+void dcgs2_step(cx<T>* Q, size_t ldv, int j, const cx<T>* w, size_t n, ArnoldiState* st, ReduceScratch& rs,
+                c64* dots, double breakdown_eps, hipStream_t s) {
+  size_t nch = (n + 511) / 512;
+  unsigned nb = (unsigned)(nch < 1 ? 1 : (nch > kMaxDotBlocks ? kMaxDotBlocks : nch));
+  const cx<T>* u = Q + (size_t)j * ldv;
+  hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, j, u, w, n, rs.partial, kMaxDotBlocks);
+  int tot = w ? 2 * j + 3 : j + 1;
+  hipLaunchKernelGGL(k_finish, dim3((tot + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, tot, dots);
+  hipLaunchKernelGGL(k_dc_mid, dim3(1), dim3(KWG), 0, s, st, j, w ? 1 : 0, dots, breakdown_eps);
+  if (w) hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, j, n, st);
+}
 void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s) {
   hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2, inner_target);
 }
@@ -557,7 +860,7 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
   if (c == 0) return;
   hipLaunchKernelGGL((k_cond_prep<T>), dim3(gblocks(c)), dim3(256), 0, s, vc, cv.C, c, precond, cv.zc);
   if (cv.nv) {
-    // B z_v in row chunks of kMaxKeys, Jacobi applied to v_v on the fly
+    // B z_v in row chunks of kMaxKeys keys, Jacobi applied to v_v on the fly
     JacobiLite<T> J{cv.nbox, cv.eps, {cv.self[0], cv.self[1], cv.self[2]}};
     unsigned nb = dot_blocks<T>(cv.nv);
     for (int r0 = 0; r0 < c; r0 += kMaxKeys) {
@@ -565,11 +868,11 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
       const cx<T>* Br = cv.B + (size_t)r0 * cv.nv;
       if (precond)
         hipLaunchKernelGGL((k_dots<T, false, true>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
-                           rs.partial, kMaxKeys, J);
+                           rs.partial, kMaxDotBlocks, J);
       else
         hipLaunchKernelGGL((k_dots<T, false, false>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
-                           rs.partial, kMaxKeys, J);
-      hipLaunchKernelGGL(k_finish, dim3((rows + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxKeys, nb, rows,
+                           rs.partial, kMaxDotBlocks, J);
+      hipLaunchKernelGGL(k_finish, dim3((rows + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, rows,
                          cv.bz + r0);
     }
   } else {
@@ -592,7 +895,9 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
   template void to_box<T>(const c64*, size_t, size_t, const size_t*, size_t, cx<T>*, hipStream_t);                \
   template void from_box<T>(const cx<T>*, size_t, size_t, const size_t*, size_t, c64*, hipStream_t);              \
   template void coupling_apply<T>(const CouplingView<T>&, const cx<T>*, const cx<T>*, cx<T>*, cx<T>*, int,         \
-                                  ReduceScratch&, hipStream_t);
+                                  ReduceScratch&, hipStream_t);                                                  \
+  template void dcgs2_step<T>(cx<T>*, size_t, int, const cx<T>*, size_t, ArnoldiState*, ReduceScratch&, c64*,    \
+                              double, hipStream_t);
 VIE_INST(float)
 VIE_INST(double)
 

@@ -40,9 +40,12 @@ template <typename T> __device__ __forceinline__ cx<T> jacobi_scale(cx<T> v, cx<
 template <typename T> __device__ __forceinline__ cx<T> ld(const cx<T>* p) { return *p; }
 
 // ---------------------------------------------------------------- pass 1
+// All three components' inputs are loaded into registers before the first
+// transform so their HBM latency overlaps; each transformed component is
+// stored as soon as it is done.
 template <typename T, int PZ, int LW>
 __global__ __launch_bounds__(256) void k_zfwd(OpArgs<T> a) {
-  constexpr int WG = 256, E = LW * PZ;
+  constexpr int WG = 256, E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
   const int tid = threadIdx.x;
@@ -52,25 +55,41 @@ __global__ __launch_bounds__(256) void k_zfwd(OpArgs<T> a) {
   const cx<T>* twz = a.tw + g.Px + g.Py;
   for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
   const size_t nbox = g.nbox();
-  for (int comp = 0; comp < 3; comp++) {
-    const cx<T> sgn = {(T)(comp == 2 ? 1 : -1), (T)0};
-    for (int f = tid; f < E; f += WG) {
-      int l = f / PZ, e = f % PZ;
-      long line = line0 + l;
-      cx<T> v = {(T)0, (T)0};
-      if (line < lines) {
-        int z = -1;
-        bool image = false;
-        if (e >= g.zoff && e - g.zoff < g.nz) z = e - g.zoff;
-        else if (g.pec && e < g.nz) { z = g.nz - 1 - e; image = true; }
-        if (z >= 0) {
-          size_t cell = (size_t)line * g.nz + z;
-          v = a.xin[comp * nbox + cell];
-          if (a.precond) v = jacobi_scale(v, a.eps[cell], a.self[comp]);
-          if (image) v = sgn * v;
-        }
+  cx<T> in[3][EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int f = tid + i * WG;
+    const int l = f / PZ, e = f % PZ;
+    const long line = line0 + l;
+    int z = -1;
+    bool image = false;
+    if ((E % WG == 0 || f < E) && line < lines) {
+      if (e >= g.zoff && e - g.zoff < g.nz) z = e - g.zoff;
+      else if (g.pec && e < g.nz) {
+        z = g.nz - 1 - e;
+        image = true;
       }
-      lds[l * PZ + e] = v;
+    }
+    cx<T> ev = {(T)1, (T)0};
+    size_t cell = z >= 0 ? (size_t)line * g.nz + z : 0;
+    if (z >= 0 && a.precond) ev = a.eps[cell];
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+      cx<T> v = {(T)0, (T)0};
+      if (z >= 0) {
+        v = a.xin[comp * nbox + cell];
+        if (a.precond) v = jacobi_scale(v, ev, a.self[comp]);
+        if (image && comp != 2) v = cx<T>{-v.re, -v.im};
+      }
+      in[comp][i] = v;
+    }
+  }
+#pragma unroll
+  for (int comp = 0; comp < 3; comp++) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int f = tid + i * WG;
+      if (E % WG == 0 || f < E) lds[f] = in[comp][i];  // f = l * PZ + e
     }
     __syncthreads();
     fft_lines<T, PZ, LW, WG, 1, PZ, false>(lds, tw, tid);
@@ -142,16 +161,28 @@ __global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
   const long line0 = (long)blockIdx.x * LW;
   const size_t plane = (size_t)g.Py * g.Pz;
   for (int i = tid; i < PX; i += WG) tw[i] = a.tw[i];
-  cx<T> hold[3][EPT];
-  for (int comp = 0; comp < 3; comp++) {
-    for (int f = tid; f < E; f += WG) {
-      int e = f / LW, l = f % LW;
-      long line = line0 + l;
+  cx<T> hold[3][EPT], next[EPT];
+  auto load_comp = [&](int comp, cx<T>* dst) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int f = tid + i * WG;
+      const int e = f / LW, l = f % LW;
+      const long line = line0 + l;
       cx<T> v = {(T)0, (T)0};
-      if (line < lines && e < g.nx) v = a.B[((size_t)comp * g.nx + e) * plane + line];
-      lds[e * LW + l] = v;
+      if ((E % WG == 0 || f < E) && line < lines && e < g.nx) v = a.B[((size_t)comp * g.nx + e) * plane + line];
+      dst[i] = v;
+    }
+  };
+  load_comp(0, next);
+#pragma unroll
+  for (int comp = 0; comp < 3; comp++) {
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int f = tid + i * WG;
+      if (E % WG == 0 || f < E) lds[f] = next[i];  // f = e * LW + l
     }
     __syncthreads();
+    if (comp < 2) load_comp(comp + 1, next);  // in flight during this transform
     fft_lines<T, PX, LW, WG, LW, 1, true>(lds, tw, tid);
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
@@ -189,6 +220,7 @@ __global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
       }
     }
   }
+#pragma unroll
   for (int comp = 0; comp < 3; comp++) {
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
@@ -209,7 +241,7 @@ __global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
 // ---------------------------------------------------------------- pass 5
 template <typename T, int PZ, int LW>
 __global__ __launch_bounds__(256) void k_zinv(OpArgs<T> a) {
-  constexpr int WG = 256, E = LW * PZ;
+  constexpr int WG = 256, E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
   const int tid = threadIdx.x;
@@ -220,13 +252,22 @@ __global__ __launch_bounds__(256) void k_zinv(OpArgs<T> a) {
   for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
   const size_t nbox = g.nbox();
   const cx<T> one = {(T)1, (T)0};
+  cx<T> in[3][EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int f = tid + i * WG;
+    const long line = line0 + f / PZ;
+    const bool ok = (E % WG == 0 || f < E) && line < lines;
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++)
+      in[comp][i] = ok ? conj(a.A[((size_t)comp * lines + line) * PZ + f % PZ]) : cx<T>{(T)0, (T)0};
+  }
+#pragma unroll
   for (int comp = 0; comp < 3; comp++) {
-    for (int f = tid; f < E; f += WG) {
-      int l = f / PZ, e = f % PZ;
-      long line = line0 + l;
-      cx<T> v = {(T)0, (T)0};
-      if (line < lines) v = conj(a.A[((size_t)comp * lines + line) * PZ + e]);
-      lds[l * PZ + e] = v;
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int f = tid + i * WG;
+      if (E % WG == 0 || f < E) lds[f] = in[comp][i];
     }
     __syncthreads();
     fft_lines<T, PZ, LW, WG, 1, PZ, false>(lds, tw, tid);
@@ -250,8 +291,10 @@ __global__ __launch_bounds__(256) void k_zinv(OpArgs<T> a) {
 // ---------------------------------------------------------------- launch
 template <typename T, int P> struct PassLauncher {
   static constexpr int E = PassShape<T>::E;
-  static constexpr int LWZ = (E / P) < 16 ? (E / P) : 16;
+  static constexpr int LWZ = (E / P) < 8 ? (E / P) : 8;
   static constexpr int LWY = (E / P) < 16 ? (E / P) : 16;
+  static constexpr int XCAP = 16;
+  static constexpr int LWX = (E / P) < XCAP ? (E / P) : XCAP;
   static void zfwd(const OpArgs<T>& a, hipStream_t s) {
     long lines = (long)a.g.nx * a.g.ny;
     hipLaunchKernelGGL((k_zfwd<T, P, LWZ>), dim3((lines + LWZ - 1) / LWZ), dim3(256), 0, s, a);
@@ -268,7 +311,7 @@ template <typename T, int P> struct PassLauncher {
   }
   static void xpass(const OpArgs<T>& a, hipStream_t s) {
     long lines = (long)a.g.Py * a.g.Pz;
-    hipLaunchKernelGGL((k_xpass<T, P, LWY>), dim3((lines + LWY - 1) / LWY), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_xpass<T, P, LWX>), dim3((lines + LWX - 1) / LWX), dim3(256), 0, s, a);
   }
 };
 
