@@ -91,6 +91,47 @@ int dielectric_fft_time_apply(dielectric_fft_system *system, vie_hip_precision p
 int dielectric_fft_apply_bytes(const dielectric_fft_system *system, vie_hip_precision precision,
                                double *pass_bytes, double *total_bytes);
 
+/* ---- batched solves (new: the reference solves one right-hand side at a
+   time, each frequency in its own forked process, dielectric_wire.c:1296-1304)
+   Many independent solves in flight on one GPU: each submitted job is a
+   device-resident GMRES (the vie_gmres contract, x0 = 0) with its own pooled
+   workspace; the host interleaves the Arnoldi steps of all jobs, jobs of
+   one system run in submission order on its stream and jobs of different
+   systems overlap on the device.  A job's result does not depend on what
+   else is in flight.  One host thread drives a batch; systems must outlive
+   the jobs submitted on them. */
+typedef struct vie_hip_batch vie_hip_batch;
+vie_hip_batch *vie_hip_batch_create(void);
+/* Waits for queued device work, then frees every job and workspace. */
+void vie_hip_batch_destroy(vie_hip_batch *batch);
+/* Device bytes one job of this system and options holds while in flight
+   (Krylov basis, solver vectors, reduction scratch): the admission unit. */
+size_t vie_hip_batch_job_bytes(const dielectric_fft_system *system, const vie_gmres_options *options,
+                               const vie_hip_solve_options *solve);
+/* Pre-allocates `count` pooled workspaces large enough for jobs of this
+   system and options, so later submissions allocate nothing. */
+int vie_hip_batch_reserve(vie_hip_batch *batch, size_t count, dielectric_fft_system *system,
+                          const vie_gmres_options *options);
+/* Queues a solve of system x = rhs (user layout, copied before return).
+   Returns the job id (> 0), or 0 with vie_hip_last_error set. */
+long long vie_hip_batch_submit(vie_hip_batch *batch, dielectric_fft_system *system, const VIE_CDOUBLE *rhs,
+                               const vie_gmres_options *options, const vie_hip_solve_options *solve);
+/* Advances every job as far as it can without waiting; writes up to
+   max_ready ids of jobs finished since they were last reported and returns
+   their count; *running (optional) receives the jobs still in flight. */
+size_t vie_hip_batch_progress(vie_hip_batch *batch, long long *ready, size_t max_ready, size_t *running);
+size_t vie_hip_batch_running(const vie_hip_batch *batch);
+/* Drives all jobs until job `id` (or every job) has finished. */
+int vie_hip_batch_wait(vie_hip_batch *batch, long long id);
+int vie_hip_batch_wait_all(vie_hip_batch *batch);
+/* Result of a finished job: returns what vie_gmres would (1 on convergence);
+   x (optional) receives the solution, report (optional) the report and
+   seconds (optional) the time from submission to completion. */
+int vie_hip_batch_result(vie_hip_batch *batch, long long id, VIE_CDOUBLE *x, vie_gmres_report *report,
+                         double *seconds);
+/* Returns a finished job's workspace to the pool. */
+int vie_hip_batch_release(vie_hip_batch *batch, long long id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,10 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <chrono>
+#include <map>
+#include <thread>
+#include <memory>
 #include <cmath>
 #include <complex>
 #include <mutex>
@@ -417,16 +421,40 @@ struct StatusBlock {
   int breakdown, nonfinite, singular, used, done, pad_;
 };
 
+// Host-visible landing area of one solve (pinned, written by async copies).
+struct PinnedStatus {
+  StatusBlock ring[2];  // status after Arnoldi steps j (slot j & 1)
+  StatusBlock fin;      // status after back substitution
+  c64 scal[4];          // ||b||^2, ||x||^2, ||r||^2
+};
+
+struct dielectric_fft_system;
+
+// One device-resident GMRES solve: its buffers and a resumable state machine
+// (every host wait is an event query, so many solves can be interleaved).
 struct SolverWork {
   size_t n = 0, ldv = 0;
   int m = 0, prec = -1;
   DevBuf V, x, b, r, ax, state, dots, partial, scal;
-  void* status_host = nullptr;      // pinned ring of status blocks
-  hipEvent_t status_ev[2] = {nullptr, nullptr};
+  PinnedStatus* pin = nullptr;
+  hipEvent_t ev_ring[2] = {nullptr, nullptr}, ev_misc = nullptr;
+  c64* host_io = nullptr;  // pinned user-layout staging of a batched solve
+  size_t host_io_cap = 0;
+  // job state
+  dielectric_fft_system* sys = nullptr;
+  vie_gmres_options o{};
+  vie_hip_solve_options so{};
+  vie_gmres_report rep{};
+  int phase = 0, m_eff = 0, steps = 0, issued = 0, checked = 0, result = 0;
+  bool after_cycle = false, breakdown = false, cycle_done = false, dcgs2 = true;
+  double bnorm = 0, target = 0, beta = 0, reduction = 1e-5, t_submit = 0, t_end = 0;
+  size_t it0 = 0;
   ~SolverWork() {
-    if (status_host) (void)hipHostFree(status_host);
-    for (auto& e : status_ev)
+    if (pin) (void)hipHostFree(pin);
+    if (host_io) (void)hipHostFree(host_io);
+    for (auto& e : ev_ring)
       if (e) (void)hipEventDestroy(e);
+    if (ev_misc) (void)hipEventDestroy(ev_misc);
   }
 };
 
@@ -748,31 +776,28 @@ static vie::JacobiView jacobi_view(dielectric_fft_system* s) {
   return J;
 }
 
-static int ensure_scratch(dielectric_fft_system* s, int m) {
-  SolverWork& w = s->sw;
-  size_t n = s->nsys();
-  (void)m;
-  (void)n;
+static int ensure_scratch(SolverWork& w) {
   if (!w.partial.alloc(vie::kReduceScratchBytes)) return 0;
   if (!w.scal.alloc(16 * sizeof(c64))) return 0;
-  if (!w.status_host) {
-    HIP_OK(hipHostMalloc(&w.status_host, 2 * sizeof(StatusBlock), hipHostMallocDefault));
-    for (auto& e : w.status_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!w.pin) {
+    HIP_OK(hipHostMalloc((void**)&w.pin, sizeof(PinnedStatus), hipHostMallocDefault));
+    for (auto& e : w.ev_ring) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&w.ev_misc, hipEventDisableTiming));
   }
   return 1;
 }
 
-static int ensure_solver(dielectric_fft_system* s, int m, int prec) {
-  SolverWork& w = s->sw;
+static int ensure_solver(dielectric_fft_system* s, SolverWork& w, int m, int prec) {
   size_t n = s->nsys();
   size_t ldv = (n + 63) / 64 * 64;
   size_t es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
-  if (!ensure_scratch(s, m)) return 0;
+  if (!ensure_scratch(w)) return 0;
   // a larger basis from an earlier reserve or FP64 solve is reused as is
   if (!(w.V.alloc((size_t)(m + 1) * ldv * es) && w.x.alloc(n * 16) && w.b.alloc(n * 16) && w.r.alloc(n * 16) &&
         w.ax.alloc(n * 16) && w.state.alloc(sizeof(vie::ArnoldiState)) &&
         w.dots.alloc(2 * (vie::kMaxRestart + 2) * sizeof(c64))))
     return 0;
+  w.sys = s;
   w.n = n;
   w.ldv = ldv;
   w.m = m;
@@ -780,191 +805,264 @@ static int ensure_solver(dielectric_fft_system* s, int m, int prec) {
   return 1;
 }
 
+static size_t solver_bytes(size_t n, int m, int prec) {
+  size_t ldv = (n + 63) / 64 * 64, es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
+  return (size_t)(m + 1) * ldv * es + 4 * n * 16 + vie::kReduceScratchBytes + sizeof(vie::ArnoldiState);
+}
+
 // ------------------------------------------------------------------ device-resident GMRES
 // The restarted GMRES of vie_iterative.c:80-199 with every vector operation on
 // the GPU.  T is the storage / matvec precision of the Krylov cycle; the
 // solution, right-hand side, true residual and all reductions are FP64.
-// FP64: the reference's algorithm (two Gram-Schmidt passes, Givens, true
-// residual acceptance).  MIXED: the same outer loop with FP32 cycles that stop
-// once |g| <= max(target, inner_reduction * beta); each restart's FP64 true
-// residual refines the solution.
+// FP64: the reference's algorithm (reorthogonalised Gram-Schmidt, Givens,
+// true residual acceptance).  MIXED: the same outer loop with FP32 cycles that
+// stop once |g| <= max(target, inner_reduction * beta); each restart's FP64
+// true residual refines the solution.
+enum { PH_IDLE, PH_NORMS, PH_RESID, PH_CYCLE, PH_BACKSOLVE, PH_FINISHED };
 
-static int read_status(dielectric_fft_system* s, StatusBlock* sb) {
-  HIP_OK(hipMemcpyAsync(sb, s->sw.state.p, sizeof(StatusBlock), hipMemcpyDeviceToHost, s->stream));
-  HIP_OK(hipStreamSynchronize(s->stream));
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void job_finish(SolverWork& w, vie_iterative_status st, int ok) {
+  w.rep.status = st;
+  w.result = ok;
+  w.phase = PH_FINISHED;
+  w.t_end = now_s();
+}
+
+// 1: event complete; 0: still running (non-blocking); -1: runtime error.
+static int job_ready(hipEvent_t e, bool blocking) {
+  hipError_t r = blocking ? hipEventSynchronize(e) : hipEventQuery(e);
+  if (r == hipSuccess) return 1;
+  if (r == hipErrorNotReady) return 0;
+  set_error("event wait failed: %s", hipGetErrorString(r));
+  return -1;
+}
+
+static int job_issue_residual(SolverWork& w) {
+  dielectric_fft_system* s = w.sys;
+  vie::ReduceScratch rs;
+  rs.partial = w.partial.as<c64>();
+  w.rep.operator_applications++;
+  if (!sys_apply<double>(s, w.x.as<c64>(), w.ax.as<c64>(), 0, rs)) return 0;
+  vie::residual(w.b.as<c64>(), w.ax.as<c64>(), w.r.as<c64>(), w.n, rs, w.scal.as<c64>() + 2, s->stream);
+  HIP_OK(hipMemcpyAsync(&w.pin->scal[2], w.scal.as<c64>() + 2, sizeof(c64), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipEventRecord(w.ev_misc, s->stream));
   return 1;
 }
 
-static int read_norm(dielectric_fft_system* s, const c64* d, double* out) {
-  c64 v;
-  HIP_OK(hipMemcpyAsync(&v, d, sizeof(c64), hipMemcpyDeviceToHost, s->stream));
-  HIP_OK(hipStreamSynchronize(s->stream));
-  *out = sqrt(v.re);
+// Queues the first work of a solve whose b and x are staged in w.
+static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_solve_options* so) {
+  dielectric_fft_system* s = w.sys;
+  memset(&w.rep, 0, sizeof(w.rep));
+  w.rep.status = VIE_ITERATIVE_INVALID_INPUT;
+  w.rep.residual_norm = w.rep.relative_residual = INFINITY;
+  w.o = *o;
+  w.so = *so;
+  w.reduction = so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
+  const char* ortho = getenv("VIE_HIP_ORTHO");
+  w.dcgs2 = !(ortho && !strcmp(ortho, "cgs2"));
+  w.after_cycle = w.breakdown = w.cycle_done = false;
+  w.result = 0;
+  w.t_submit = now_s();
+  vie::ReduceScratch rs;
+  rs.partial = w.partial.as<c64>();
+  int mm = w.m;
+  HIP_OK(hipMemcpyAsync((char*)w.state.p + offsetof(vie::ArnoldiState, m), &mm, sizeof(int), hipMemcpyHostToDevice,
+                        s->stream));
+  vie::norm2<double>(w.b.as<c64>(), w.n, rs, w.scal.as<c64>() + 0, s->stream);
+  vie::norm2<double>(w.x.as<c64>(), w.n, rs, w.scal.as<c64>() + 1, s->stream);
+  HIP_OK(hipMemcpyAsync(&w.pin->scal[0], w.scal.p, 2 * sizeof(c64), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipEventRecord(w.ev_misc, s->stream));
+  w.phase = PH_NORMS;
   return 1;
 }
 
-template <typename T>
-static int gmres_device(dielectric_fft_system* s, const vie_gmres_options* o, const vie_hip_solve_options* so,
-                        vie_gmres_report* rep) {
-  SolverWork& w = s->sw;
-  const size_t n = w.n, ldv = w.ldv;
-  const int m = w.m;
+// One Arnoldi step j of the current cycle, followed by a status copy.
+template <typename T> static int job_issue_step(SolverWork& w, int j) {
+  dielectric_fft_system* s = w.sys;
   hipStream_t st = s->stream;
   vie::ReduceScratch rs;
   rs.partial = w.partial.as<c64>();
-  c64* x = w.x.as<c64>();
-  c64* b = w.b.as<c64>();
-  c64* r = w.r.as<c64>();
-  c64* ax = w.ax.as<c64>();
   vie::cx<T>* V = w.V.as<vie::cx<T>>();
   vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
   c64* dots1 = w.dots.as<c64>();
   c64* dots2 = dots1 + vie::kMaxRestart + 2;
-  c64* scal = w.scal.as<c64>();  // [0] ||b||^2 [1] ||x||^2 [2] ||r||^2
-  const vie::JacobiView J = jacobi_view(s);
-  const double reduction = so && so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
-  const char* ortho = getenv("VIE_HIP_ORTHO");
-  const bool dcgs2 = !(ortho && !strcmp(ortho, "cgs2"));
+  const size_t ldv = w.ldv, n = w.n;
+  vie::cx<T>* vj = V + (size_t)j * ldv;
+  vie::cx<T>* wv = V + (size_t)(j + 1) * ldv;
+  if (w.dcgs2) {
+    // delayed CGS2: m_eff matvec steps, then one closing step that only
+    // finalises the last column
+    if (j < w.m_eff) {
+      if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
+      w.rep.operator_applications++;
+      vie::dcgs2_step<T>(V, ldv, j, wv, n, sd, rs, dots1, Prec<T>::eps_b, st);
+    } else {
+      vie::dcgs2_step<T>(V, ldv, j, (const vie::cx<T>*)nullptr, n, sd, rs, dots1, Prec<T>::eps_b, st);
+    }
+  } else {
+    if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
+    w.rep.operator_applications++;
+    // two classical Gram-Schmidt passes (same role as the reference's two
+    // MGS passes, vie_iterative.c:151-159); ||w|| before and after
+    vie::multidot<T>(V, ldv, j + 1, wv, n, 1, rs, dots1, st);
+    vie::update<T>(V, ldv, j + 1, dots1, wv, n, 0, rs, st);
+    vie::multidot<T>(V, ldv, j + 1, wv, n, 0, rs, dots2, st);
+    unsigned nb = vie::update<T>(V, ldv, j + 1, dots2, wv, n, 1, rs, st);
+    vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, rs, nb, Prec<T>::eps_b, st);
+    vie::scale_next<T>(wv, n, sd, st);
+  }
+  HIP_OK(hipMemcpyAsync(&w.pin->ring[j & 1], sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipEventRecord(w.ev_ring[j & 1], st));
+  return 1;
+}
+
+// Advances the solve as far as it can without waiting (or, when blocking, to
+// the end).  Returns 1 once the solve has finished.
+template <typename T> static int job_advance(SolverWork& w, bool blocking) {
+  dielectric_fft_system* s = w.sys;
+  hipStream_t st = s->stream;
+  vie_gmres_report& rep = w.rep;
+  vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
   const bool mixed = sizeof(T) == sizeof(float);
-  // m must be set in the state once
-  {
-    int mm = m;
-    HIP_OK(hipMemcpyAsync((char*)sd + offsetof(vie::ArnoldiState, m), &mm, sizeof(int), hipMemcpyHostToDevice, st));
-  }
-  double bnorm, xnorm;
-  vie::norm2<double>(b, n, rs, scal + 0, st);
-  vie::norm2<double>(x, n, rs, scal + 1, st);
-  if (!read_norm(s, scal + 0, &bnorm) || !read_norm(s, scal + 1, &xnorm)) {
-    rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-    return 0;
-  }
-  const double target = std::max(o->absolute_tolerance, o->relative_tolerance * bnorm);
-  if (!std::isfinite(bnorm) || !std::isfinite(xnorm) || !std::isfinite(target)) {
-    rep->status = VIE_ITERATIVE_NONFINITE;
-    return 0;
-  }
-  auto true_residual = [&](double* beta) -> int {
-    rep->operator_applications++;
-    if (!sys_apply<double>(s, x, ax, 0, rs)) {
-      rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-      return 0;
-    }
-    vie::residual(b, ax, r, n, rs, scal + 2, st);
-    if (!read_norm(s, scal + 2, beta)) {
-      rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-      return 0;
-    }
-    rep->residual_norm = *beta;
-    if (!std::isfinite(*beta)) {
-      rep->status = VIE_ITERATIVE_NONFINITE;
-      return 0;
-    }
-    return 1;
-  };
-  double beta;
-  if (!true_residual(&beta)) return 0;
   for (;;) {
-    rep->relative_residual = bnorm > 0 ? beta / bnorm : beta;
-    if (beta <= target) {
-      rep->status = VIE_ITERATIVE_CONVERGED;
-      return 1;
-    }
-    if (rep->iterations >= o->maximum_iterations) {
-      rep->status = VIE_ITERATIVE_ITERATION_LIMIT;
-      return 0;
-    }
-    const double inner_target = mixed ? std::max(target, reduction * beta) : target;
-    vie::cycle_init(sd, scal + 2, inner_target, st);
-    vie::scale_from<T>(V, r, n, sd, st);
-    // Arnoldi steps with one step of lookahead: step j is queued before the
-    // status of step j-1 is read, so the GPU never waits for the host.  The
-    // device tail marks the cycle done; a step queued after that is a no-op
-    // for the state (its vector work lands in an unused basis slot).
-    StatusBlock* ring = (StatusBlock*)w.status_host;
-    const size_t it0 = rep->iterations;
-    // columns this cycle may add (the reference counts Arnoldi steps, vie_iterative.c:134)
-    const int m_eff = (int)std::min<size_t>((size_t)m, o->maximum_iterations - it0);
-    // Delayed CGS2 issues m_eff matvec steps plus one closing step that only
-    // finalises the last column; CGS2 issues m_eff steps.
-    const int steps = dcgs2 ? m_eff + 1 : m_eff;
-    int issued = 0, checked = 0;
-    bool done = false;
-    auto issue = [&](int j) -> int {
-      vie::cx<T>* vj = V + (size_t)j * ldv;
-      vie::cx<T>* wv = V + (size_t)(j + 1) * ldv;
-      if (dcgs2) {
-        if (j < m_eff) {
-          if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
-          rep->operator_applications++;
-          vie::dcgs2_step<T>(V, ldv, j, wv, n, sd, rs, dots1, Prec<T>::eps_b, st);
-        } else {
-          vie::dcgs2_step<T>(V, ldv, j, (const vie::cx<T>*)nullptr, n, sd, rs, dots1, Prec<T>::eps_b, st);
+    switch (w.phase) {
+      case PH_FINISHED:
+        return 1;
+      case PH_NORMS: {
+        int r = job_ready(w.ev_misc, blocking);
+        if (r <= 0) {
+          if (r < 0) job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+          return r < 0;
         }
-      } else {
-        if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
-        rep->operator_applications++;
-        // two classical Gram-Schmidt passes (same role as the reference's two
-        // MGS passes, vie_iterative.c:151-159); ||w|| before and after
-        vie::multidot<T>(V, ldv, j + 1, wv, n, 1, rs, dots1, st);
-        vie::update<T>(V, ldv, j + 1, dots1, wv, n, 0, rs, st);
-        vie::multidot<T>(V, ldv, j + 1, wv, n, 0, rs, dots2, st);
-        unsigned nb = vie::update<T>(V, ldv, j + 1, dots2, wv, n, 1, rs, st);
-        vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, rs, nb, Prec<T>::eps_b, st);
-        vie::scale_next<T>(wv, n, sd, st);
-      }
-      HIP_OK(hipMemcpyAsync(&ring[j & 1], sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
-      HIP_OK(hipEventRecord(w.status_ev[j & 1], st));
-      return 1;
-    };
-    auto check = [&](int j) -> int {
-      HIP_OK(hipEventSynchronize(w.status_ev[j & 1]));
-      const StatusBlock& sb = ring[j & 1];
-      if (sb.nonfinite) {
-        rep->status = VIE_ITERATIVE_NONFINITE;
-        return 0;
-      }
-      done = sb.done != 0;
-      return 1;
-    };
-    while (!done) {
-      bool can_issue = issued < steps;
-      if (can_issue) {
-        if (!issue(issued)) {
-          rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-          return 0;
+        w.bnorm = sqrt(w.pin->scal[0].re);
+        double xnorm = sqrt(w.pin->scal[1].re);
+        w.target = std::max(w.o.absolute_tolerance, w.o.relative_tolerance * w.bnorm);
+        if (!std::isfinite(w.bnorm) || !std::isfinite(xnorm) || !std::isfinite(w.target)) {
+          job_finish(w, VIE_ITERATIVE_NONFINITE, 0);
+          return 1;
         }
-        issued++;
-      }
-      if (checked < issued && (issued - checked > 1 || !can_issue)) {
-        if (!check(checked)) return 0;
-        checked++;
-      } else if (!can_issue) {
+        if (!job_issue_residual(w)) {
+          job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+          return 1;
+        }
+        w.phase = PH_RESID;
         break;
       }
+      case PH_RESID: {
+        int r = job_ready(w.ev_misc, blocking);
+        if (r <= 0) {
+          if (r < 0) job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+          return r < 0;
+        }
+        w.beta = sqrt(w.pin->scal[2].re);
+        rep.residual_norm = w.beta;
+        if (!std::isfinite(w.beta)) {
+          job_finish(w, VIE_ITERATIVE_NONFINITE, 0);
+          return 1;
+        }
+        rep.relative_residual = w.bnorm > 0 ? w.beta / w.bnorm : w.beta;
+        if (w.after_cycle) {
+          if (w.breakdown && w.beta > w.target) {
+            job_finish(w, VIE_ITERATIVE_BREAKDOWN, 0);
+            return 1;
+          }
+          if (w.beta > w.target && rep.iterations < w.o.maximum_iterations) rep.restarts++;
+        }
+        if (w.beta <= w.target) {
+          job_finish(w, VIE_ITERATIVE_CONVERGED, 1);
+          return 1;
+        }
+        if (rep.iterations >= w.o.maximum_iterations) {
+          job_finish(w, VIE_ITERATIVE_ITERATION_LIMIT, 0);
+          return 1;
+        }
+        // new cycle
+        const double inner_target = mixed ? std::max(w.target, w.reduction * w.beta) : w.target;
+        vie::cycle_init(sd, w.scal.as<c64>() + 2, inner_target, st);
+        vie::scale_from<T>(w.V.as<vie::cx<T>>(), w.r.as<c64>(), w.n, sd, st);
+        w.it0 = rep.iterations;
+        // columns this cycle may add (the reference counts Arnoldi steps, vie_iterative.c:134)
+        w.m_eff = (int)std::min<size_t>((size_t)w.m, w.o.maximum_iterations - w.it0);
+        w.steps = w.dcgs2 ? w.m_eff + 1 : w.m_eff;
+        w.issued = w.checked = 0;
+        w.cycle_done = false;
+        w.phase = PH_CYCLE;
+        break;
+      }
+      case PH_CYCLE: {
+        // one step of lookahead: step j is queued before the status of step
+        // j-1 is read, so the GPU never idles on the host; the device marks
+        // the cycle done and later queued steps leave the state untouched
+        bool progressed = true;
+        while (progressed) {
+          progressed = false;
+          if (!w.cycle_done && w.issued < w.steps && w.issued - w.checked < 2) {
+            if (!job_issue_step<T>(w, w.issued)) {
+              job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+              return 1;
+            }
+            w.issued++;
+            progressed = true;
+            continue;
+          }
+          if (w.checked < w.issued) {
+            int r = job_ready(w.ev_ring[w.checked & 1], blocking);
+            if (r < 0) {
+              job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+              return 1;
+            }
+            if (r == 0) return 0;
+            const StatusBlock& sb = w.pin->ring[w.checked & 1];
+            if (sb.nonfinite) {
+              job_finish(w, VIE_ITERATIVE_NONFINITE, 0);
+              return 1;
+            }
+            if (sb.done) w.cycle_done = true;
+            w.checked++;
+            progressed = true;
+          }
+        }
+        // all queued steps are complete: the last status holds the cycle's end
+        const StatusBlock& last = w.pin->ring[(w.issued - 1) & 1];
+        rep.iterations = w.it0 + (size_t)last.used;
+        w.breakdown = last.breakdown != 0;
+        vie::backsolve(sd, st);
+        HIP_OK(hipMemcpyAsync(&w.pin->fin, sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipEventRecord(w.ev_misc, st));
+        w.phase = PH_BACKSOLVE;
+        break;
+      }
+      case PH_BACKSOLVE: {
+        int r = job_ready(w.ev_misc, blocking);
+        if (r <= 0) {
+          if (r < 0) job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+          return r < 0;
+        }
+        if (w.pin->fin.singular) {
+          job_finish(w, VIE_ITERATIVE_BREAKDOWN, 0);
+          return 1;
+        }
+        vie::combine<T>(w.x.as<c64>(), w.V.as<vie::cx<T>>(), w.ldv, w.n, sd, jacobi_view(s), 1, st);
+        if (!job_issue_residual(w)) {
+          job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+          return 1;
+        }
+        w.after_cycle = true;
+        w.phase = PH_RESID;
+        break;
+      }
+      default:
+        job_finish(w, VIE_ITERATIVE_INVALID_INPUT, 0);
+        return 1;
     }
-    StatusBlock sb;
-    if (!read_status(s, &sb)) return 0;
-    if (sb.nonfinite) {
-      rep->status = VIE_ITERATIVE_NONFINITE;
-      return 0;
-    }
-    rep->iterations = it0 + (size_t)sb.used;
-    const bool breakdown = sb.breakdown != 0;
-    vie::backsolve(sd, st);
-    if (!read_status(s, &sb)) return 0;
-    if (sb.singular) {
-      rep->status = VIE_ITERATIVE_BREAKDOWN;
-      return 0;
-    }
-    vie::combine<T>(x, V, ldv, n, sd, J, 1, st);
-    if (!true_residual(&beta)) return 0;
-    rep->relative_residual = bnorm > 0 ? beta / bnorm : beta;
-    if (breakdown && beta > target) {
-      rep->status = VIE_ITERATIVE_BREAKDOWN;
-      return 0;
-    }
-    if (beta > target && rep->iterations < o->maximum_iterations) rep->restarts++;
   }
+}
+
+static int job_advance_any(SolverWork& w, bool blocking) {
+  return w.prec == VIE_HIP_MIXED ? job_advance<float>(w, blocking) : job_advance<double>(w, blocking);
 }
 
 static int validate_options(const vie_gmres_options* o) {
@@ -979,6 +1077,31 @@ static void report_init(vie_gmres_report* r) {
   r->residual_norm = r->relative_residual = INFINITY;
 }
 
+// Restart length the device solver uses for these options (0 when too long).
+static size_t device_restart(size_t n, const vie_gmres_options* o) {
+  size_t m = o->restart;
+  if (m > n) m = n;
+  if (m > o->maximum_iterations) m = o->maximum_iterations;
+  if (m > (size_t)vie::kMaxRestart) {
+    set_error("restart %zu exceeds the device solver's %d", m, vie::kMaxRestart);
+    return 0;
+  }
+  return m;
+}
+
+// Readies w for a solve of s (basis sized for the options, blocks and
+// permittivities on the device).  0 with rep->status set on failure.
+static int job_prepare(dielectric_fft_system* s, SolverWork& w, const vie_gmres_options* o,
+                       const vie_hip_solve_options* so, vie_gmres_report* rep) {
+  size_t m = device_restart(s->c + s->v, o);
+  int prec = so->precision == VIE_HIP_MIXED ? VIE_HIP_MIXED : VIE_HIP_FP64;
+  if (!m || !ensure_solver(s, w, (int)m, prec) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan))) {
+    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
+    return 0;
+  }
+  return 1;
+}
+
 // Solve with b already in the solver's device buffer (system layout) and x
 // holding the initial guess.
 static int solve_resident(dielectric_fft_system* s, const vie_gmres_options* options,
@@ -990,21 +1113,15 @@ static int solve_resident(dielectric_fft_system* s, const vie_gmres_options* opt
     init_solve_defaults();
     so = &g_solve;
   }
-  size_t n = s->c + s->v;
-  size_t m = options->restart;
-  if (m > n) m = n;
-  if (m > options->maximum_iterations) m = options->maximum_iterations;
-  if (m > (size_t)vie::kMaxRestart) {
-    set_error("restart %zu exceeds the device solver's %d", m, vie::kMaxRestart);
-    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
+  SolverWork& w = s->sw;
+  if (!job_prepare(s, w, options, so, rep)) return 0;
+  if (!job_start(w, options, so)) {
+    rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
     return 0;
   }
-  int prec = so->precision == VIE_HIP_MIXED ? VIE_HIP_MIXED : VIE_HIP_FP64;
-  if (!ensure_solver(s, (int)m, prec) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan))) {
-    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
-    return 0;
-  }
-  return prec == VIE_HIP_MIXED ? gmres_device<float>(s, options, so, rep) : gmres_device<double>(s, options, so, rep);
+  job_advance_any(w, true);
+  *rep = w.rep;
+  return w.result;
 }
 
 static int stage_vector(dielectric_fft_system* s, const double _Complex* user, c64* dst) {
@@ -1112,7 +1229,7 @@ extern "C" int dielectric_fft_reserve(dielectric_fft_system* s, const vie_gmres_
   if (m > options->maximum_iterations) m = options->maximum_iterations;
   if (m > (size_t)vie::kMaxRestart) return 0;
   // FP64-sized basis serves both precisions
-  if (!ensure_solver(s, (int)m, VIE_HIP_FP64) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)))
+  if (!ensure_solver(s, s->sw, (int)m, VIE_HIP_FP64) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)))
     return 0;
   HIP_OK(hipStreamSynchronize(s->stream));
   return 1;
@@ -1141,7 +1258,7 @@ static int apply_host(dielectric_fft_system* s, const double _Complex* in, doubl
   if (!s || !in || !out) return 0;
   size_t n = s->nsys();
   if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
-      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s, 1))
+      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw))
     return 0;
   if (!stage_vector(s, in, s->sw.x.as<c64>())) return 0;
   if (precond) {
@@ -1159,7 +1276,7 @@ extern "C" int dielectric_fft_apply_ex(dielectric_fft_system* s, vie_hip_precisi
   if (!s || !in || !out) return 0;
   size_t n = s->nsys();
   if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
-      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s, 1))
+      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw))
     return 0;
   if (!stage_vector(s, in, s->sw.x.as<c64>())) return 0;
   vie::ReduceScratch rs;
@@ -1468,7 +1585,7 @@ extern "C" int dielectric_fft_time_apply(dielectric_fft_system* s, vie_hip_preci
                                          double* ms_per_apply, double* pass_ms) {
   if (!s || repeats < 1) return 0;
   size_t n = s->nsys();
-  if (!ensure_vectors(s) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s, 1))
+  if (!ensure_vectors(s) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw))
     return 0;
   vie::ReduceScratch rs;
   rs.partial = s->sw.partial.as<c64>();
@@ -1559,5 +1676,267 @@ extern "C" int dielectric_fft_apply_bytes(const dielectric_fft_system* s, vie_hi
     t += p[q];
   }
   if (total) *total = t;
+  return 1;
+}
+
+// ------------------------------------------------------------------ batched solves
+// Many independent solves in flight at once.  Each job owns a pooled solver
+// workspace (Krylov basis, vectors, reduction scratch, pinned status) and is
+// advanced by event queries only, so the host interleaves the Arnoldi steps
+// of every job: jobs of one system share its stream (they run in submission
+// order), jobs of different systems run on different streams and overlap on
+// the GPU.  The object is not reentrant; one host thread drives it.
+namespace {
+enum { JB_RUNNING, JB_DOWNLOAD, JB_READY };
+struct BatchJob {
+  std::unique_ptr<SolverWork> w;
+  int state = JB_RUNNING;
+  bool reported = false;
+  size_t nu = 0;
+  double t_ready = 0;
+};
+}  // namespace
+
+struct vie_hip_batch {
+  std::vector<std::unique_ptr<SolverWork>> pool;
+  std::map<long long, BatchJob> jobs;
+  std::vector<long long> order;  // submission order of unfinished jobs
+  long long next_id = 1;
+};
+
+extern "C" vie_hip_batch* vie_hip_batch_create(void) {
+  g_last_error.clear();
+  return new (std::nothrow) vie_hip_batch();
+}
+
+extern "C" void vie_hip_batch_destroy(vie_hip_batch* b) {
+  if (!b) return;
+  // queued device work must end before its buffers are freed
+  for (auto& kv : b->jobs)
+    if (kv.second.w && kv.second.w->sys) (void)hipStreamSynchronize(kv.second.w->sys->stream);
+  delete b;
+}
+
+static std::unique_ptr<SolverWork> pool_take(vie_hip_batch* b) {
+  if (b->pool.empty()) return std::unique_ptr<SolverWork>(new (std::nothrow) SolverWork());
+  std::unique_ptr<SolverWork> w = std::move(b->pool.back());
+  b->pool.pop_back();
+  return w;
+}
+
+static int host_io_reserve(SolverWork& w, size_t nu) {
+  if (w.host_io_cap >= nu && w.host_io) return 1;
+  if (w.host_io) (void)hipHostFree(w.host_io);
+  w.host_io = nullptr;
+  w.host_io_cap = 0;
+  HIP_OK(hipHostMalloc((void**)&w.host_io, std::max<size_t>(nu, 1) * 16, hipHostMallocDefault));
+  w.host_io_cap = nu;
+  return 1;
+}
+
+extern "C" size_t vie_hip_batch_job_bytes(const dielectric_fft_system* s, const vie_gmres_options* options,
+                                          const vie_hip_solve_options* so) {
+  if (!s) return 0;
+  vie_gmres_options defaults = dielectric_gmres_settings;
+  if (!options) options = &defaults;
+  size_t m = options->restart, n = s->c + s->v;
+  if (m > n) m = n;
+  if (m > options->maximum_iterations) m = options->maximum_iterations;
+  int prec = so && so->precision == VIE_HIP_MIXED ? VIE_HIP_MIXED : VIE_HIP_FP64;
+  return solver_bytes(s->nsys(), (int)m, prec);
+}
+
+extern "C" int vie_hip_batch_reserve(vie_hip_batch* b, size_t count, dielectric_fft_system* s,
+                                     const vie_gmres_options* options) {
+  if (!b || !s) return 0;
+  vie_gmres_options defaults = dielectric_gmres_settings;
+  if (!options) options = &defaults;
+  if (!validate_options(options)) return 0;
+  size_t m = device_restart(s->c + s->v, options);
+  if (!m) return 0;
+  while (b->pool.size() < count) {
+    std::unique_ptr<SolverWork> w(new (std::nothrow) SolverWork());
+    if (!w) return 0;
+    b->pool.push_back(std::move(w));
+  }
+  // FP64-sized bases serve either precision
+  for (size_t i = 0; i < count; i++) {
+    SolverWork& w = *b->pool[b->pool.size() - 1 - i];
+    if (!ensure_solver(s, w, (int)m, VIE_HIP_FP64) || !host_io_reserve(w, s->c + s->v)) return 0;
+    w.sys = nullptr;
+  }
+  HIP_OK(hipDeviceSynchronize());
+  return 1;
+}
+
+extern "C" long long vie_hip_batch_submit(vie_hip_batch* b, dielectric_fft_system* s, const double _Complex* rhs,
+                                          const vie_gmres_options* options, const vie_hip_solve_options* so) {
+  g_last_error.clear();
+  if (!b || !s || !rhs) {
+    set_error("batch submit: null argument");
+    return 0;
+  }
+  vie_gmres_options defaults = dielectric_gmres_settings;
+  if (!options) options = &defaults;
+  if (!validate_options(options)) {
+    set_error("batch submit: invalid GMRES options");
+    return 0;
+  }
+  if (!so) {
+    init_solve_defaults();
+    so = &g_solve;
+  }
+  BatchJob job;
+  job.w = pool_take(b);
+  if (!job.w) return 0;
+  SolverWork& w = *job.w;
+  vie_gmres_report rep;
+  report_init(&rep);
+  size_t nu = s->c + s->v, n = s->nsys();
+  if (!job_prepare(s, w, options, so, &rep) || !host_io_reserve(w, nu)) {
+    b->pool.push_back(std::move(job.w));
+    return 0;
+  }
+  // b: user layout -> pinned -> device -> system layout; x0 = 0
+  memcpy(w.host_io, rhs, nu * 16);
+  hipStream_t st = s->stream;
+  if (hipMemcpyAsync(w.ax.p, w.host_io, nu * 16, hipMemcpyHostToDevice, st) != hipSuccess) {
+    set_error("batch submit: right-hand side copy failed");
+    b->pool.push_back(std::move(job.w));
+    return 0;
+  }
+  vie::vec_zero<double>(w.b.as<c64>(), n, st);
+  vie::to_box<double>(w.ax.as<c64>(), s->c, s->cells, s->plan ? s->plan->map_dev.as<size_t>() : nullptr, s->nbox(),
+                      w.b.as<c64>(), st);
+  vie::vec_zero<double>(w.x.as<c64>(), n, st);
+  if (!job_start(w, options, so)) {
+    b->pool.push_back(std::move(job.w));
+    return 0;
+  }
+  job.nu = nu;
+  long long id = b->next_id++;
+  b->jobs.emplace(id, std::move(job));
+  b->order.push_back(id);
+  return id;
+}
+
+// Advances one job; returns 1 when it changed state.
+static int batch_step(BatchJob& j, bool blocking) {
+  SolverWork& w = *j.w;
+  if (j.state == JB_RUNNING) {
+    if (!job_advance_any(w, blocking)) return 0;
+    // solution: system layout -> user layout -> pinned
+    dielectric_fft_system* s = w.sys;
+    hipStream_t st = s->stream;
+    vie::from_box<double>(w.x.as<c64>(), s->c, s->cells, s->plan ? s->plan->map_dev.as<size_t>() : nullptr, s->nbox(),
+                          w.r.as<c64>(), st);
+    if (hipMemcpyAsync(w.host_io, w.r.p, j.nu * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(w.ev_misc, st) != hipSuccess) {
+      w.result = 0;
+      w.rep.status = VIE_ITERATIVE_CALLBACK_FAILED;
+      j.state = JB_READY;
+      j.t_ready = now_s();
+      return 1;
+    }
+    j.state = JB_DOWNLOAD;
+    return 1;
+  }
+  if (j.state == JB_DOWNLOAD) {
+    int r = job_ready(w.ev_misc, blocking);
+    if (r == 0) return 0;
+    if (r < 0) {
+      w.result = 0;
+      w.rep.status = VIE_ITERATIVE_CALLBACK_FAILED;
+    }
+    j.state = JB_READY;
+    j.t_ready = now_s();
+    return 1;
+  }
+  return 0;
+}
+
+// One non-blocking sweep over the unfinished jobs; returns how many changed.
+static int batch_sweep(vie_hip_batch* b) {
+  int changed = 0;
+  size_t keep = 0;
+  for (size_t i = 0; i < b->order.size(); i++) {
+    long long id = b->order[i];
+    BatchJob& j = b->jobs[id];
+    while (j.state != JB_READY && batch_step(j, false)) changed++;
+    if (j.state != JB_READY) b->order[keep++] = id;
+  }
+  b->order.resize(keep);
+  return changed;
+}
+
+extern "C" size_t vie_hip_batch_progress(vie_hip_batch* b, long long* ready, size_t max_ready, size_t* running) {
+  if (!b) return 0;
+  batch_sweep(b);
+  size_t count = 0;
+  for (auto& kv : b->jobs) {
+    if (count >= max_ready) break;
+    if (kv.second.state == JB_READY && !kv.second.reported) {
+      kv.second.reported = true;
+      if (ready) ready[count] = kv.first;
+      count++;
+    }
+  }
+  if (running) *running = b->order.size();
+  return count;
+}
+
+extern "C" size_t vie_hip_batch_running(const vie_hip_batch* b) { return b ? b->order.size() : 0; }
+
+extern "C" int vie_hip_batch_wait(vie_hip_batch* b, long long id) {
+  if (!b) return 0;
+  auto it = b->jobs.find(id);
+  if (it == b->jobs.end()) {
+    set_error("batch wait: unknown job %lld", id);
+    return 0;
+  }
+  while (it->second.state != JB_READY) {
+    if (!batch_sweep(b)) std::this_thread::yield();
+  }
+  return 1;
+}
+
+extern "C" int vie_hip_batch_wait_all(vie_hip_batch* b) {
+  if (!b) return 0;
+  while (!b->order.empty())
+    if (!batch_sweep(b)) std::this_thread::yield();
+  return 1;
+}
+
+// 1 when the job has converged (the vie_gmres return value); x (may be NULL)
+// receives the solution in the user layout, report (may be NULL) the report,
+// seconds (may be NULL) the time from submission to completion.
+extern "C" int vie_hip_batch_result(vie_hip_batch* b, long long id, double _Complex* x, vie_gmres_report* report,
+                                    double* seconds) {
+  if (!b) return 0;
+  auto it = b->jobs.find(id);
+  if (it == b->jobs.end() || it->second.state != JB_READY) {
+    set_error("batch result: job %lld is not finished", id);
+    if (report) report_init(report);
+    return 0;
+  }
+  BatchJob& j = it->second;
+  if (x && j.w->rep.status != VIE_ITERATIVE_INVALID_INPUT) memcpy(x, j.w->host_io, j.nu * 16);
+  if (report) *report = j.w->rep;
+  if (seconds) *seconds = j.t_ready - j.w->t_submit;
+  j.reported = true;
+  return j.w->result;
+}
+
+extern "C" int vie_hip_batch_release(vie_hip_batch* b, long long id) {
+  if (!b) return 0;
+  auto it = b->jobs.find(id);
+  if (it == b->jobs.end()) return 0;
+  if (it->second.state != JB_READY) {
+    set_error("batch release: job %lld is still running", id);
+    return 0;
+  }
+  it->second.w->sys = nullptr;
+  b->pool.push_back(std::move(it->second.w));
+  b->jobs.erase(it);
   return 1;
 }

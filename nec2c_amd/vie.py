@@ -21,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libvie_hip.so")
 
 FP64, MIXED = 0, 1
+KMAX_RESTART = 128  # longest Krylov cycle of the device solver (krylov.h kMaxRestart)
 STATUS = ("converged", "iteration limit", "Krylov breakdown",
           "operator/preconditioner failure", "invalid input",
           "workspace allocation/size failure", "nonfinite arithmetic")
@@ -121,6 +122,9 @@ EXPORTS = (
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
     "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
     "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_reserve",
+    "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
+    "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
+    "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release",
 )
 
 _LIB = None
@@ -186,6 +190,19 @@ def lib() -> ctypes.CDLL:
         "dielectric_fft_apply_ex": (i, [vp, i, i, vp, vp]),
         "dielectric_fft_reserve": (i, [vp, ctypes.POINTER(GmresOptions)]),
         "vie_hip_copy_probe": (d, [sz, i]),
+        "vie_hip_batch_create": (vp, []),
+        "vie_hip_batch_destroy": (None, [vp]),
+        "vie_hip_batch_job_bytes": (sz, [vp, ctypes.POINTER(GmresOptions), ctypes.POINTER(SolveOptions)]),
+        "vie_hip_batch_reserve": (i, [vp, sz, vp, ctypes.POINTER(GmresOptions)]),
+        "vie_hip_batch_submit": (ctypes.c_longlong, [vp, vp, vp, ctypes.POINTER(GmresOptions),
+                                                      ctypes.POINTER(SolveOptions)]),
+        "vie_hip_batch_progress": (sz, [vp, ctypes.POINTER(ctypes.c_longlong), sz, szp]),
+        "vie_hip_batch_running": (sz, [vp]),
+        "vie_hip_batch_wait": (i, [vp, ctypes.c_longlong]),
+        "vie_hip_batch_wait_all": (i, [vp]),
+        "vie_hip_batch_result": (i, [vp, ctypes.c_longlong, vp, ctypes.POINTER(GmresReport),
+                                     ctypes.POINTER(d)]),
+        "vie_hip_batch_release": (i, [vp, ctypes.c_longlong]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -526,6 +543,79 @@ class System:
         if getattr(self, "s", None):
             lib().dielectric_fft_free(self.s)
             self.s = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Many device-resident solves in flight (vie_hip_batch_*).
+
+    Jobs are submitted per (system, right-hand side); the library interleaves
+    their Arnoldi steps and overlaps jobs of different systems on the GPU.
+    """
+
+    def __init__(self):
+        self.b = lib().vie_hip_batch_create()
+        if not self.b:
+            raise RuntimeError(f"batch create failed: {last_error()}")
+        self._orders = {}
+
+    @staticmethod
+    def job_bytes(system, precision=FP64, **opts):
+        o = _options(**opts)
+        so = SolveOptions(precision, 0.0)
+        return lib().vie_hip_batch_job_bytes(system.s, ctypes.byref(o), ctypes.byref(so))
+
+    def reserve(self, count, system, **opts):
+        o = _options(**opts)
+        if not lib().vie_hip_batch_reserve(self.b, count, system.s, ctypes.byref(o)):
+            raise RuntimeError(f"batch reserve failed: {last_error()}")
+
+    def submit(self, system, rhs, precision=FP64, inner_reduction=0.0, **opts):
+        rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+        if rhs.size != system.order:
+            raise ValueError(f"right-hand side has {rhs.size} entries, the system {system.order}")
+        o = _options(**opts)
+        so = SolveOptions(precision, inner_reduction)
+        jid = lib().vie_hip_batch_submit(self.b, system.s, _ptr(rhs), ctypes.byref(o), ctypes.byref(so))
+        if jid <= 0:
+            raise RuntimeError(f"batch submit failed: {last_error()}")
+        self._orders[jid] = system.order
+        return jid
+
+    def progress(self, max_ready=1024):
+        ids = (ctypes.c_longlong * max_ready)()
+        running = ctypes.c_size_t()
+        k = lib().vie_hip_batch_progress(self.b, ids, max_ready, ctypes.byref(running))
+        return [ids[q] for q in range(k)], running.value
+
+    @property
+    def running(self):
+        return lib().vie_hip_batch_running(self.b)
+
+    def wait(self, jid):
+        if not lib().vie_hip_batch_wait(self.b, jid):
+            raise RuntimeError(last_error())
+
+    def wait_all(self):
+        lib().vie_hip_batch_wait_all(self.b)
+
+    def result(self, jid, release=True):
+        """(converged, x, report, seconds from submission to completion)."""
+        x = np.zeros(self._orders[jid], dtype=np.complex128)
+        rep = GmresReport()
+        sec = ctypes.c_double()
+        ok = lib().vie_hip_batch_result(self.b, jid, _ptr(x), ctypes.byref(rep), ctypes.byref(sec))
+        if release:
+            lib().vie_hip_batch_release(self.b, jid)
+            self._orders.pop(jid, None)
+        return bool(ok), x, rep, sec.value
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib().vie_hip_batch_destroy(self.b)
+            self.b = None
 
     def __del__(self):
         self.close()

@@ -149,3 +149,59 @@ def cube_cells(n, pitch, origin=(0.0, 0.0, 0.0)):
     shape = (n, n, n) if np.isscalar(n) else tuple(n)
     g = np.stack(np.meshgrid(*[np.arange(s) for s in shape], indexing="ij"), -1).reshape(-1, 3)
     return np.asarray(origin)[None, :] + pitch * (g + 0.5)
+
+
+@dataclass
+class BatchCase:
+    """One solve of the batched workload: a homogeneous block, a dipole and
+    its feed ports (one right-hand side per port)."""
+    shape: tuple
+    eps_r: float
+    freq: float
+    pitch: float
+    dipole: Dipole
+    ports: tuple        # feed segment of each right-hand side
+
+    @property
+    def k0(self):
+        return k0_of(self.freq)
+
+    def centers(self):
+        return cube_cells(self.shape, self.pitch)
+
+    def rhs(self, port):
+        v = np.zeros(self.dipole.segments, dtype=np.complex128)
+        v[self.ports[port]] = 1.0
+        return v
+
+
+def batch_cases(count, seed, lo=8, hi=48, max_ports=8, seg_lo=11, seg_hi=31, f_lo=50e6, f_hi=500e6,
+                eps_lo=2.0, eps_hi=12.0, kh=0.3):
+    """Draws of the batched workload: per case the grid extent per axis is
+    uniform on [lo, hi] (a cube or a cuboid with equal odds), eps_r uniform on
+    [eps_lo, eps_hi], the frequency uniform on [f_lo, f_hi], the pitch chosen
+    so k0 sqrt(eps_r) h = kh (kh <= 0.5), and a z-directed dipole of 0.47
+    free-space wavelengths with seg_lo..seg_hi segments, a tenth of a
+    wavelength (at least two cells) from the block's x = 0 face, fed at
+    1..max_ports distinct segments."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        if rng.random() < 0.5:
+            n = int(rng.integers(lo, hi + 1))
+            shape = (n, n, n)
+        else:
+            shape = tuple(int(v) for v in rng.integers(lo, hi + 1, 3))
+        er = float(rng.uniform(eps_lo, eps_hi))
+        f = float(rng.uniform(f_lo, f_hi))
+        k0 = k0_of(f)
+        h = kh / (k0 * math.sqrt(er))
+        lam = 2 * math.pi / k0
+        segs = int(rng.integers(seg_lo, seg_hi + 1))
+        gap = max(0.1 * lam, 2 * h)
+        dip = Dipole(center=(-gap, 0.5 * shape[1] * h, 0.5 * shape[2] * h), length=0.47 * lam, segments=segs,
+                     radius=lam / 1000)
+        p = int(rng.integers(1, max_ports + 1))
+        ports = tuple(int(v) for v in rng.choice(segs, size=min(p, segs), replace=False))
+        out.append(BatchCase(shape, er, f, h, dip, ports))
+    return out
