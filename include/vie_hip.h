@@ -63,6 +63,14 @@ int vie_fft_set_permittivities(vie_fft_plan *plan, const VIE_CDOUBLE *epsilon_r,
    device work and returns y widened to double. */
 int dielectric_fft_apply_ex(dielectric_fft_system *system, vie_hip_precision precision, int jacobi,
                             const VIE_CDOUBLE *x, VIE_CDOUBLE *y);
+/* Pass-shape autotuner: times each launch-shape variant (lines per
+   workgroup, workgroup size) of the five volume passes on this plan's
+   geometry and keeps the fastest per pass and precision.  Choices are cached
+   per size class (box, padded extents, PEC) for the life of the process.
+   Runs by itself before a plan's first solve or apply; VIE_HIP_TUNE=0 in the
+   environment keeps variant 0.  variants (optional, 10 ints) receives the
+   FP64 then the FP32 choice of each pass. */
+int vie_fft_tune(vie_fft_plan *plan, int *variants);
 /* Waits for all work on the current device. */
 int vie_hip_synchronize(void);
 /* Copies `bytes` (rounded down to 16) between two device buffers `repeats`

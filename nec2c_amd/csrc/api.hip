@@ -12,6 +12,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <map>
 #include <thread>
@@ -137,6 +138,8 @@ struct vie_fft_plan {
   DevBuf spec64, spec32, work, tw64, tw32, eps64, eps32, map_dev;
   DevBuf io_user, io_sys_in, io_sys_out;  // host-pointer apply staging
   dielectric_fft_system* wrap = nullptr;  // c = 0 system around this plan, for solves
+  unsigned char var[2][5] = {};          // pass launch variants: [0] FP64, [1] FP32
+  bool tuned[2] = {false, false};
   ~vie_fft_plan();
   size_t nbox() const { return g.nbox(); }
   size_t oct() const { return (size_t)(g.Px / 2 + 1) * (g.Py / 2 + 1) * (g.Pz / 2 + 1); }
@@ -153,6 +156,7 @@ template <> vie::VolumeView<double> vie_fft_plan::view<double>() {
   v.eps = eps64.as<c64>();
   v.tw = tw64.as<c64>();
   for (int i = 0; i < 3; i++) v.self[i] = self[i];
+  for (int i = 0; i < 5; i++) v.var[i] = var[0][i];
   return v;
 }
 template <> vie::VolumeView<float> vie_fft_plan::view<float>() {
@@ -165,6 +169,7 @@ template <> vie::VolumeView<float> vie_fft_plan::view<float>() {
   v.eps = eps32.as<c32>();
   v.tw = tw32.as<c32>();
   for (int i = 0; i < 3; i++) v.self[i] = c32{(float)self[i].re, (float)self[i].im};
+  for (int i = 0; i < 5; i++) v.var[i] = var[1][i];
   return v;
 }
 
@@ -218,6 +223,113 @@ static int plan_upload_eps(vie_fft_plan* p) {
   HIP_OK(hipMemcpyAsync(p->eps32.p, box32.data(), nbox * sizeof(c32), hipMemcpyHostToDevice, p->stream));
   HIP_OK(hipStreamSynchronize(p->stream));
   p->eps_dirty = false;
+  return 1;
+}
+
+// ------------------------------------------------------------------ pass-shape tuner
+// Picks, per pass and precision, the launch variant (lines per workgroup,
+// workgroup size) with the shortest measured time on this geometry.  Results
+// are cached per (precision, box, padded extents, PEC) for the life of the
+// process, so every plan of one size class is tuned once.  VIE_HIP_TUNE=0
+// keeps variant 0 everywhere.
+namespace {
+struct TuneKey {
+  int prec, nx, ny, nz, Px, Py, Pz, pec;
+  bool operator<(const TuneKey& o) const {
+    return memcmp(this, &o, sizeof(TuneKey)) < 0;
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::array<unsigned char, 5>> g_tune_cache;
+}  // namespace
+
+static bool tuning_enabled() {
+  const char* e = getenv("VIE_HIP_TUNE");
+  return !(e && !strcmp(e, "0"));
+}
+
+template <typename T> static int plan_tune(vie_fft_plan* p) {
+  const int slot = sizeof(T) == sizeof(double) ? 0 : 1;
+  if (p->tuned[slot]) return 1;
+  if (!tuning_enabled()) {
+    p->tuned[slot] = true;
+    return 1;
+  }
+  TuneKey key;
+  memset(&key, 0, sizeof(key));
+  key = TuneKey{slot, p->g.nx, p->g.ny, p->g.nz, p->g.Px, p->g.Py, p->g.Pz, p->g.pec};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      memcpy(p->var[slot], it->second.data(), 5);
+      p->tuned[slot] = true;
+      return 1;
+    }
+  }
+  const size_t n = 3 * p->nbox();
+  DevBuf xin, yout;
+  if (!xin.alloc(n * sizeof(vie::cx<T>)) || !yout.alloc(n * sizeof(vie::cx<T>))) return 0;
+  hipStream_t st = p->stream;
+  HIP_OK(hipMemsetAsync(xin.p, 0, n * sizeof(vie::cx<T>), st));
+  memset(p->var[slot], 0, 5);
+  vie::VolumeView<T> v = p->view<T>();
+  // a zero input leaves zeros in the workspace, so every pass maps zeros to zeros
+  if (!vie::volume_apply<T>(v, xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st)) return 0;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  const int reps = 10;
+  bool ok = true;
+  for (int pass = 0; pass < 5 && ok; pass++) {
+    int P = pass == 2 ? p->g.Px : (pass == 1 || pass == 3) ? p->g.Py : p->g.Pz;
+    int nv = vie::pass_variants(P);
+    float best = 0;
+    int best_v = 0;
+    for (int var = 0; var < nv && ok; var++) {
+      for (int r = 0; r < 2; r++)
+        ok = ok && vie::pass_launch<T>(v, pass, var, xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st);
+      ok = ok && hipEventRecord(e0, st) == hipSuccess;
+      for (int r = 0; r < reps; r++)
+        ok = ok && vie::pass_launch<T>(v, pass, var, xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st);
+      ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+      float ms = 0;
+      ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+      // a later variant must win by 2% to displace an earlier one
+      if (ok && (var == 0 || ms < 0.98f * best)) {
+        best = ms;
+        best_v = var;
+      }
+    }
+    p->var[slot][pass] = (unsigned char)best_v;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!ok) {
+    memset(p->var[slot], 0, 5);
+    set_error("pass tuning failed");
+    return 0;
+  }
+  p->tuned[slot] = true;
+  std::array<unsigned char, 5> a;
+  memcpy(a.data(), p->var[slot], 5);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = a;
+  return 1;
+}
+
+static int plan_tune_prec(vie_fft_plan* p, int prec) {
+  if (!p) return 1;
+  return prec == VIE_HIP_MIXED ? plan_tune<float>(p) : plan_tune<double>(p);
+}
+
+extern "C" int vie_fft_tune(vie_fft_plan* p, int* variants) {
+  if (!p || !plan_upload_eps(p) || !plan_tune<double>(p) || !plan_tune<float>(p)) return 0;
+  if (variants)
+    for (int i = 0; i < 5; i++) {
+      variants[i] = p->var[0][i];
+      variants[5 + i] = p->var[1][i];
+    }
   return 1;
 }
 
@@ -1099,6 +1211,10 @@ static int job_prepare(dielectric_fft_system* s, SolverWork& w, const vie_gmres_
     rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
     return 0;
   }
+  if (!plan_tune_prec(s->plan, prec) || !plan_tune_prec(s->plan, VIE_HIP_FP64)) {
+    rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
+    return 0;
+  }
   return 1;
 }
 
@@ -1229,7 +1345,8 @@ extern "C" int dielectric_fft_reserve(dielectric_fft_system* s, const vie_gmres_
   if (m > options->maximum_iterations) m = options->maximum_iterations;
   if (m > (size_t)vie::kMaxRestart) return 0;
   // FP64-sized basis serves both precisions
-  if (!ensure_solver(s, s->sw, (int)m, VIE_HIP_FP64) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)))
+  if (!ensure_solver(s, s->sw, (int)m, VIE_HIP_FP64) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)) ||
+      !plan_tune_prec(s->plan, VIE_HIP_FP64) || !plan_tune_prec(s->plan, VIE_HIP_MIXED))
     return 0;
   HIP_OK(hipStreamSynchronize(s->stream));
   return 1;
@@ -1258,7 +1375,7 @@ static int apply_host(dielectric_fft_system* s, const double _Complex* in, doubl
   if (!s || !in || !out) return 0;
   size_t n = s->nsys();
   if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
-      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw))
+      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw) || !plan_tune_prec(s->plan, VIE_HIP_FP64))
     return 0;
   if (!stage_vector(s, in, s->sw.x.as<c64>())) return 0;
   if (precond) {
@@ -1276,7 +1393,7 @@ extern "C" int dielectric_fft_apply_ex(dielectric_fft_system* s, vie_hip_precisi
   if (!s || !in || !out) return 0;
   size_t n = s->nsys();
   if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
-      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw))
+      (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw) || !plan_tune_prec(s->plan, prec))
     return 0;
   if (!stage_vector(s, in, s->sw.x.as<c64>())) return 0;
   vie::ReduceScratch rs;
@@ -1585,7 +1702,8 @@ extern "C" int dielectric_fft_time_apply(dielectric_fft_system* s, vie_hip_preci
                                          double* ms_per_apply, double* pass_ms) {
   if (!s || repeats < 1) return 0;
   size_t n = s->nsys();
-  if (!ensure_vectors(s) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)) || !ensure_scratch(s->sw))
+  if (!ensure_vectors(s) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)) ||
+      !ensure_scratch(s->sw) || !plan_tune_prec(s->plan, prec))
     return 0;
   vie::ReduceScratch rs;
   rs.partial = s->sw.partial.as<c64>();

@@ -14,9 +14,18 @@ template <typename T> struct VolumeView {
   const cx<T>* eps;   // [nbox]
   const cx<T>* tw;    // twiddle tables, x then y then z
   cx<T> self[3];
+  unsigned char var[5] = {0, 0, 0, 0, 0};  // launch-shape variant of each pass
 };
 
-bool max_fft_extent_ok(int P);
+inline bool max_fft_extent_ok(int P) { return P >= 1 && P <= 1024 && (P & (P - 1)) == 0; }
+// Launch-shape variants available to a pass over lines of length P.
+constexpr int kPassVariants = 4;
+inline int pass_variants(int P) { return P >= 16 ? kPassVariants : 1; }
+// One pass (0 Z forward, 1 Y forward, 2 X, 3 Y inverse, 4 Z inverse) at a
+// given variant, for the tuner.
+template <typename T>
+int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin, cx<T>* yout, int precond,
+                hipStream_t s);
 
 // y = eps z - (eps-1) conv(z), z = precond ? D^{-1} xin : xin.  Volume part only.
 // ev (optional, 6 events): recorded before each of the five passes and after the last.

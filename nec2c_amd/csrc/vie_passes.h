@@ -1,4 +1,5 @@
-// Fused five-pass FFT volume operator for gfx950.
+// Fused five-pass FFT volume operator for gfx950 (kernels and launch
+// dispatch; instantiated per precision by vie_passes_f32.hip / _f64.hip).
 //
 // Computes, for the unknown box [3][nx][ny][nz] (component-major, z fastest),
 //   y = eps z - (eps - 1) IFFT( T^ . FFT( pad(z) ) ),   z = D^{-1} v or v,
@@ -13,6 +14,7 @@
 // Only the region that can hold nonzeros is ever read or written, so each
 // pass touches the minimum the pruned pencil pipeline allows.  The 1/(PxPyPz)
 // inverse scaling lives in the spectrum.
+#pragma once
 #include "fft_core.h"
 #include "vie_kernels.h"
 
@@ -43,9 +45,9 @@ template <typename T> __device__ __forceinline__ cx<T> ld(const cx<T>* p) { retu
 // All three components' inputs are loaded into registers before the first
 // transform so their HBM latency overlaps; each transformed component is
 // stored as soon as it is done.
-template <typename T, int PZ, int LW>
-__global__ __launch_bounds__(256) void k_zfwd(OpArgs<T> a) {
-  constexpr int WG = 256, E = LW * PZ, EPT = (E + WG - 1) / WG;
+template <typename T, int PZ, int LW, int WG>
+__global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
+  constexpr int E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
   const int tid = threadIdx.x;
@@ -105,9 +107,9 @@ __global__ __launch_bounds__(256) void k_zfwd(OpArgs<T> a) {
 // ---------------------------------------------------------------- pass 2 / 4
 // Lines are (comp, x, kz) with kz fastest; a workgroup takes LW consecutive
 // lines so each y row it loads is LW contiguous values.
-template <typename T, int PY, int LW, bool INV>
-__global__ __launch_bounds__(256) void k_ypass(OpArgs<T> a) {
-  constexpr int WG = 256, E = LW * PY;
+template <typename T, int PY, int LW, int WG, bool INV>
+__global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
+  constexpr int E = LW * PY;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PY];
   const int tid = threadIdx.x;
@@ -150,9 +152,9 @@ __global__ __launch_bounds__(256) void k_ypass(OpArgs<T> a) {
 // Lines are (ky, kz) with kz fastest.  Forward x FFT of each component is
 // staged through LDS and parked in registers; the 3x3 multiply runs in
 // registers; each product component goes back through LDS for the inverse.
-template <typename T, int PX, int LW>
-__global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
-  constexpr int WG = 256, E = LW * PX, EPT = (E + WG - 1) / WG;
+template <typename T, int PX, int LW, int WG>
+__global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
+  constexpr int E = LW * PX, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PX];
   const int tid = threadIdx.x;
@@ -162,18 +164,17 @@ __global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
   const size_t plane = (size_t)g.Py * g.Pz;
   for (int i = tid; i < PX; i += WG) tw[i] = a.tw[i];
   cx<T> hold[3][EPT], next[EPT];
-  auto load_comp = [&](int comp, cx<T>* dst) {
+  // element f = e * LW + l of component comp (zero beyond the nx physical rows)
+#define VIE_XLOAD(comp, i)                                                                   \
+  do {                                                                                       \
+    const int f_ = tid + (i) * WG, e_ = f_ / LW, l_ = f_ % LW;                               \
+    const long line_ = line0 + l_;                                                           \
+    next[i] = (E % WG == 0 || f_ < E) && line_ < lines && e_ < g.nx                          \
+                  ? a.B[((size_t)(comp) * g.nx + e_) * plane + line_]                         \
+                  : cx<T>{(T)0, (T)0};                                                       \
+  } while (0)
 #pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      const int f = tid + i * WG;
-      const int e = f / LW, l = f % LW;
-      const long line = line0 + l;
-      cx<T> v = {(T)0, (T)0};
-      if ((E % WG == 0 || f < E) && line < lines && e < g.nx) v = a.B[((size_t)comp * g.nx + e) * plane + line];
-      dst[i] = v;
-    }
-  };
-  load_comp(0, next);
+  for (int i = 0; i < EPT; i++) VIE_XLOAD(0, i);
 #pragma unroll
   for (int comp = 0; comp < 3; comp++) {
 #pragma unroll
@@ -182,7 +183,10 @@ __global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
       if (E % WG == 0 || f < E) lds[f] = next[i];  // f = e * LW + l
     }
     __syncthreads();
-    if (comp < 2) load_comp(comp + 1, next);  // in flight during this transform
+    if (comp < 2) {  // next component in flight during this transform
+#pragma unroll
+      for (int i = 0; i < EPT; i++) VIE_XLOAD(comp + 1, i);
+    }
     fft_lines<T, PX, LW, WG, LW, 1, true>(lds, tw, tid);
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
@@ -191,6 +195,7 @@ __global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
     }
     __syncthreads();
   }
+#undef VIE_XLOAD
   // spectral multiply: dst_i = sum_j T^_ij src_j   (vie_fft.c:253-256)
   const int Hx = g.Px / 2 + 1, Hy = g.Py / 2 + 1, Hz = g.Pz / 2 + 1;
   const size_t oct = (size_t)Hx * Hy * Hz;
@@ -239,9 +244,9 @@ __global__ __launch_bounds__(256) void k_xpass(OpArgs<T> a) {
 }
 
 // ---------------------------------------------------------------- pass 5
-template <typename T, int PZ, int LW>
-__global__ __launch_bounds__(256) void k_zinv(OpArgs<T> a) {
-  constexpr int WG = 256, E = LW * PZ, EPT = (E + WG - 1) / WG;
+template <typename T, int PZ, int LW, int WG>
+__global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
+  constexpr int E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
   const int tid = threadIdx.x;
@@ -289,29 +294,46 @@ __global__ __launch_bounds__(256) void k_zinv(OpArgs<T> a) {
 }
 
 // ---------------------------------------------------------------- launch
+// Launch shapes of a pass over lines of length P.  Variant 0 stages the
+// widest tile the LDS budget allows (L0 lines, 256 threads); variants 1 and 2
+// keep the elements per thread and shrink the workgroup to 128 and 64
+// threads (more, smaller workgroups: better fill for small grids); variant 3
+// halves the elements per thread at 256 threads (fewer live registers).
+// A per-geometry timing tuner picks one variant per pass.
+__host__ __device__ constexpr int tile_l0(int E, int P, int cap) { return E / P < cap ? E / P : cap; }
+__host__ __device__ constexpr int tile_lw(int l0, int v) {
+  return v == 0 ? l0 : v == 2 ? (l0 / 4 > 0 ? l0 / 4 : 1) : (l0 / 2 > 0 ? l0 / 2 : 1);
+}
+__host__ __device__ constexpr int tile_wg(int v) { return v == 1 ? 128 : v == 2 ? 64 : 256; }
+constexpr int kZCap = 8, kYCap = 16, kXCap = 16;
+
 template <typename T, int P> struct PassLauncher {
   static constexpr int E = PassShape<T>::E;
-  static constexpr int LWZ = (E / P) < 8 ? (E / P) : 8;
-  static constexpr int LWY = (E / P) < 16 ? (E / P) : 16;
-  static constexpr int XCAP = 16;
-  static constexpr int LWX = (E / P) < XCAP ? (E / P) : XCAP;
-  static void zfwd(const OpArgs<T>& a, hipStream_t s) {
-    long lines = (long)a.g.nx * a.g.ny;
-    hipLaunchKernelGGL((k_zfwd<T, P, LWZ>), dim3((lines + LWZ - 1) / LWZ), dim3(256), 0, s, a);
+  template <int V> static void run(int pass, const OpArgs<T>& a, hipStream_t s) {
+    constexpr int WG = tile_wg(V);
+    constexpr int LZ = tile_lw(tile_l0(E, P, kZCap), V);
+    constexpr int LY = tile_lw(tile_l0(E, P, kYCap), V);
+    constexpr int LX = tile_lw(tile_l0(E, P, kXCap), V);
+    const long zl = (long)a.g.nx * a.g.ny, yl = 3L * a.g.nx * a.g.Pz, xl = (long)a.g.Py * a.g.Pz;
+    switch (pass) {
+      case 0: hipLaunchKernelGGL((k_zfwd<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
+      case 1: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, false>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
+      case 2: hipLaunchKernelGGL((k_xpass<T, P, LX, WG>), dim3((xl + LX - 1) / LX), dim3(WG), 0, s, a); break;
+      case 3: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, true>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
+      default: hipLaunchKernelGGL((k_zinv<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
+    }
   }
-  static void zinv(const OpArgs<T>& a, hipStream_t s) {
-    long lines = (long)a.g.nx * a.g.ny;
-    hipLaunchKernelGGL((k_zinv<T, P, LWZ>), dim3((lines + LWZ - 1) / LWZ), dim3(256), 0, s, a);
-  }
-  static void ypass(const OpArgs<T>& a, bool inv, hipStream_t s) {
-    long lines = 3L * a.g.nx * a.g.Pz;
-    dim3 grid((lines + LWY - 1) / LWY);
-    if (inv) hipLaunchKernelGGL((k_ypass<T, P, LWY, true>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_ypass<T, P, LWY, false>), grid, dim3(256), 0, s, a);
-  }
-  static void xpass(const OpArgs<T>& a, hipStream_t s) {
-    long lines = (long)a.g.Py * a.g.Pz;
-    hipLaunchKernelGGL((k_xpass<T, P, LWX>), dim3((lines + LWX - 1) / LWX), dim3(256), 0, s, a);
+  static void launch(int pass, int v, const OpArgs<T>& a, hipStream_t s) {
+    if constexpr (P >= 16) {
+      switch (v) {
+        case 1: run<1>(pass, a, s); return;
+        case 2: run<2>(pass, a, s); return;
+        case 3: run<3>(pass, a, s); return;
+        default: run<0>(pass, a, s); return;
+      }
+    } else {
+      run<0>(pass, a, s);
+    }
   }
 };
 
@@ -332,10 +354,7 @@ template <typename T, typename F> static bool dispatch_p(int P, F&& f) {
   }
 }
 
-bool max_fft_extent_ok(int P) { return P >= 1 && P <= 1024 && (P & (P - 1)) == 0; }
-
-template <typename T>
-int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, hipStream_t s, hipEvent_t* ev) {
+template <typename T> static OpArgs<T> op_args(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond) {
   OpArgs<T> a;
   a.g = v.g;
   a.xin = xin;
@@ -347,25 +366,30 @@ int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int prec
   a.tw = v.tw;
   for (int i = 0; i < 3; i++) a.self[i] = v.self[i];
   a.precond = precond;
-  // A unit axis pads to P = 1, whose DFT is the identity (zero stages).
-  auto mark = [&](int i) {
-    if (ev) (void)hipEventRecord(ev[i], s);
-  };
-  mark(0);
-  bool ok = dispatch_p<T>(v.g.Pz, [&](auto L) { L.zfwd(a, s); });
-  mark(1);
-  ok = ok && dispatch_p<T>(v.g.Py, [&](auto L) { L.ypass(a, false, s); });
-  mark(2);
-  ok = ok && dispatch_p<T>(v.g.Px, [&](auto L) { L.xpass(a, s); });
-  mark(3);
-  ok = ok && dispatch_p<T>(v.g.Py, [&](auto L) { L.ypass(a, true, s); });
-  mark(4);
-  ok = ok && dispatch_p<T>(v.g.Pz, [&](auto L) { L.zinv(a, s); });
-  mark(5);
+  return a;
+}
+
+static int pass_extent(const Geom& g, int pass) { return pass == 2 ? g.Px : (pass == 1 || pass == 3) ? g.Py : g.Pz; }
+
+template <typename T>
+int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin, cx<T>* yout, int precond,
+                hipStream_t s) {
+  OpArgs<T> a = op_args(v, xin, yout, precond);
+  bool ok = dispatch_p<T>(pass_extent(v.g, pass), [&](auto L) { L.launch(pass, variant, a, s); });
   return ok && hipGetLastError() == hipSuccess;
 }
 
-template int volume_apply<float>(const VolumeView<float>&, const c32*, c32*, int, hipStream_t, hipEvent_t*);
-template int volume_apply<double>(const VolumeView<double>&, const c64*, c64*, int, hipStream_t, hipEvent_t*);
+template <typename T>
+int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, hipStream_t s, hipEvent_t* ev) {
+  OpArgs<T> a = op_args(v, xin, yout, precond);
+  // A unit axis pads to P = 1, whose DFT is the identity (zero stages).
+  bool ok = true;
+  for (int pass = 0; pass < 5 && ok; pass++) {
+    if (ev) (void)hipEventRecord(ev[pass], s);
+    ok = dispatch_p<T>(pass_extent(v.g, pass), [&](auto L) { L.launch(pass, v.var[pass], a, s); });
+  }
+  if (ev) (void)hipEventRecord(ev[5], s);
+  return ok && hipGetLastError() == hipSuccess;
+}
 
 }  // namespace vie

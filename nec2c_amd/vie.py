@@ -124,7 +124,7 @@ EXPORTS = (
     "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_reserve",
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
-    "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release",
+    "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
 )
 
 _LIB = None
@@ -203,6 +203,7 @@ def lib() -> ctypes.CDLL:
         "vie_hip_batch_result": (i, [vp, ctypes.c_longlong, vp, ctypes.POINTER(GmresReport),
                                      ctypes.POINTER(d)]),
         "vie_hip_batch_release": (i, [vp, ctypes.c_longlong]),
+        "vie_fft_tune": (i, [vp, ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -525,6 +526,17 @@ class System:
         if not lib().dielectric_fft_download_solution(self.s, _ptr(x)):
             raise RuntimeError(last_error())
         return x
+
+    def tune(self):
+        """vie_fft_tune on the system's plan: the chosen launch variant of each
+        pass, {"fp64": [5], "fp32": [5]} (None without cells)."""
+        plan = lib().dielectric_fft_plan(self.s)
+        if not plan:
+            return None
+        v = (ctypes.c_int * 10)()
+        if not lib().vie_fft_tune(plan, v):
+            raise RuntimeError(f"tune failed: {last_error()}")
+        return {"fp64": list(v[:5]), "fp32": list(v[5:])}
 
     def time_apply(self, precision=FP64, repeats=20):
         ms = ctypes.c_double()
