@@ -1,11 +1,12 @@
 // Device-resident vector kernels for the restarted GMRES and the conductor
 // coupling blocks.
 //
-// Reductions are deterministic: every block writes its partial sums, and the
-// last block to finish (counted with one device atomic, reset by that same
-// block so no memset is ever needed between launches) adds them in block
-// order.  Dot products and norms accumulate in FP64 whatever the storage
-// precision of the Krylov basis.
+// Reductions are deterministic and atomic-free: every block writes its
+// partial sums (key-major, partial[key * kMaxDotBlocks + block]) and a
+// separate finishing launch adds them in block order, so a result never
+// depends on scheduling or on what else runs on the device.  Dot products
+// and norms accumulate in FP64 whatever the storage precision of the Krylov
+// basis.
 #include "krylov.h"
 #include "vie_kernels.h"
 
