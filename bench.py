@@ -119,6 +119,228 @@ def cpu_baseline(n, pitch, eps, case, iterations_needed, restart, sample_iters=1
     }, None
 
 
+def _case_blocks(case):
+    """Host-side inputs of one batched model (the caller's data)."""
+    from nec2c_amd import workloads as W
+    centers = case.centers()
+    eps = np.full(len(centers), case.eps_r, dtype=np.complex128)
+    C = case.dipole.impedance_matrix(case.k0)
+    B, D = W.coupling_blocks(case.dipole, centers, case.pitch, eps, case.k0)
+    return centers, C, B, D
+
+
+def bench_batch(args):
+    """256 independent models: grids 8-48 per axis (cubes and cuboids), eps_r
+    U[2, 12], 50-500 MHz, a dipole of 11-31 segments with 1-8 feed ports (one
+    right-hand side each), all solved to 1e-8, mixed and FP64, with at most
+    --inflight jobs on the device at once (continuous batching)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from nec2c_amd import vie, serve, workloads as W
+    cases = W.batch_cases(args.batch_count, seed=4, kh=args.batch_kh)
+    opts = dict(restart=args.restart, maximum_iterations=args.max_iterations, relative_tolerance=1e-8)
+    log(f"batch: {len(cases)} models, {sum(len(c.ports) for c in cases)} right-hand sides")
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(16) as ex:
+        inputs = list(ex.map(_case_blocks, cases))
+    t_host = time.perf_counter() - t0
+    systems, jobs, feeds = [], [], []
+    t1 = time.perf_counter()
+    for case, (centers, C, B, D) in zip(cases, inputs):
+        s = vie.System(centers, [vie.MaterialSpec(1, case.eps_r)], case.dipole.segments, case.k0, pitch=case.pitch)
+        s.set_blocks(C=C, B=B, D=D)
+        s.reserve(restart=1, maximum_iterations=1, relative_tolerance=1e-8)  # blocks resident, passes tuned
+        systems.append(s)
+        for port in range(len(case.ports)):
+            b = np.zeros(s.order, dtype=np.complex128)
+            b[: case.dipole.segments] = case.rhs(port)
+            jobs.append((s, b))
+            feeds.append(case.ports[port])
+    t_setup = time.perf_counter() - t1
+    del inputs
+    log(f"batch: host inputs {t_host:.1f} s, device setup (spectra, blocks, tuning) {t_setup:.1f} s")
+    biggest = max(systems, key=lambda q: q.order)
+    out = {}
+    for prec, name in ((vie.MIXED, "mixed"), (vie.FP64, "fp64")):
+        b = vie.Batch()
+        b.reserve(args.inflight, biggest, **opts)
+        res, secs = serve.run_batch(jobs, precision=prec, inflight=args.inflight, batch=b,
+                                    inner_reduction=args.inner_reduction, **opts)
+        b.close()
+        out[name] = (res, secs)
+        log(f"batch {name}: {len(jobs)} solves in {secs:.2f} s "
+            f"({sum(r[0] for r in res)} converged)")
+    rm, tm = out["mixed"]
+    rd, td = out["fp64"]
+    z_err = max(abs(1 / xm[1][f] - 1 / xd[1][f]) / abs(1 / xd[1][f]) for xm, xd, f in zip(rm, rd, feeds))
+    lat_m = np.array([r[3] for r in rm])
+    lat_d = np.array([r[3] for r in rd])
+    result = {
+        "metric": METRIC,
+        "value": len(jobs) / tm,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(jobs),
+        "warmup": 0,
+        "ms_per_step": 1e3 * tm / len(jobs),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 inner (fp32 complex) / c128 outer (fp64 complex)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {len(cases)} models, grids 8-48 per axis (cubes/cuboids), eps_r U[2,12], "
+                        f"50-500 MHz, k0 sqrt(eps_r) h = {args.batch_kh:g}, dipoles of 11-31 segments, 1-8 ports each; "
+                        f"one GMRES({args.restart}) job per port to 1e-8, {args.inflight} jobs in flight",
+            "models": len(cases), "right_hand_sides": len(jobs),
+            "cells_total": int(sum(np.prod(c.shape) for c in cases)),
+        },
+        "fp64_solves_per_s": len(jobs) / td,
+        "speedup_mixed_vs_fp64": td / tm,
+        "p50_ms_mixed": 1e3 * float(np.percentile(lat_m, 50)),
+        "p99_ms_mixed": 1e3 * float(np.percentile(lat_m, 99)),
+        "p50_ms_fp64": 1e3 * float(np.percentile(lat_d, 50)),
+        "p99_ms_fp64": 1e3 * float(np.percentile(lat_d, 99)),
+        "converged_mixed": int(sum(r[0] for r in rm)),
+        "converged_fp64": int(sum(r[0] for r in rd)),
+        "iterations_mixed_total": int(sum(r[2].iterations for r in rm)),
+        "iterations_fp64_total": int(sum(r[2].iterations for r in rd)),
+        "matvecs_per_s_mixed": sum(r[2].operator_applications for r in rm) / tm,
+        "matvecs_per_s_fp64": sum(r[2].operator_applications for r in rd) / td,
+        "residual_max_mixed": max(r[2].relative_residual for r in rm),
+        "residual_max_fp64": max(r[2].relative_residual for r in rd),
+        "port_impedance_rel_err_mixed_vs_fp64": z_err,
+        "setup_seconds": {"host_inputs": t_host, "device": t_setup},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    for s in systems:
+        s.close()
+    return 0
+
+
+def bench_serve(args):
+    """Poisson arrivals (seeded) of frequency points for models with grids
+    16^3-96^3 (log-uniform), random eps_r fields as in the sweep, 1-4 ports, 30%
+    of requests repeating an earlier (grid, frequency, eps) key; mixed precision
+    to 1e-8 under an HBM budget (288 GB or the device's free memory).  The
+    caller-side inputs (eps fields, C/B/D) are generated before the clock
+    starts; spectrum builds on cache misses, uploads and solves are served."""
+    from concurrent.futures import ThreadPoolExecutor
+    from nec2c_amd import vie, serve, workloads as W
+    rng = np.random.default_rng(5)
+    keys, reqs_spec = [], []
+    for _ in range(args.requests):
+        if keys and rng.random() < 0.3:
+            key = keys[int(rng.integers(len(keys)))]
+        else:
+            n = int(round(math.exp(rng.uniform(math.log(16), math.log(96)))))
+            key = (n, round(float(rng.uniform(100e6, 400e6)) / 1e6, 3), int(rng.integers(1 << 30)))
+            keys.append(key)
+        reqs_spec.append((key, int(rng.integers(1, 5))))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, args.requests))
+    pitch = 0.5 / 64
+
+    def model_inputs(key):
+        n, fmhz, seed = key
+        eps = W.random_permittivity((n, n, n), seed).reshape(-1)
+        centers = W.cube_cells(n, pitch)
+        dip = W.Dipole(center=(-0.1, 0.5 * n * pitch, 0.5 * n * pitch))
+        k0 = W.k0_of(fmhz * 1e6)
+        C = dip.impedance_matrix(k0)
+        B, D = W.coupling_blocks(dip, centers, pitch, eps, k0)
+        return key, (eps, centers, dip, k0, C, B, D)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(16) as ex:
+        inputs = dict(ex.map(model_inputs, keys))
+    log(f"serve: {len(keys)} distinct models, host inputs {time.perf_counter() - t0:.1f} s")
+
+    def factory(key):
+        eps, centers, dip, k0, C, B, D = inputs[key]
+
+        def make():
+            s = vie.System(centers, [vie.MaterialSpec(1, 1.5)], dip.segments, k0, pitch=pitch)
+            s.set_permittivities(eps)
+            s.set_blocks(C=C, B=B, D=D)
+            return s
+        return make
+
+    requests = []
+    for rid, ((key, ports), t) in enumerate(zip(reqs_spec, arrivals)):
+        dip = inputs[key][2]
+        segs = rng.choice(dip.segments, size=ports, replace=False)
+        rhs = []
+        for seg in segs:
+            b = np.zeros(dip.segments + 3 * key[0] ** 3, dtype=np.complex128)
+            b[int(seg)] = 1.0
+            rhs.append(b)
+        requests.append(serve.Request(rid, float(t), key, (key[0],) * 3, dip.segments, factory(key), rhs))
+    free, total = vie.memory_info()
+    budget = min(288 * 10 ** 9, int(0.9 * free))
+    srv = serve.Server(budget, precision=vie.MIXED, restart=args.restart, maximum_iterations=args.max_iterations,
+                       inner_reduction=args.inner_reduction)
+    rep = srv.run(requests)
+    srv.close()
+    result = {
+        "metric": METRIC,
+        "value": rep["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": rep["solves"],
+        "warmup": 0,
+        "ms_per_step": 1e3 / rep["solves_per_s"] if rep["solves_per_s"] else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 inner (fp32 complex) / c128 outer (fp64 complex)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"served: {args.requests} Poisson requests ({args.rate:g}/s, seed 5), grids 16^3-96^3 "
+                        "log-uniform, random eps_r fields (as the sweep), 100-400 MHz, 1-4 ports, 30% key reuse, "
+                        f"mixed GMRES({args.restart}) to 1e-8, HBM budget {budget / 1e9:.0f} GB",
+            "distinct_models": len(keys),
+        },
+        "serve": rep,
+        "p50_ms": rep["p50_ms"],
+        "p99_ms": rep["p99_ms"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_matvec(args):
+    """Fused operator applies of the first timed sweep point (FP32 and FP64,
+    resident input, Jacobi fused) and a plain device copy of known size, the
+    calibration kernel for HBM byte counters.  Short by design: counter runs
+    replay every kernel."""
+    from nec2c_amd import vie, workloads as W
+    n = args.size
+    pitch = 0.5 / n
+    eps = W.random_permittivity((n, n, n), seed=2)
+    centers = W.cube_cells(n, pitch)
+    dipole = W.Dipole(center=(-0.1, 0.25, 0.25))
+    f = sweep_frequencies(args.steps + args.warmup)[args.warmup]
+    opts = dict(restart=4, maximum_iterations=4, relative_tolerance=1e-8)
+    s, _ = build_case(n, f, eps, dipole, None, centers, pitch, opts)
+    variants = s.tune()
+    out = {"workload": f"{n}^3 sweep point {f / 1e6:.1f} MHz, fused apply (Jacobi) x {args.matvec_repeats}",
+           "tuned_variants": variants}
+    names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
+    for prec, tag in ((vie.MIXED, "fp32"), (vie.FP64, "fp64")):
+        ms, passes = s.time_apply(prec, args.matvec_repeats)
+        tot, pb = s.apply_bytes(prec)
+        out[tag] = {"ms": ms, "bytes": tot, "pass_ms": dict(zip(names, passes)), "pass_bytes": dict(zip(names, pb)),
+                    "pass_gbs": {k: (b / (t * 1e-3) / 1e9 if t > 0 else None) for k, b, t in zip(names, pb, passes)}}
+    cms = vie.copy_probe(args.copy_bytes, 5)
+    out["copy"] = {"bytes": args.copy_bytes, "ms": cms, "gbs": 2 * args.copy_bytes / (cms * 1e-3) / 1e9}
+    print(json.dumps(out), flush=True)
+    s.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +354,17 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=2000)
     ap.add_argument("--inner-reduction", type=float, default=0.0,
                     help="mixed: FP32 cycle stops at |g| <= max(target, this * beta); 0 = library default")
+    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec"), default="sweep",
+                    help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
+                         "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
+                         "applies of one sweep point plus the copy calibration kernel (for counter runs)")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="matvec: bytes of the calibration copy")
+    ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
+    ap.add_argument("--inflight", type=int, default=96, help="batch: jobs on the device at once")
+    ap.add_argument("--batch-kh", type=float, default=0.1,
+                    help="batch: k0 sqrt(eps_r) h of every model (the workload bounds it by 0.5)")
+    ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: mean arrival rate (requests/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -143,6 +376,12 @@ def main():
     if vie.device_count() < 1:
         log("no HIP device visible")
         return 1
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
+    if args.workload == "matvec":
+        return bench_matvec(args)
 
     n = args.size
     pitch = 0.5 / n

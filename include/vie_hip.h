@@ -52,6 +52,9 @@ const char *vie_hip_last_error(void);
 /* Number of HIP devices visible (0 without a GPU; never initialises one
    when the runtime reports none). */
 int vie_hip_device_count(void);
+/* Free and total memory of the current device (for admission control);
+   0 without a device. */
+int vie_hip_memory_info(size_t *free_bytes, size_t *total_bytes);
 
 /* Bulk per-cell permittivity of a plan (count == cells); each value passes
    the same checks as vie_fft_set_permittivity.  Returns 0 and changes

@@ -77,6 +77,15 @@ extern "C" int vie_hip_device_count(void) {
   return n;
 }
 
+extern "C" int vie_hip_memory_info(size_t* free_bytes, size_t* total_bytes) {
+  if (vie_hip_device_count() <= 0) return 0;
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return 0;
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return 1;
+}
+
 static int ensure_device() {
   int n = vie_hip_device_count();
   if (n <= 0) {
@@ -528,10 +537,7 @@ extern "C" double vie_hip_copy_probe(size_t bytes, int repeats) {
 }
 
 // ------------------------------------------------------------------ system
-struct StatusBlock {
-  double gabs, beta;
-  int breakdown, nonfinite, singular, used, done, pad_;
-};
+using StatusBlock = vie::StepStatus;
 
 // Host-visible landing area of one solve (pinned, written by async copies).
 struct PinnedStatus {
@@ -549,7 +555,12 @@ struct SolverWork {
   int m = 0, prec = -1;
   DevBuf V, x, b, r, ax, state, dots, partial, scal;
   PinnedStatus* pin = nullptr;
+  PinnedStatus* pin_dev = nullptr;  // the same memory as the device addresses it
   hipEvent_t ev_ring[2] = {nullptr, nullptr}, ev_misc = nullptr;
+  // recorded Arnoldi step (matvec + delayed CGS2 step) of this workspace
+  hipGraphExec_t step_exec = nullptr;
+  std::vector<const void*> step_key;
+  bool graph_off = false;
   c64* host_io = nullptr;  // pinned user-layout staging of a batched solve
   size_t host_io_cap = 0;
   // job state
@@ -562,6 +573,7 @@ struct SolverWork {
   double bnorm = 0, target = 0, beta = 0, reduction = 1e-5, t_submit = 0, t_end = 0;
   size_t it0 = 0;
   ~SolverWork() {
+    if (step_exec) (void)hipGraphExecDestroy(step_exec);
     if (pin) (void)hipHostFree(pin);
     if (host_io) (void)hipHostFree(host_io);
     for (auto& e : ev_ring)
@@ -850,11 +862,11 @@ template <> struct Prec<float> {
 // y = A z, z = precond ? M^{-1} x : x, on system vectors [c | 3][nbox].
 template <typename T>
 static int sys_apply(dielectric_fft_system* s, const vie::cx<T>* x, vie::cx<T>* y, int precond,
-                     vie::ReduceScratch& rs, hipEvent_t* ev = nullptr) {
+                     vie::ReduceScratch& rs, hipEvent_t* ev = nullptr, vie::DevIndex di = {}) {
   const size_t c = s->c;
   if (s->plan) {
     vie::VolumeView<T> vv = s->plan->view<T>();
-    if (!vie::volume_apply<T>(vv, x + c, y + c, precond, s->stream, ev)) {
+    if (!vie::volume_apply<T>(vv, x + c, y + c, precond, s->stream, ev, di)) {
       set_error("volume kernels failed to launch");
       return 0;
     }
@@ -872,7 +884,7 @@ static int sys_apply(dielectric_fft_system* s, const vie::cx<T>* x, vie::cx<T>* 
     cv.eps = s->plan ? Prec<T>::eps(s->plan).template as<vie::cx<T>>() : nullptr;
     for (int i = 0; i < 3; i++)
       cv.self[i] = s->plan ? vie::cx<T>{(T)s->plan->self[i].re, (T)s->plan->self[i].im} : vie::cx<T>{0, 0};
-    vie::coupling_apply<T>(cv, x, x + c, y, y + c, precond, rs, s->stream);
+    vie::coupling_apply<T>(cv, x, x + c, y, y + c, precond, rs, s->stream, di);
     if (ev) HIP_OK(hipEventRecord(ev[6], s->stream));
   }
   return hipGetLastError() == hipSuccess;
@@ -892,7 +904,9 @@ static int ensure_scratch(SolverWork& w) {
   if (!w.partial.alloc(vie::kReduceScratchBytes)) return 0;
   if (!w.scal.alloc(16 * sizeof(c64))) return 0;
   if (!w.pin) {
-    HIP_OK(hipHostMalloc((void**)&w.pin, sizeof(PinnedStatus), hipHostMallocDefault));
+    // coherent: the step kernels write the status ring straight into it
+    HIP_OK(hipHostMalloc((void**)&w.pin, sizeof(PinnedStatus), hipHostMallocMapped | hipHostMallocCoherent));
+    HIP_OK(hipHostGetDevicePointer((void**)&w.pin_dev, w.pin, 0));
     for (auto& e : w.ev_ring) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&w.ev_misc, hipEventDisableTiming));
   }
@@ -991,7 +1005,76 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   return 1;
 }
 
-// One Arnoldi step j of the current cycle, followed by a status copy.
+// Replaying a recorded step (VIE_HIP_GRAPH=1) is opt-in: on the workloads
+// measured here the device, not launch issue, bounds the step rate, and the
+// direct launches run the same kernels.
+static bool graphs_enabled() {
+  const char* e = getenv("VIE_HIP_GRAPH");
+  return e && !strcmp(e, "1");
+}
+
+// Everything a recorded step depends on: if any of it changes, record again.
+template <typename T> static std::vector<const void*> step_graph_key(SolverWork& w) {
+  dielectric_fft_system* s = w.sys;
+  std::vector<const void*> k = {s, (const void*)(uintptr_t)sizeof(T), w.V.p, w.partial.p, w.dots.p, w.state.p,
+                                w.pin_dev, (const void*)(uintptr_t)w.n, (const void*)(uintptr_t)w.ldv,
+                                (const void*)(uintptr_t)w.m, s->stream, s->C64.p, s->B64.p, s->D64.p, s->C32.p,
+                                s->B32.p, s->D32.p, s->zc64.p, s->zc32.p, s->bz.p};
+  if (s->plan) {
+    vie_fft_plan* p = s->plan;
+    const void* more[] = {p->work.p, p->spec64.p, p->spec32.p, p->eps64.p, p->eps32.p, p->tw64.p, p->tw32.p};
+    k.insert(k.end(), more, more + 7);
+    const int slot = sizeof(T) == sizeof(double) ? 0 : 1;
+    for (int i = 0; i < 5; i++) k.push_back((const void*)(uintptr_t)p->var[slot][i]);
+  }
+  return k;
+}
+
+// The launch sequence of one matvec Arnoldi step; j is read on the device.
+template <typename T> static int issue_matvec_step(SolverWork& w) {
+  dielectric_fft_system* s = w.sys;
+  vie::ReduceScratch rs;
+  rs.partial = w.partial.as<c64>();
+  vie::cx<T>* V = w.V.as<vie::cx<T>>();
+  vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
+  vie::DevIndex di;
+  di.j = &sd->j;
+  di.stride = w.ldv;
+  di.skip = &sd->done;
+  if (!sys_apply<T>(s, V, V + w.ldv, 1, rs, nullptr, di)) return 0;
+  vie::dcgs2_step<T>(V, w.ldv, true, w.n, sd, rs, w.dots.as<c64>(), Prec<T>::eps_b, w.m, w.pin_dev->ring,
+                     s->stream);
+  return hipGetLastError() == hipSuccess;
+}
+
+// Records the matvec step once per workspace / system / precision.
+template <typename T> static int ensure_step_graph(SolverWork& w) {
+  std::vector<const void*> key = step_graph_key<T>(w);
+  if (w.step_exec && key == w.step_key) return 1;
+  if (w.step_exec) (void)hipGraphExecDestroy(w.step_exec);
+  w.step_exec = nullptr;
+  hipStream_t st = w.sys->stream;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return 0;
+  int ok = issue_matvec_step<T>(w);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (!ok || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return 0;
+  }
+  e = hipGraphInstantiate(&w.step_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    w.step_exec = nullptr;
+    (void)hipGetLastError();
+    return 0;
+  }
+  w.step_key.swap(key);
+  return 1;
+}
+
+// One Arnoldi step j of the current cycle, followed by its status.
 template <typename T> static int job_issue_step(SolverWork& w, int j) {
   dielectric_fft_system* s = w.sys;
   hipStream_t st = s->stream;
@@ -1005,14 +1088,20 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
   vie::cx<T>* vj = V + (size_t)j * ldv;
   vie::cx<T>* wv = V + (size_t)(j + 1) * ldv;
   if (w.dcgs2) {
-    // delayed CGS2: m_eff matvec steps, then one closing step that only
-    // finalises the last column
+    // delayed CGS2: m_eff matvec steps (recorded once, replayed), then one
+    // closing step that only finalises the last column
     if (j < w.m_eff) {
-      if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
       w.rep.operator_applications++;
-      vie::dcgs2_step<T>(V, ldv, j, wv, n, sd, rs, dots1, Prec<T>::eps_b, st);
+      if (!w.graph_off && graphs_enabled()) {
+        if (!ensure_step_graph<T>(w)) w.graph_off = true;  // record failed: launch directly from now on
+      }
+      if (w.step_exec && !w.graph_off && graphs_enabled()) {
+        HIP_OK(hipGraphLaunch(w.step_exec, st));
+      } else if (!issue_matvec_step<T>(w)) {
+        return 0;
+      }
     } else {
-      vie::dcgs2_step<T>(V, ldv, j, (const vie::cx<T>*)nullptr, n, sd, rs, dots1, Prec<T>::eps_b, st);
+      vie::dcgs2_step<T>(V, ldv, false, n, sd, rs, dots1, Prec<T>::eps_b, w.m, w.pin_dev->ring, st);
     }
   } else {
     if (!sys_apply<T>(s, vj, wv, 1, rs)) return 0;
@@ -1025,8 +1114,8 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
     unsigned nb = vie::update<T>(V, ldv, j + 1, dots2, wv, n, 1, rs, st);
     vie::arnoldi_tail(sd, dots1, dots2, dots1 + j + 1, rs, nb, Prec<T>::eps_b, st);
     vie::scale_next<T>(wv, n, sd, st);
+    HIP_OK(hipMemcpyAsync(&w.pin->ring[j & 1], sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
   }
-  HIP_OK(hipMemcpyAsync(&w.pin->ring[j & 1], sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
   HIP_OK(hipEventRecord(w.ev_ring[j & 1], st));
   return 1;
 }

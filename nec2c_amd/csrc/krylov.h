@@ -21,6 +21,7 @@ struct ArnoldiState {
   int pad_;
   // --- working data
   int j, m;
+  int step, pad2_;  // Arnoldi steps issued in this cycle (status ring slot)
   double inv_beta, inv_next;
   double inner_target;  // cycle stops once |g_{j+1}| <= inner_target
   // delayed (DCGS2) orthogonalisation: coefficients of the fused update,
@@ -34,6 +35,12 @@ struct ArnoldiState {
   c64 sn[kMaxRestart];
   c64 y[kMaxRestart];
   c64 H[(kMaxRestart + 1) * kMaxRestart];
+};
+
+// The leading status words of ArnoldiState, as the host reads them.
+struct StepStatus {
+  double gabs, beta;
+  int breakdown, nonfinite, singular, used, done, pad_;
 };
 
 constexpr int kMaxKeys = 320;  // sums per reduction launch (>= 2 kMaxRestart + 3)
@@ -92,12 +99,16 @@ void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, 
 void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, ReduceScratch& rs,
                   unsigned nblk, double breakdown_eps, hipStream_t s);
 void backsolve(ArnoldiState* st, hipStream_t s);
-// Delayed CGS2 Arnoldi step j (slot j holds the once-projected candidate u,
-// slot j+1 holds w = A M^{-1} u, or nothing when w is null: the closing step
-// of a cycle).  One reduction pass, one small solve, one fused update pass.
+// Delayed CGS2 Arnoldi step j = st->j, read on the device (slot j holds the
+// once-projected candidate u, slot j+1 holds w = A M^{-1} u, or nothing
+// without has_w: the closing step of a cycle).  One reduction pass, one small
+// solve, one fused update pass; the step then writes the status block to
+// ring[st->step & 1] (device-visible host memory) and advances j unless the
+// cycle is done.  The launch sequence does not depend on j, so it can be
+// recorded once and replayed for every step.  max_j bounds j (the restart).
 template <typename T>
-void dcgs2_step(cx<T>* Q, size_t ldv, int j, const cx<T>* w, size_t n, ArnoldiState* st, ReduceScratch& rs,
-                c64* dots, double breakdown_eps, hipStream_t s);
+void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, ReduceScratch& rs, c64* dots,
+                double breakdown_eps, int max_j, StepStatus* ring, hipStream_t s);
 void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s);
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s);
 template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st, hipStream_t s);
@@ -109,8 +120,9 @@ void to_box(const c64* user, size_t c, size_t cells, const size_t* map, size_t n
 template <typename T>
 void from_box(const cx<T>* sys, size_t c, size_t cells, const size_t* map, size_t nbox, c64* user, hipStream_t s);
 void precondition_apply(const c64* x, c64* y, size_t n, const JacobiView& J, hipStream_t s);
+// di offsets vc, vv, yc, yv on the device and skips the whole product.
 template <typename T>
 void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv, cx<T>* yc, cx<T>* yv, int precond,
-                    ReduceScratch& rs, hipStream_t s);
+                    ReduceScratch& rs, hipStream_t s, DevIndex di = {});
 
 }  // namespace vie

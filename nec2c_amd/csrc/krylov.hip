@@ -64,12 +64,15 @@ template <typename T> struct DotShape { static constexpr int CPL = 128 / (int)si
 
 template <typename T, bool CONJ, bool JAC>
 __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n,
-                                              int self_norm, c64* partial, int ldp, JacobiLite<T> J) {
+                                              int self_norm, c64* partial, int ldp, JacobiLite<T> J,
+                                              DevIndex di) {
   constexpr int CPL = DotShape<T>::CPL;
   constexpr int WAVES = KWG / 64;
   constexpr int CH = 64 * CPL;
   __shared__ c64 lds[kMaxKeys];
   __shared__ c64 zs[CH];
+  if (di.skipped()) return;
+  w += di.off();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tot = nk + (self_norm ? 1 : 0);
   for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
@@ -132,6 +135,23 @@ __global__ __launch_bounds__(KWG) void k_finish(const c64* partial, int ldp, uns
     s.im += __shfl_xor(s.im, m, 32);
   }
   if (key < nk && sub == 0) out[key] = s;
+}
+
+// k_finish for the delayed-CGS2 reduction, key count read from the state.
+__global__ __launch_bounds__(KWG) void k_finish_dc(const c64* partial, int ldp, unsigned nblk,
+                                                   const ArnoldiState* st, int has_w, c64* out) {
+  if (st->done) return;
+  const int j = st->j, nk = has_w ? 2 * j + 3 : j + 1;
+  const int key = blockIdx.x * (KWG / 32) + (threadIdx.x >> 5);
+  if (key >= nk) return;  // whole 32-lane groups leave together
+  const int sub = threadIdx.x & 31;
+  c64 s = {0, 0};
+  for (unsigned b = sub; b < nblk; b += 32) s = s + partial[(size_t)key * ldp + b];
+  for (int m = 16; m >= 1; m >>= 1) {
+    s.re += __shfl_xor(s.re, m, 32);
+    s.im += __shfl_xor(s.im, m, 32);
+  }
+  if (sub == 0) out[key] = s;
 }
 
 // ---------------------------------------------------------------- update
@@ -311,6 +331,7 @@ __global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_ta
     st->beta = beta;
     st->inv_beta = beta > 0 ? 1.0 / beta : 0.0;
     st->j = 0;
+    st->step = 0;
     st->used = 0;
     st->breakdown = 0;
     st->nonfinite = !isfinite(beta);
@@ -324,15 +345,19 @@ __global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_ta
 //   a_k = q_k^H u, b_k = q_k^H w (k < j), nu = u^H u, c = u^H w, d = w^H w.
 // Layout: [a_0..a_{j-1}, b_0..b_{j-1}, nu, c, d]; without w only [a, nu].
 template <typename T>
-__global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, int j, const cx<T>* u,
-                                                 const cx<T>* w, size_t n, c64* partial, int ldp) {
+__global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, const ArnoldiState* st, int has_w,
+                                                 size_t n, c64* partial, int ldp) {
   constexpr int CPL = 8;  // u, w and q in flight per lane
   constexpr int WAVES = KWG / 64;
   constexpr int CH = 64 * CPL;
   __shared__ c64 lds[kMaxKeys];
   __shared__ c64 us[CH], ws[CH];
+  if (st->done) return;
+  const int j = st->j;
+  const cx<T>* u = Q + (size_t)j * ldv;
+  const cx<T>* w = has_w ? Q + (size_t)(j + 1) * ldv : nullptr;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const bool hw = w != nullptr;
+  const bool hw = has_w != 0;
   const int tot = hw ? 2 * j + 3 : j + 1;
   for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
   const size_t nchunks = (n + CH - 1) / CH;
@@ -410,7 +435,7 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
 //     projections are h = [(b - t)/alpha ; ((c - a^H b)/alpha - t_j)/alpha];
 //  4. next candidate u' = (w - Q_j e - f u)/alpha with s = t + alpha h,
 //     e = s(0:j-1) - (s_j/alpha) a, f = s_j/alpha; q_j = (u - Q_j a)/alpha.
-__global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int j, int has_w, const c64* dots,
+__global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, const c64* dots,
                                                 double breakdown_eps) {
   __shared__ c64 a[kMaxRestart], b[kMaxRestart], t[kMaxRestart + 1], col[kMaxRestart + 1];
   __shared__ c64 sn[kMaxRestart];
@@ -419,6 +444,7 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int j, int has
   __shared__ double alpha_s;
   __shared__ int stop;
   if (st->done) return;
+  const int j = st->j;
   const int m = st->m;
   c64* Hraw = st->Hraw;
   for (int k = threadIdx.x; k < j; k += KWG) {
@@ -527,9 +553,10 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int j, int has
 
 // Fused update of step j: slot j <- (u - Q_j a)/alpha, slot j+1 <- (w - Q_j e - f u)/alpha.
 template <typename T>
-__global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, int j, size_t n, const ArnoldiState* st) {
+__global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t n, const ArnoldiState* st) {
   __shared__ c64 ca[kMaxRestart], ce[kMaxRestart];
   if (st->done) return;
+  const int j = st->j;
   for (int k = threadIdx.x; k < j; k += KWG) {
     ca[k] = st->dc_a[k];
     ce[k] = st->dc_e[k];
@@ -601,6 +628,24 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, int j, 
       }
     }
   }
+}
+
+// End of an Arnoldi step: status block to the host ring slot of this step,
+// then j advances unless the cycle is done (later steps are then no-ops).
+__global__ void k_step_end(ArnoldiState* st, StepStatus* ring) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  StepStatus* r = ring + (st->step & 1);
+  r->gabs = st->gabs;
+  r->beta = st->beta;
+  r->breakdown = st->breakdown;
+  r->nonfinite = st->nonfinite;
+  r->singular = st->singular;
+  r->used = st->used;
+  r->done = st->done;
+  r->pad_ = st->step;
+  __threadfence_system();
+  st->step = st->step + 1;
+  if (!st->done) st->j = st->j + 1;
 }
 
 // ---------------------------------------------------------------- elementwise
@@ -689,13 +734,19 @@ __global__ __launch_bounds__(KWG) void k_norm2(const cx<T>* x, size_t n, c64* pa
 
 // ---------------------------------------------------------------- conductor coupling
 // z_c = v_c / C_ii (Jacobi) or v_c   (dielectric_fft.c:77-81)
-template <typename T> __global__ void k_cond_prep(const cx<T>* vc, const cx<T>* Cm, int c, int precond, cx<T>* zc) {
+template <typename T>
+__global__ void k_cond_prep(const cx<T>* vc, const cx<T>* Cm, int c, int precond, cx<T>* zc, DevIndex di) {
+  if (di.skipped()) return;
+  vc += di.off();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < c) zc[i] = precond ? cdiv(vc[i], Cm[(size_t)i * c + i]) : vc[i];
 }
 
 // y_c = C z_c + (B z_v)   (dielectric_fft.c:56-60)
-template <typename T> __global__ void k_cond_finish(const cx<T>* Cm, const cx<T>* zc, const c64* bz, int c, cx<T>* yc) {
+template <typename T>
+__global__ void k_cond_finish(const cx<T>* Cm, const cx<T>* zc, const c64* bz, int c, cx<T>* yc, DevIndex di) {
+  if (di.skipped()) return;
+  yc += di.off();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c) return;
   c64 s = bz[i];
@@ -708,8 +759,10 @@ template <typename T> __global__ void k_cond_finish(const cx<T>* Cm, const cx<T>
 
 // y_v += sum_j D[j] z_c[j]   (dielectric_fft.c:63-64), D stored column-wise.
 template <typename T>
-__global__ void k_dadd(const cx<T>* D, int c, const cx<T>* zc, cx<T>* yv, size_t nv) {
+__global__ void k_dadd(const cx<T>* D, int c, const cx<T>* zc, cx<T>* yv, size_t nv, DevIndex di) {
   __shared__ cx<T> zs[kMaxConductorsShared];
+  if (di.skipped()) return;
+  yv += di.off();
   for (int j = threadIdx.x; j < c && j < kMaxConductorsShared; j += blockDim.x) zs[j] = zc[j];
   __syncthreads();
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -782,7 +835,7 @@ void multidot(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n, int 
   unsigned nb = dot_blocks<T>(n);
   JacobiLite<T> none{};
   hipLaunchKernelGGL((k_dots<T, true, false>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, w, n, self_norm, rs.partial,
-                     kMaxDotBlocks, none);
+                     kMaxDotBlocks, none, DevIndex{});
   int tot = nk + (self_norm ? 1 : 0);
   hipLaunchKernelGGL(k_finish, dim3((tot + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, tot, dots);
 }
@@ -815,16 +868,20 @@ void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* bef
 void backsolve(ArnoldiState* st, hipStream_t s) { hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(KWG), 0, s, st); }
 
 template <typename T>
-void dcgs2_step(cx<T>* Q, size_t ldv, int j, const cx<T>* w, size_t n, ArnoldiState* st, ReduceScratch& rs,
-                c64* dots, double breakdown_eps, hipStream_t s) {
+void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, ReduceScratch& rs, c64* dots,
+                double breakdown_eps, int max_j, StepStatus* ring, hipStream_t s) {
   size_t nch = (n + 511) / 512;
   unsigned nb = (unsigned)(nch < 1 ? 1 : (nch > kMaxDotBlocks ? kMaxDotBlocks : nch));
-  const cx<T>* u = Q + (size_t)j * ldv;
-  hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, j, u, w, n, rs.partial, kMaxDotBlocks);
-  int tot = w ? 2 * j + 3 : j + 1;
-  hipLaunchKernelGGL(k_finish, dim3((tot + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, tot, dots);
-  hipLaunchKernelGGL(k_dc_mid, dim3(1), dim3(KWG), 0, s, st, j, w ? 1 : 0, dots, breakdown_eps);
-  if (w) hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, j, n, st);
+  const int hw = has_w ? 1 : 0;
+  hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, (const ArnoldiState*)st, hw, n, rs.partial,
+                     kMaxDotBlocks);
+  const int max_keys = 2 * max_j + 3;
+  hipLaunchKernelGGL(k_finish_dc, dim3((max_keys + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb,
+                     (const ArnoldiState*)st, hw, dots);
+  hipLaunchKernelGGL(k_dc_mid, dim3(1), dim3(KWG), 0, s, st, hw, dots, breakdown_eps);
+  if (has_w) hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, n,
+                                (const ArnoldiState*)st);
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, st, ring);
 }
 void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s) {
   hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2, inner_target);
@@ -856,10 +913,10 @@ void precondition_apply(const c64* x, c64* y, size_t n, const JacobiView& J, hip
 
 template <typename T>
 void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv, cx<T>* yc, cx<T>* yv, int precond,
-                    ReduceScratch& rs, hipStream_t s) {
+                    ReduceScratch& rs, hipStream_t s, DevIndex di) {
   const int c = cv.c;
   if (c == 0) return;
-  hipLaunchKernelGGL((k_cond_prep<T>), dim3(gblocks(c)), dim3(256), 0, s, vc, cv.C, c, precond, cv.zc);
+  hipLaunchKernelGGL((k_cond_prep<T>), dim3(gblocks(c)), dim3(256), 0, s, vc, cv.C, c, precond, cv.zc, di);
   if (cv.nv) {
     // B z_v in row chunks of kMaxKeys keys, Jacobi applied to v_v on the fly
     JacobiLite<T> J{cv.nbox, cv.eps, {cv.self[0], cv.self[1], cv.self[2]}};
@@ -869,18 +926,18 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
       const cx<T>* Br = cv.B + (size_t)r0 * cv.nv;
       if (precond)
         hipLaunchKernelGGL((k_dots<T, false, true>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
-                           rs.partial, kMaxDotBlocks, J);
+                           rs.partial, kMaxDotBlocks, J, di);
       else
         hipLaunchKernelGGL((k_dots<T, false, false>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
-                           rs.partial, kMaxDotBlocks, J);
+                           rs.partial, kMaxDotBlocks, J, di);
       hipLaunchKernelGGL(k_finish, dim3((rows + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, rows,
                          cv.bz + r0);
     }
   } else {
     hipLaunchKernelGGL((k_zero<double>), dim3(gblocks(c)), dim3(256), 0, s, cv.bz, (size_t)c);
   }
-  hipLaunchKernelGGL((k_cond_finish<T>), dim3(gblocks(c)), dim3(256), 0, s, cv.C, cv.zc, cv.bz, c, yc);
-  if (cv.nv) hipLaunchKernelGGL((k_dadd<T>), dim3(gblocks(cv.nv)), dim3(256), 0, s, cv.D, c, cv.zc, yv, cv.nv);
+  hipLaunchKernelGGL((k_cond_finish<T>), dim3(gblocks(c)), dim3(256), 0, s, cv.C, cv.zc, cv.bz, c, yc, di);
+  if (cv.nv) hipLaunchKernelGGL((k_dadd<T>), dim3(gblocks(cv.nv)), dim3(256), 0, s, cv.D, c, cv.zc, yv, cv.nv, di);
 }
 
 #define VIE_INST(T)                                                                                              \
@@ -896,9 +953,9 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
   template void to_box<T>(const c64*, size_t, size_t, const size_t*, size_t, cx<T>*, hipStream_t);                \
   template void from_box<T>(const cx<T>*, size_t, size_t, const size_t*, size_t, c64*, hipStream_t);              \
   template void coupling_apply<T>(const CouplingView<T>&, const cx<T>*, const cx<T>*, cx<T>*, cx<T>*, int,         \
-                                  ReduceScratch&, hipStream_t);                                                  \
-  template void dcgs2_step<T>(cx<T>*, size_t, int, const cx<T>*, size_t, ArnoldiState*, ReduceScratch&, c64*,    \
-                              double, hipStream_t);
+                                  ReduceScratch&, hipStream_t, DevIndex);                                        \
+  template void dcgs2_step<T>(cx<T>*, size_t, bool, size_t, ArnoldiState*, ReduceScratch&, c64*, double, int,    \
+                              StepStatus*, hipStream_t);
 VIE_INST(float)
 VIE_INST(double)
 

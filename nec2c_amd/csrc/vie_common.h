@@ -45,6 +45,18 @@ template <typename T> struct PassShape;
 template <> struct PassShape<float> { static constexpr int E = 4096; static constexpr int WG = 256; };
 template <> struct PassShape<double> { static constexpr int E = 2048; static constexpr int WG = 256; };
 
+// Device-side operand indexing, so one launch sequence can be recorded once
+// (hipGraph) and replayed for every Arnoldi step: a kernel given a DevIndex
+// offsets its vector operands by (*j) * stride elements, and returns at once
+// when *skip is nonzero.  A default DevIndex means plain pointers.
+struct DevIndex {
+  const int* j = nullptr;
+  size_t stride = 0;
+  const int* skip = nullptr;
+  __device__ inline size_t off() const { return j ? (size_t)(*j) * stride : 0; }
+  __device__ inline bool skipped() const { return skip && *skip; }
+};
+
 // Geometry of one FFT operator instance (one lattice).
 struct Geom {
   int nx, ny, nz;        // unknown box (physical cells; holes carry eps = 1, x = 0)

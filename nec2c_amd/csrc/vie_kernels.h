@@ -29,9 +29,11 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
 
 // y = eps z - (eps-1) conv(z), z = precond ? D^{-1} xin : xin.  Volume part only.
 // ev (optional, 6 events): recorded before each of the five passes and after the last.
+// di (optional): xin / yout offset by (*di.j) * di.stride on the device, whole
+// apply skipped while *di.skip is set.
 template <typename T>
 int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, hipStream_t s,
-                 hipEvent_t* ev = nullptr);
+                 hipEvent_t* ev = nullptr, DevIndex di = {});
 
 // ---- kernel spectrum (FP64) ----
 struct SpectrumResult {

@@ -31,6 +31,7 @@ template <typename T> struct OpArgs {
   const cx<T>* tw;    // twiddles: x table (Px), then y (Py), then z (Pz)
   cx<T> self[3];      // self-term diagonal N_ii(0)
   int precond;
+  DevIndex di;        // xin / yout offset and skip flag (Arnoldi step replay)
 };
 
 template <typename T> __device__ __forceinline__ cx<T> jacobi_scale(cx<T> v, cx<T> e, cx<T> s) {
@@ -50,6 +51,7 @@ __global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
   constexpr int E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
+  if (a.di.skipped()) return;
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.nx * g.ny;  // (x, y) pencils
@@ -57,6 +59,7 @@ __global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
   const cx<T>* twz = a.tw + g.Px + g.Py;
   for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
   const size_t nbox = g.nbox();
+  const cx<T>* xin = a.xin + a.di.off();
   cx<T> in[3][EPT];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
     for (int comp = 0; comp < 3; comp++) {
       cx<T> v = {(T)0, (T)0};
       if (z >= 0) {
-        v = a.xin[comp * nbox + cell];
+        v = xin[comp * nbox + cell];
         if (a.precond) v = jacobi_scale(v, ev, a.self[comp]);
         if (image && comp != 2) v = cx<T>{-v.re, -v.im};
       }
@@ -112,6 +115,7 @@ __global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
   constexpr int E = LW * PY;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PY];
+  if (a.di.skipped()) return;
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = 3L * g.nx * g.Pz;
@@ -152,11 +156,16 @@ __global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
 // Lines are (ky, kz) with kz fastest.  Forward x FFT of each component is
 // staged through LDS and parked in registers; the 3x3 multiply runs in
 // registers; each product component goes back through LDS for the inverse.
+// A workgroup reads LW consecutive kz of each x row (one 128-byte segment in
+// FP32 at LW = 16).  The octant spectrum (13 MB at 128^3 in FP32) is read by
+// the eight mirror points of each entry from different workgroups; it stays
+// resident in the last-level cache, so those re-reads cost no HBM traffic.
 template <typename T, int PX, int LW, int WG>
 __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   constexpr int E = LW * PX, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PX];
+  if (a.di.skipped()) return;
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.Py * g.Pz;
@@ -249,6 +258,7 @@ __global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
   constexpr int E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
+  if (a.di.skipped()) return;
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.nx * g.ny;
@@ -257,6 +267,8 @@ __global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
   for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
   const size_t nbox = g.nbox();
   const cx<T> one = {(T)1, (T)0};
+  const cx<T>* xin = a.xin + a.di.off();
+  cx<T>* yout = a.yout + a.di.off();
   cx<T> in[3][EPT];
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -283,10 +295,10 @@ __global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
       if (line < lines) {
         size_t cell = (size_t)line * g.nz + z;
         cx<T> e = a.eps[cell];
-        cx<T> v = a.xin[comp * nbox + cell];
+        cx<T> v = xin[comp * nbox + cell];
         if (a.precond) v = jacobi_scale(v, e, a.self[comp]);
         cx<T> d = conj(lds[l * PZ + g.zoff + z]);
-        a.yout[comp * nbox + cell] = e * v - (e - one) * d;
+        yout[comp * nbox + cell] = e * v - (e - one) * d;
       }
     }
     __syncthreads();
@@ -354,8 +366,10 @@ template <typename T, typename F> static bool dispatch_p(int P, F&& f) {
   }
 }
 
-template <typename T> static OpArgs<T> op_args(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond) {
+template <typename T>
+static OpArgs<T> op_args(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, DevIndex di = {}) {
   OpArgs<T> a;
+  a.di = di;
   a.g = v.g;
   a.xin = xin;
   a.yout = yout;
@@ -380,8 +394,9 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
 }
 
 template <typename T>
-int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, hipStream_t s, hipEvent_t* ev) {
-  OpArgs<T> a = op_args(v, xin, yout, precond);
+int volume_apply(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, hipStream_t s, hipEvent_t* ev,
+                 DevIndex di) {
+  OpArgs<T> a = op_args(v, xin, yout, precond, di);
   // A unit axis pads to P = 1, whose DFT is the identity (zero stages).
   bool ok = true;
   for (int pass = 0; pass < 5 && ok; pass++) {

@@ -2,6 +2,7 @@
 #include "vie_passes.h"
 
 namespace vie {
-template int volume_apply<float>(const VolumeView<float>&, const c32*, c32*, int, hipStream_t, hipEvent_t*);
+template int volume_apply<float>(const VolumeView<float>&, const c32*, c32*, int, hipStream_t, hipEvent_t*,
+                                   DevIndex);
 template int pass_launch<float>(const VolumeView<float>&, int, int, const c32*, c32*, int, hipStream_t);
 }  // namespace vie

@@ -1,0 +1,248 @@
+"""Continuous-batching request loop for dielectric VIE solves on one GPU.
+
+A request asks for the solution of one antenna model at one frequency for
+one or more feed ports (one right-hand side each).  The server
+
+* keeps prepared systems (kernel spectrum, permittivity, C/B/D blocks, all in
+  HBM) in a cache keyed by (grid, frequency, permittivity key), so a request
+  for a model already seen skips the spectrum build;
+* admits requests in arrival order while the HBM footprint of the work in
+  flight (cached systems + one pooled Krylov workspace per port) stays under a
+  budget, evicting idle cached systems least-recently-used first;
+* submits every port of an admitted request as a job of one
+  ``vie.Batch`` (the ``vie_hip_batch_*`` C-ABI), so the Arnoldi steps of all
+  requests in flight are interleaved on the device while new requests keep
+  arriving;
+* records per-request latency (arrival -> last port finished) and service time
+  (admission -> last port finished).
+
+The reference has no server: it solves one frequency per forked process,
+one right-hand side at a time (dielectric_wire.c:1296-1304, main.c:586-660).
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Hashable, List, Optional, Sequence
+
+import numpy as np
+
+from . import vie
+
+KRYLOV_SCRATCH = 1024 * 320 * 16  # reduction partials per job (krylov.h kReduceScratchBytes)
+
+
+def system_device_bytes(shape, conductors, pec=False):
+    """Device bytes of a prepared system: octant spectrum (FP64 + FP32), FFT
+    workspace (FP64-sized), permittivity, cross blocks (FP64 + FP32)."""
+    nx, ny, nz = shape
+    kz = 2 * nz + 1 if pec else nz
+    P = [1 << max(0, int(np.ceil(np.log2(max(2 * e - 1, 1))))) for e in (nx, ny, kz)]
+    nbox = nx * ny * nz
+    octant = 6 * (P[0] // 2 + 1) * (P[1] // 2 + 1) * (P[2] // 2 + 1)
+    work = 3 * nx * ny * P[2] + 3 * nx * P[1] * P[2]
+    c, nv = conductors, 3 * nbox
+    blocks = c * c + 2 * c * nv
+    return octant * 24 + work * 16 + nbox * 24 + blocks * 24 + (c + nv) * 16 * 4
+
+
+def job_device_bytes(shape, conductors, restart, precision=vie.MIXED):
+    """Device bytes one in-flight solve holds (Krylov basis, FP64 vectors,
+    reduction scratch); the pooled basis is FP64-sized so it serves either
+    precision."""
+    n = conductors + 3 * int(np.prod(shape))
+    ldv = (n + 63) // 64 * 64
+    return (restart + 1) * ldv * 16 + 4 * n * 16 + KRYLOV_SCRATCH
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                      # seconds after the server clock starts
+    key: Hashable                       # (grid, frequency, permittivity key)
+    shape: tuple
+    conductors: int
+    make_system: Callable[[], "vie.System"]
+    rhs: Sequence[np.ndarray]           # one right-hand side per port
+    # filled by the server
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    cache_hit: bool = False
+    results: list = field(default_factory=list)
+
+
+class Server:
+    def __init__(self, budget_bytes: int, precision=vie.MIXED, restart=120, maximum_iterations=2000,
+                 relative_tolerance=1e-8, inner_reduction=0.0, clock=time.perf_counter):
+        self.budget = int(budget_bytes)
+        self.precision = precision
+        self.inner_reduction = inner_reduction
+        self.opts = dict(restart=restart, maximum_iterations=maximum_iterations,
+                         relative_tolerance=relative_tolerance)
+        self.restart = restart
+        self.batch = vie.Batch()
+        self.cache: "collections.OrderedDict[Hashable, list]" = collections.OrderedDict()  # key -> [system, bytes, users]
+        self.queue: "collections.deque[Request]" = collections.deque()
+        self.inflight = {}       # job id -> (request, port)
+        self.pending = {}        # rid -> ports still running
+        self.used = 0            # bytes: cached systems + in-flight jobs
+        self.done: List[Request] = []
+        self.rejected: List[Request] = []
+        self.clock = clock
+        self.t0 = None
+        self.builds = 0
+        self.build_seconds = 0.0
+
+    # ---------------------------------------------------------------- helpers
+    def now(self):
+        return self.clock() - self.t0
+
+    def _job_bytes(self, req):
+        return job_device_bytes(req.shape, req.conductors, self.restart, self.precision)
+
+    def _footprint(self, req):
+        sys_bytes = 0 if req.key in self.cache else system_device_bytes(req.shape, req.conductors)
+        return sys_bytes + len(req.rhs) * self._job_bytes(req)
+
+    def _evict_for(self, need):
+        for key in list(self.cache):
+            if self.used + need <= self.budget:
+                return True
+            s, nbytes, users = self.cache[key]
+            if users == 0:
+                s.close()
+                del self.cache[key]
+                self.used -= nbytes
+        return self.used + need <= self.budget
+
+    def _admit(self, req):
+        now = self.now()
+        entry = self.cache.get(req.key)
+        if entry is None:
+            t = self.clock()
+            s = req.make_system()
+            self.build_seconds += self.clock() - t
+            self.builds += 1
+            entry = [s, system_device_bytes(req.shape, req.conductors), 0]
+            self.cache[req.key] = entry
+            self.used += entry[1]
+        else:
+            req.cache_hit = True
+            self.cache.move_to_end(req.key)
+        entry[2] += 1
+        req.admitted = now
+        req.results = [None] * len(req.rhs)
+        self.pending[req.rid] = len(req.rhs)
+        for port, b in enumerate(req.rhs):
+            jid = self.batch.submit(entry[0], b, precision=self.precision, inner_reduction=self.inner_reduction,
+                                    **self.opts)
+            self.inflight[jid] = (req, port)
+            self.used += self._job_bytes(req)
+
+    def _collect(self, jid):
+        req, port = self.inflight.pop(jid)
+        ok, x, rep, sec = self.batch.result(jid)
+        req.results[port] = (ok, x, rep, sec)
+        self.used -= self._job_bytes(req)
+        self.pending[req.rid] -= 1
+        if self.pending[req.rid] == 0:
+            del self.pending[req.rid]
+            req.finished = self.now()
+            self.cache[req.key][2] -= 1
+            self.done.append(req)
+
+    # ---------------------------------------------------------------- loop
+    def step(self, arrived: "collections.deque[Request]"):
+        """Admits what fits, advances every job once, collects finished jobs.
+        Returns True when something happened."""
+        busy = False
+        now = self.now()
+        while arrived and arrived[0].arrival <= now:
+            self.queue.append(arrived.popleft())
+        while self.queue:
+            req = self.queue[0]
+            need = self._footprint(req)
+            if self.used + need > self.budget and not self._evict_for(need):
+                if not self.inflight and need > self.budget:
+                    self.rejected.append(self.queue.popleft())  # can never fit
+                    continue
+                break  # wait for memory: arrival order is kept
+            self._admit(self.queue.popleft())
+            busy = True
+        ready, _ = self.batch.progress()
+        for jid in ready:
+            self._collect(jid)
+            busy = True
+        return busy
+
+    def run(self, requests: Sequence[Request]):
+        """Serves `requests` (sorted by arrival) to completion in real time."""
+        arrived = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        total = len(arrived)
+        self.t0 = self.clock()
+        while len(self.done) + len(self.rejected) < total:
+            if not self.step(arrived) and not self.inflight and not self.queue and arrived:
+                wait = arrived[0].arrival - self.now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+        return self.report()
+
+    def report(self):
+        lat = np.array([r.finished - r.arrival for r in self.done])
+        svc = np.array([r.finished - r.admitted for r in self.done])
+        solves = sum(len(r.rhs) for r in self.done)
+        span = max(r.finished for r in self.done) if self.done else 0.0
+        reps = [res[2] for r in self.done for res in r.results]
+        return {
+            "requests": len(self.done),
+            "rejected": len(self.rejected),
+            "solves": solves,
+            "seconds": span,
+            "solves_per_s": solves / span if span else 0.0,
+            "requests_per_s": len(self.done) / span if span else 0.0,
+            "p50_ms": 1e3 * float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)) if len(lat) else None,
+            "service_p50_ms": 1e3 * float(np.percentile(svc, 50)) if len(svc) else None,
+            "service_p99_ms": 1e3 * float(np.percentile(svc, 99)) if len(svc) else None,
+            "cache_hits": sum(r.cache_hit for r in self.done),
+            "system_builds": self.builds,
+            "system_build_seconds": self.build_seconds,
+            "converged": sum(bool(res[0]) for r in self.done for res in r.results),
+            "max_relative_residual": max((rep.relative_residual for rep in reps), default=None),
+            "iterations_total": sum(rep.iterations for rep in reps),
+        }
+
+    def close(self):
+        self.batch.close()
+        for s, _, _ in self.cache.values():
+            s.close()
+        self.cache.clear()
+
+
+def run_batch(jobs, precision=vie.MIXED, inflight=128, inner_reduction=0.0, batch=None, **opts):
+    """Solves a closed batch: `jobs` is a list of (system, rhs).  At most
+    `inflight` jobs are on the device at once; a finished job's slot is refilled
+    at once (continuous batching).  `batch` may be a caller's ``vie.Batch``
+    (e.g. with reserved workspaces).  Returns (results in job order, seconds);
+    each result is (converged, x, report, seconds in flight)."""
+    own = batch is None
+    if own:
+        batch = vie.Batch()
+    try:
+        results = [None] * len(jobs)
+        ids = {}
+        nxt = 0
+        t0 = time.perf_counter()
+        while nxt < len(jobs) or ids:
+            while nxt < len(jobs) and len(ids) < inflight:
+                s, b = jobs[nxt]
+                ids[batch.submit(s, b, precision=precision, inner_reduction=inner_reduction, **opts)] = nxt
+                nxt += 1
+            ready, _ = batch.progress()
+            for jid in ready:
+                results[ids.pop(jid)] = batch.result(jid)
+        return results, time.perf_counter() - t0
+    finally:
+        if own:
+            batch.close()

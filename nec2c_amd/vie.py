@@ -125,6 +125,7 @@ EXPORTS = (
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
+    "vie_hip_memory_info",
 )
 
 _LIB = None
@@ -204,6 +205,7 @@ def lib() -> ctypes.CDLL:
                                      ctypes.POINTER(d)]),
         "vie_hip_batch_release": (i, [vp, ctypes.c_longlong]),
         "vie_fft_tune": (i, [vp, ctypes.POINTER(ctypes.c_int)]),
+        "vie_hip_memory_info": (i, [szp, szp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -219,6 +221,14 @@ def last_error() -> str:
 
 def device_count() -> int:
     return lib().vie_hip_device_count()
+
+
+def memory_info():
+    """(free, total) bytes of the current device."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    if not lib().vie_hip_memory_info(ctypes.byref(f), ctypes.byref(t)):
+        raise RuntimeError("no HIP device")
+    return f.value, t.value
 
 
 def _ptr(a: np.ndarray):
