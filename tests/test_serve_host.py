@@ -1,0 +1,126 @@
+"""Host logic of the serving loop (nec2c_amd/serve.py) on the CPU: arrival-order
+admission under an HBM footprint budget, LRU eviction of idle cached systems,
+cache hits on repeated (grid, frequency, eps) keys, rejection of a request that
+can never fit, and the latency report.  The device batch is replaced by a
+stand-in that finishes each job a fixed number of progress calls after
+submission, so no GPU is touched."""
+import types
+
+import numpy as np
+import pytest
+
+from nec2c_amd import serve
+
+
+class FakeSystem:
+    def __init__(self, key):
+        self.key = key
+        self.closed = False
+
+    def close(self):
+        self.closed = True
+
+
+class FakeBatch:
+    def __init__(self, delay=3):
+        self.delay = delay
+        self.jobs = {}
+        self.next = 1
+        self.in_flight_max = 0
+
+    def submit(self, system, b, **kw):
+        jid = self.next
+        self.next += 1
+        self.jobs[jid] = [system, self.delay]
+        self.in_flight_max = max(self.in_flight_max, len(self.jobs))
+        return jid
+
+    def progress(self):
+        ready = []
+        for jid, j in list(self.jobs.items()):
+            j[1] -= 1
+            if j[1] <= 0:
+                ready.append(jid)
+        return ready, len(self.jobs) - len(ready)
+
+    def result(self, jid):
+        self.jobs.pop(jid)
+        rep = types.SimpleNamespace(relative_residual=5e-9, iterations=10)
+        return True, np.zeros(1), rep, 0.001
+
+    def close(self):
+        pass
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1e-4
+        return self.t
+
+
+def make_server(budget, delay=3):
+    srv = serve.Server(budget, clock=Clock())
+    srv.batch.close()
+    srv.batch = FakeBatch(delay)
+    return srv
+
+
+def request(rid, key, n, ports, built, arrival=0.0):
+    def make():
+        built.append(key)
+        return FakeSystem(key)
+    return serve.Request(rid, arrival, key, (n, n, n), 21, make, [np.zeros(4)] * ports)
+
+
+def test_footprint_model_is_monotone_and_positive():
+    a = serve.system_device_bytes((16, 16, 16), 21)
+    b = serve.system_device_bytes((32, 32, 32), 21)
+    assert 0 < a < b
+    assert serve.job_device_bytes((16, 16, 16), 21, 120) > serve.job_device_bytes((16, 16, 16), 21, 30)
+
+
+def test_cache_hits_and_arrival_order():
+    built = []
+    srv = make_server(budget=10 ** 12)
+    reqs = [request(0, "A", 16, 2, built), request(1, "B", 16, 1, built),
+            request(2, "A", 16, 3, built, arrival=0.0)]
+    rep = srv.run(reqs)
+    assert rep["requests"] == 3 and rep["solves"] == 6
+    assert built == ["A", "B"]          # the repeated key reuses the cached system
+    assert rep["cache_hits"] == 1 and rep["system_builds"] == 2
+    assert [r.rid for r in sorted(srv.done, key=lambda r: r.admitted)] == [0, 1, 2]
+    assert rep["p50_ms"] is not None and rep["p99_ms"] >= rep["p50_ms"]
+
+
+def test_budget_limits_concurrency_and_evicts_idle_systems():
+    built = []
+    n = 16
+    sys_b = serve.system_device_bytes((n,) * 3, 21)
+    one = sys_b + serve.job_device_bytes((n,) * 3, 21, 120)
+    srv = make_server(budget=one + sys_b // 2)   # room for one model and one port at a time
+    reqs = [request(i, f"K{i}", n, 1, built) for i in range(4)]
+    rep = srv.run(reqs)
+    assert rep["requests"] == 4 and rep["rejected"] == 0
+    assert srv.batch.in_flight_max == 1      # never two requests' jobs on the device together
+    assert len(srv.cache) == 1               # idle systems were evicted to make room
+    assert srv.used <= srv.budget
+
+
+def test_request_that_can_never_fit_is_rejected():
+    built = []
+    srv = make_server(budget=1000)
+    rep = srv.run([request(0, "huge", 32, 1, built)])
+    assert rep["rejected"] == 1 and rep["requests"] == 0
+    assert built == []
+
+
+@pytest.mark.parametrize("delay", [1, 5])
+def test_run_batch_keeps_inflight_cap(delay, monkeypatch):
+    fb = FakeBatch(delay)
+    jobs = [(FakeSystem(i), np.zeros(2)) for i in range(10)]
+    res, secs = serve.run_batch(jobs, inflight=3, batch=fb)
+    assert len(res) == 10 and all(r[0] for r in res)
+    assert fb.in_flight_max <= 3
