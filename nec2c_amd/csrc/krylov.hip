@@ -91,7 +91,7 @@ __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int
     auto load_key = [&](int k, cx<T>* dst) {
       const cx<T>* vk = V + (size_t)k * ldv + base + lane;
 #pragma unroll
-      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? vk[e * 64] : cx<T>{0, 0};
+      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? ld_stream(vk + e * 64) : cx<T>{0, 0};
     };
     if (wave < nk) load_key(wave, a);
     for (int k = wave; k < tot; k += WAVES) {
@@ -122,19 +122,27 @@ __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int
   for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + blockIdx.x] = lds[k];
 }
 
-// out[k] = sum_b partial[k * ldp + b] (key-major, contiguous per key),
-// 32 lanes per key, fixed summation order.
-__global__ __launch_bounds__(KWG) void k_finish(const c64* partial, int ldp, unsigned nblk, int nk, c64* out) {
-  const int key = blockIdx.x * (KWG / 32) + (threadIdx.x >> 5);
-  const int sub = threadIdx.x & 31;
-  c64 s = {0, 0};
-  if (key < nk)
-    for (unsigned b = sub; b < nblk; b += 32) s = s + partial[(size_t)key * ldp + b];
-  for (int m = 16; m >= 1; m >>= 1) {
-    s.re += __shfl_xor(s.re, m, 32);
-    s.im += __shfl_xor(s.im, m, 32);
+// out[k] = sum_b partial[k * ldp + b] (key-major, contiguous per key): one
+// wave per key, four loads in flight per lane, fixed summation order.
+__device__ __forceinline__ c64 sum_partials(const c64* p, unsigned nblk, int lane) {
+  c64 s0 = {0, 0}, s1 = {0, 0}, s2 = {0, 0}, s3 = {0, 0};
+  unsigned b = lane;
+  for (; b + 192 < nblk; b += 256) {
+    s0 = s0 + p[b];
+    s1 = s1 + p[b + 64];
+    s2 = s2 + p[b + 128];
+    s3 = s3 + p[b + 192];
   }
-  if (key < nk && sub == 0) out[key] = s;
+  for (; b < nblk; b += 64) s0 = s0 + p[b];
+  return wave_reduce((s0 + s1) + (s2 + s3));
+}
+
+__global__ __launch_bounds__(KWG) void k_finish(const c64* partial, int ldp, unsigned nblk, int nk, c64* out) {
+  const int key = blockIdx.x * (KWG / 64) + (threadIdx.x >> 6);
+  if (key >= nk) return;  // whole waves leave together
+  const int lane = threadIdx.x & 63;
+  const c64 s = sum_partials(partial + (size_t)key * ldp, nblk, lane);
+  if (lane == 0) out[key] = s;
 }
 
 // k_finish for the delayed-CGS2 reduction, key count read from the state.
@@ -142,16 +150,11 @@ __global__ __launch_bounds__(KWG) void k_finish_dc(const c64* partial, int ldp, 
                                                    const ArnoldiState* st, int has_w, c64* out) {
   if (st->done) return;
   const int j = st->j, nk = has_w ? 2 * j + 3 : j + 1;
-  const int key = blockIdx.x * (KWG / 32) + (threadIdx.x >> 5);
-  if (key >= nk) return;  // whole 32-lane groups leave together
-  const int sub = threadIdx.x & 31;
-  c64 s = {0, 0};
-  for (unsigned b = sub; b < nblk; b += 32) s = s + partial[(size_t)key * ldp + b];
-  for (int m = 16; m >= 1; m >>= 1) {
-    s.re += __shfl_xor(s.re, m, 32);
-    s.im += __shfl_xor(s.im, m, 32);
-  }
-  if (sub == 0) out[key] = s;
+  const int key = blockIdx.x * (KWG / 64) + (threadIdx.x >> 6);
+  if (key >= nk) return;
+  const int lane = threadIdx.x & 63;
+  const c64 s = sum_partials(partial + (size_t)key * ldp, nblk, lane);
+  if (lane == 0) out[key] = s;
 }
 
 // ---------------------------------------------------------------- update
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(KWG) void k_update(const cx<T>* V, size_t ldv, int 
 #pragma unroll
         for (int e = 0; e < UEPT; e++) {
           size_t i = base + (size_t)e * KWG;
-          a[q][e] = in[e] ? V[(size_t)(k + q) * ldv + i] : cx<T>{0, 0};
+          a[q][e] = in[e] ? ld_stream(V + (size_t)(k + q) * ldv + i) : cx<T>{0, 0};
         }
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -207,7 +210,7 @@ __global__ __launch_bounds__(KWG) void k_update(const cx<T>* V, size_t ldv, int 
       for (int e = 0; e < UEPT; e++) {
         size_t i = base + (size_t)e * KWG;
         if (in[e]) {
-          cx<T> a = V[(size_t)k * ldv + i];
+          cx<T> a = ld_stream(V + (size_t)k * ldv + i);
           ar[e] -= hr * a.re - hi * a.im;
           ai[e] -= hr * a.im + hi * a.re;
         }
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
     auto load_key = [&](int k, cx<T>* dst) {
       const cx<T>* qk = Q + (size_t)k * ldv + base + lane;
 #pragma unroll
-      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? qk[e * 64] : cx<T>{0, 0};
+      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? ld_stream(qk + e * 64) : cx<T>{0, 0};
     };
     if (wave < j) load_key(wave, q);
     for (int k = wave; k < j; k += WAVES) {
@@ -438,32 +441,45 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
 __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, const c64* dots,
                                                 double breakdown_eps) {
   __shared__ c64 a[kMaxRestart], b[kMaxRestart], t[kMaxRestart + 1], col[kMaxRestart + 1];
+  __shared__ c64 tpart[2][kMaxRestart + 1];
   __shared__ c64 sn[kMaxRestart];
   __shared__ double cs[kMaxRestart];
   __shared__ c64 red[KWG / 64];
+  __shared__ c64 sc[4];        // nu, c, d, g_{j-1}
+  __shared__ double sd[2];     // dc_before, inner_target
   __shared__ double alpha_s;
   __shared__ int stop;
   if (st->done) return;
   const int j = st->j;
-  const int m = st->m;
+  const int m = st->m, ld = m + 1;  // Hraw is column-major: Hraw[l * (m + 1) + k] = row k, column l
+  const int tid = threadIdx.x;
   c64* Hraw = st->Hraw;
-  for (int k = threadIdx.x; k < j; k += KWG) {
+  // every operand in one round of parallel loads
+  for (int k = tid; k < j; k += KWG) {
     a[k] = dots[k];
     if (has_w) b[k] = dots[j + k];
     sn[k] = st->sn[k];
     cs[k] = st->cs[k];
   }
+  if (tid == 0) sc[0] = dots[has_w ? 2 * j : j];
+  if (tid == 1) sc[1] = has_w ? dots[2 * j + 1] : c64{0, 0};
+  if (tid == 2) sc[2] = has_w ? dots[2 * j + 2] : c64{0, 0};
+  if (tid == 3) sc[3] = st->g[j > 0 ? j - 1 : 0];
+  if (tid == 4) sd[0] = st->dc_before;
+  if (tid == 5) sd[1] = st->inner_target;
+  // old column j-1 of Hraw, rows 0..j-1 (contiguous)
+  if (j > 0)
+    for (int k = tid; k < j; k += KWG) col[k] = Hraw[(size_t)(j - 1) * ld + k];
   __syncthreads();
   // |a|^2 and a^H b in parallel
   c64 p2 = {0, 0}, pab = {0, 0};
-  for (int k = threadIdx.x; k < j; k += KWG) {
+  for (int k = tid; k < j; k += KWG) {
     p2.re += a[k].re * a[k].re + a[k].im * a[k].im;
     if (has_w) pab = pab + conj(a[k]) * b[k];
   }
   p2 = block_reduce(p2, red);
-  if (threadIdx.x == 0) {
-    const c64 nu = dots[has_w ? 2 * j : j];
-    double al2 = nu.re - p2.re;
+  if (tid == 0) {
+    double al2 = sc[0].re - p2.re;
     alpha_s = al2 > 0 ? sqrt(al2) : 0.0;
   }
   __syncthreads();
@@ -471,24 +487,25 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   const double alpha = alpha_s;
   // column j-1 becomes final: Hraw(0:j, j-1) += a, Hraw(j, j-1) = alpha
   if (j > 0) {
-    for (int k = threadIdx.x; k <= j; k += KWG) {
-      c64 v = k < j ? Hraw[k * m + j - 1] + a[k] : c64{alpha, 0};
-      Hraw[k * m + j - 1] = v;
+    for (int k = tid; k <= j; k += KWG) {
+      c64 v = k < j ? col[k] + a[k] : c64{alpha, 0};
+      Hraw[(size_t)(j - 1) * ld + k] = v;
       col[k] = v;
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     stop = 0;
     if (!isfinite(alpha)) {
       st->nonfinite = 1;
       stop = 1;
     } else if (j == 0) {
       // q_0 = u / alpha: the least-squares right-hand side is beta * alpha
-      st->g[0] = alpha * st->g[0];
+      st->g[0] = alpha * sc[3];
     } else {
       const int c = j - 1;
-      st->breakdown = alpha <= breakdown_eps * st->dc_before;
+      const int brk = alpha <= breakdown_eps * sd[0];
+      st->breakdown = brk;
       for (int k = 0; k < c; k++) {
         c64 f = col[k], gg = col[k + 1];
         col[k] = cs[k] * f + sn[k] * gg;
@@ -501,54 +518,74 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
       st->sn[c] = z;
       col[c] = cr * col[c] + z * col[c + 1];
       col[c + 1] = c64{0, 0};
-      c64 gc = st->g[c];
-      st->g[c + 1] = c64{0, 0} - conj(z) * gc;
+      const c64 gc = sc[3];
+      const c64 gn = c64{0, 0} - conj(z) * gc;
+      st->g[c + 1] = gn;
       st->g[c] = cr * gc;
-      st->gabs = hypot(st->g[c + 1].re, st->g[c + 1].im);
+      const double gabs = hypot(gn.re, gn.im);
+      st->gabs = gabs;
       st->used = j;
       st->j = j;
-      if (st->breakdown || st->gabs <= st->inner_target || j >= m) stop = 1;
+      if (brk || gabs <= sd[1] || j >= m) stop = 1;
     }
     if (!has_w) stop = 1;
     if (stop) st->done = 1;
   }
   __syncthreads();
   if (j > 0)
-    for (int k = threadIdx.x; k <= j; k += KWG) st->H[k * m + j - 1] = col[k];
+    for (int k = tid; k <= j; k += KWG) st->H[k * m + j - 1] = col[k];
   if (stop) return;
-  // t = Hraw(0:j, 0:j-1) a  (Hessenberg: row k only has columns >= k-1)
-  for (int k = threadIdx.x; k <= j; k += KWG) {
-    c64 s = {0, 0};
-    for (int l = (k > 0 ? k - 1 : 0); l < j; l++) s = s + Hraw[k * m + l] * a[l];
-    t[k] = s;
+  // t = Hraw(0:j, 0:j-1) a (Hessenberg: row k only has columns >= k-1).
+  // Threads own rows (consecutive rows are contiguous in a column, so each
+  // column read is coalesced); the two halves of the block split the columns.
+  {
+    const int half = tid >> 7, r0 = tid & 127;
+    const int mid = j / 2;
+    for (int k = r0; k <= j; k += 128) {
+      const int lo = half == 0 ? (k > 0 ? k - 1 : 0) : (mid > k - 1 ? mid : (k > 0 ? k - 1 : 0));
+      const int hi = half == 0 ? (mid < j ? mid : j) : j;
+      c64 s0 = {0, 0}, s1 = {0, 0};
+      int l = lo;
+      for (; l + 1 < hi; l += 2) {
+        const c64 h0 = Hraw[(size_t)l * ld + k], h1 = Hraw[(size_t)(l + 1) * ld + k];
+        s0 = s0 + h0 * a[l];
+        s1 = s1 + h1 * a[l + 1];
+      }
+      if (l < hi) s0 = s0 + Hraw[(size_t)l * ld + k] * a[l];
+      tpart[half][k] = s0 + s1;
+    }
   }
+  __syncthreads();
+  for (int k = tid; k <= j; k += KWG) t[k] = tpart[0][k] + tpart[1][k];
   __syncthreads();
   const double ia = 1.0 / alpha;
   __shared__ c64 f_s;
-  if (threadIdx.x == 0) {
-    const c64 cc = dots[2 * j + 1], dd = dots[2 * j + 2];
+  if (tid == 0) {
+    const c64 cc = sc[1], dd = sc[2];
     c64 hj = ia * (ia * (cc - pab) - t[j]);
     c64 sj = t[j] + alpha * hj;  // f = s_j / alpha
     f_s = ia * sj;
     st->dc_alpha = alpha;
     st->dc_f_re = f_s.re;
     st->dc_f_im = f_s.im;
-    st->dc_before = sqrt(dd.re) * ia;
-    Hraw[j * m + j] = hj;
-    st->nonfinite = !(isfinite(hj.re) && isfinite(hj.im) && isfinite(st->dc_before));
-    if (st->nonfinite) st->done = 1;
+    const double before = sqrt(dd.re) * ia;
+    st->dc_before = before;
+    Hraw[(size_t)j * ld + j] = hj;
+    const int bad = !(isfinite(hj.re) && isfinite(hj.im) && isfinite(before));
+    st->nonfinite = bad;
+    if (bad) st->done = 1;
   }
   __syncthreads();
   const c64 f = f_s;
-  for (int k = threadIdx.x; k < j; k += KWG) {
+  for (int k = tid; k < j; k += KWG) {
     c64 hk = ia * (b[k] - t[k]);
-    Hraw[k * m + j] = hk;
+    Hraw[(size_t)j * ld + k] = hk;
     c64 sk = t[k] + alpha * hk;
     st->dc_a[k] = a[k];
     st->dc_e[k] = sk - f * a[k];
   }
   // rows below the new column stay zero
-  for (int k = j + 1 + threadIdx.x; k <= m; k += KWG) Hraw[k * m + j] = c64{0, 0};
+  for (int k = j + 1 + tid; k <= m; k += KWG) Hraw[(size_t)j * ld + k] = c64{0, 0};
 }
 
 // Fused update of step j: slot j <- (u - Q_j a)/alpha, slot j+1 <- (w - Q_j e - f u)/alpha.
@@ -589,7 +626,7 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
 #pragma unroll
         for (int e = 0; e < UEPT; e++) {
           size_t i = base + (size_t)e * KWG;
-          q[p][e] = in[e] ? Q[(size_t)(k + p) * ldv + i] : cx<T>{0, 0};
+          q[p][e] = in[e] ? ld_stream(Q + (size_t)(k + p) * ldv + i) : cx<T>{0, 0};
         }
 #pragma unroll
       for (int p = 0; p < 4; p++) {
@@ -610,7 +647,7 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
       for (int e = 0; e < UEPT; e++) {
         size_t i = base + (size_t)e * KWG;
         if (in[e]) {
-          cx<T> qq = Q[(size_t)k * ldv + i];
+          cx<T> qq = ld_stream(Q + (size_t)k * ldv + i);
           double qr = qq.re, qi = qq.im;
           ur[e] -= ar * qr - ai * qi;
           ui[e] -= ar * qi + ai * qr;
@@ -697,7 +734,7 @@ __global__ void k_combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const Ar
   if (i >= n) return;
   c64 s = {0, 0};
   for (int k = 0; k < used; k++) {
-    cx<T> a = V[(size_t)k * ldv + i];
+    cx<T> a = ld_stream(V + (size_t)k * ldv + i);
     s = s + ys[k] * c64{(double)a.re, (double)a.im};
   }
   if (precond) s = jacobi_inv(J, i, s);
@@ -769,7 +806,7 @@ __global__ void k_dadd(const cx<T>* D, int c, const cx<T>* zc, cx<T>* yv, size_t
   if (i >= nv) return;
   c64 s = {(double)yv[i].re, (double)yv[i].im};
   for (int j = 0; j < c; j++) {
-    cx<T> a = D[(size_t)j * nv + i], z = j < kMaxConductorsShared ? zs[j] : zc[j];
+    cx<T> a = ld_stream(D + (size_t)j * nv + i), z = j < kMaxConductorsShared ? zs[j] : zc[j];
     s = s + c64{a.re, a.im} * c64{z.re, z.im};
   }
   yv[i] = cx<T>{(T)s.re, (T)s.im};
@@ -837,7 +874,7 @@ void multidot(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n, int 
   hipLaunchKernelGGL((k_dots<T, true, false>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, w, n, self_norm, rs.partial,
                      kMaxDotBlocks, none, DevIndex{});
   int tot = nk + (self_norm ? 1 : 0);
-  hipLaunchKernelGGL(k_finish, dim3((tot + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, tot, dots);
+  hipLaunchKernelGGL(k_finish, dim3((tot + 3) / 4), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, tot, dots);
 }
 template <typename T>
 unsigned update(const cx<T>* V, size_t ldv, int nk, const c64* h, cx<T>* w, size_t n, int want_norm,
@@ -876,7 +913,7 @@ void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, Re
   hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, (const ArnoldiState*)st, hw, n, rs.partial,
                      kMaxDotBlocks);
   const int max_keys = 2 * max_j + 3;
-  hipLaunchKernelGGL(k_finish_dc, dim3((max_keys + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb,
+  hipLaunchKernelGGL(k_finish_dc, dim3((max_keys + 3) / 4), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb,
                      (const ArnoldiState*)st, hw, dots);
   hipLaunchKernelGGL(k_dc_mid, dim3(1), dim3(KWG), 0, s, st, hw, dots, breakdown_eps);
   if (has_w) hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, n,
@@ -930,7 +967,7 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
       else
         hipLaunchKernelGGL((k_dots<T, false, false>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
                            rs.partial, kMaxDotBlocks, J, di);
-      hipLaunchKernelGGL(k_finish, dim3((rows + 7) / 8), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, rows,
+      hipLaunchKernelGGL(k_finish, dim3((rows + 3) / 4), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, rows,
                          cv.bz + r0);
     }
   } else {

@@ -39,6 +39,15 @@ template <typename To, typename From> __host__ __device__ inline cx<To> cast(cx<
   return {(To)a.re, (To)a.im};
 }
 
+// Streaming loads for data read once per pass (Krylov basis, coupling
+// blocks): the nontemporal hint keeps them from displacing reused data in
+// the caches and lets the memory pipeline stream at a higher rate.
+template <typename T> __device__ __forceinline__ cx<T> ld_stream(const cx<T>* p) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  return cx<T>{v.x, v.y};
+}
+
 // Elements staged in LDS per workgroup by the pencil-FFT passes, and the
 // workgroup size.  FP32: 4096 x 8 B = 32 KiB; FP64: 2048 x 16 B = 32 KiB.
 template <typename T> struct PassShape;
