@@ -591,7 +591,7 @@ struct dielectric_fft_system {
   bool blocks_dirty = true;
   hipStream_t stream = nullptr;  // the plan's stream, or an own one without cells
   bool own_stream = false;
-  DevBuf C64, B64, D64, C32, B32, D32, zc64, zc32, bz, cdiag;
+  DevBuf C64, B64, D64, C32, B32, D32, bz, cdiag;
   SolverWork sw;
   bool rhs_loaded = false;
   ~dielectric_fft_system() {
@@ -811,7 +811,7 @@ static int upload_blocks(dielectric_fft_system* s) {
           D[r * nv + bi] = s->blocks[c * (c + v) + ui * c + r];
         }
       }
-    if (!(s->C64.alloc(c * c * 16) && s->C32.alloc(c * c * 8) && s->zc64.alloc(c * 16) && s->zc32.alloc(c * 8) &&
+    if (!(s->C64.alloc(c * c * 16) && s->C32.alloc(c * c * 8) &&
           s->bz.alloc(c * 16) && s->cdiag.alloc(c * 16)))
       return 0;
     std::vector<c64> cd(c);
@@ -846,7 +846,6 @@ template <> struct Prec<double> {
   static DevBuf& C(dielectric_fft_system* s) { return s->C64; }
   static DevBuf& B(dielectric_fft_system* s) { return s->B64; }
   static DevBuf& D(dielectric_fft_system* s) { return s->D64; }
-  static DevBuf& zc(dielectric_fft_system* s) { return s->zc64; }
   static DevBuf& eps(vie_fft_plan* p) { return p->eps64; }
   static constexpr double eps_b = 32 * 2.220446049250313e-16;
 };
@@ -854,7 +853,6 @@ template <> struct Prec<float> {
   static DevBuf& C(dielectric_fft_system* s) { return s->C32; }
   static DevBuf& B(dielectric_fft_system* s) { return s->B32; }
   static DevBuf& D(dielectric_fft_system* s) { return s->D32; }
-  static DevBuf& zc(dielectric_fft_system* s) { return s->zc32; }
   static DevBuf& eps(vie_fft_plan* p) { return p->eps32; }
   static constexpr double eps_b = 32 * 1.1920928955078125e-07;
 };
@@ -879,7 +877,6 @@ static int sys_apply(dielectric_fft_system* s, const vie::cx<T>* x, vie::cx<T>* 
     cv.C = Prec<T>::C(s).template as<vie::cx<T>>();
     cv.B = Prec<T>::B(s).template as<vie::cx<T>>();
     cv.D = Prec<T>::D(s).template as<vie::cx<T>>();
-    cv.zc = Prec<T>::zc(s).template as<vie::cx<T>>();
     cv.bz = s->bz.as<c64>();
     cv.eps = s->plan ? Prec<T>::eps(s->plan).template as<vie::cx<T>>() : nullptr;
     for (int i = 0; i < 3; i++)
@@ -1019,7 +1016,7 @@ template <typename T> static std::vector<const void*> step_graph_key(SolverWork&
   std::vector<const void*> k = {s, (const void*)(uintptr_t)sizeof(T), w.V.p, w.partial.p, w.dots.p, w.state.p,
                                 w.pin_dev, (const void*)(uintptr_t)w.n, (const void*)(uintptr_t)w.ldv,
                                 (const void*)(uintptr_t)w.m, s->stream, s->C64.p, s->B64.p, s->D64.p, s->C32.p,
-                                s->B32.p, s->D32.p, s->zc64.p, s->zc32.p, s->bz.p};
+                                s->B32.p, s->D32.p, s->bz.p};
   if (s->plan) {
     vie_fft_plan* p = s->plan;
     const void* more[] = {p->work.p, p->spec64.p, p->spec32.p, p->eps64.p, p->eps32.p, p->tw64.p, p->tw32.p};

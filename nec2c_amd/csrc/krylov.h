@@ -76,8 +76,7 @@ template <typename T> struct CouplingView {
   const cx<T>* C;  // c x c row-major
   const cx<T>* B;  // c rows of nv (volume -> conductor)
   const cx<T>* D;  // c columns of nv (conductor -> volume)
-  cx<T>* zc;       // c scratch
-  c64* bz;         // c scratch (B z_v)
+  c64* bz;         // c scratch (B z_v when c exceeds one reduction launch)
   const cx<T>* eps;
   cx<T> self[3];
 };

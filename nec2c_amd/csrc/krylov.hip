@@ -770,46 +770,66 @@ __global__ __launch_bounds__(KWG) void k_norm2(const cx<T>* x, size_t n, c64* pa
 }
 
 // ---------------------------------------------------------------- conductor coupling
-// z_c = v_c / C_ii (Jacobi) or v_c   (dielectric_fft.c:77-81)
+// z_c = v_c / C_ii (Jacobi) or v_c   (dielectric_fft.c:77-81), for conductors
+// j0 .. j0 + cnt - 1, into LDS.
 template <typename T>
-__global__ void k_cond_prep(const cx<T>* vc, const cx<T>* Cm, int c, int precond, cx<T>* zc, DevIndex di) {
-  if (di.skipped()) return;
-  vc += di.off();
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < c) zc[i] = precond ? cdiv(vc[i], Cm[(size_t)i * c + i]) : vc[i];
-}
-
-// y_c = C z_c + (B z_v)   (dielectric_fft.c:56-60)
-template <typename T>
-__global__ void k_cond_finish(const cx<T>* Cm, const cx<T>* zc, const c64* bz, int c, cx<T>* yc, DevIndex di) {
-  if (di.skipped()) return;
-  yc += di.off();
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c) return;
-  c64 s = bz[i];
-  for (int j = 0; j < c; j++) {
-    cx<T> a = Cm[(size_t)i * c + j], z = zc[j];
-    s = s + c64{a.re, a.im} * c64{z.re, z.im};
+__device__ __forceinline__ void stage_zc(cx<T>* zs, const cx<T>* vc, const cx<T>* Cm, int c, int j0, int cnt,
+                                         int precond) {
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const int jj = j0 + q;
+    zs[q] = precond ? cdiv(vc[jj], Cm[(size_t)jj * c + jj]) : vc[jj];
   }
-  yc[i] = cx<T>{(T)s.re, (T)s.im};
 }
 
-// y_v += sum_j D[j] z_c[j]   (dielectric_fft.c:63-64), D stored column-wise.
+// y_v += sum_j D[j] z_c[j]   (dielectric_fft.c:63-64), D stored column-wise,
+// and, in the first ceil(c / 4) blocks (one wave per conductor row),
+// y_c = C z_c + B z_v   (dielectric_fft.c:56-60) with B z_v summed from the
+// partials of the preceding B reduction (or taken from bz when given).
 template <typename T>
-__global__ void k_dadd(const cx<T>* D, int c, const cx<T>* zc, cx<T>* yv, size_t nv, DevIndex di) {
+__global__ __launch_bounds__(256) void k_couple_out(const cx<T>* vc, const cx<T>* Cm, int c, int precond,
+                                                    const cx<T>* D, cx<T>* yv, size_t nv, const c64* bpart, int ldp,
+                                                    unsigned nblk, const c64* bz, cx<T>* yc, DevIndex di) {
   __shared__ cx<T> zs[kMaxConductorsShared];
   if (di.skipped()) return;
+  vc += di.off();
   yv += di.off();
-  for (int j = threadIdx.x; j < c && j < kMaxConductorsShared; j += blockDim.x) zs[j] = zc[j];
-  __syncthreads();
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  c64 s = {(double)yv[i].re, (double)yv[i].im};
-  for (int j = 0; j < c; j++) {
-    cx<T> a = ld_stream(D + (size_t)j * nv + i), z = j < kMaxConductorsShared ? zs[j] : zc[j];
-    s = s + c64{a.re, a.im} * c64{z.re, z.im};
+  yc += di.off();
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if ((size_t)blockIdx.x * 256 < nv) {
+    c64 s = {0, 0};
+    if (i < nv) s = c64{(double)yv[i].re, (double)yv[i].im};
+    for (int j0 = 0; j0 < c; j0 += kMaxConductorsShared) {
+      const int cnt = c - j0 < kMaxConductorsShared ? c - j0 : kMaxConductorsShared;
+      __syncthreads();
+      stage_zc(zs, vc, Cm, c, j0, cnt, precond);
+      __syncthreads();
+      if (i < nv)
+        for (int j = 0; j < cnt; j++) {
+          const cx<T> a = ld_stream(D + (size_t)(j0 + j) * nv + i), z = zs[j];
+          s = s + c64{a.re, a.im} * c64{z.re, z.im};
+        }
+    }
+    if (i < nv) yv[i] = cx<T>{(T)s.re, (T)s.im};
   }
-  yv[i] = cx<T>{(T)s.re, (T)s.im};
+  if ((int)blockIdx.x * 4 < c) {
+    const int lane = threadIdx.x & 63, row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    c64 acc = {0, 0};
+    if (row < c) acc = bz ? bz[row] : sum_partials(bpart + (size_t)row * ldp, nblk, lane);
+    c64 part = {0, 0};
+    for (int j0 = 0; j0 < c; j0 += kMaxConductorsShared) {
+      const int cnt = c - j0 < kMaxConductorsShared ? c - j0 : kMaxConductorsShared;
+      __syncthreads();
+      stage_zc(zs, vc, Cm, c, j0, cnt, precond);
+      __syncthreads();
+      if (row < c)
+        for (int j = lane; j < cnt; j += 64) {
+          const cx<T> a = Cm[(size_t)row * c + j0 + j], z = zs[j];
+          part = part + c64{a.re, a.im} * c64{z.re, z.im};
+        }
+    }
+    part = wave_reduce(part);
+    if (row < c && lane == 0) yc[row] = cx<T>{(T)(acc.re + part.re), (T)(acc.im + part.im)};
+  }
 }
 
 // ---------------------------------------------------------------- layout
@@ -953,11 +973,14 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
                     ReduceScratch& rs, hipStream_t s, DevIndex di) {
   const int c = cv.c;
   if (c == 0) return;
-  hipLaunchKernelGGL((k_cond_prep<T>), dim3(gblocks(c)), dim3(256), 0, s, vc, cv.C, c, precond, cv.zc, di);
+  // B z_v: one reduction launch for up to kMaxKeys conductor rows, whose
+  // partials the output kernel sums; more rows go through bz chunk by chunk
+  const c64* bpart = rs.partial;
+  const c64* bz = nullptr;
+  unsigned nb = 0;
   if (cv.nv) {
-    // B z_v in row chunks of kMaxKeys keys, Jacobi applied to v_v on the fly
     JacobiLite<T> J{cv.nbox, cv.eps, {cv.self[0], cv.self[1], cv.self[2]}};
-    unsigned nb = dot_blocks<T>(cv.nv);
+    nb = dot_blocks<T>(cv.nv);
     for (int r0 = 0; r0 < c; r0 += kMaxKeys) {
       int rows = c - r0 < kMaxKeys ? c - r0 : kMaxKeys;
       const cx<T>* Br = cv.B + (size_t)r0 * cv.nv;
@@ -967,14 +990,19 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
       else
         hipLaunchKernelGGL((k_dots<T, false, false>), dim3(nb), dim3(KWG), 0, s, Br, cv.nv, rows, vv, cv.nv, 0,
                            rs.partial, kMaxDotBlocks, J, di);
-      hipLaunchKernelGGL(k_finish, dim3((rows + 3) / 4), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, rows,
-                         cv.bz + r0);
+      if (c > kMaxKeys) {
+        hipLaunchKernelGGL(k_finish, dim3((rows + 3) / 4), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb, rows,
+                           cv.bz + r0);
+        bz = cv.bz;
+      }
     }
   } else {
     hipLaunchKernelGGL((k_zero<double>), dim3(gblocks(c)), dim3(256), 0, s, cv.bz, (size_t)c);
+    bz = cv.bz;
   }
-  hipLaunchKernelGGL((k_cond_finish<T>), dim3(gblocks(c)), dim3(256), 0, s, cv.C, cv.zc, cv.bz, c, yc, di);
-  if (cv.nv) hipLaunchKernelGGL((k_dadd<T>), dim3(gblocks(cv.nv)), dim3(256), 0, s, cv.D, c, cv.zc, yv, cv.nv, di);
+  const size_t vblocks = (cv.nv + 255) / 256, cblocks = ((size_t)c + 3) / 4;
+  hipLaunchKernelGGL((k_couple_out<T>), dim3((unsigned)(vblocks > cblocks ? vblocks : cblocks)), dim3(256), 0, s, vc,
+                     cv.C, c, precond, cv.D, yv, cv.nv, bpart, kMaxDotBlocks, nb, bz, yc, di);
 }
 
 #define VIE_INST(T)                                                                                              \

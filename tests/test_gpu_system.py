@@ -75,6 +75,33 @@ def test_coupled_apply_matches_reference(ref):
     assert rel(s.apply_ex(x, vie.FP64, jacobi=True), rc.apply(pz)) < 1e-11
 
 
+@pytest.mark.parametrize("c", [3, 300, 330])
+def test_many_conductors_coupling(ref, c):
+    """Conductor counts below, at and above one B-reduction launch (320 rows):
+    the products with random C/B/D against the reference's block loops."""
+    n = 4
+    pitch = 0.5 / 64
+    eps = W.random_permittivity((n, n, n), 7).reshape(-1)
+    centers = W.cube_cells(n, pitch)
+    k0 = W.k0_of(200e6)
+    rng = np.random.default_rng(c)
+    nv = 3 * n ** 3
+    C = rng.standard_normal((c, c)) + 1j * rng.standard_normal((c, c)) + 50 * np.eye(c)
+    B = 1e-3 * (rng.standard_normal((c, nv)) + 1j * rng.standard_normal((c, nv)))
+    D = 1e-3 * (rng.standard_normal((nv, c)) + 1j * rng.standard_normal((nv, c)))
+    s = vie.System(centers, [vie.MaterialSpec(1, 1.5)], c, k0, pitch=pitch)
+    s.set_permittivities(eps)
+    s.set_blocks(C=C, B=B, D=D)
+    rp, rc = ref_coupled(ref, n, pitch, k0, eps, (C, B, D))
+    x = rng.standard_normal(s.order) + 1j * rng.standard_normal(s.order)
+    yr = rc.apply(x)
+    assert rel(s.apply(x), yr) < 1e-11
+    assert rel(s.apply_ex(x, vie.MIXED), yr) < 2e-6
+    pz = s.precondition(x)
+    assert rel(s.apply_ex(x, vie.FP64, jacobi=True), rc.apply(pz)) < 1e-11
+    s.close()
+
+
 @pytest.mark.parametrize("precision", [vie.FP64, vie.MIXED])
 def test_coupled_solve(ref, precision):
     n = 6
