@@ -32,6 +32,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_64cube_matvec.json")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by its reading on a copy of known
+    size); None when that kernel variant was not profiled."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    name = kernel.split(" ")[0].replace(" ", "")
+    for k, v in table.items():
+        if k.replace(" ", "") == name:
+            return v["bytes_per_launch"]
+    return None
 METRIC = "VIE matvecs/s & solves/s at 1e-8 residual, mixed vs FP64 GPU; p50/p99 latency"
 
 
@@ -334,6 +351,12 @@ def bench_matvec(args):
         tot, pb = s.apply_bytes(prec)
         out[tag] = {"ms": ms, "bytes": tot, "pass_ms": dict(zip(names, passes)), "pass_bytes": dict(zip(names, pb)),
                     "pass_gbs": {k: (b / (t * 1e-3) / 1e9 if t > 0 else None) for k, b, t in zip(names, pb, passes)}}
+    if args.arnoldi_column > 0:
+        s.reserve(restart=min(args.arnoldi_column + 1, vie.KMAX_RESTART), maximum_iterations=1000,
+                  relative_tolerance=1e-8)
+        out["arnoldi"] = {tag: s.time_arnoldi(prec, args.arnoldi_column, args.matvec_repeats)
+                          for prec, tag in ((vie.MIXED, "fp32"), (vie.FP64, "fp64"))}
+        out["arnoldi"]["j"] = args.arnoldi_column
     cms = vie.copy_probe(args.copy_bytes, 5)
     out["copy"] = {"bytes": args.copy_bytes, "ms": cms, "gbs": 2 * args.copy_bytes / (cms * 1e-3) / 1e9}
     print(json.dumps(out), flush=True)
@@ -359,6 +382,8 @@ def main():
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
                          "applies of one sweep point plus the copy calibration kernel (for counter runs)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="matvec: bytes of the calibration copy")
+    ap.add_argument("--arnoldi-column", type=int, default=0,
+                    help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
     ap.add_argument("--inflight", type=int, default=96, help="batch: jobs on the device at once")
     ap.add_argument("--batch-kh", type=float, default=0.1,
@@ -420,7 +445,11 @@ def main():
         return time.perf_counter() - t, lat, reps
 
     log("timing mixed-precision solves")
+    for q in timed:
+        q.solve_stats(reset=True)
     t_mixed, lat_mixed, rep_mixed = run(vie.MIXED)
+    col_stats = [q.solve_stats(reset=True) for q in timed]
+    mean_col = sum(m * c for m, c in col_stats) / max(sum(c for _, c in col_stats), 1)
     x_mixed = [s.download_solution() for s in timed]
     log("timing FP64 solves")
     t_fp64, lat_fp64, rep_fp64 = run(vie.FP64)
@@ -442,6 +471,16 @@ def main():
     names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
     dom = int(np.argmax(pass32[:5]))
     achieved = pbytes32[dom] / (pass32[dom] * 1e-3) / 1e9
+
+    # the two streaming kernels of an Arnoldi step (the solve's dominant
+    # kernels), timed at the mean basis column of the timed mixed solves
+    arn = {}
+    jbar = max(1, int(round(mean_col)))
+    for prec, tag in ((vie.MIXED, "fp32"), (vie.FP64, "fp64")):
+        t = s0.time_arnoldi(prec, jbar, args.matvec_repeats)
+        t["gbs_reduce"] = t["bytes_reduce"] / (t["ms_reduce"] * 1e-3) / 1e9
+        t["gbs_update"] = t["bytes_update"] / (t["ms_update"] * 1e-3) / 1e9
+        arn[tag] = t
 
     iters_mixed = [r.iterations for r in rep_mixed]
     apps_mixed = sum(r.operator_applications for r in rep_mixed)
@@ -495,6 +534,16 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
+            "kernel": f"k_dots_dc<float> (delayed-CGS2 reduction; basis column {jbar}, the mean over the "
+                      f"timed mixed solves; ({jbar} + 2) x {s0.order} complex float read per launch)",
+            "achieved": arn["fp32"]["gbs_reduce"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": arn["fp32"]["gbs_reduce"] / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "roofline_matvec": {
+            "bound": "hbm",
             "kernel": pass_kernel_name(dom, 2 * n, variants["fp32"][dom]),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -503,7 +552,14 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": None,
+        "arnoldi": {"j": jbar, "mean_column": mean_col, **arn},
     }
+    traffic = profiled_traffic(result["roofline_matvec"]["kernel"])
+    if traffic is not None:
+        result["roofline_matvec"]["traffic"] = traffic
+    traffic = profiled_traffic(f"k_dots_dc<float>@{jbar}")
+    if traffic is not None:
+        result["roofline"]["traffic"] = traffic
     if not args.no_cpu:
         log("CPU baseline: reference FFT-GMRES on one frequency point")
         cb, _ = cpu_baseline(n, pitch, eps, cases[args.warmup], rep_fp64[0].iterations, args.restart)

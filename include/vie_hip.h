@@ -97,6 +97,17 @@ int dielectric_fft_download_solution(dielectric_fft_system *system, VIE_CDOUBLE 
    the five volume passes and the conductor coupling. */
 int dielectric_fft_time_apply(dielectric_fft_system *system, vie_hip_precision precision, int repeats,
                               double *ms_per_apply, double *pass_ms);
+/* Mean basis column j of the matvec Arnoldi steps that the system's own
+   resident solver issued since the last call with reset != 0, and their count. */
+int dielectric_fft_solve_stats(dielectric_fft_system *system, double *mean_column, unsigned long long *steps,
+                               int reset);
+/* Times the two streaming kernels of one delayed-CGS2 Arnoldi step at basis
+   column j (the reduction and the fused update), averaged over `repeats`
+   launches, with HIP events on the system's stream; overwrites the resident
+   solver workspace.  bytes_* receive the algorithmic bytes per launch:
+   (j + 2) n S and (j + 4) n S for n unknowns of S bytes. */
+int dielectric_fft_time_arnoldi(dielectric_fft_system *system, vie_hip_precision precision, int j, int repeats,
+                                double *ms_reduce, double *ms_update, double *bytes_reduce, double *bytes_update);
 /* Algorithmic HBM bytes of one fused apply at a precision (per pass and total),
    from the pruned five-pass model: see the implementation for the formula. */
 int dielectric_fft_apply_bytes(const dielectric_fft_system *system, vie_hip_precision precision,

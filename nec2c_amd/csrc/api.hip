@@ -570,6 +570,8 @@ struct SolverWork {
   vie_gmres_report rep{};
   int phase = 0, m_eff = 0, steps = 0, issued = 0, checked = 0, result = 0;
   bool after_cycle = false, breakdown = false, cycle_done = false, dcgs2 = true;
+  double col_sum = 0;       // sum of basis columns j over the matvec Arnoldi steps issued
+  unsigned long long col_steps = 0;
   double bnorm = 0, target = 0, beta = 0, reduction = 1e-5, t_submit = 0, t_end = 0;
   size_t it0 = 0;
   ~SolverWork() {
@@ -1089,6 +1091,8 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
     // closing step that only finalises the last column
     if (j < w.m_eff) {
       w.rep.operator_applications++;
+      w.col_sum += j;
+      w.col_steps++;
       if (!w.graph_off && graphs_enabled()) {
         if (!ensure_step_graph<T>(w)) w.graph_off = true;  // record failed: launch directly from now on
       }
@@ -1846,6 +1850,49 @@ extern "C" int dielectric_fft_time_apply(dielectric_fft_system* s, vie_hip_preci
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   return ok;
+}
+
+// Mean basis column of the matvec Arnoldi steps the system's own solver
+// issued since the last reset (the column sets the reduction's width).
+extern "C" int dielectric_fft_solve_stats(dielectric_fft_system* s, double* mean_column, unsigned long long* steps,
+                                          int reset) {
+  if (!s) return 0;
+  if (mean_column) *mean_column = s->sw.col_steps ? s->sw.col_sum / (double)s->sw.col_steps : 0.0;
+  if (steps) *steps = s->sw.col_steps;
+  if (reset) {
+    s->sw.col_sum = 0;
+    s->sw.col_steps = 0;
+  }
+  return 1;
+}
+
+// Times the two streaming kernels of a delayed-CGS2 step at column j on the
+// system's resident solver workspace (which it overwrites).  Bytes per launch
+// with n unknowns and S bytes a value:
+//   reduction : columns 0..j-1, u and w read                   (j + 2) n S
+//   update    : columns 0..j-1 read, u and w read and written  (j + 4) n S
+extern "C" int dielectric_fft_time_arnoldi(dielectric_fft_system* s, vie_hip_precision prec, int j, int repeats,
+                                           double* ms_reduce, double* ms_update, double* bytes_reduce,
+                                           double* bytes_update) {
+  if (!s || j < 1 || j + 1 > vie::kMaxRestart || repeats < 1) return 0;
+  const int p = prec == VIE_HIP_MIXED ? VIE_HIP_MIXED : VIE_HIP_FP64;
+  SolverWork& w = s->sw;
+  if (!ensure_solver(s, w, std::max(w.prec >= 0 ? w.m : 0, j + 1), p)) return 0;
+  vie::ReduceScratch rs;
+  rs.partial = w.partial.as<c64>();
+  float a = 0, b = 0;
+  int ok = p == VIE_HIP_MIXED
+               ? vie::dcgs2_time<float>(w.V.as<c32>(), w.ldv, w.n, w.state.as<vie::ArnoldiState>(), rs, j, repeats,
+                                        s->stream, &a, &b)
+               : vie::dcgs2_time<double>(w.V.as<c64>(), w.ldv, w.n, w.state.as<vie::ArnoldiState>(), rs, j, repeats,
+                                         s->stream, &a, &b);
+  if (!ok) return 0;
+  const double S = p == VIE_HIP_MIXED ? 8.0 : 16.0, n = (double)w.n;
+  if (ms_reduce) *ms_reduce = a;
+  if (ms_update) *ms_update = b;
+  if (bytes_reduce) *bytes_reduce = (j + 2) * n * S;
+  if (bytes_update) *bytes_update = (j + 4) * n * S;
+  return 1;
 }
 
 // Algorithmic bytes per fused apply (pruned five-pass pipeline):

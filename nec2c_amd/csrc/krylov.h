@@ -108,6 +108,11 @@ void backsolve(ArnoldiState* st, hipStream_t s);
 template <typename T>
 void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, ReduceScratch& rs, c64* dots,
                 double breakdown_eps, int max_j, StepStatus* ring, hipStream_t s);
+// Times the reduction and the fused update of a delayed-CGS2 step at column
+// j (averages over reps launches); overwrites the state and columns 0..j+1.
+template <typename T>
+int dcgs2_time(cx<T>* Q, size_t ldv, size_t n, ArnoldiState* st, ReduceScratch& rs, int j, int reps, hipStream_t s,
+               float* ms_reduce, float* ms_update);
 void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s);
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s);
 template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st, hipStream_t s);

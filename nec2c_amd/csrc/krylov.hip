@@ -940,6 +940,61 @@ void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, Re
                                 (const ArnoldiState*)st);
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, st, ring);
 }
+// Timing hook: a state at column j with finite update coefficients, and a
+// basis whose first j + 2 columns hold a constant, so the two streaming
+// kernels of a delayed-CGS2 step can be timed on their own.
+__global__ void k_time_state(ArnoldiState* st, int j) {
+  for (int k = threadIdx.x; k < j; k += blockDim.x) {
+    st->dc_a[k] = c64{1e-3, 0};
+    st->dc_e[k] = c64{1e-3, 0};
+  }
+  if (threadIdx.x == 0) {
+    st->j = j;
+    st->done = 0;
+    st->dc_alpha = 1.0;
+    st->dc_f_re = 0.0;
+    st->dc_f_im = 0.0;
+  }
+}
+
+template <typename T> __global__ void k_fill(cx<T>* x, size_t n, T v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = cx<T>{v, (T)0};
+}
+
+template <typename T>
+int dcgs2_time(cx<T>* Q, size_t ldv, size_t n, ArnoldiState* st, ReduceScratch& rs, int j, int reps, hipStream_t s,
+               float* ms_reduce, float* ms_update) {
+  size_t nch = (n + 511) / 512;
+  unsigned nb = (unsigned)(nch < 1 ? 1 : (nch > kMaxDotBlocks ? kMaxDotBlocks : nch));
+  const size_t cells = (size_t)(j + 2) * ldv;
+  hipLaunchKernelGGL((k_fill<T>), dim3(gblocks(cells)), dim3(256), 0, s, Q, cells, (T)1e-3);
+  hipLaunchKernelGGL(k_time_state, dim3(1), dim3(256), 0, s, st, j);
+  hipEvent_t e[3];
+  for (auto& x : e)
+    if (hipEventCreate(&x) != hipSuccess) return 0;
+  for (int r = 0; r < 2; r++) {
+    hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, (const ArnoldiState*)st, 1, n, rs.partial,
+                       kMaxDotBlocks);
+    hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, n, (const ArnoldiState*)st);
+  }
+  (void)hipEventRecord(e[0], s);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, (const ArnoldiState*)st, 1, n, rs.partial,
+                       kMaxDotBlocks);
+  (void)hipEventRecord(e[1], s);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, n, (const ArnoldiState*)st);
+  (void)hipEventRecord(e[2], s);
+  int ok = hipEventSynchronize(e[2]) == hipSuccess;
+  ok = ok && hipEventElapsedTime(ms_reduce, e[0], e[1]) == hipSuccess &&
+       hipEventElapsedTime(ms_update, e[1], e[2]) == hipSuccess;
+  *ms_reduce /= reps;
+  *ms_update /= reps;
+  for (auto& x : e) (void)hipEventDestroy(x);
+  return ok && hipGetLastError() == hipSuccess;
+}
+
 void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s) {
   hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2, inner_target);
 }
@@ -1020,7 +1075,9 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
   template void coupling_apply<T>(const CouplingView<T>&, const cx<T>*, const cx<T>*, cx<T>*, cx<T>*, int,         \
                                   ReduceScratch&, hipStream_t, DevIndex);                                        \
   template void dcgs2_step<T>(cx<T>*, size_t, bool, size_t, ArnoldiState*, ReduceScratch&, c64*, double, int,    \
-                              StepStatus*, hipStream_t);
+                              StepStatus*, hipStream_t);                                                         \
+  template int dcgs2_time<T>(cx<T>*, size_t, size_t, ArnoldiState*, ReduceScratch&, int, int, hipStream_t, float*, \
+                             float*);
 VIE_INST(float)
 VIE_INST(double)
 

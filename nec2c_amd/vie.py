@@ -125,7 +125,7 @@ EXPORTS = (
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
-    "vie_hip_memory_info",
+    "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
 )
 
 _LIB = None
@@ -206,6 +206,8 @@ def lib() -> ctypes.CDLL:
         "vie_hip_batch_release": (i, [vp, ctypes.c_longlong]),
         "vie_fft_tune": (i, [vp, ctypes.POINTER(ctypes.c_int)]),
         "vie_hip_memory_info": (i, [szp, szp]),
+        "dielectric_fft_time_arnoldi": (i, [vp, i, i, i] + [ctypes.POINTER(d)] * 4),
+        "dielectric_fft_solve_stats": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_ulonglong), i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -554,6 +556,20 @@ class System:
         if not lib().dielectric_fft_time_apply(self.s, precision, repeats, ctypes.byref(ms), passes):
             raise RuntimeError(last_error())
         return ms.value, list(passes)
+
+    def solve_stats(self, reset=False):
+        """(mean basis column, matvec Arnoldi steps) of the resident solves since the last reset."""
+        mj, st = ctypes.c_double(), ctypes.c_ulonglong()
+        lib().dielectric_fft_solve_stats(self.s, ctypes.byref(mj), ctypes.byref(st), int(reset))
+        return mj.value, st.value
+
+    def time_arnoldi(self, precision=FP64, j=60, repeats=20):
+        """(ms, bytes) of the delayed-CGS2 reduction and fused update at column j."""
+        v = [ctypes.c_double() for _ in range(4)]
+        if not lib().dielectric_fft_time_arnoldi(self.s, precision, j, repeats, *[ctypes.byref(x) for x in v]):
+            raise RuntimeError(f"time_arnoldi failed: {last_error()}")
+        return {"ms_reduce": v[0].value, "ms_update": v[1].value, "bytes_reduce": v[2].value,
+                "bytes_update": v[3].value}
 
     def apply_bytes(self, precision=FP64):
         passes = (ctypes.c_double * 6)()
