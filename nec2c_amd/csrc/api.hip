@@ -1004,6 +1004,16 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   return 1;
 }
 
+// VIE_HIP_TRACE=1: one line per GMRES cycle on stderr (diagnostics).
+static bool trace_enabled() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("VIE_HIP_TRACE");
+    on = e && !strcmp(e, "1");
+  }
+  return on == 1;
+}
+
 // Replaying a recorded step (VIE_HIP_GRAPH=1) is opt-in: on the workloads
 // measured here the device, not launch issue, bounds the step rate, and the
 // direct launches run the same kernels.
@@ -1166,6 +1176,11 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           return 1;
         }
         rep.relative_residual = w.bnorm > 0 ? w.beta / w.bnorm : w.beta;
+        if (trace_enabled() && w.after_cycle)
+          fprintf(stderr, "[vie_hip] %s cycle end: iterations %zu (cycle %zu) true relative residual %.3e "
+                          "estimate %.3e\n", mixed ? "fp32" : "fp64", (size_t)rep.iterations,
+                  (size_t)(rep.iterations - w.it0), rep.relative_residual,
+                  w.bnorm > 0 ? w.pin->fin.gabs / w.bnorm : 0.0);
         if (w.after_cycle) {
           if (w.breakdown && w.beta > w.target) {
             job_finish(w, VIE_ITERATIVE_BREAKDOWN, 0);

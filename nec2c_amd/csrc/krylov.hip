@@ -727,6 +727,7 @@ template <typename T>
 __global__ void k_combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const ArnoldiState* st, JacobiView J,
                           int precond) {
   __shared__ c64 ys[kMaxRestart];
+  if (st->singular) return;  // no least-squares solution: x stays as it is
   const int used = st->used;
   for (int k = threadIdx.x; k < used; k += blockDim.x) ys[k] = st->y[k];
   __syncthreads();
