@@ -328,6 +328,94 @@ def bench_serve(args):
     return 0
 
 
+def bench_slab(args):
+    """The 128^3 slab: a homogeneous block (eps_r 6, sigma 0.01 S/m, 1 cm
+    voxels, 1.28 m edge) driven by a 4-element Yagi (reflector 0.50 m, driven
+    0.47 m, directors 0.44 m and 0.43 m, 21 segments each, 0.2 m spacing) 0.1 m
+    from its x = 0 face, at 200 MHz; one mixed and one FP64 solve to 1e-8
+    after a warm-up solve of each.  The reference's FFT path has no real-ground
+    kernel (dielectric_ground.c:28-29), so the slab is in free space here."""
+    from nec2c_amd import vie, workloads as W
+    n = args.slab_size
+    h = 0.01
+    f = 200e6
+    k0 = W.k0_of(f)
+    edge = n * h
+    yc, zc = 0.5 * edge, 0.5 * edge
+    yagi = W.WireArray(tuple((-0.1, yc + (i - 1.5) * 0.2, zc, L) for i, L in enumerate((0.50, 0.47, 0.44, 0.43))),
+                       segments_per_wire=21, radius=1e-3, feed_wire=1)
+    opts = dict(restart=args.restart, maximum_iterations=args.max_iterations, relative_tolerance=1e-8)
+    t0 = time.perf_counter()
+    centers = W.cube_cells(n, h)
+    s = vie.System(centers, [vie.MaterialSpec(1, 6.0, 0.0, 0.01)], yagi.segments, k0, pitch=h)
+    t_prep = time.perf_counter() - t0
+    log(f"slab: {n}^3 prepared (spectrum) in {t_prep:.1f} s; building Yagi blocks (c = {yagi.segments})")
+    eps_cell = 6.0 - 1j * 0.01 / (k0 * 299792458.0 * W.EPS0)
+    C = yagi.impedance_matrix(k0)
+    B, D = W.coupling_blocks(yagi, centers, h, eps_cell, k0, threads=16)
+    s.set_blocks(C=C, B=B, D=D)
+    del B, D, centers
+    b = np.zeros(s.order, dtype=np.complex128)
+    b[: yagi.segments] = yagi.rhs()
+    s.upload_rhs(b)
+    s.reserve(**opts)
+    log(f"slab: ready in {time.perf_counter() - t0:.1f} s")
+    res = {}
+    for prec, tag in ((vie.MIXED, "mixed"), (vie.FP64, "fp64")):
+        s.solve_resident(prec, args.inner_reduction, **opts)  # warm-up
+        vie.synchronize()
+        t = time.perf_counter()
+        ok, rep = s.solve_resident(prec, args.inner_reduction, **opts)
+        vie.synchronize()
+        dt = time.perf_counter() - t
+        res[tag] = (ok, rep, dt, s.download_solution())
+        log(f"slab {tag}: {'converged' if ok else 'NOT converged'} in {rep.iterations} iterations, {dt:.3f} s, "
+            f"relative residual {rep.relative_residual:.2e}")
+    ms32, p32 = s.time_apply(vie.MIXED, args.matvec_repeats)
+    ms64, p64 = s.time_apply(vie.FP64, args.matvec_repeats)
+    by32, pb32 = s.apply_bytes(vie.MIXED)
+    by64, _ = s.apply_bytes(vie.FP64)
+    fi = yagi.feed_index()
+    zm, zd = 1 / res["mixed"][3][fi], 1 / res["fp64"][3][fi]
+    names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
+    result = {
+        "metric": METRIC,
+        "value": 1.0 / res["mixed"][2],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": 1e3 * res["mixed"][2],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 inner (fp32 complex) / c128 outer (fp64 complex)",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 slab (eps_r 6, 0.01 S/m, h 1 cm) in free space + 4-element 84-segment Yagi "
+                               f"0.1 m away, 200 MHz, GMRES({args.restart}) to 1e-8",
+                   "unknowns": s.order, "conductors": yagi.segments, "padded": [2 * n, 2 * n, 2 * n]},
+        "fp64_solves_per_s": 1.0 / res["fp64"][2],
+        "speedup_mixed_vs_fp64": res["fp64"][2] / res["mixed"][2],
+        "iterations_mixed": res["mixed"][1].iterations,
+        "iterations_fp64": res["fp64"][1].iterations,
+        "residual_mixed": res["mixed"][1].relative_residual,
+        "residual_fp64": res["fp64"][1].relative_residual,
+        "converged": [bool(res["mixed"][0]), bool(res["fp64"][0])],
+        "feed_impedance_ohm_fp64": str(complex(np.round(zd, 6))),
+        "feed_impedance_rel_err_mixed_vs_fp64": float(abs(zm - zd) / abs(zd)),
+        "matvec": {"ms_mixed": ms32, "ms_fp64": ms64, "bytes_mixed": by32, "bytes_fp64": by64,
+                   "effective_gbs_mixed": by32 / (ms32 * 1e-3) / 1e9, "effective_gbs_fp64": by64 / (ms64 * 1e-3) / 1e9,
+                   "pass_ms_mixed": dict(zip(names, p32)), "pass_ms_fp64": dict(zip(names, p64)),
+                   "pass_bytes_mixed": dict(zip(names, pb32))},
+        "setup_seconds": {"prepare": t_prep},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    s.close()
+    return 0
+
+
 def bench_matvec(args):
     """Fused operator applies of the first timed sweep point (FP32 and FP64,
     resident input, Jacobi fused) and a plain device copy of known size, the
@@ -377,11 +465,12 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=2000)
     ap.add_argument("--inner-reduction", type=float, default=0.0,
                     help="mixed: FP32 cycle stops at |g| <= max(target, this * beta); 0 = library default")
-    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab"), default="sweep",
                     help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
                          "applies of one sweep point plus the copy calibration kernel (for counter runs)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="matvec: bytes of the calibration copy")
+    ap.add_argument("--slab-size", type=int, default=128, help="slab: voxels per edge (128 = the 128^3 slab)")
     ap.add_argument("--arnoldi-column", type=int, default=0,
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
@@ -407,6 +496,8 @@ def main():
         return bench_serve(args)
     if args.workload == "matvec":
         return bench_matvec(args)
+    if args.workload == "slab":
+        return bench_slab(args)
 
     n = args.size
     pitch = 0.5 / n

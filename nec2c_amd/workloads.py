@@ -127,20 +127,107 @@ def dyadic(rvec, k0):
     return G
 
 
-def coupling_blocks(dipole: Dipole, centers: np.ndarray, pitch: float, eps: np.ndarray, k0: float):
-    """B (c x 3N) and D (3N x c) in the reference's cell-major layout."""
-    pts = dipole.points()
+@dataclass
+class WireArray:
+    """Parallel z-directed thin wires (a Yagi-Uda array, say), each cut into
+    the same number of segments, discretised like ``Dipole`` (Harrington's
+    pulse-basis point matching); between wires the reduced kernel uses the
+    axis separation in place of the radius.  ``wires`` holds (x, y, z_centre,
+    length) per wire; the feed is the centre segment of wire ``feed_wire``."""
+    wires: tuple
+    segments_per_wire: int = 21
+    radius: float = 1e-3
+    feed_wire: int = 1
+
+    @property
+    def segments(self):
+        return len(self.wires) * self.segments_per_wire
+
+    def deltas(self):
+        return np.repeat([w[3] / self.segments_per_wire for w in self.wires], self.segments_per_wire)
+
+    def points(self):
+        out = []
+        for x, y, zc, length in self.wires:
+            d = length / self.segments_per_wire
+            z0 = zc - 0.5 * length + 0.5 * d
+            out += [[x, y, z0 + i * d] for i in range(self.segments_per_wire)]
+        return np.array(out)
+
+    def impedance_matrix(self, k0: float) -> np.ndarray:
+        pts, dl = self.points(), self.deltas()
+        N, a, omega = len(pts), self.radius, k0 * C0
+        xg, wg = _gauss01(16)
+
+        def psi(pm, zm, n):
+            # segment-averaged exp(-jkR)/(4 pi R) over segment n seen from (pm_xy, zm)
+            dn, zc = dl[n], pts[n, 2]
+            rho = np.hypot(pm[0] - pts[n, 0], pm[1] - pts[n, 1])
+            same = rho < 1e-12
+            r0 = a if same else rho
+            if same and abs(zm - zc) < 1e-12 * dn:
+                return math.log(dn / a) / (2 * math.pi * dn) - 1j * k0 / (4 * math.pi)
+            zp = zc - 0.5 * dn + dn * xg
+            R = np.sqrt((zm - zp) ** 2 + r0 * r0)
+            return np.sum(wg * np.exp(-1j * k0 * R) / (4 * math.pi * R))
+
+        Z = np.zeros((N, N), dtype=np.complex128)
+        for m in range(N):
+            pm, zm, dm = pts[m], pts[m, 2], dl[m]
+            for n in range(N):
+                dn = dl[n]
+                t1 = 1j * omega * MU0 * dm * dn * psi(pm, zm, n)
+                # the charge terms: psi between the ends of segment m and the
+                # half-segment-shifted segment n (the Dipole recipe with a shift)
+                def psi_s(zs, sn):
+                    zc = pts[n, 2] + sn * 0.5 * dn
+                    rho = np.hypot(pm[0] - pts[n, 0], pm[1] - pts[n, 1])
+                    same = rho < 1e-12
+                    if same and abs(zs - zc) < 1e-12 * dn:
+                        return math.log(dn / a) / (2 * math.pi * dn) - 1j * k0 / (4 * math.pi)
+                    r0 = a if same else rho
+                    zp = zc - 0.5 * dn + dn * xg
+                    R = np.sqrt((zs - zp) ** 2 + r0 * r0)
+                    return np.sum(wg * np.exp(-1j * k0 * R) / (4 * math.pi * R))
+                t2 = (psi_s(zm + dm / 2, 1) - psi_s(zm - dm / 2, 1) - psi_s(zm + dm / 2, -1)
+                      + psi_s(zm - dm / 2, -1)) / (1j * omega * EPS0)
+                Z[m, n] = t1 + t2
+        return Z
+
+    def feed_index(self):
+        return self.feed_wire * self.segments_per_wire + self.segments_per_wire // 2
+
+    def rhs(self, volt=1.0):
+        v = np.zeros(self.segments, dtype=np.complex128)
+        v[self.feed_index()] = volt
+        return v
+
+
+def coupling_blocks(wire, centers: np.ndarray, pitch: float, eps: np.ndarray, k0: float, threads: int = 1):
+    """B (c x 3N) and D (3N x c) in the reference's cell-major layout for a
+    ``Dipole`` or a ``WireArray`` (segment lengths from ``deltas()``)."""
+    pts = wire.points()
     c = len(pts)
     N = len(centers)
     omega = k0 * C0
-    B = np.zeros((c, 3 * N), dtype=np.complex128)
-    D = np.zeros((3 * N, c), dtype=np.complex128)
-    contrast = (eps - 1.0).reshape(N)
-    for n in range(c):
+    dl = wire.deltas() if hasattr(wire, "deltas") else np.full(c, wire.delta)
+    B = np.empty((c, 3 * N), dtype=np.complex128)
+    D = np.empty((3 * N, c), dtype=np.complex128)
+    contrast = (np.broadcast_to(eps, (N,)) - 1.0).reshape(N)
+
+    def one(n):
         G = dyadic(centers - pts[n][None, :], k0)            # [N, 3, 3]
-        inc = (1j * omega * MU0 * dipole.delta / (k0 * k0)) * G[:, :, 2]   # field per amp, [N, 3]
+        inc = (1j * omega * MU0 * dl[n] / (k0 * k0)) * G[:, :, 2]   # field per amp, [N, 3]
         D[:, n] = (contrast[:, None] * inc).reshape(-1)
-        B[n, :] = (-(pitch ** 3) * dipole.delta * G[:, 2, :]).reshape(-1)
+        B[n, :] = (-(pitch ** 3) * dl[n] * G[:, 2, :]).reshape(-1)
+
+    if threads > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(one, range(c)))
+    else:
+        for n in range(c):
+            one(n)
     return B, D
 
 
