@@ -15,7 +15,9 @@ typedef enum {
   VIE_HIP_FP64 = 0,  /* every matvec, basis vector and reduction in FP64 */
   VIE_HIP_MIXED = 1  /* FP32 matvecs and Krylov basis inside each cycle; FP64
                         true residual b - A x, solution update, dot-product
-                        accumulation and Givens least squares */
+                        accumulation and Givens least squares.  A cycle that
+                        cannot lower the true residual in FP32 hands the rest
+                        of the solve to FP64 cycles. */
 } vie_hip_precision;
 
 typedef struct {
@@ -97,6 +99,10 @@ int dielectric_fft_download_solution(dielectric_fft_system *system, VIE_CDOUBLE 
    the five volume passes and the conductor coupling. */
 int dielectric_fft_time_apply(dielectric_fft_system *system, vie_hip_precision precision, int repeats,
                               double *ms_per_apply, double *pass_ms);
+/* 1 when the system's last device solve in VIE_HIP_MIXED ran out of FP32
+   accuracy (an FP32 cycle gained less than 2x on the FP64 true residual) and
+   finished with FP64 cycles. */
+int dielectric_fft_last_solve_escalated(const dielectric_fft_system *system);
 /* Mean basis column j of the matvec Arnoldi steps that the system's own
    resident solver issued since the last call with reset != 0, and their count. */
 int dielectric_fft_solve_stats(dielectric_fft_system *system, double *mean_column, unsigned long long *steps,

@@ -570,6 +570,8 @@ struct SolverWork {
   vie_gmres_report rep{};
   int phase = 0, m_eff = 0, steps = 0, issued = 0, checked = 0, result = 0;
   bool after_cycle = false, breakdown = false, cycle_done = false, dcgs2 = true;
+  double cycle_beta = 0;    // true residual at the start of the current cycle
+  bool escalated = false;   // a MIXED solve that switched to FP64 cycles
   double col_sum = 0;       // sum of basis columns j over the matvec Arnoldi steps issued
   unsigned long long col_steps = 0;
   double bnorm = 0, target = 0, beta = 0, reduction = 1e-5, t_submit = 0, t_end = 0;
@@ -988,7 +990,8 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   w.reduction = so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
   const char* ortho = getenv("VIE_HIP_ORTHO");
   w.dcgs2 = !(ortho && !strcmp(ortho, "cgs2"));
-  w.after_cycle = w.breakdown = w.cycle_done = false;
+  w.after_cycle = w.breakdown = w.cycle_done = w.escalated = false;
+  w.cycle_beta = INFINITY;
   w.result = 0;
   w.t_submit = now_s();
   vie::ReduceScratch rs;
@@ -1182,7 +1185,8 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
                   (size_t)(rep.iterations - w.it0), rep.relative_residual,
                   w.bnorm > 0 ? w.pin->fin.gabs / w.bnorm : 0.0);
         if (w.after_cycle) {
-          if (w.breakdown && w.beta > w.target) {
+          // an FP32 breakdown is a precision artefact: FP64 cycles take over below
+          if (w.breakdown && w.beta > w.target && !mixed) {
             job_finish(w, VIE_ITERATIVE_BREAKDOWN, 0);
             return 1;
           }
@@ -1196,6 +1200,25 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           job_finish(w, VIE_ITERATIVE_ITERATION_LIMIT, 0);
           return 1;
         }
+        // An FP32 cycle that gained less than 2x on the FP64 true residual has
+        // hit the FP32 floor of this system (its solution is too large against
+        // the tolerance for FP32 products): the remaining cycles run in FP64.
+        if (mixed && w.after_cycle && (w.breakdown || w.beta > 0.5 * w.cycle_beta)) {
+          if (!w.V.alloc((size_t)(w.m + 1) * w.ldv * sizeof(c64))) {
+            job_finish(w, VIE_ITERATIVE_ALLOCATION_FAILED, 0);
+            return 1;
+          }
+          w.prec = VIE_HIP_FP64;
+          w.escalated = true;
+          if (trace_enabled())
+            fprintf(stderr, "[vie_hip] fp32 cycle stalled at %.3e: continuing in fp64\n", rep.relative_residual);
+          w.cycle_beta = w.beta;
+          w.breakdown = false;
+          w.phase = PH_RESID;  // re-entered in FP64: the event has completed
+          w.after_cycle = false;
+          return job_advance<double>(w, blocking);
+        }
+        w.cycle_beta = w.beta;
         // new cycle
         const double inner_target = mixed ? std::max(w.target, w.reduction * w.beta) : w.target;
         vie::cycle_init(sd, w.scal.as<c64>() + 2, inner_target, st);
@@ -1865,6 +1888,11 @@ extern "C" int dielectric_fft_time_apply(dielectric_fft_system* s, vie_hip_preci
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   return ok;
+}
+
+// 1 when the system's last resident MIXED solve switched to FP64 cycles.
+extern "C" int dielectric_fft_last_solve_escalated(const dielectric_fft_system* s) {
+  return s && s->sw.escalated ? 1 : 0;
 }
 
 // Mean basis column of the matvec Arnoldi steps the system's own solver

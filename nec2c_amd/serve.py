@@ -211,6 +211,10 @@ class Server:
             "converged": sum(bool(res[0]) for r in self.done for res in r.results),
             "max_relative_residual": max((rep.relative_residual for rep in reps), default=None),
             "iterations_total": sum(rep.iterations for rep in reps),
+            "failures": [{"request": r.rid, "key": list(r.key) if isinstance(r.key, tuple) else str(r.key),
+                          "port": p, "status": int(res[2].status), "iterations": int(res[2].iterations),
+                          "relative_residual": float(res[2].relative_residual)}
+                         for r in self.done for p, res in enumerate(r.results) if not res[0]],
         }
 
     def close(self):

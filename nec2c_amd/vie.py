@@ -126,6 +126,7 @@ EXPORTS = (
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
     "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
+    "dielectric_fft_last_solve_escalated",
 )
 
 _LIB = None
@@ -208,6 +209,7 @@ def lib() -> ctypes.CDLL:
         "vie_hip_memory_info": (i, [szp, szp]),
         "dielectric_fft_time_arnoldi": (i, [vp, i, i, i] + [ctypes.POINTER(d)] * 4),
         "dielectric_fft_solve_stats": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_ulonglong), i]),
+        "dielectric_fft_last_solve_escalated": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -556,6 +558,11 @@ class System:
         if not lib().dielectric_fft_time_apply(self.s, precision, repeats, ctypes.byref(ms), passes):
             raise RuntimeError(last_error())
         return ms.value, list(passes)
+
+    @property
+    def escalated(self):
+        """True when the last MIXED device solve finished in FP64 cycles."""
+        return bool(lib().dielectric_fft_last_solve_escalated(self.s))
 
     def solve_stats(self, reset=False):
         """(mean basis column, matvec Arnoldi steps) of the resident solves since the last reset."""

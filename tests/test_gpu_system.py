@@ -148,3 +148,34 @@ def test_pec_larger_block(ref):
     x = np.random.default_rng(5).standard_normal(s.order) + 0.5j
     assert rel(s.apply(x), rs.apply(x)) < 1e-11
     assert rel(s.apply_ex(x, vie.MIXED), rs.apply(x)) < 2e-6
+
+
+def test_mixed_escalates_to_fp64_when_fp32_stalls(ref):
+    """A system whose volume response dwarfs the excitation (D scaled by 1e7,
+    so |x_v| / |b| ~ 1e7): FP32 products cannot lower the true residual at
+    all, and a MIXED solve must hand over to FP64 cycles and still converge
+    against the reference operator."""
+    n, c = 6, 5
+    pitch = 0.5 / 64
+    eps = W.random_permittivity((n, n, n), 9).reshape(-1)
+    centers = W.cube_cells(n, pitch)
+    k0 = W.k0_of(200e6)
+    rng = np.random.default_rng(5)
+    nv = 3 * n ** 3
+    C = rng.standard_normal((c, c)) + 1j * rng.standard_normal((c, c)) + 20 * np.eye(c)
+    B = 1e-9 * (rng.standard_normal((c, nv)) + 1j * rng.standard_normal((c, nv)))
+    D = 1e7 * (rng.standard_normal((nv, c)) + 1j * rng.standard_normal((nv, c)))
+    s = vie.System(centers, [vie.MaterialSpec(1, 1.5)], c, k0, pitch=pitch)
+    s.set_permittivities(eps)
+    s.set_blocks(C=C, B=B, D=D)
+    rp, rc = ref_coupled(ref, n, pitch, k0, eps, (C, B, D))
+    b = np.zeros(s.order, complex)
+    b[0] = 1.0
+    ok, x, rep = s.solve(b, precision=vie.MIXED, relative_tolerance=1e-6, restart=30, maximum_iterations=600)
+    print("mixed", rep.as_dict(), "escalated", s.escalated, flush=True)
+    assert ok and s.escalated
+    r = b - rc.apply(x)
+    assert np.linalg.norm(r) <= 1.05e-6 * np.linalg.norm(b)
+    ok64, x64, rep64 = s.solve(b, precision=vie.FP64, relative_tolerance=1e-6, restart=30, maximum_iterations=600)
+    assert ok64 and not s.escalated
+    s.close()
