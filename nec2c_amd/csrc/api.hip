@@ -1083,6 +1083,7 @@ template <typename T> static int ensure_step_graph(SolverWork& w) {
     return 0;
   }
   w.step_key.swap(key);
+  if (trace_enabled()) fprintf(stderr, "[vie_hip] step graph recorded\n");
   return 1;
 }
 
@@ -1107,7 +1108,10 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
       w.col_sum += j;
       w.col_steps++;
       if (!w.graph_off && graphs_enabled()) {
-        if (!ensure_step_graph<T>(w)) w.graph_off = true;  // record failed: launch directly from now on
+        if (!ensure_step_graph<T>(w)) {
+          w.graph_off = true;  // record failed: launch directly from now on
+          if (trace_enabled()) fprintf(stderr, "[vie_hip] step graph could not be recorded: direct launches\n");
+        }
       }
       if (w.step_exec && !w.graph_off && graphs_enabled()) {
         HIP_OK(hipGraphLaunch(w.step_exec, st));

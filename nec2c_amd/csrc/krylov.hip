@@ -25,8 +25,21 @@ template <typename T> static unsigned dot_blocks(size_t n) {
   size_t b = (n + ch - 1) / ch;
   return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
 }
-static unsigned update_blocks(size_t n) {
-  size_t ch = (size_t)KWG * 2;
+static unsigned update_blocks(size_t n, int ept = 2) {
+  size_t ch = (size_t)KWG * ept;
+  size_t b = (n + ch - 1) / ch;
+  return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
+}
+
+// Delayed-CGS2 streaming shapes: basis elements per lane per key in the
+// reduction, elements per thread in the fused update (FP32 takes twice as
+// many so each lane keeps the same bytes in flight).
+template <typename T> struct DcShape {
+  static constexpr int CPL = 8;
+  static constexpr int UE = 2;
+};
+template <typename T> static unsigned dc_blocks(size_t n) {
+  size_t ch = (size_t)64 * DcShape<T>::CPL;
   size_t b = (n + ch - 1) / ch;
   return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
 }
@@ -350,7 +363,7 @@ __global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_ta
 template <typename T>
 __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, const ArnoldiState* st, int has_w,
                                                  size_t n, c64* partial, int ldp) {
-  constexpr int CPL = 8;  // u, w and q in flight per lane
+  constexpr int CPL = DcShape<T>::CPL;  // u, w and q in flight per lane
   constexpr int WAVES = KWG / 64;
   constexpr int CH = 64 * CPL;
   __shared__ c64 lds[kMaxKeys];
@@ -377,14 +390,15 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
     // basis vectors: one load feeds both projections; the next key's load is
     // issued before this key's reduction (software pipelining)
     cx<T> q[CPL], qn[CPL];
-    auto load_key = [&](int k, cx<T>* dst) {
-      const cx<T>* qk = Q + (size_t)k * ldv + base + lane;
-#pragma unroll
-      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? ld_stream(qk + e * 64) : cx<T>{0, 0};
-    };
-    if (wave < j) load_key(wave, q);
+#define VIE_DC_LOAD(k, dst)                                                                                \
+  do {                                                                                                     \
+    const cx<T>* qk_ = Q + (size_t)(k) * ldv + base + lane;                                                \
+    _Pragma("unroll") for (int e_ = 0; e_ < CPL; e_++) dst[e_] =                                           \
+        base + lane + (size_t)e_ * 64 < n ? ld_stream(qk_ + e_ * 64) : cx<T>{0, 0};                       \
+  } while (0)
+    if (wave < j) VIE_DC_LOAD(wave, q);
     for (int k = wave; k < j; k += WAVES) {
-      if (k + WAVES < j) load_key(k + WAVES, qn);
+      if (k + WAVES < j) VIE_DC_LOAD(k + WAVES, qn);
       c64 sa = {0, 0}, sb = {0, 0};
 #pragma unroll
       for (int e = 0; e < CPL; e++) {
@@ -407,6 +421,7 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
 #pragma unroll
       for (int e = 0; e < CPL; e++) q[e] = qn[e];
     }
+#undef VIE_DC_LOAD
     // the scalars nu, c, d: one wave each
     const int nscal = hw ? 3 : 1;
     for (int q = wave; q < nscal; q += WAVES) {
@@ -594,6 +609,7 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
   __shared__ c64 ca[kMaxRestart], ce[kMaxRestart];
   if (st->done) return;
   const int j = st->j;
+  constexpr int UE = DcShape<T>::UE;
   for (int k = threadIdx.x; k < j; k += KWG) {
     ca[k] = st->dc_a[k];
     ce[k] = st->dc_e[k];
@@ -603,12 +619,12 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
   const double fr = st->dc_f_re, fi = st->dc_f_im;
   cx<T>* u = Q + (size_t)j * ldv;
   cx<T>* w = Q + (size_t)(j + 1) * ldv;
-  const size_t step = (size_t)gridDim.x * KWG * UEPT;
-  for (size_t base = (size_t)blockIdx.x * KWG * UEPT + threadIdx.x; base < n; base += step) {
-    double ur[UEPT], ui[UEPT], wr[UEPT], wi[UEPT];
-    bool in[UEPT];
+  const size_t step = (size_t)gridDim.x * KWG * UE;
+  for (size_t base = (size_t)blockIdx.x * KWG * UE + threadIdx.x; base < n; base += step) {
+    double ur[UE], ui[UE], wr[UE], wi[UE];
+    bool in[UE];
 #pragma unroll
-    for (int e = 0; e < UEPT; e++) {
+    for (int e = 0; e < UE; e++) {
       size_t i = base + (size_t)e * KWG;
       in[e] = i < n;
       cx<T> uu = in[e] ? u[i] : cx<T>{0, 0}, ww = in[e] ? w[i] : cx<T>{0, 0};
@@ -620,11 +636,11 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
     }
     int k = 0;
     for (; k + 4 <= j; k += 4) {
-      cx<T> q[4][UEPT];
+      cx<T> q[4][UE];
 #pragma unroll
       for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int e = 0; e < UEPT; e++) {
+        for (int e = 0; e < UE; e++) {
           size_t i = base + (size_t)e * KWG;
           q[p][e] = in[e] ? ld_stream(Q + (size_t)(k + p) * ldv + i) : cx<T>{0, 0};
         }
@@ -632,7 +648,7 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
       for (int p = 0; p < 4; p++) {
         const double ar = ca[k + p].re, ai = ca[k + p].im, er = ce[k + p].re, ei = ce[k + p].im;
 #pragma unroll
-        for (int e = 0; e < UEPT; e++) {
+        for (int e = 0; e < UE; e++) {
           double qr = q[p][e].re, qi = q[p][e].im;
           ur[e] -= ar * qr - ai * qi;
           ui[e] -= ar * qi + ai * qr;
@@ -644,7 +660,7 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
     for (; k < j; k++) {
       const double ar = ca[k].re, ai = ca[k].im, er = ce[k].re, ei = ce[k].im;
 #pragma unroll
-      for (int e = 0; e < UEPT; e++) {
+      for (int e = 0; e < UE; e++) {
         size_t i = base + (size_t)e * KWG;
         if (in[e]) {
           cx<T> qq = ld_stream(Q + (size_t)k * ldv + i);
@@ -657,7 +673,7 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
       }
     }
 #pragma unroll
-    for (int e = 0; e < UEPT; e++) {
+    for (int e = 0; e < UE; e++) {
       size_t i = base + (size_t)e * KWG;
       if (in[e]) {
         u[i] = cx<T>{(T)(ur[e] * ia), (T)(ui[e] * ia)};
@@ -928,8 +944,7 @@ void backsolve(ArnoldiState* st, hipStream_t s) { hipLaunchKernelGGL(k_backsolve
 template <typename T>
 void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, ReduceScratch& rs, c64* dots,
                 double breakdown_eps, int max_j, StepStatus* ring, hipStream_t s) {
-  size_t nch = (n + 511) / 512;
-  unsigned nb = (unsigned)(nch < 1 ? 1 : (nch > kMaxDotBlocks ? kMaxDotBlocks : nch));
+  const unsigned nb = dc_blocks<T>(n);
   const int hw = has_w ? 1 : 0;
   hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, (const ArnoldiState*)st, hw, n, rs.partial,
                      kMaxDotBlocks);
@@ -937,8 +952,8 @@ void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, Re
   hipLaunchKernelGGL(k_finish_dc, dim3((max_keys + 3) / 4), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb,
                      (const ArnoldiState*)st, hw, dots);
   hipLaunchKernelGGL(k_dc_mid, dim3(1), dim3(KWG), 0, s, st, hw, dots, breakdown_eps);
-  if (has_w) hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, n,
-                                (const ArnoldiState*)st);
+  if (has_w) hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n, DcShape<T>::UE)), dim3(KWG), 0, s, Q, ldv,
+                                n, (const ArnoldiState*)st);
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, st, ring);
 }
 // Timing hook: a state at column j with finite update coefficients, and a
@@ -966,8 +981,7 @@ template <typename T> __global__ void k_fill(cx<T>* x, size_t n, T v) {
 template <typename T>
 int dcgs2_time(cx<T>* Q, size_t ldv, size_t n, ArnoldiState* st, ReduceScratch& rs, int j, int reps, hipStream_t s,
                float* ms_reduce, float* ms_update) {
-  size_t nch = (n + 511) / 512;
-  unsigned nb = (unsigned)(nch < 1 ? 1 : (nch > kMaxDotBlocks ? kMaxDotBlocks : nch));
+  const unsigned nb = dc_blocks<T>(n);
   const size_t cells = (size_t)(j + 2) * ldv;
   hipLaunchKernelGGL((k_fill<T>), dim3(gblocks(cells)), dim3(256), 0, s, Q, cells, (T)1e-3);
   hipLaunchKernelGGL(k_time_state, dim3(1), dim3(256), 0, s, st, j);
@@ -977,7 +991,8 @@ int dcgs2_time(cx<T>* Q, size_t ldv, size_t n, ArnoldiState* st, ReduceScratch& 
   for (int r = 0; r < 2; r++) {
     hipLaunchKernelGGL((k_dots_dc<T>), dim3(nb), dim3(KWG), 0, s, Q, ldv, (const ArnoldiState*)st, 1, n, rs.partial,
                        kMaxDotBlocks);
-    hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, n, (const ArnoldiState*)st);
+    hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n, DcShape<T>::UE)), dim3(KWG), 0, s, Q, ldv, n,
+                       (const ArnoldiState*)st);
   }
   (void)hipEventRecord(e[0], s);
   for (int r = 0; r < reps; r++)
@@ -985,7 +1000,8 @@ int dcgs2_time(cx<T>* Q, size_t ldv, size_t n, ArnoldiState* st, ReduceScratch& 
                        kMaxDotBlocks);
   (void)hipEventRecord(e[1], s);
   for (int r = 0; r < reps; r++)
-    hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n)), dim3(KWG), 0, s, Q, ldv, n, (const ArnoldiState*)st);
+    hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n, DcShape<T>::UE)), dim3(KWG), 0, s, Q, ldv, n,
+                       (const ArnoldiState*)st);
   (void)hipEventRecord(e[2], s);
   int ok = hipEventSynchronize(e[2]) == hipSuccess;
   ok = ok && hipEventElapsedTime(ms_reduce, e[0], e[1]) == hipSuccess &&
