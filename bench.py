@@ -181,7 +181,7 @@ def bench_batch(args):
         b = vie.Batch()
         b.reserve(args.inflight, biggest, **opts)
         res, secs = serve.run_batch(jobs, precision=prec, inflight=args.inflight, batch=b,
-                                    inner_reduction=args.inner_reduction, **opts)
+                                    inner_reduction=args.inner_reduction, recycle=args.recycle, **opts)
         b.close()
         out[name] = (res, secs)
         log(f"batch {name}: {len(jobs)} solves in {secs:.2f} s "
@@ -296,7 +296,7 @@ def bench_serve(args):
     free, total = vie.memory_info()
     budget = min(288 * 10 ** 9, int(0.9 * free))
     srv = serve.Server(budget, precision=vie.MIXED, restart=args.restart, maximum_iterations=args.max_iterations,
-                       inner_reduction=args.inner_reduction)
+                       inner_reduction=args.inner_reduction, recycle=args.recycle)
     rep = srv.run(requests)
     srv.close()
     result = {
@@ -362,10 +362,10 @@ def bench_slab(args):
     log(f"slab: ready in {time.perf_counter() - t0:.1f} s")
     res = {}
     for prec, tag in ((vie.MIXED, "mixed"), (vie.FP64, "fp64")):
-        s.solve_resident(prec, args.inner_reduction, **opts)  # warm-up
+        s.solve_resident(prec, args.inner_reduction, args.recycle, **opts)  # warm-up
         vie.synchronize()
         t = time.perf_counter()
-        ok, rep = s.solve_resident(prec, args.inner_reduction, **opts)
+        ok, rep = s.solve_resident(prec, args.inner_reduction, args.recycle, **opts)
         vie.synchronize()
         dt = time.perf_counter() - t
         res[tag] = (ok, rep, dt, s.download_solution())
@@ -465,6 +465,8 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=2000)
     ap.add_argument("--inner-reduction", type=float, default=0.0,
                     help="mixed: FP32 cycle stops at |g| <= max(target, this * beta); 0 = library default")
+    ap.add_argument("--recycle", type=int, default=0,
+                    help="mixed: recycled-subspace size of the GMRES cycles (0 = library default, -1 = off)")
     ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab"), default="sweep",
                     help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
@@ -517,7 +519,7 @@ def main():
     warm, timed = systems[: args.warmup], systems[args.warmup:]
 
     for s in warm:
-        s.solve_resident(vie.MIXED, args.inner_reduction, **opts)
+        s.solve_resident(vie.MIXED, args.inner_reduction, args.recycle, **opts)
         s.solve_resident(vie.FP64, **opts)
     vie.synchronize()
 
@@ -527,7 +529,7 @@ def main():
         t = time.perf_counter()
         for s in timed:
             a = time.perf_counter()
-            ok, rep = s.solve_resident(prec, args.inner_reduction, **opts)
+            ok, rep = s.solve_resident(prec, args.inner_reduction, args.recycle if prec == vie.MIXED else 0, **opts)
             lat.append(time.perf_counter() - a)
             if not ok:
                 raise RuntimeError(f"solve failed: {vie.STATUS[rep.status]} rel {rep.relative_residual}")

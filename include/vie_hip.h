@@ -25,6 +25,14 @@ typedef struct {
   /* MIXED: an FP32 cycle ends once |g_{j+1}| <= max(target, inner_reduction * beta);
      0 selects the default (1e-5). */
   double inner_reduction;
+  /* Recycled Krylov subspace (GCRO with a subspace fixed after the first
+     cycle): after the first cycle the k harmonic Ritz vectors of smallest
+     modulus span U, with C = A M^{-1} U orthonormal, and every later cycle
+     minimises over span(U) + its own Krylov space of m - k columns.  k > 0
+     sets k (at most 32 and m / 2), -1 turns it off, 0 selects the default:
+     off for VIE_HIP_FP64 (the reference's restarted GMRES), on for
+     VIE_HIP_MIXED, where the restarts of the FP32 cycles lose most. */
+  int recycle;
 } vie_hip_solve_options;
 
 /* Device-resident solve of a system prepared by dielectric_fft_prepare[_pec]
@@ -103,6 +111,15 @@ int dielectric_fft_time_apply(dielectric_fft_system *system, vie_hip_precision p
    accuracy (an FP32 cycle gained less than 2x on the FP64 true residual) and
    finished with FP64 cycles. */
 int dielectric_fft_last_solve_escalated(const dielectric_fft_system *system);
+/* Size of the recycled subspace the system's last device solve built and used
+   (0 when its cycles were plain: recycling off, one cycle was enough, or the
+   first cycle was too short to supply one). */
+int dielectric_fft_last_solve_recycled(const dielectric_fft_system *system);
+/* Host-side step of the recycled subspace, exported for tests: the k harmonic
+   Ritz vectors of smallest modulus (Morgan 2002) of the (m+1) x m Arnoldi
+   Hessenberg hbar (column-major), as the m x k column-major p with unit
+   columns.  1 on success.  No device work. */
+int vie_hip_harmonic_ritz(const VIE_CDOUBLE *hbar, int m, int k, VIE_CDOUBLE *p);
 /* Mean basis column j of the matvec Arnoldi steps that the system's own
    resident solver issued since the last call with reset != 0, and their count. */
 int dielectric_fft_solve_stats(dielectric_fft_system *system, double *mean_column, unsigned long long *steps,

@@ -27,6 +27,7 @@
 
 #include "../../include/vie_hip.h"
 #include "krylov.h"
+#include "small_dense.h"
 #include "vie_kernels.h"
 
 using vie::c32;
@@ -57,7 +58,7 @@ extern "C" const char* vie_hip_last_error(void) { return g_last_error.c_str(); }
 enum { BACKEND_AUTO, BACKEND_HIP };
 static int g_requested = BACKEND_AUTO;
 static int g_last_used = -1;
-static vie_hip_solve_options g_solve = {VIE_HIP_FP64, 0.0};
+static vie_hip_solve_options g_solve = {VIE_HIP_FP64, 0.0, 0};
 static bool g_solve_init = false;
 static size_t g_budget = 0;
 static std::mutex g_mutex;
@@ -576,9 +577,21 @@ struct SolverWork {
   unsigned long long col_steps = 0;
   double bnorm = 0, target = 0, beta = 0, reduction = 1e-5, t_submit = 0, t_end = 0;
   size_t it0 = 0;
+  // Recycled subspace.  Basis columns (stride ldv) are laid out as
+  // [U (rk) | C (rk) | Krylov]: the first cycle builds its basis from column
+  // 2 rk on, U and C are then formed from it in place, and every later cycle
+  // sees C as its columns 0..rk-1 (basis pointer at column rk).
+  int rk = 0;              // recycled vectors of this solve (0: plain restarted GMRES)
+  int cyc_base = 0;        // first Krylov column of the current cycle
+  int cycles = 0;          // cycles started in this solve
+  int rec_used = 0;        // columns of the first cycle
+  bool rec_breakdown = false, rec_ready = false;
+  DevBuf rcoef;            // combination coefficients of U and C
+  c64* rec_host = nullptr; // pinned: first cycle's Hraw, dc_a, dc_alpha, then the coefficients
   ~SolverWork() {
     if (step_exec) (void)hipGraphExecDestroy(step_exec);
     if (pin) (void)hipHostFree(pin);
+    if (rec_host) (void)hipHostFree(rec_host);
     if (host_io) (void)hipHostFree(host_io);
     for (auto& e : ev_ring)
       if (e) (void)hipEventDestroy(e);
@@ -914,13 +927,28 @@ static int ensure_scratch(SolverWork& w) {
   return 1;
 }
 
-static int ensure_solver(dielectric_fft_system* s, SolverWork& w, int m, int prec) {
+// Pinned host area of the recycled-subspace construction (c64 entries).
+static size_t rec_host_entries() {
+  const size_t m = vie::kMaxRestart;
+  return (m + 1) * m + m + 1 + 2 * (m + 1) * vie::kMaxRecycle;
+}
+
+// Host and device areas of the recycled-subspace construction.
+static int ensure_recycle(SolverWork& w) {
+  if (!w.rec_host)
+    HIP_OK(hipHostMalloc((void**)&w.rec_host, rec_host_entries() * sizeof(c64), hipHostMallocDefault));
+  return w.rcoef.alloc(2 * (vie::kMaxRestart + 1) * vie::kMaxRecycle * sizeof(c64));
+}
+
+static int ensure_solver(dielectric_fft_system* s, SolverWork& w, int m, int prec, int rk = 0) {
   size_t n = s->nsys();
   size_t ldv = (n + 63) / 64 * 64;
   size_t es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
   if (!ensure_scratch(w)) return 0;
+  if (rk > 0 && !ensure_recycle(w)) return 0;
   // a larger basis from an earlier reserve or FP64 solve is reused as is
-  if (!(w.V.alloc((size_t)(m + 1) * ldv * es) && w.x.alloc(n * 16) && w.b.alloc(n * 16) && w.r.alloc(n * 16) &&
+  if (!(w.V.alloc((size_t)(m + 1 + 2 * rk) * ldv * es) && w.x.alloc(n * 16) && w.b.alloc(n * 16) &&
+        w.r.alloc(n * 16) &&
         w.ax.alloc(n * 16) && w.state.alloc(sizeof(vie::ArnoldiState)) &&
         w.dots.alloc(2 * (vie::kMaxRestart + 2) * sizeof(c64))))
     return 0;
@@ -929,12 +957,26 @@ static int ensure_solver(dielectric_fft_system* s, SolverWork& w, int m, int pre
   w.ldv = ldv;
   w.m = m;
   w.prec = prec;
+  w.rk = rk;
   return 1;
 }
 
-static size_t solver_bytes(size_t n, int m, int prec) {
+static size_t solver_bytes(size_t n, int m, int prec, int rk = 0) {
   size_t ldv = (n + 63) / 64 * 64, es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
-  return (size_t)(m + 1) * ldv * es + 4 * n * 16 + vie::kReduceScratchBytes + sizeof(vie::ArnoldiState);
+  return (size_t)(m + 1 + 2 * rk) * ldv * es + 4 * n * 16 + vie::kReduceScratchBytes + sizeof(vie::ArnoldiState);
+}
+
+// Recycled vectors a solve keeps: the option's k (0: default per precision,
+// -1: none), capped at kMaxRecycle and at half the restart so every later
+// cycle still has a Krylov space of its own.
+constexpr int kDefaultRecycleMixed = 20;
+static int recycle_count(const vie_hip_solve_options* so, int prec, int m) {
+  int k = so ? so->recycle : 0;
+  if (k < 0) return 0;
+  if (k == 0) k = prec == VIE_HIP_MIXED ? kDefaultRecycleMixed : 0;
+  if (k > vie::kMaxRecycle) k = vie::kMaxRecycle;
+  if (k > m / 2) k = m / 2;
+  return k;
 }
 
 // ------------------------------------------------------------------ device-resident GMRES
@@ -990,6 +1032,9 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   w.reduction = so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
   const char* ortho = getenv("VIE_HIP_ORTHO");
   w.dcgs2 = !(ortho && !strcmp(ortho, "cgs2"));
+  if (!w.dcgs2) w.rk = 0;  // the recycled cycles are built on the delayed-CGS2 step
+  w.cyc_base = w.cycles = w.rec_used = 0;
+  w.rec_breakdown = w.rec_ready = false;
   w.after_cycle = w.breakdown = w.cycle_done = w.escalated = false;
   w.cycle_beta = INFINITY;
   w.result = 0;
@@ -1025,10 +1070,18 @@ static bool graphs_enabled() {
   return e && !strcmp(e, "1");
 }
 
+// Column 0 of the current cycle's basis (the layout of SolverWork::rk).
+template <typename T> static vie::cx<T>* cycle_basis(SolverWork& w) {
+  vie::cx<T>* V = w.V.as<vie::cx<T>>();
+  if (!w.rk) return V;
+  return V + (size_t)(w.rec_ready ? w.rk : 2 * w.rk) * w.ldv;
+}
+
 // Everything a recorded step depends on: if any of it changes, record again.
 template <typename T> static std::vector<const void*> step_graph_key(SolverWork& w) {
   dielectric_fft_system* s = w.sys;
-  std::vector<const void*> k = {s, (const void*)(uintptr_t)sizeof(T), w.V.p, w.partial.p, w.dots.p, w.state.p,
+  std::vector<const void*> k = {s, (const void*)(uintptr_t)sizeof(T), cycle_basis<T>(w), w.partial.p, w.dots.p,
+                                w.state.p,
                                 w.pin_dev, (const void*)(uintptr_t)w.n, (const void*)(uintptr_t)w.ldv,
                                 (const void*)(uintptr_t)w.m, s->stream, s->C64.p, s->B64.p, s->D64.p, s->C32.p,
                                 s->B32.p, s->D32.p, s->bz.p};
@@ -1047,7 +1100,7 @@ template <typename T> static int issue_matvec_step(SolverWork& w) {
   dielectric_fft_system* s = w.sys;
   vie::ReduceScratch rs;
   rs.partial = w.partial.as<c64>();
-  vie::cx<T>* V = w.V.as<vie::cx<T>>();
+  vie::cx<T>* V = cycle_basis<T>(w);
   vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
   vie::DevIndex di;
   di.j = &sd->j;
@@ -1093,7 +1146,7 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
   hipStream_t st = s->stream;
   vie::ReduceScratch rs;
   rs.partial = w.partial.as<c64>();
-  vie::cx<T>* V = w.V.as<vie::cx<T>>();
+  vie::cx<T>* V = cycle_basis<T>(w);
   vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
   c64* dots1 = w.dots.as<c64>();
   c64* dots2 = dots1 + vie::kMaxRestart + 2;
@@ -1105,7 +1158,7 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
     // closing step that only finalises the last column
     if (j < w.m_eff) {
       w.rep.operator_applications++;
-      w.col_sum += j;
+      w.col_sum += j + w.cyc_base;
       w.col_steps++;
       if (!w.graph_off && graphs_enabled()) {
         if (!ensure_step_graph<T>(w)) {
@@ -1136,6 +1189,75 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
   }
   HIP_OK(hipEventRecord(w.ev_ring[j & 1], st));
   return 1;
+}
+
+// Builds the recycled subspace from the finished first cycle (its basis from
+// column 2 rk, its unrotated Hessenberg, and the last step's a and alpha in
+// w.rec_host): P = the rk harmonic Ritz vectors of smallest modulus,
+// Hbar P = Q R (thin QR), C = V_{J+1} Q, U = V_J P R^{-1}, so that
+// A M^{-1} U = C with C orthonormal (Parks, de Sturler, Mackey, Johnson, Maiti,
+// "Recycling Krylov subspaces for sequences of linear systems", SIAM J. Sci.
+// Comput. 28 (2006), the GCRO-DR start; here the subspace then stays fixed).
+// The first cycle leaves its last basis vector as u = alpha q_J + Q_J a, so
+// the coefficients of column J are folded onto u and the earlier columns.
+// False (the solve continues with plain cycles) when the cycle is too short
+// or the small problems fail.
+template <typename T> static bool recycle_build(SolverWork& w) {
+  namespace D = vie::dense;
+  const int J = w.rec_used, k = w.rk, m = w.m;
+  if (J < k + 1 || w.rec_breakdown) return false;
+  const double t0 = now_s();
+  const c64* hr = w.rec_host;  // column-major, ld m + 1
+  const c64* a = hr + (size_t)(m + 1) * m;
+  const double alpha = hr[(size_t)(m + 1) * m + m].re;
+  if (!(alpha > 0) || !std::isfinite(alpha)) return false;
+  D::Mat Hb(J + 1, J);
+  for (int c = 0; c < J; c++)
+    for (int r = 0; r <= c + 1 && r <= J; r++) {
+      const c64 v = hr[(size_t)c * (m + 1) + r];
+      Hb(r, c) = D::zd(v.re, v.im);
+    }
+  D::Mat P, G(J + 1, k), Qg, R, Ri;
+  if (!D::harmonic_ritz(Hb, k, P)) return false;
+  for (int i = 0; i < k; i++)
+    for (int r = 0; r <= J; r++) {
+      D::zd acc(0, 0);
+      for (int c = (r > 0 ? r - 1 : 0); c < J; c++) acc = acc + Hb(r, c) * P(c, i);
+      G(r, i) = acc;
+    }
+  if (!D::thin_qr(G, Qg, R) || !D::upper_inverse(R, Ri)) return false;
+  c64* cc = w.rec_host + (size_t)(m + 1) * m + m + 1;  // C coefficients, (J+1) x k
+  c64* uc = cc + (size_t)(J + 1) * k;                  // U coefficients, J x k
+  bool finite = true;
+  for (int i = 0; i < k; i++) {
+    const D::zd last = (1.0 / alpha) * Qg(J, i);
+    for (int c = 0; c < J; c++) {
+      const D::zd v = Qg(c, i) - D::zd(a[c].re, a[c].im) * last;
+      cc[(size_t)i * (J + 1) + c] = c64{v.re, v.im};
+    }
+    cc[(size_t)i * (J + 1) + J] = c64{last.re, last.im};
+    for (int c = 0; c < J; c++) {
+      D::zd acc(0, 0);
+      for (int l = 0; l <= i; l++) acc = acc + P(c, l) * Ri(l, i);
+      uc[(size_t)i * J + c] = c64{acc.re, acc.im};
+      finite = finite && std::isfinite(acc.re) && std::isfinite(acc.im);
+    }
+  }
+  if (!finite) return false;
+  hipStream_t st = w.sys->stream;
+  const size_t nc = (size_t)(J + 1) * k + (size_t)J * k;
+  if (hipMemcpyAsync(w.rcoef.p, cc, nc * sizeof(c64), hipMemcpyHostToDevice, st) != hipSuccess) return false;
+  vie::cx<T>* V = w.V.as<vie::cx<T>>();
+  const size_t ldv = w.ldv;
+  const vie::cx<T>* first = V + (size_t)2 * k * ldv;
+  vie::block_combine<T>(V + (size_t)k * ldv, k, first, J + 1, ldv, w.rcoef.as<c64>(), w.n, st);
+  vie::block_combine<T>(V, k, first, J, ldv, w.rcoef.as<c64>() + (size_t)(J + 1) * k, w.n, st);
+  if (hipGetLastError() != hipSuccess) return false;
+  w.rec_ready = true;
+  if (trace_enabled())
+    fprintf(stderr, "[vie_hip] recycled subspace of %d vectors from a %d-column cycle (host %.2f ms)\n", k, J,
+            (now_s() - t0) * 1e3);
+  return true;
 }
 
 // Advances the solve as far as it can without waiting (or, when blocking, to
@@ -1214,6 +1336,8 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           }
           w.prec = VIE_HIP_FP64;
           w.escalated = true;
+          w.rk = 0;  // the FP32 recycled subspace does not carry over
+          w.rec_ready = false;
           if (trace_enabled())
             fprintf(stderr, "[vie_hip] fp32 cycle stalled at %.3e: continuing in fp64\n", rep.relative_residual);
           w.cycle_beta = w.beta;
@@ -1223,13 +1347,28 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           return job_advance<double>(w, blocking);
         }
         w.cycle_beta = w.beta;
-        // new cycle
+        // new cycle; the second one first forms the recycled subspace
+        if (w.rk && !w.rec_ready && w.cycles == 1 && !recycle_build<T>(w)) {
+          if (trace_enabled()) fprintf(stderr, "[vie_hip] no recycled subspace: plain cycles\n");
+          w.rec_used = 0;
+        }
+        const int base = w.rec_ready ? w.rk : 0;
         const double inner_target = mixed ? std::max(w.target, w.reduction * w.beta) : w.target;
-        vie::cycle_init(sd, w.scal.as<c64>() + 2, inner_target, st);
-        vie::scale_from<T>(w.V.as<vie::cx<T>>(), w.r.as<c64>(), w.n, sd, st);
+        vie::cycle_init(sd, w.scal.as<c64>() + 2, inner_target, base, st);
+        {
+          vie::cx<T>* Q = cycle_basis<T>(w);
+          vie::scale_from<T>(Q + (size_t)base * w.ldv, w.r.as<c64>(), w.n, sd, st);
+          if (base) {
+            vie::ReduceScratch rs;
+            rs.partial = w.partial.as<c64>();
+            vie::recycle_project<T>(Q, w.ldv, base, w.n, sd, rs, st);
+          }
+        }
+        w.cyc_base = base;
+        w.cycles++;
         w.it0 = rep.iterations;
         // columns this cycle may add (the reference counts Arnoldi steps, vie_iterative.c:134)
-        w.m_eff = (int)std::min<size_t>((size_t)w.m, w.o.maximum_iterations - w.it0);
+        w.m_eff = (int)std::min<size_t>((size_t)(w.m - base), w.o.maximum_iterations - w.it0);
         w.steps = w.dcgs2 ? w.m_eff + 1 : w.m_eff;
         w.issued = w.checked = 0;
         w.cycle_done = false;
@@ -1271,10 +1410,26 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
         }
         // all queued steps are complete: the last status holds the cycle's end
         const StatusBlock& last = w.pin->ring[(w.issued - 1) & 1];
-        rep.iterations = w.it0 + (size_t)last.used;
+        rep.iterations = w.it0 + (size_t)(last.used - w.cyc_base);
         w.breakdown = last.breakdown != 0;
         vie::backsolve(sd, st);
         HIP_OK(hipMemcpyAsync(&w.pin->fin, sd, sizeof(StatusBlock), hipMemcpyDeviceToHost, st));
+        if (w.rk && !w.rec_ready && w.cycles == 1) {
+          // the first cycle's Hessenberg and last-step coefficients, for the
+          // recycled subspace should the solve need another cycle
+          const int J = last.used, m = w.m;
+          w.rec_used = J;
+          w.rec_breakdown = last.breakdown != 0;
+          const char* sp = (const char*)sd;
+          if (J > 0) {
+            HIP_OK(hipMemcpyAsync(w.rec_host, sp + offsetof(vie::ArnoldiState, Hraw),
+                                  (size_t)J * (m + 1) * sizeof(c64), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipMemcpyAsync(w.rec_host + (size_t)(m + 1) * m, sp + offsetof(vie::ArnoldiState, dc_a),
+                                  (size_t)J * sizeof(c64), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipMemcpyAsync(w.rec_host + (size_t)(m + 1) * m + m, sp + offsetof(vie::ArnoldiState, dc_alpha),
+                                  sizeof(double), hipMemcpyDeviceToHost, st));
+          }
+        }
         HIP_OK(hipEventRecord(w.ev_misc, st));
         w.phase = PH_BACKSOLVE;
         break;
@@ -1289,7 +1444,8 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           job_finish(w, VIE_ITERATIVE_BREAKDOWN, 0);
           return 1;
         }
-        vie::combine<T>(w.x.as<c64>(), w.V.as<vie::cx<T>>(), w.ldv, w.n, sd, jacobi_view(s), 1, st);
+        vie::combine<T>(w.x.as<c64>(), cycle_basis<T>(w), w.ldv, w.n, sd, jacobi_view(s), 1, st,
+                        w.cyc_base ? w.V.as<vie::cx<T>>() : nullptr);
         if (!job_issue_residual(w)) {
           job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
           return 1;
@@ -1339,7 +1495,8 @@ static int job_prepare(dielectric_fft_system* s, SolverWork& w, const vie_gmres_
                        const vie_hip_solve_options* so, vie_gmres_report* rep) {
   size_t m = device_restart(s->c + s->v, o);
   int prec = so->precision == VIE_HIP_MIXED ? VIE_HIP_MIXED : VIE_HIP_FP64;
-  if (!m || !ensure_solver(s, w, (int)m, prec) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan))) {
+  if (!m || !ensure_solver(s, w, (int)m, prec, recycle_count(so, prec, (int)m)) || !upload_blocks(s) ||
+      (s->plan && !plan_upload_eps(s->plan))) {
     rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
     return 0;
   }
@@ -1476,8 +1633,9 @@ extern "C" int dielectric_fft_reserve(dielectric_fft_system* s, const vie_gmres_
   if (m > n) m = n;
   if (m > options->maximum_iterations) m = options->maximum_iterations;
   if (m > (size_t)vie::kMaxRestart) return 0;
-  // FP64-sized basis serves both precisions
-  if (!ensure_solver(s, s->sw, (int)m, VIE_HIP_FP64) || !upload_blocks(s) || (s->plan && !plan_upload_eps(s->plan)) ||
+  // FP64-sized basis serves both precisions (and a MIXED one with its recycled subspace)
+  if (!ensure_solver(s, s->sw, (int)m, VIE_HIP_FP64) || !ensure_recycle(s->sw) || !upload_blocks(s) ||
+      (s->plan && !plan_upload_eps(s->plan)) ||
       !plan_tune_prec(s->plan, VIE_HIP_FP64) || !plan_tune_prec(s->plan, VIE_HIP_MIXED))
     return 0;
   HIP_OK(hipStreamSynchronize(s->stream));
@@ -1899,6 +2057,24 @@ extern "C" int dielectric_fft_last_solve_escalated(const dielectric_fft_system* 
   return s && s->sw.escalated ? 1 : 0;
 }
 
+extern "C" int dielectric_fft_last_solve_recycled(const dielectric_fft_system* s) {
+  return s && s->sw.rec_ready ? s->sw.rk : 0;
+}
+
+extern "C" int vie_hip_harmonic_ritz(const double _Complex* hbar, int m, int k, double _Complex* p) {
+  namespace D = vie::dense;
+  if (!hbar || !p || m < 1 || m > vie::kMaxRestart || k < 1 || k > m) return 0;
+  const c64* h = (const c64*)hbar;
+  D::Mat Hb(m + 1, m), P;
+  for (int c = 0; c < m; c++)
+    for (int r = 0; r <= m; r++) Hb(r, c) = D::zd(h[(size_t)c * (m + 1) + r].re, h[(size_t)c * (m + 1) + r].im);
+  if (!D::harmonic_ritz(Hb, k, P)) return 0;
+  c64* out = (c64*)p;
+  for (int c = 0; c < k; c++)
+    for (int r = 0; r < m; r++) out[(size_t)c * m + r] = c64{P(r, c).re, P(r, c).im};
+  return 1;
+}
+
 // Mean basis column of the matvec Arnoldi steps the system's own solver
 // issued since the last reset (the column sets the reduction's width).
 extern "C" int dielectric_fft_solve_stats(dielectric_fft_system* s, double* mean_column, unsigned long long* steps,
@@ -2041,7 +2217,7 @@ extern "C" size_t vie_hip_batch_job_bytes(const dielectric_fft_system* s, const 
   if (m > n) m = n;
   if (m > options->maximum_iterations) m = options->maximum_iterations;
   int prec = so && so->precision == VIE_HIP_MIXED ? VIE_HIP_MIXED : VIE_HIP_FP64;
-  return solver_bytes(s->nsys(), (int)m, prec);
+  return solver_bytes(s->nsys(), (int)m, prec, recycle_count(so, prec, (int)m));
 }
 
 extern "C" int vie_hip_batch_reserve(vie_hip_batch* b, size_t count, dielectric_fft_system* s,
@@ -2060,7 +2236,8 @@ extern "C" int vie_hip_batch_reserve(vie_hip_batch* b, size_t count, dielectric_
   // FP64-sized bases serve either precision
   for (size_t i = 0; i < count; i++) {
     SolverWork& w = *b->pool[b->pool.size() - 1 - i];
-    if (!ensure_solver(s, w, (int)m, VIE_HIP_FP64) || !host_io_reserve(w, s->c + s->v)) return 0;
+    if (!ensure_solver(s, w, (int)m, VIE_HIP_FP64) || !ensure_recycle(w) || !host_io_reserve(w, s->c + s->v))
+      return 0;
     w.sys = nullptr;
   }
   HIP_OK(hipDeviceSynchronize());

@@ -5,6 +5,7 @@
 namespace vie {
 
 constexpr int kMaxRestart = 128;           // largest Krylov cycle the device solver keeps
+constexpr int kMaxRecycle = 32;            // largest recycled subspace (GCRO cycles)
 constexpr int kMaxConductorsShared = 256;  // conductor coefficients staged in LDS
 
 // Small per-solve state kept in device memory.  The host reads the leading
@@ -21,7 +22,11 @@ struct ArnoldiState {
   int pad_;
   // --- working data
   int j, m;
-  int step, pad2_;  // Arnoldi steps issued in this cycle (status ring slot)
+  int step;  // Arnoldi steps issued in this cycle (status ring slot)
+  // first Krylov column of the cycle: columns 0..base-1 of the basis hold the
+  // recycled images C (A M^{-1} U = C, orthonormal) and are never expanded;
+  // Hessenberg rows 0..base-1 are the projections C^H A M^{-1} q_j (unrotated)
+  int base;
   double inv_beta, inv_next;
   double inner_target;  // cycle stops once |g_{j+1}| <= inner_target
   // delayed (DCGS2) orthogonalisation: coefficients of the fused update,
@@ -35,6 +40,10 @@ struct ArnoldiState {
   c64 sn[kMaxRestart];
   c64 y[kMaxRestart];
   c64 H[(kMaxRestart + 1) * kMaxRestart];
+  // recycled cycles: projection of the cycle's start residual on C (in units
+  // of beta), and the coefficients of U in the cycle's solution update
+  c64 zc[kMaxRecycle];
+  c64 uc[kMaxRecycle];
 };
 
 // The leading status words of ArnoldiState, as the host reads them.
@@ -113,12 +122,23 @@ void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, Re
 template <typename T>
 int dcgs2_time(cx<T>* Q, size_t ldv, size_t n, ArnoldiState* st, ReduceScratch& rs, int j, int reps, hipStream_t s,
                float* ms_reduce, float* ms_update);
-void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s);
+// Starts a cycle at basis column `base` (0 for plain GMRES cycles).
+void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, int base, hipStream_t s);
+// Recycled cycle start: with the scaled residual u in column k of Q and the
+// recycled images C in columns 0..k-1, st->zc = C^H u and u -= C zc.
+template <typename T>
+void recycle_project(cx<T>* Q, size_t ldv, int k, size_t n, ArnoldiState* st, ReduceScratch& rs, hipStream_t s);
+// out[:, i] = sum_c in[:, c] coef[c + i * nin] for i < nout (column stride ldv).
+template <typename T>
+void block_combine(cx<T>* out, int nout, const cx<T>* in, int nin, size_t ldv, const c64* coef, size_t n,
+                   hipStream_t s);
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s);
 template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st, hipStream_t s);
+// x += M^{-1} (sum_k y_k V[k] + sum_l uc_l U[l]); U (columns of stride ldv)
+// is read only in a recycled cycle (st->base > 0).
 template <typename T>
 void combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const ArnoldiState* st, const JacobiView& J, int precond,
-             hipStream_t s);
+             hipStream_t s, const cx<T>* U = nullptr);
 template <typename T>
 void to_box(const c64* user, size_t c, size_t cells, const size_t* map, size_t nbox, cx<T>* sys, hipStream_t s);
 template <typename T>

@@ -310,16 +310,19 @@ __global__ __launch_bounds__(KWG) void k_arnoldi_tail(ArnoldiState* st, const c6
   st->done = st->breakdown || st->nonfinite || st->gabs <= st->inner_target || j + 1 >= m;
 }
 
-// y = H(0:used,0:used)^{-1} g by back substitution (vie_iterative.c:179-186):
-// one row dot product per step, spread over the block.
+// y = H(base:used,base:used)^{-1} g by back substitution
+// (vie_iterative.c:179-186): one row dot product per step, spread over the
+// block.  A recycled cycle also forms the U coefficients of its update,
+// uc = beta zc - B y, with B = H(0:base, base:used) (unrotated rows).
 __global__ __launch_bounds__(KWG) void k_backsolve(ArnoldiState* st) {
   __shared__ c64 y[kMaxRestart];
   __shared__ c64 lds[KWG / 64];
   __shared__ int bad;
-  const int used = st->used, m = st->m;
+  const int used = st->used, m = st->m, base = st->base;
   const c64* H = st->H;
   if (threadIdx.x == 0) bad = 0;
-  for (int i = used - 1; i >= 0; i--) {
+  __syncthreads();
+  for (int i = used - 1; i >= base; i--) {
     c64 part = {0, 0};
     for (int k = i + 1 + threadIdx.x; k < used; k += KWG) part = part + H[i * m + k] * y[k];
     part = block_reduce(part, lds);
@@ -331,24 +334,36 @@ __global__ __launch_bounds__(KWG) void k_backsolve(ArnoldiState* st) {
     __syncthreads();
     if (bad) break;
   }
+  __syncthreads();
   if (threadIdx.x == 0) st->singular = bad;
-  for (int k = threadIdx.x; k < used && !bad; k += KWG) st->y[k] = y[k];
+  if (bad) return;
+  for (int k = base + threadIdx.x; k < used; k += KWG) st->y[k] = y[k];
+  const double beta = st->beta;
+  for (int l = threadIdx.x; l < base; l += KWG) {
+    c64 s = beta * st->zc[l];
+    for (int c = base; c < used; c++) s = s - H[l * m + c] * y[c];
+    st->uc[l] = s;
+  }
 }
 
-// Starts a cycle: g = [beta, 0, ...], H = 0, j = 0.
-__global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_target) {
+// Starts a cycle at column base: g = beta e_base, H = 0, j = base, and the
+// columns of Hraw that stand for the recycled images zeroed (they have no
+// Arnoldi relation of their own).
+__global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, int base) {
   const int m = st->m;
   for (int i = threadIdx.x; i < (m + 1) * m; i += blockDim.x) st->H[i] = c64{0, 0};
   for (int i = threadIdx.x; i <= m; i += blockDim.x) st->g[i] = c64{0, 0};
+  for (int i = threadIdx.x; i < (m + 1) * base; i += blockDim.x) st->Hraw[i] = c64{0, 0};
   __syncthreads();
   if (threadIdx.x == 0) {
     double beta = sqrt(beta2->re);
-    st->g[0] = c64{beta, 0};
+    st->g[base] = c64{beta, 0};
     st->beta = beta;
     st->inv_beta = beta > 0 ? 1.0 / beta : 0.0;
-    st->j = 0;
+    st->j = base;
+    st->base = base;
     st->step = 0;
-    st->used = 0;
+    st->used = base;
     st->breakdown = 0;
     st->nonfinite = !isfinite(beta);
     st->done = 0;
@@ -465,7 +480,7 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   __shared__ double alpha_s;
   __shared__ int stop;
   if (st->done) return;
-  const int j = st->j;
+  const int j = st->j, base = st->base;
   const int m = st->m, ld = m + 1;  // Hraw is column-major: Hraw[l * (m + 1) + k] = row k, column l
   const int tid = threadIdx.x;
   c64* Hraw = st->Hraw;
@@ -479,11 +494,11 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   if (tid == 0) sc[0] = dots[has_w ? 2 * j : j];
   if (tid == 1) sc[1] = has_w ? dots[2 * j + 1] : c64{0, 0};
   if (tid == 2) sc[2] = has_w ? dots[2 * j + 2] : c64{0, 0};
-  if (tid == 3) sc[3] = st->g[j > 0 ? j - 1 : 0];
+  if (tid == 3) sc[3] = st->g[j > base ? j - 1 : base];
   if (tid == 4) sd[0] = st->dc_before;
   if (tid == 5) sd[1] = st->inner_target;
   // old column j-1 of Hraw, rows 0..j-1 (contiguous)
-  if (j > 0)
+  if (j > base)
     for (int k = tid; k < j; k += KWG) col[k] = Hraw[(size_t)(j - 1) * ld + k];
   __syncthreads();
   // |a|^2 and a^H b in parallel
@@ -501,7 +516,7 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   pab = block_reduce(pab, red);  // thread 0 holds a^H b
   const double alpha = alpha_s;
   // column j-1 becomes final: Hraw(0:j, j-1) += a, Hraw(j, j-1) = alpha
-  if (j > 0) {
+  if (j > base) {
     for (int k = tid; k <= j; k += KWG) {
       c64 v = k < j ? col[k] + a[k] : c64{alpha, 0};
       Hraw[(size_t)(j - 1) * ld + k] = v;
@@ -514,14 +529,22 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
     if (!isfinite(alpha)) {
       st->nonfinite = 1;
       stop = 1;
-    } else if (j == 0) {
-      // q_0 = u / alpha: the least-squares right-hand side is beta * alpha
-      st->g[0] = alpha * sc[3];
+    } else if (j == base) {
+      // q_base = (u - Q a) / alpha: the least-squares right-hand side is
+      // beta * alpha.  In a recycled cycle u was projected on C once and a
+      // holds the second projection; an alpha at rounding level means the
+      // residual lies in span(C), which the U update alone then removes.
+      st->g[base] = alpha * sc[3];
+      if (base > 0 && alpha <= breakdown_eps) {
+        st->breakdown = 1;
+        st->used = base;
+        stop = 1;
+      }
     } else {
       const int c = j - 1;
       const int brk = alpha <= breakdown_eps * sd[0];
       st->breakdown = brk;
-      for (int k = 0; k < c; k++) {
+      for (int k = base; k < c; k++) {
         c64 f = col[k], gg = col[k + 1];
         col[k] = cs[k] * f + sn[k] * gg;
         col[k + 1] = cs[k] * gg - conj(sn[k]) * f;
@@ -547,9 +570,17 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
     if (stop) st->done = 1;
   }
   __syncthreads();
-  if (j > 0)
+  if (j > base)
     for (int k = tid; k <= j; k += KWG) st->H[k * m + j - 1] = col[k];
-  if (stop) return;
+  if (j == base)
+    for (int k = tid; k < base; k += KWG) st->zc[k] = st->zc[k] + a[k];
+  if (stop) {
+    // the last basis vector is left as u = alpha q_j + Q_j a: keep a and alpha
+    // so the host can still form q_j (recycled-subspace construction)
+    for (int k = tid; k < j; k += KWG) st->dc_a[k] = a[k];
+    if (tid == 0) st->dc_alpha = alpha;
+    return;
+  }
   // t = Hraw(0:j, 0:j-1) a (Hessenberg: row k only has columns >= k-1).
   // Threads own rows (consecutive rows are contiguous in a column, so each
   // column read is coalesced); the two halves of the block split the columns.
@@ -738,21 +769,27 @@ __device__ inline c64 jacobi_inv(const JacobiView& J, size_t i, c64 v) {
   return cdiv(v, e - (e - one) * J.self[comp]);
 }
 
-// x += M^{-1} (sum_k y_k V[k])   (vie_iterative.c:187-188 with Z = M^{-1} V)
+// x += M^{-1} (sum_k y_k V[k] + sum_l uc_l U[l])   (vie_iterative.c:187-188
+// with Z = M^{-1} V; the U term only in a recycled cycle)
 template <typename T>
 __global__ void k_combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const ArnoldiState* st, JacobiView J,
-                          int precond) {
-  __shared__ c64 ys[kMaxRestart];
+                          int precond, const cx<T>* U) {
+  __shared__ c64 ys[kMaxRestart], us[kMaxRecycle];
   if (st->singular) return;  // no least-squares solution: x stays as it is
-  const int used = st->used;
-  for (int k = threadIdx.x; k < used; k += blockDim.x) ys[k] = st->y[k];
+  const int used = st->used, base = U ? st->base : 0;
+  for (int k = base + threadIdx.x; k < used; k += blockDim.x) ys[k] = st->y[k];
+  for (int k = threadIdx.x; k < base; k += blockDim.x) us[k] = st->uc[k];
   __syncthreads();
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   c64 s = {0, 0};
-  for (int k = 0; k < used; k++) {
+  for (int k = base; k < used; k++) {
     cx<T> a = ld_stream(V + (size_t)k * ldv + i);
     s = s + ys[k] * c64{(double)a.re, (double)a.im};
+  }
+  for (int k = 0; k < base; k++) {
+    cx<T> a = ld_stream(U + (size_t)k * ldv + i);
+    s = s + us[k] * c64{(double)a.re, (double)a.im};
   }
   if (precond) s = jacobi_inv(J, i, s);
   x[i] = x[i] + s;
@@ -875,6 +912,34 @@ __global__ void k_from_box(const cx<T>* sys, size_t c, size_t cells, const size_
 __global__ void k_precondition(const c64* x, c64* y, size_t n, JacobiView J) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = jacobi_inv(J, i, x[i]);
+}
+
+// out[:, i] = sum_c in[:, c] coef[c + i * nin]: the recycled subspace U, C as
+// combinations of a finished cycle's basis.  Each thread owns one element and
+// keeps BC_OUT outputs in registers; the coefficients are uniform (scalar
+// loads).  out and in do not overlap.
+constexpr int BC_OUT = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_combine(cx<T>* out, int nout, const cx<T>* in, int nin, size_t ldv,
+                                                       const c64* coef, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  for (int i0 = 0; i0 < nout; i0 += BC_OUT) {
+    const int cnt = nout - i0 < BC_OUT ? nout - i0 : BC_OUT;
+    c64 acc[BC_OUT];
+#pragma unroll
+    for (int q = 0; q < BC_OUT; q++) acc[q] = c64{0, 0};
+    for (int c = 0; c < nin; c++) {
+      const cx<T> a = in[(size_t)c * ldv + i];
+      const c64 v = {(double)a.re, (double)a.im};
+#pragma unroll
+      for (int q = 0; q < BC_OUT; q++)
+        if (q < cnt) acc[q] = acc[q] + coef[(size_t)(i0 + q) * nin + c] * v;
+    }
+#pragma unroll
+    for (int q = 0; q < BC_OUT; q++)
+      if (q < cnt) out[(size_t)(i0 + q) * ldv + i] = cx<T>{(T)acc[q].re, (T)acc[q].im};
+  }
 }
 
 // Streaming 16-byte copy (HBM traffic calibration probe).
@@ -1012,8 +1077,20 @@ int dcgs2_time(cx<T>* Q, size_t ldv, size_t n, ArnoldiState* st, ReduceScratch& 
   return ok && hipGetLastError() == hipSuccess;
 }
 
-void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, hipStream_t s) {
-  hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2, inner_target);
+void cycle_init(ArnoldiState* st, const c64* beta2, double inner_target, int base, hipStream_t s) {
+  hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(256), 0, s, st, beta2, inner_target, base);
+}
+template <typename T>
+void recycle_project(cx<T>* Q, size_t ldv, int k, size_t n, ArnoldiState* st, ReduceScratch& rs, hipStream_t s) {
+  c64* zc = st->zc;  // device address of the state's projection coefficients
+  multidot<T>(Q, ldv, k, Q + (size_t)k * ldv, n, 0, rs, zc, s);
+  update<T>(Q, ldv, k, zc, Q + (size_t)k * ldv, n, 0, rs, s);
+}
+template <typename T>
+void block_combine(cx<T>* out, int nout, const cx<T>* in, int nin, size_t ldv, const c64* coef, size_t n,
+                   hipStream_t s) {
+  if (n && nout > 0 && nin > 0)
+    hipLaunchKernelGGL((k_block_combine<T>), dim3(gblocks(n)), dim3(256), 0, s, out, nout, in, nin, ldv, coef, n);
 }
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s) {
   hipLaunchKernelGGL((k_scale_from<T>), dim3(gblocks(n)), dim3(256), 0, s, v, r, n, st);
@@ -1023,8 +1100,8 @@ template <typename T> void scale_next(cx<T>* v, size_t n, const ArnoldiState* st
 }
 template <typename T>
 void combine(c64* x, const cx<T>* V, size_t ldv, size_t n, const ArnoldiState* st, const JacobiView& J, int precond,
-             hipStream_t s) {
-  hipLaunchKernelGGL((k_combine<T>), dim3(gblocks(n)), dim3(256), 0, s, x, V, ldv, n, st, J, precond);
+             hipStream_t s, const cx<T>* U) {
+  hipLaunchKernelGGL((k_combine<T>), dim3(gblocks(n)), dim3(256), 0, s, x, V, ldv, n, st, J, precond, U);
 }
 template <typename T>
 void to_box(const c64* user, size_t c, size_t cells, const size_t* map, size_t nbox, cx<T>* sys, hipStream_t s) {
@@ -1086,7 +1163,9 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
   template void scale_from<T>(cx<T>*, const c64*, size_t, const ArnoldiState*, hipStream_t);                     \
   template void scale_next<T>(cx<T>*, size_t, const ArnoldiState*, hipStream_t);                                 \
   template void combine<T>(c64*, const cx<T>*, size_t, size_t, const ArnoldiState*, const JacobiView&, int,       \
-                           hipStream_t);                                                                         \
+                           hipStream_t, const cx<T>*);                                                           \
+  template void recycle_project<T>(cx<T>*, size_t, int, size_t, ArnoldiState*, ReduceScratch&, hipStream_t);      \
+  template void block_combine<T>(cx<T>*, int, const cx<T>*, int, size_t, const c64*, size_t, hipStream_t);        \
   template void to_box<T>(const c64*, size_t, size_t, const size_t*, size_t, cx<T>*, hipStream_t);                \
   template void from_box<T>(const cx<T>*, size_t, size_t, const size_t*, size_t, c64*, hipStream_t);              \
   template void coupling_apply<T>(const CouplingView<T>&, const cx<T>*, const cx<T>*, cx<T>*, cx<T>*, int,         \

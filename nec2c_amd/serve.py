@@ -50,7 +50,8 @@ def system_device_bytes(shape, conductors, pec=False):
 def job_device_bytes(shape, conductors, restart, precision=vie.MIXED):
     """Device bytes one in-flight solve holds (Krylov basis, FP64 vectors,
     reduction scratch); the pooled basis is FP64-sized so it serves either
-    precision."""
+    precision (a mixed basis with its recycled subspace, restart + 1 + 2k FP32
+    columns with k <= restart / 2, fits in it)."""
     n = conductors + 3 * int(np.prod(shape))
     ldv = (n + 63) // 64 * 64
     return (restart + 1) * ldv * 16 + 4 * n * 16 + KRYLOV_SCRATCH
@@ -74,10 +75,11 @@ class Request:
 
 class Server:
     def __init__(self, budget_bytes: int, precision=vie.MIXED, restart=120, maximum_iterations=2000,
-                 relative_tolerance=1e-8, inner_reduction=0.0, clock=time.perf_counter):
+                 relative_tolerance=1e-8, inner_reduction=0.0, recycle=0, clock=time.perf_counter):
         self.budget = int(budget_bytes)
         self.precision = precision
         self.inner_reduction = inner_reduction
+        self.recycle = recycle
         self.opts = dict(restart=restart, maximum_iterations=maximum_iterations,
                          relative_tolerance=relative_tolerance)
         self.restart = restart
@@ -136,6 +138,7 @@ class Server:
         self.pending[req.rid] = len(req.rhs)
         for port, b in enumerate(req.rhs):
             jid = self.batch.submit(entry[0], b, precision=self.precision, inner_reduction=self.inner_reduction,
+                                    recycle=self.recycle,
                                     **self.opts)
             self.inflight[jid] = (req, port)
             self.used += self._job_bytes(req)
@@ -224,7 +227,7 @@ class Server:
         self.cache.clear()
 
 
-def run_batch(jobs, precision=vie.MIXED, inflight=128, inner_reduction=0.0, batch=None, **opts):
+def run_batch(jobs, precision=vie.MIXED, inflight=128, inner_reduction=0.0, batch=None, recycle=0, **opts):
     """Solves a closed batch: `jobs` is a list of (system, rhs).  At most
     `inflight` jobs are on the device at once; a finished job's slot is refilled
     at once (continuous batching).  `batch` may be a caller's ``vie.Batch``
@@ -241,7 +244,8 @@ def run_batch(jobs, precision=vie.MIXED, inflight=128, inner_reduction=0.0, batc
         while nxt < len(jobs) or ids:
             while nxt < len(jobs) and len(ids) < inflight:
                 s, b = jobs[nxt]
-                ids[batch.submit(s, b, precision=precision, inner_reduction=inner_reduction, **opts)] = nxt
+                ids[batch.submit(s, b, precision=precision, inner_reduction=inner_reduction, recycle=recycle,
+                                 **opts)] = nxt
                 nxt += 1
             ready, _ = batch.progress()
             for jid in ready:

@@ -60,7 +60,9 @@ class GmresReport(ctypes.Structure):           # vie_iterative.h:39-47
 
 
 class SolveOptions(ctypes.Structure):
-    _fields_ = [("precision", ctypes.c_int), ("inner_reduction", ctypes.c_double)]
+    """vie_hip_solve_options; recycle: recycled subspace size (0 default per
+    precision, -1 off)."""
+    _fields_ = [("precision", ctypes.c_int), ("inner_reduction", ctypes.c_double), ("recycle", ctypes.c_int)]
 
 
 class Material(ctypes.Structure):              # dielectric.h:26-35
@@ -126,7 +128,7 @@ EXPORTS = (
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
     "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
-    "dielectric_fft_last_solve_escalated",
+    "dielectric_fft_last_solve_escalated", "dielectric_fft_last_solve_recycled", "vie_hip_harmonic_ritz",
 )
 
 _LIB = None
@@ -210,6 +212,8 @@ def lib() -> ctypes.CDLL:
         "dielectric_fft_time_arnoldi": (i, [vp, i, i, i] + [ctypes.POINTER(d)] * 4),
         "dielectric_fft_solve_stats": (i, [vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_ulonglong), i]),
         "dielectric_fft_last_solve_escalated": (i, [vp]),
+        "dielectric_fft_last_solve_recycled": (i, [vp]),
+        "vie_hip_harmonic_ritz": (i, [vp, i, i, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -217,6 +221,19 @@ def lib() -> ctypes.CDLL:
         f.argtypes = args
     _LIB = L
     return L
+
+
+def harmonic_ritz(hbar, k):
+    """vie_hip_harmonic_ritz: the k harmonic Ritz vectors of smallest modulus
+    of an (m+1) x m Arnoldi Hessenberg, as an m x k array (host computation)."""
+    hbar = np.asfortranarray(hbar, dtype=np.complex128)
+    m = hbar.shape[1]
+    if hbar.shape != (m + 1, m):
+        raise ValueError("hbar must be (m+1) x m")
+    p = np.zeros((m, k), dtype=np.complex128, order="F")
+    if not lib().vie_hip_harmonic_ritz(hbar.ctypes.data, m, k, p.ctypes.data):
+        raise RuntimeError("harmonic Ritz vectors failed")
+    return p
 
 
 def last_error() -> str:
@@ -495,7 +512,7 @@ class System:
             raise RuntimeError(f"precondition failed: {last_error()}")
         return y
 
-    def solve(self, b, x=None, precision=None, inner_reduction=0.0, **opts):
+    def solve(self, b, x=None, precision=None, inner_reduction=0.0, recycle=0, **opts):
         """dielectric_fft_solve; precision None keeps the library default."""
         b = np.ascontiguousarray(b, dtype=np.complex128)
         x = np.zeros_like(b) if x is None else np.array(x, dtype=np.complex128)
@@ -505,7 +522,7 @@ class System:
         if precision is not None:
             old = SolveOptions()
             L.vie_hip_get_solve_options(ctypes.byref(old))
-            L.vie_hip_set_solve_options(ctypes.byref(SolveOptions(precision, inner_reduction)))
+            L.vie_hip_set_solve_options(ctypes.byref(SolveOptions(precision, inner_reduction, recycle)))
         try:
             ok = L.dielectric_fft_solve(self.s, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(rep))
         finally:
@@ -528,9 +545,9 @@ class System:
         if not lib().dielectric_fft_reserve(self.s, ctypes.byref(o)):
             raise RuntimeError(last_error())
 
-    def solve_resident(self, precision=FP64, inner_reduction=0.0, **opts):
+    def solve_resident(self, precision=FP64, inner_reduction=0.0, recycle=0, **opts):
         o = _options(**opts)
-        so = SolveOptions(precision, inner_reduction)
+        so = SolveOptions(precision, inner_reduction, recycle)
         rep = GmresReport()
         ok = lib().dielectric_fft_solve_resident(self.s, ctypes.byref(o), ctypes.byref(so), ctypes.byref(rep))
         return bool(ok), rep
@@ -563,6 +580,11 @@ class System:
     def escalated(self):
         """True when the last MIXED device solve finished in FP64 cycles."""
         return bool(lib().dielectric_fft_last_solve_escalated(self.s))
+
+    @property
+    def recycled(self):
+        """Recycled-subspace size the last device solve built and used (0: plain cycles)."""
+        return int(lib().dielectric_fft_last_solve_recycled(self.s))
 
     def solve_stats(self, reset=False):
         """(mean basis column, matvec Arnoldi steps) of the resident solves since the last reset."""
@@ -607,9 +629,9 @@ class Batch:
         self._orders = {}
 
     @staticmethod
-    def job_bytes(system, precision=FP64, **opts):
+    def job_bytes(system, precision=FP64, recycle=0, **opts):
         o = _options(**opts)
-        so = SolveOptions(precision, 0.0)
+        so = SolveOptions(precision, 0.0, recycle)
         return lib().vie_hip_batch_job_bytes(system.s, ctypes.byref(o), ctypes.byref(so))
 
     def reserve(self, count, system, **opts):
@@ -617,12 +639,12 @@ class Batch:
         if not lib().vie_hip_batch_reserve(self.b, count, system.s, ctypes.byref(o)):
             raise RuntimeError(f"batch reserve failed: {last_error()}")
 
-    def submit(self, system, rhs, precision=FP64, inner_reduction=0.0, **opts):
+    def submit(self, system, rhs, precision=FP64, inner_reduction=0.0, recycle=0, **opts):
         rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
         if rhs.size != system.order:
             raise ValueError(f"right-hand side has {rhs.size} entries, the system {system.order}")
         o = _options(**opts)
-        so = SolveOptions(precision, inner_reduction)
+        so = SolveOptions(precision, inner_reduction, recycle)
         jid = lib().vie_hip_batch_submit(self.b, system.s, _ptr(rhs), ctypes.byref(o), ctypes.byref(so))
         if jid <= 0:
             raise RuntimeError(f"batch submit failed: {last_error()}")

@@ -223,6 +223,54 @@ def test_struct_layouts_match_c():
     assert ctypes.sizeof(vie.Model) == 112
     assert ctypes.sizeof(vie.Material) == 64
     assert ctypes.sizeof(vie.GmresReport) == 48
+    assert ctypes.sizeof(vie.SolveOptions) == 24
+
+
+def _arnoldi_hessenberg(A, b, m):
+    n = len(b)
+    V = np.zeros((n, m + 1), complex)
+    H = np.zeros((m + 1, m), complex)
+    V[:, 0] = b / np.linalg.norm(b)
+    for j in range(m):
+        w = A @ V[:, j]
+        for _ in range(2):
+            h = V[:, :j + 1].conj().T @ w
+            w = w - V[:, :j + 1] @ h
+            H[:j + 1, j] += h
+        H[j + 1, j] = np.linalg.norm(w)
+        V[:, j + 1] = w / H[j + 1, j]
+    return H
+
+
+@pytest.mark.parametrize("m,k", [(12, 4), (60, 10), (120, 20)])
+def test_harmonic_ritz_vectors(m, k):
+    """Host step of the recycled subspace against its definition (Morgan 2002):
+    each column is a unit eigenvector of H_m + |h_{m+1,m}|^2 H_m^{-H} e_m e_m^T,
+    and the eigenvalues are its k of smallest modulus (numpy's eig as the check).
+    The Hessenberg comes from an Arnoldi run on a shifted random operator with a
+    few small outlying eigenvalues, the shape the solver's cycles produce."""
+    rng = np.random.default_rng(m)
+    n = 400
+    A = (2 + 0.5j) * np.eye(n) + 0.3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    A[:6, :6] += np.diag([-1.9, -1.8, -1.95, -1.7, -1.85, -1.6])
+    H = _arnoldi_hessenberg(A, rng.standard_normal(n) + 0j, m)
+    P = vie.harmonic_ritz(H, k)
+    Hm = H[:m]
+    em = np.zeros(m)
+    em[-1] = 1
+    f = np.linalg.solve(Hm.conj().T, em)
+    M = Hm + abs(H[m, m - 1]) ** 2 * np.outer(f, em)
+    ref = np.sort(np.abs(np.linalg.eigvals(M)))[:k]
+    got = []
+    for c in range(k):
+        v = P[:, c]
+        assert abs(np.linalg.norm(v) - 1) < 1e-12
+        lam = v.conj() @ M @ v
+        assert np.linalg.norm(M @ v - lam * v) <= 1e-10 * np.linalg.norm(M)
+        got.append(abs(lam))
+    assert np.allclose(np.sort(got), ref, rtol=1e-9)
+    # a wrong selection (largest modulus) or a dropped rank-one term fails:
+    assert not np.allclose(np.sort(got), np.sort(np.abs(np.linalg.eigvals(Hm)))[:k], rtol=1e-6)
 
 
 def test_bench_refuses_more_than_one_gpu():

@@ -179,3 +179,33 @@ def test_mixed_escalates_to_fp64_when_fp32_stalls(ref):
     ok64, x64, rep64 = s.solve(b, precision=vie.FP64, relative_tolerance=1e-6, restart=30, maximum_iterations=600)
     assert ok64 and not s.escalated
     s.close()
+
+
+@pytest.mark.parametrize("precision", [vie.FP64, vie.MIXED])
+def test_recycled_cycles_converge_to_reference(ref, precision):
+    """GMRES cycles over a recycled subspace (GCRO with the subspace of the
+    first cycle's harmonic Ritz vectors kept fixed): restart 16 forces several
+    cycles on the coupled 6^3 system.  The recycled solve must reach the
+    reference operator's residual target and the reference GMRES solution, and
+    plain cycles of the same restart must still do so too."""
+    n = 6
+    s, eps, pitch, k0, blocks, dip = coupled_case(n)
+    rp, rc = ref_coupled(ref, n, pitch, k0, eps, blocks)
+    b = np.zeros(s.order, complex)
+    b[:21] = dip.rhs()
+    okr, xr, _ = rc.gmres(b, rtol=1e-10)
+    assert okr
+    opts = dict(relative_tolerance=1e-10, restart=16, maximum_iterations=3000)
+    ok0, x0, rep0 = s.solve(b, precision=precision, recycle=-1, **opts)
+    assert ok0 and s.recycled == 0
+    ok, x, rep = s.solve(b, precision=precision, recycle=6, **opts)
+    print(precision, "plain", rep0.as_dict(), "recycled", rep.as_dict(), "k", s.recycled, flush=True)
+    assert ok and rep.restarts >= 1 and s.recycled == 6
+    r = b - rc.apply(x)
+    assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(b)
+    fi = dip.feed_index()
+    for sol in (x0, x):
+        zerr = abs(1 / sol[fi] - 1 / xr[fi]) / abs(1 / xr[fi])
+        assert zerr < 1e-6, zerr
+        assert rel(sol[:21], xr[:21]) < 1e-6
+    s.close()
