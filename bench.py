@@ -523,13 +523,13 @@ def main():
         s.solve_resident(vie.FP64, **opts)
     vie.synchronize()
 
-    def run(prec):
+    def run(prec, recycle=0):
         lat, reps, sols = [], [], []
         vie.synchronize()
         t = time.perf_counter()
         for s in timed:
             a = time.perf_counter()
-            ok, rep = s.solve_resident(prec, args.inner_reduction, args.recycle if prec == vie.MIXED else 0, **opts)
+            ok, rep = s.solve_resident(prec, args.inner_reduction, recycle, **opts)
             lat.append(time.perf_counter() - a)
             if not ok:
                 raise RuntimeError(f"solve failed: {vie.STATUS[rep.status]} rel {rep.relative_residual}")
@@ -540,13 +540,20 @@ def main():
     log("timing mixed-precision solves")
     for q in timed:
         q.solve_stats(reset=True)
-    t_mixed, lat_mixed, rep_mixed = run(vie.MIXED)
+    t_mixed, lat_mixed, rep_mixed = run(vie.MIXED, args.recycle)
+    rec_k = timed[0].recycled
     col_stats = [q.solve_stats(reset=True) for q in timed]
     mean_col = sum(m * c for m, c in col_stats) / max(sum(c for _, c in col_stats), 1)
     x_mixed = [s.download_solution() for s in timed]
     log("timing FP64 solves")
     t_fp64, lat_fp64, rep_fp64 = run(vie.FP64)
     x_fp64 = [s.download_solution() for s in timed]
+    # the same recycled cycles in FP64, to separate the precision gain from the recycling gain
+    k_fp64r = args.recycle if args.recycle > 0 else 20
+    log(f"timing FP64 solves with a recycled subspace of {k_fp64r}")
+    for q in timed:  # untimed: sizes each basis for FP64 plus the recycled columns
+        q.solve_resident(vie.FP64, 0.0, k_fp64r, **dict(opts, relative_tolerance=1.0))
+    t_fp64r, _, rep_fp64r = run(vie.FP64, k_fp64r)
 
     # parity between the two precisions (feed impedance, conductor currents)
     fi = dipole.feed_index()
@@ -600,6 +607,10 @@ def main():
         },
         "fp64_solves_per_s": K / t_fp64,
         "speedup_mixed_vs_fp64": t_fp64 / t_mixed,
+        "recycled_subspace_mixed": rec_k,
+        "fp64_recycled_solves_per_s": K / t_fp64r,
+        "speedup_mixed_vs_fp64_recycled": t_fp64r / t_mixed,
+        "iterations_fp64_recycled": [r.iterations for r in rep_fp64r],
         "matvecs_per_s_mixed": 1e3 / mv_ms32,
         "matvecs_per_s_fp64": 1e3 / mv_ms64,
         "solve_matvecs_per_s_mixed": apps_mixed / t_mixed,

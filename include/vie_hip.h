@@ -23,7 +23,7 @@ typedef enum {
 typedef struct {
   vie_hip_precision precision;
   /* MIXED: an FP32 cycle ends once |g_{j+1}| <= max(target, inner_reduction * beta);
-     0 selects the default (1e-5). */
+     0 selects the default (1e-4 with a recycled subspace, else 1e-5). */
   double inner_reduction;
   /* Recycled Krylov subspace (GCRO with a subspace fixed after the first
      cycle): after the first cycle the k harmonic Ritz vectors of smallest

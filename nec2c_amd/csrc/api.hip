@@ -1029,10 +1029,11 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   w.rep.residual_norm = w.rep.relative_residual = INFINITY;
   w.o = *o;
   w.so = *so;
-  w.reduction = so->inner_reduction > 0 ? so->inner_reduction : 1e-5;
   const char* ortho = getenv("VIE_HIP_ORTHO");
   w.dcgs2 = !(ortho && !strcmp(ortho, "cgs2"));
   if (!w.dcgs2) w.rk = 0;  // the recycled cycles are built on the delayed-CGS2 step
+  // with a recycled subspace a restart loses little, so FP32 cycles may end earlier
+  w.reduction = so->inner_reduction > 0 ? so->inner_reduction : (w.rk ? 1e-4 : 1e-5);
   w.cyc_base = w.cycles = w.rec_used = 0;
   w.rec_breakdown = w.rec_ready = false;
   w.after_cycle = w.breakdown = w.cycle_done = w.escalated = false;
