@@ -942,7 +942,7 @@ static int ensure_recycle(SolverWork& w) {
 
 static int ensure_solver(dielectric_fft_system* s, SolverWork& w, int m, int prec, int rk = 0) {
   size_t n = s->nsys();
-  size_t ldv = (n + 63) / 64 * 64;
+  size_t ldv = vie::basis_ld(n);
   size_t es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
   if (!ensure_scratch(w)) return 0;
   if (rk > 0 && !ensure_recycle(w)) return 0;
@@ -962,7 +962,7 @@ static int ensure_solver(dielectric_fft_system* s, SolverWork& w, int m, int pre
 }
 
 static size_t solver_bytes(size_t n, int m, int prec, int rk = 0) {
-  size_t ldv = (n + 63) / 64 * 64, es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
+  size_t ldv = vie::basis_ld(n), es = prec == VIE_HIP_MIXED ? sizeof(c32) : sizeof(c64);
   return (size_t)(m + 1 + 2 * rk) * ldv * es + 4 * n * 16 + vie::kReduceScratchBytes + sizeof(vie::ArnoldiState);
 }
 
@@ -1356,6 +1356,9 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
         const int base = w.rec_ready ? w.rk : 0;
         const double inner_target = mixed ? std::max(w.target, w.reduction * w.beta) : w.target;
         vie::cycle_init(sd, w.scal.as<c64>() + 2, inner_target, base, st);
+        // the streaming Arnoldi kernels need zero column padding; the buffer
+        // may last have held the other precision's columns
+        vie::zero_padding<T>(w.V.as<vie::cx<T>>(), w.ldv, w.n, w.V.bytes / (w.ldv * sizeof(vie::cx<T>)), st);
         {
           vie::cx<T>* Q = cycle_basis<T>(w);
           vie::scale_from<T>(Q + (size_t)base * w.ldv, w.r.as<c64>(), w.n, sd, st);

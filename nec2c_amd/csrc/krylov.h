@@ -53,6 +53,12 @@ struct StepStatus {
 };
 
 constexpr int kMaxKeys = 320;  // sums per reduction launch (>= 2 kMaxRestart + 3)
+// Krylov basis columns are kBasisAlign-element aligned and zero past the
+// vector length n: the delayed-CGS2 kernels stream whole 512-element chunks
+// with no per-element bounds test (the padding adds zeros to every sum and is
+// rewritten as zeros by the update).
+constexpr size_t kBasisAlign = 512;
+inline size_t basis_ld(size_t n) { return (n + kBasisAlign - 1) / kBasisAlign * kBasisAlign; }
 
 struct ReduceScratch {
   c64* partial = nullptr;  // kMaxDotBlocksHost x kMaxKeys partial sums
@@ -107,6 +113,8 @@ void residual(const c64* b, const c64* ax, c64* r, size_t n, ReduceScratch& rs, 
 void arnoldi_tail(ArnoldiState* st, const c64* h1, const c64* h2, const c64* before2, ReduceScratch& rs,
                   unsigned nblk, double breakdown_eps, hipStream_t s);
 void backsolve(ArnoldiState* st, hipStream_t s);
+// Zeroes entries [n, ldv) of the first cols columns of a basis.
+template <typename T> void zero_padding(cx<T>* V, size_t ldv, size_t n, size_t cols, hipStream_t s);
 // Delayed CGS2 Arnoldi step j = st->j, read on the device (slot j holds the
 // once-projected candidate u, slot j+1 holds w = A M^{-1} u, or nothing
 // without has_w: the closing step of a cycle).  One reduction pass, one small
@@ -114,6 +122,7 @@ void backsolve(ArnoldiState* st, hipStream_t s);
 // ring[st->step & 1] (device-visible host memory) and advances j unless the
 // cycle is done.  The launch sequence does not depend on j, so it can be
 // recorded once and replayed for every step.  max_j bounds j (the restart).
+// ldv = basis_ld(n) and the padding of every column read is zero.
 template <typename T>
 void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, ReduceScratch& rs, c64* dots,
                 double breakdown_eps, int max_j, StepStatus* ring, hipStream_t s);

@@ -32,12 +32,17 @@ static unsigned update_blocks(size_t n, int ept = 2) {
 }
 
 // Delayed-CGS2 streaming shapes: basis elements per lane per key in the
-// reduction, elements per thread in the fused update (FP32 takes twice as
-// many so each lane keeps the same bytes in flight).
+// reduction (64 bytes per lane either way), elements per thread in the fused
+// update.
 template <typename T> struct DcShape {
-  static constexpr int CPL = 8;
+  static constexpr int CPL = sizeof(T) == 4 ? 8 : 4;  // 64 bytes per lane per key either way
   static constexpr int UE = 2;
 };
+// the padded basis covers whole reduction chunks and update groups
+static_assert(kBasisAlign % (64 * DcShape<float>::CPL) == 0 && kBasisAlign % (64 * DcShape<double>::CPL) == 0,
+              "basis padding must cover whole delayed-CGS2 reduction chunks");
+static_assert(kBasisAlign % (KWG * DcShape<float>::UE) == 0 && kBasisAlign % (KWG * DcShape<double>::UE) == 0,
+              "basis padding must cover whole delayed-CGS2 update groups");
 template <typename T> static unsigned dc_blocks(size_t n) {
   size_t ch = (size_t)64 * DcShape<T>::CPL;
   size_t b = (n + ch - 1) / ch;
@@ -51,6 +56,29 @@ __device__ inline c64 wave_reduce(c64 v) {
   }
   return v;
 }
+
+// Wave sum by DPP row operations (no LDS crossbar), total in lane 63 only:
+// quad swaps (xor 1, xor 2), half-row and row mirrors give every row its sum,
+// then row_bcast15 / row_bcast31 carry row sums upward (GCN DPP controls
+// 0xB1, 0x4E, 0x141, 0x140, 0x142 with rows 1 and 3, 0x143 with rows 2 and 3).
+// Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add_src(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_last(double v) {
+  v += dpp_add_src<0xB1, 0xf>(v);
+  v += dpp_add_src<0x4E, 0xf>(v);
+  v += dpp_add_src<0x141, 0xf>(v);
+  v += dpp_add_src<0x140, 0xf>(v);
+  v += dpp_add_src<0x142, 0xa>(v);
+  v += dpp_add_src<0x143, 0xc>(v);
+  return v;
+}
+__device__ __forceinline__ c64 wave_sum_last(c64 v) { return c64{wave_sum_last(v.re), wave_sum_last(v.im)}; }
 
 // Reduces per-wave values through LDS; returns the block total on thread 0.
 __device__ inline c64 block_reduce(c64 v, c64* lds) {
@@ -127,8 +155,8 @@ __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int
           s.re += z.re * z.re + z.im * z.im;
         }
       }
-      s = wave_reduce(s);
-      if (lane == 0) lds[k] = lds[k] + s;  // keys are owned by one wave
+      s = wave_sum_last(s);
+      if (lane == 63) lds[k] = lds[k] + s;  // keys are owned by one wave
     }
   }
   __syncthreads();
@@ -391,13 +419,14 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
   const bool hw = has_w != 0;
   const int tot = hw ? 2 * j + 3 : j + 1;
   for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
+  // whole chunks: the basis padding past n is zero (krylov.h, kBasisAlign)
   const size_t nchunks = (n + CH - 1) / CH;
   for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
     const size_t base = ch * CH;
     __syncthreads();
     for (int e = threadIdx.x; e < CH; e += KWG) {
       size_t i = base + e;
-      cx<T> a = i < n ? u[i] : cx<T>{0, 0}, b = (hw && i < n) ? w[i] : cx<T>{0, 0};
+      cx<T> a = u[i], b = hw ? w[i] : cx<T>{0, 0};
       us[e] = c64{(double)a.re, (double)a.im};
       ws[e] = c64{(double)b.re, (double)b.im};
     }
@@ -408,34 +437,50 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
 #define VIE_DC_LOAD(k, dst)                                                                                \
   do {                                                                                                     \
     const cx<T>* qk_ = Q + (size_t)(k) * ldv + base + lane;                                                \
-    _Pragma("unroll") for (int e_ = 0; e_ < CPL; e_++) dst[e_] =                                           \
-        base + lane + (size_t)e_ * 64 < n ? ld_stream(qk_ + e_ * 64) : cx<T>{0, 0};                       \
+    _Pragma("unroll") for (int e_ = 0; e_ < CPL; e_++) dst[e_] = ld_stream(qk_ + e_ * 64);                 \
   } while (0)
-    if (wave < j) VIE_DC_LOAD(wave, q);
-    for (int k = wave; k < j; k += WAVES) {
-      if (k + WAVES < j) VIE_DC_LOAD(k + WAVES, qn);
-      c64 sa = {0, 0}, sb = {0, 0};
-#pragma unroll
-      for (int e = 0; e < CPL; e++) {
-        const c64 uu = us[lane + e * 64];
-        double qr = q[e].re, qi = q[e].im;
-        sa.re += qr * uu.re + qi * uu.im;
-        sa.im += qr * uu.im - qi * uu.re;
-        if (hw) {
-          const c64 ww = ws[lane + e * 64];
-          sb.re += qr * ww.re + qi * ww.im;
-          sb.im += qr * ww.im - qi * ww.re;
-        }
-      }
-      sa = wave_reduce(sa);
-      if (hw) sb = wave_reduce(sb);
-      if (lane == 0) {
-        lds[k] = lds[k] + sa;
-        if (hw) lds[j + k] = lds[j + k] + sb;
-      }
-#pragma unroll
-      for (int e = 0; e < CPL; e++) q[e] = qn[e];
+    // one key's two projections, reduced over the wave and added to the block's sums
+#define VIE_DC_KEY(qb, k)                                                                                  \
+  do {                                                                                                     \
+    c64 sa = {0, 0}, sb = {0, 0};                                                                          \
+    _Pragma("unroll") for (int e = 0; e < CPL; e++) {                                                      \
+      const c64 uu = us[lane + e * 64];                                                                    \
+      const double qr = qb[e].re, qi = qb[e].im;                                                           \
+      sa.re += qr * uu.re + qi * uu.im;                                                                    \
+      sa.im += qr * uu.im - qi * uu.re;                                                                    \
+      if (hw) {                                                                                            \
+        const c64 ww = ws[lane + e * 64];                                                                  \
+        sb.re += qr * ww.re + qi * ww.im;                                                                  \
+        sb.im += qr * ww.im - qi * ww.re;                                                                  \
+      }                                                                                                    \
+    }                                                                                                      \
+    sa = wave_sum_last(sa);                                                                                \
+    if (hw) sb = wave_sum_last(sb);                                                                        \
+    if (lane == 63) {                                                                                      \
+      lds[k] = lds[k] + sa;                                                                                \
+      if (hw) lds[j + k] = lds[j + k] + sb;                                                                \
+    }                                                                                                      \
+  } while (0)
+    // Two register sets in turn: the next key's loads are issued before this
+    // key's arithmetic and stay in flight through it (the loop body holds no
+    // conditional load and no register copy, so the wait before the
+    // arithmetic covers only the older set).
+    int k = wave;
+    if (k < j) VIE_DC_LOAD(k, q);
+    for (; k + 2 * WAVES < j; k += 2 * WAVES) {
+      VIE_DC_LOAD(k + WAVES, qn);
+      VIE_DC_KEY(q, k);
+      VIE_DC_LOAD(k + 2 * WAVES, q);
+      VIE_DC_KEY(qn, k + WAVES);
     }
+    if (k + WAVES < j) {
+      VIE_DC_LOAD(k + WAVES, qn);
+      VIE_DC_KEY(q, k);
+      VIE_DC_KEY(qn, k + WAVES);
+    } else if (k < j) {
+      VIE_DC_KEY(q, k);
+    }
+#undef VIE_DC_KEY
 #undef VIE_DC_LOAD
     // the scalars nu, c, d: one wave each
     const int nscal = hw ? 3 : 1;
@@ -450,9 +495,9 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
           s.im += uu.re * ww.im - uu.im * ww.re;
         } else s.re += ww.re * ww.re + ww.im * ww.im;
       }
-      s = wave_reduce(s);
+      s = wave_sum_last(s);
       int key = (hw ? 2 * j : j) + q;
-      if (lane == 0) lds[key] = lds[key] + s;
+      if (lane == 63) lds[key] = lds[key] + s;
     }
   }
   __syncthreads();
@@ -650,66 +695,81 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
   const double fr = st->dc_f_re, fi = st->dc_f_im;
   cx<T>* u = Q + (size_t)j * ldv;
   cx<T>* w = Q + (size_t)(j + 1) * ldv;
+  // whole KWG * UE groups: the basis padding past n is zero and stays zero
+  const size_t npad = (n + KWG * UE - 1) / (KWG * UE) * (KWG * UE);
   const size_t step = (size_t)gridDim.x * KWG * UE;
-  for (size_t base = (size_t)blockIdx.x * KWG * UE + threadIdx.x; base < n; base += step) {
+  for (size_t base = (size_t)blockIdx.x * KWG * UE + threadIdx.x; base < npad; base += step) {
     double ur[UE], ui[UE], wr[UE], wi[UE];
-    bool in[UE];
 #pragma unroll
     for (int e = 0; e < UE; e++) {
       size_t i = base + (size_t)e * KWG;
-      in[e] = i < n;
-      cx<T> uu = in[e] ? u[i] : cx<T>{0, 0}, ww = in[e] ? w[i] : cx<T>{0, 0};
+      cx<T> uu = u[i], ww = w[i];
       ur[e] = uu.re;
       ui[e] = uu.im;
       // w - f u uses the original u
       wr[e] = ww.re - (fr * uu.re - fi * uu.im);
       wi[e] = ww.im - (fr * uu.im + fi * uu.re);
     }
+    // groups of KG basis vectors, two register sets in turn: the next group's
+    // loads are in flight while this group's are applied
+    constexpr int KG = 4;
+#define VIE_UP_LOAD(k0, qb)                                                                                \
+  do {                                                                                                     \
+    _Pragma("unroll") for (int p = 0; p < KG; p++) _Pragma("unroll") for (int e = 0; e < UE; e++)          \
+        qb[p][e] = ld_stream(Q + (size_t)((k0) + p) * ldv + base + (size_t)e * KWG);                       \
+  } while (0)
+#define VIE_UP_USE(k0, qb)                                                                                 \
+  do {                                                                                                     \
+    _Pragma("unroll") for (int p = 0; p < KG; p++) {                                                       \
+      const double ar = ca[(k0) + p].re, ai = ca[(k0) + p].im, er = ce[(k0) + p].re, ei = ce[(k0) + p].im; \
+      _Pragma("unroll") for (int e = 0; e < UE; e++) {                                                     \
+        const double qr = qb[p][e].re, qi = qb[p][e].im;                                                   \
+        ur[e] -= ar * qr - ai * qi;                                                                        \
+        ui[e] -= ar * qi + ai * qr;                                                                        \
+        wr[e] -= er * qr - ei * qi;                                                                        \
+        wi[e] -= er * qi + ei * qr;                                                                        \
+      }                                                                                                    \
+    }                                                                                                      \
+  } while (0)
+    cx<T> qa[KG][UE], qb[KG][UE];
+    const int jg = j / KG * KG;
     int k = 0;
-    for (; k + 4 <= j; k += 4) {
-      cx<T> q[4][UE];
-#pragma unroll
-      for (int p = 0; p < 4; p++)
-#pragma unroll
-        for (int e = 0; e < UE; e++) {
-          size_t i = base + (size_t)e * KWG;
-          q[p][e] = in[e] ? ld_stream(Q + (size_t)(k + p) * ldv + i) : cx<T>{0, 0};
-        }
-#pragma unroll
-      for (int p = 0; p < 4; p++) {
-        const double ar = ca[k + p].re, ai = ca[k + p].im, er = ce[k + p].re, ei = ce[k + p].im;
-#pragma unroll
-        for (int e = 0; e < UE; e++) {
-          double qr = q[p][e].re, qi = q[p][e].im;
-          ur[e] -= ar * qr - ai * qi;
-          ui[e] -= ar * qi + ai * qr;
-          wr[e] -= er * qr - ei * qi;
-          wi[e] -= er * qi + ei * qr;
-        }
-      }
+    if (k < jg) VIE_UP_LOAD(k, qa);
+    for (; k + 2 * KG < jg; k += 2 * KG) {
+      VIE_UP_LOAD(k + KG, qb);
+      VIE_UP_USE(k, qa);
+      VIE_UP_LOAD(k + 2 * KG, qa);
+      VIE_UP_USE(k + KG, qb);
     }
+    if (k + KG < jg) {
+      VIE_UP_LOAD(k + KG, qb);
+      VIE_UP_USE(k, qa);
+      VIE_UP_USE(k + KG, qb);
+      k += 2 * KG;
+    } else if (k < jg) {
+      VIE_UP_USE(k, qa);
+      k += KG;
+    }
+#undef VIE_UP_LOAD
+#undef VIE_UP_USE
     for (; k < j; k++) {
       const double ar = ca[k].re, ai = ca[k].im, er = ce[k].re, ei = ce[k].im;
 #pragma unroll
       for (int e = 0; e < UE; e++) {
         size_t i = base + (size_t)e * KWG;
-        if (in[e]) {
-          cx<T> qq = ld_stream(Q + (size_t)k * ldv + i);
-          double qr = qq.re, qi = qq.im;
-          ur[e] -= ar * qr - ai * qi;
-          ui[e] -= ar * qi + ai * qr;
-          wr[e] -= er * qr - ei * qi;
-          wi[e] -= er * qi + ei * qr;
-        }
+        cx<T> qq = ld_stream(Q + (size_t)k * ldv + i);
+        double qr = qq.re, qi = qq.im;
+        ur[e] -= ar * qr - ai * qi;
+        ui[e] -= ar * qi + ai * qr;
+        wr[e] -= er * qr - ei * qi;
+        wi[e] -= er * qi + ei * qr;
       }
     }
 #pragma unroll
     for (int e = 0; e < UE; e++) {
       size_t i = base + (size_t)e * KWG;
-      if (in[e]) {
-        u[i] = cx<T>{(T)(ur[e] * ia), (T)(ui[e] * ia)};
-        w[i] = cx<T>{(T)(wr[e] * ia), (T)(wi[e] * ia)};
-      }
+      u[i] = cx<T>{(T)(ur[e] * ia), (T)(ui[e] * ia)};
+      w[i] = cx<T>{(T)(wr[e] * ia), (T)(wi[e] * ia)};
     }
   }
 }
@@ -733,6 +793,12 @@ __global__ void k_step_end(ArnoldiState* st, StepStatus* ring) {
 }
 
 // ---------------------------------------------------------------- elementwise
+// Entries [n, ldv) of column blockIdx.x.
+template <typename T> __global__ void k_zero_pad(cx<T>* V, size_t ldv, size_t n) {
+  cx<T>* col = V + (size_t)blockIdx.x * ldv;
+  for (size_t i = n + threadIdx.x; i < ldv; i += blockDim.x) col[i] = cx<T>{0, 0};
+}
+
 template <typename T> __global__ void k_zero(cx<T>* x, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = cx<T>{0, 0};
@@ -1092,6 +1158,9 @@ void block_combine(cx<T>* out, int nout, const cx<T>* in, int nin, size_t ldv, c
   if (n && nout > 0 && nin > 0)
     hipLaunchKernelGGL((k_block_combine<T>), dim3(gblocks(n)), dim3(256), 0, s, out, nout, in, nin, ldv, coef, n);
 }
+template <typename T> void zero_padding(cx<T>* V, size_t ldv, size_t n, size_t cols, hipStream_t s) {
+  if (cols && ldv > n) hipLaunchKernelGGL((k_zero_pad<T>), dim3((unsigned)cols), dim3(256), 0, s, V, ldv, n);
+}
 template <typename T> void scale_from(cx<T>* v, const c64* r, size_t n, const ArnoldiState* st, hipStream_t s) {
   hipLaunchKernelGGL((k_scale_from<T>), dim3(gblocks(n)), dim3(256), 0, s, v, r, n, st);
 }
@@ -1161,6 +1230,7 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
                               hipStream_t);                                                                      \
   template void norm2<T>(const cx<T>*, size_t, ReduceScratch&, c64*, hipStream_t);                               \
   template void scale_from<T>(cx<T>*, const c64*, size_t, const ArnoldiState*, hipStream_t);                     \
+  template void zero_padding<T>(cx<T>*, size_t, size_t, size_t, hipStream_t);                                    \
   template void scale_next<T>(cx<T>*, size_t, const ArnoldiState*, hipStream_t);                                 \
   template void combine<T>(c64*, const cx<T>*, size_t, size_t, const ArnoldiState*, const JacobiView&, int,       \
                            hipStream_t, const cx<T>*);                                                           \

@@ -53,7 +53,7 @@ def job_device_bytes(shape, conductors, restart, precision=vie.MIXED):
     precision (a mixed basis with its recycled subspace, restart + 1 + 2k FP32
     columns with k <= restart / 2, fits in it)."""
     n = conductors + 3 * int(np.prod(shape))
-    ldv = (n + 63) // 64 * 64
+    ldv = (n + 511) // 512 * 512   # krylov.h basis_ld
     return (restart + 1) * ldv * 16 + 4 * n * 16 + KRYLOV_SCRATCH
 
 
