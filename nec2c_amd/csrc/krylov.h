@@ -27,6 +27,7 @@ struct ArnoldiState {
   // recycled images C (A M^{-1} U = C, orthonormal) and are never expanded;
   // Hessenberg rows 0..base-1 are the projections C^H A M^{-1} q_j (unrotated)
   int base;
+  int ju;  // column that the step's fused update finishes (its reduction's j)
   double inv_beta, inv_next;
   double inner_target;  // cycle stops once |g_{j+1}| <= inner_target
   // delayed (DCGS2) orthogonalisation: coefficients of the fused update,
@@ -118,8 +119,8 @@ template <typename T> void zero_padding(cx<T>* V, size_t ldv, size_t n, size_t c
 // Delayed CGS2 Arnoldi step j = st->j, read on the device (slot j holds the
 // once-projected candidate u, slot j+1 holds w = A M^{-1} u, or nothing
 // without has_w: the closing step of a cycle).  One reduction pass, one small
-// solve, one fused update pass; the step then writes the status block to
-// ring[st->step & 1] (device-visible host memory) and advances j unless the
+// solve, one fused update pass; the small solve also writes the status block
+// to ring[st->step & 1] (device-visible host memory) and advances j unless the
 // cycle is done.  The launch sequence does not depend on j, so it can be
 // recorded once and replayed for every step.  max_j bounds j (the restart).
 // ldv = basis_ld(n) and the padding of every column read is zero.

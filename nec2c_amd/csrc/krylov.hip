@@ -57,28 +57,6 @@ __device__ inline c64 wave_reduce(c64 v) {
   return v;
 }
 
-// Wave sum by DPP row operations (no LDS crossbar), total in lane 63 only:
-// quad swaps (xor 1, xor 2), half-row and row mirrors give every row its sum,
-// then row_bcast15 / row_bcast31 carry row sums upward (GCN DPP controls
-// 0xB1, 0x4E, 0x141, 0x140, 0x142 with rows 1 and 3, 0x143 with rows 2 and 3).
-// Every lane of the wave must be active.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_add_src(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ __forceinline__ double wave_sum_last(double v) {
-  v += dpp_add_src<0xB1, 0xf>(v);
-  v += dpp_add_src<0x4E, 0xf>(v);
-  v += dpp_add_src<0x141, 0xf>(v);
-  v += dpp_add_src<0x140, 0xf>(v);
-  v += dpp_add_src<0x142, 0xa>(v);
-  v += dpp_add_src<0x143, 0xc>(v);
-  return v;
-}
-__device__ __forceinline__ c64 wave_sum_last(c64 v) { return c64{wave_sum_last(v.re), wave_sum_last(v.im)}; }
 
 // Reduces per-wave values through LDS; returns the block total on thread 0.
 __device__ inline c64 block_reduce(c64 v, c64* lds) {
@@ -504,6 +482,25 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
   for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + blockIdx.x] = lds[k];
 }
 
+// End of an Arnoldi step (thread 0): status block to the host ring slot of
+// this step, then j advances unless the cycle is done (later steps are then
+// no-ops); ju keeps the column the step's update finishes.
+__device__ __forceinline__ void end_step(ArnoldiState* st, StepStatus* ring, int j) {
+  StepStatus* r = ring + (st->step & 1);
+  r->gabs = st->gabs;
+  r->beta = st->beta;
+  r->breakdown = st->breakdown;
+  r->nonfinite = st->nonfinite;
+  r->singular = st->singular;
+  r->used = st->used;
+  r->done = st->done;
+  r->pad_ = st->step;
+  __threadfence_system();
+  st->step = st->step + 1;
+  st->ju = j;
+  if (!st->done) st->j = j + 1;
+}
+
 // Small solve of the delayed step (one block).  With u = alpha q_j + Q a:
 //  1. alpha^2 = nu - |a|^2 (Q orthonormal);
 //  2. column j-1 becomes final: Hraw(0:j, j-1) += a, Hraw(j, j-1) = alpha;
@@ -514,7 +511,7 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
 //  4. next candidate u' = (w - Q_j e - f u)/alpha with s = t + alpha h,
 //     e = s(0:j-1) - (s_j/alpha) a, f = s_j/alpha; q_j = (u - Q_j a)/alpha.
 __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, const c64* dots,
-                                                double breakdown_eps) {
+                                                double breakdown_eps, StepStatus* ring) {
   __shared__ c64 a[kMaxRestart], b[kMaxRestart], t[kMaxRestart + 1], col[kMaxRestart + 1];
   __shared__ c64 tpart[2][kMaxRestart + 1];
   __shared__ c64 sn[kMaxRestart];
@@ -524,7 +521,10 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   __shared__ double sd[2];     // dc_before, inner_target
   __shared__ double alpha_s;
   __shared__ int stop;
-  if (st->done) return;
+  if (st->done) {  // a step queued after the cycle ended: status only
+    if (threadIdx.x == 0) end_step(st, ring, st->j);
+    return;
+  }
   const int j = st->j, base = st->base;
   const int m = st->m, ld = m + 1;  // Hraw is column-major: Hraw[l * (m + 1) + k] = row k, column l
   const int tid = threadIdx.x;
@@ -623,7 +623,10 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
     // the last basis vector is left as u = alpha q_j + Q_j a: keep a and alpha
     // so the host can still form q_j (recycled-subspace construction)
     for (int k = tid; k < j; k += KWG) st->dc_a[k] = a[k];
-    if (tid == 0) st->dc_alpha = alpha;
+    if (tid == 0) {
+      st->dc_alpha = alpha;
+      end_step(st, ring, j);
+    }
     return;
   }
   // t = Hraw(0:j, 0:j-1) a (Hessenberg: row k only has columns >= k-1).
@@ -635,15 +638,20 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
     for (int k = r0; k <= j; k += 128) {
       const int lo = half == 0 ? (k > 0 ? k - 1 : 0) : (mid > k - 1 ? mid : (k > 0 ? k - 1 : 0));
       const int hi = half == 0 ? (mid < j ? mid : j) : j;
-      c64 s0 = {0, 0}, s1 = {0, 0};
+      // eight columns per iteration: eight independent loads in flight
+      c64 sp[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) sp[q] = c64{0, 0};
       int l = lo;
-      for (; l + 1 < hi; l += 2) {
-        const c64 h0 = Hraw[(size_t)l * ld + k], h1 = Hraw[(size_t)(l + 1) * ld + k];
-        s0 = s0 + h0 * a[l];
-        s1 = s1 + h1 * a[l + 1];
+      for (; l + 7 < hi; l += 8) {
+        c64 h[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) h[q] = Hraw[(size_t)(l + q) * ld + k];
+#pragma unroll
+        for (int q = 0; q < 8; q++) sp[q] = sp[q] + h[q] * a[l + q];
       }
-      if (l < hi) s0 = s0 + Hraw[(size_t)l * ld + k] * a[l];
-      tpart[half][k] = s0 + s1;
+      for (; l < hi; l++) sp[0] = sp[0] + Hraw[(size_t)l * ld + k] * a[l];
+      tpart[half][k] = ((sp[0] + sp[1]) + (sp[2] + sp[3])) + ((sp[4] + sp[5]) + (sp[6] + sp[7]));
     }
   }
   __syncthreads();
@@ -677,6 +685,7 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   }
   // rows below the new column stay zero
   for (int k = j + 1 + tid; k <= m; k += KWG) Hraw[(size_t)j * ld + k] = c64{0, 0};
+  if (tid == 0) end_step(st, ring, j);
 }
 
 // Fused update of step j: slot j <- (u - Q_j a)/alpha, slot j+1 <- (w - Q_j e - f u)/alpha.
@@ -684,7 +693,7 @@ template <typename T>
 __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t n, const ArnoldiState* st) {
   __shared__ c64 ca[kMaxRestart], ce[kMaxRestart];
   if (st->done) return;
-  const int j = st->j;
+  const int j = st->ju;
   constexpr int UE = DcShape<T>::UE;
   for (int k = threadIdx.x; k < j; k += KWG) {
     ca[k] = st->dc_a[k];
@@ -772,24 +781,6 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
       w[i] = cx<T>{(T)(wr[e] * ia), (T)(wi[e] * ia)};
     }
   }
-}
-
-// End of an Arnoldi step: status block to the host ring slot of this step,
-// then j advances unless the cycle is done (later steps are then no-ops).
-__global__ void k_step_end(ArnoldiState* st, StepStatus* ring) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  StepStatus* r = ring + (st->step & 1);
-  r->gabs = st->gabs;
-  r->beta = st->beta;
-  r->breakdown = st->breakdown;
-  r->nonfinite = st->nonfinite;
-  r->singular = st->singular;
-  r->used = st->used;
-  r->done = st->done;
-  r->pad_ = st->step;
-  __threadfence_system();
-  st->step = st->step + 1;
-  if (!st->done) st->j = st->j + 1;
 }
 
 // ---------------------------------------------------------------- elementwise
@@ -1082,10 +1073,9 @@ void dcgs2_step(cx<T>* Q, size_t ldv, bool has_w, size_t n, ArnoldiState* st, Re
   const int max_keys = 2 * max_j + 3;
   hipLaunchKernelGGL(k_finish_dc, dim3((max_keys + 3) / 4), dim3(KWG), 0, s, rs.partial, kMaxDotBlocks, nb,
                      (const ArnoldiState*)st, hw, dots);
-  hipLaunchKernelGGL(k_dc_mid, dim3(1), dim3(KWG), 0, s, st, hw, dots, breakdown_eps);
+  hipLaunchKernelGGL(k_dc_mid, dim3(1), dim3(KWG), 0, s, st, hw, dots, breakdown_eps, ring);
   if (has_w) hipLaunchKernelGGL((k_update_dc<T>), dim3(update_blocks(n, DcShape<T>::UE)), dim3(KWG), 0, s, Q, ldv,
                                 n, (const ArnoldiState*)st);
-  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, st, ring);
 }
 // Timing hook: a state at column j with finite update coefficients, and a
 // basis whose first j + 2 columns hold a constant, so the two streaming
@@ -1097,6 +1087,7 @@ __global__ void k_time_state(ArnoldiState* st, int j) {
   }
   if (threadIdx.x == 0) {
     st->j = j;
+    st->ju = j;
     st->done = 0;
     st->dc_alpha = 1.0;
     st->dc_f_re = 0.0;

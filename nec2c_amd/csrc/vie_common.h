@@ -66,6 +66,29 @@ struct DevIndex {
   __device__ inline bool skipped() const { return skip && *skip; }
 };
 
+// Wave sum by DPP row operations (no LDS crossbar), total in lane 63 only:
+// quad swaps (xor 1, xor 2), half-row and row mirrors give every row its sum,
+// then row_bcast15 / row_bcast31 carry row sums upward (GCN DPP controls
+// 0xB1, 0x4E, 0x141, 0x140, 0x142 with rows 1 and 3, 0x143 with rows 2 and 3).
+// Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add_src(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_last(double v) {
+  v += dpp_add_src<0xB1, 0xf>(v);
+  v += dpp_add_src<0x4E, 0xf>(v);
+  v += dpp_add_src<0x141, 0xf>(v);
+  v += dpp_add_src<0x140, 0xf>(v);
+  v += dpp_add_src<0x142, 0xa>(v);
+  v += dpp_add_src<0x143, 0xc>(v);
+  return v;
+}
+__device__ __forceinline__ c64 wave_sum_last(c64 v) { return c64{wave_sum_last(v.re), wave_sum_last(v.im)}; }
+
 // Geometry of one FFT operator instance (one lattice).
 struct Geom {
   int nx, ny, nz;        // unknown box (physical cells; holes carry eps = 1, x = 0)
