@@ -77,29 +77,26 @@ inline void givens(zd x, zd y, double* c, zd* s) {
   *s = (1.0 / (ax * r)) * (x * conj(y));
 }
 
-// Solves A x = b (n x n, partial pivoting) in place of b; false when A is singular.
-inline bool solve(Mat A, std::vector<zd>& b) {
-  const int n = A.n;
-  for (int k = 0; k < n; k++) {
-    int p = k;
-    for (int i = k + 1; i < n; i++)
-      if (norm2(A(i, k)) > norm2(A(p, k))) p = i;
-    if (norm2(A(p, k)) == 0) return false;
-    if (p != k) {
-      for (int j = 0; j < n; j++) std::swap(A(k, j), A(p, j));
-      std::swap(b[k], b[p]);
+// Solves U x = b for an upper Hessenberg U (entries below the subdiagonal
+// ignored), Gaussian elimination with adjacent-row pivoting: O(n^2).  False
+// when U is singular.
+inline bool hessenberg_solve(Mat U, std::vector<zd>& b) {
+  const int n = U.n;
+  for (int k = 0; k + 1 < n; k++) {
+    if (norm2(U(k + 1, k)) > norm2(U(k, k))) {
+      for (int j = k; j < n; j++) std::swap(U(k, j), U(k + 1, j));
+      std::swap(b[k], b[k + 1]);
     }
-    for (int i = k + 1; i < n; i++) {
-      const zd f = A(i, k) / A(k, k);
-      if (f.re == 0 && f.im == 0) continue;
-      for (int j = k; j < n; j++) A(i, j) = A(i, j) - f * A(k, j);
-      b[i] = b[i] - f * b[k];
-    }
+    if (norm2(U(k, k)) == 0) return false;
+    const zd f = U(k + 1, k) / U(k, k);
+    for (int j = k + 1; j < n; j++) U(k + 1, j) = U(k + 1, j) - f * U(k, j);
+    b[k + 1] = b[k + 1] - f * b[k];
   }
   for (int i = n - 1; i >= 0; i--) {
-    zd s = b[i];
-    for (int j = i + 1; j < n; j++) s = s - A(i, j) * b[j];
-    b[i] = s / A(i, i);
+    if (norm2(U(i, i)) == 0) return false;
+    zd t = b[i];
+    for (int j = i + 1; j < n; j++) t = t - U(i, j) * b[j];
+    b[i] = t / U(i, i);
   }
   return true;
 }
@@ -217,12 +214,15 @@ inline std::vector<zd> hessenberg_eigvec(const Mat& H, zd lambda) {
 inline bool harmonic_ritz(const Mat& Hbar, int k, Mat& P) {
   const int m = Hbar.m;
   if (k < 1 || k > m || Hbar.n != m + 1) return false;
-  Mat Hh(m, m);
+  // f = H_m^{-H} e_m.  H_m^H is lower Hessenberg; reversing its row and
+  // column order makes it upper Hessenberg, solved in O(m^2).
+  Mat R(m, m);
   for (int j = 0; j < m; j++)
-    for (int i = 0; i < m; i++) Hh(i, j) = conj(Hbar(j, i));  // H_m^H
+    for (int i = 0; i < m; i++) R(i, j) = conj(Hbar(m - 1 - j, m - 1 - i));
   std::vector<zd> f(m, zd(0, 0));
-  f[m - 1] = {1, 0};
-  if (!solve(Hh, f)) return false;
+  f[0] = {1, 0};
+  if (!hessenberg_solve(R, f)) return false;
+  std::reverse(f.begin(), f.end());
   Mat M(m, m);
   for (int j = 0; j < m; j++)
     for (int i = 0; i < m; i++) M(i, j) = Hbar(i, j);
