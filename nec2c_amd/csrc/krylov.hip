@@ -59,11 +59,12 @@ __device__ inline c64 wave_reduce(c64 v) {
 
 
 // Reduces per-wave values through LDS; returns the block total on thread 0.
+// Every thread of the block calls it.
 __device__ inline c64 block_reduce(c64 v, c64* lds) {
-  v = wave_reduce(v);
+  v = wave_sum_last(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  if (lane == 0) lds[wave] = v;
+  if (lane == 63) lds[wave] = v;
   __syncthreads();
   c64 t = {0, 0};
   if (threadIdx.x == 0)
@@ -529,22 +530,38 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   const int m = st->m, ld = m + 1;  // Hraw is column-major: Hraw[l * (m + 1) + k] = row k, column l
   const int tid = threadIdx.x;
   c64* Hraw = st->Hraw;
-  // every operand in one round of parallel loads
-  for (int k = tid; k < j; k += KWG) {
-    a[k] = dots[k];
-    if (has_w) b[k] = dots[j + k];
-    sn[k] = st->sn[k];
-    cs[k] = st->cs[k];
+  // every operand in one round of independent loads: j <= kMaxRestart <= KWG,
+  // so each thread takes one entry (threads past j read entry 0 and keep
+  // nothing); no load sits under a lane-divergent branch
+  {
+    static_assert(kMaxRestart <= KWG, "one Arnoldi column entry per thread");
+    const int k = tid < j ? tid : 0;
+    const c64 va = dots[k];
+    const c64 vb = has_w ? dots[j + k] : c64{0, 0};
+    const c64 vsn = st->sn[k];
+    const double vcs = st->cs[k];
+    const c64 vcol = Hraw[(size_t)(j > base ? j - 1 : 0) * ld + k];  // old column j-1, rows 0..j-1
+    const c64 s0 = dots[has_w ? 2 * j : j];
+    const c64 s1 = has_w ? dots[2 * j + 1] : c64{0, 0};
+    const c64 s2 = has_w ? dots[2 * j + 2] : c64{0, 0};
+    const c64 s3 = st->g[j > base ? j - 1 : base];
+    const double d0 = st->dc_before, d1 = st->inner_target;
+    if (tid < j) {
+      a[tid] = va;
+      b[tid] = vb;
+      sn[tid] = vsn;
+      cs[tid] = vcs;
+      col[tid] = vcol;
+    }
+    if (tid == 0) {
+      sc[0] = s0;
+      sc[1] = s1;
+      sc[2] = s2;
+      sc[3] = s3;
+      sd[0] = d0;
+      sd[1] = d1;
+    }
   }
-  if (tid == 0) sc[0] = dots[has_w ? 2 * j : j];
-  if (tid == 1) sc[1] = has_w ? dots[2 * j + 1] : c64{0, 0};
-  if (tid == 2) sc[2] = has_w ? dots[2 * j + 2] : c64{0, 0};
-  if (tid == 3) sc[3] = st->g[j > base ? j - 1 : base];
-  if (tid == 4) sd[0] = st->dc_before;
-  if (tid == 5) sd[1] = st->inner_target;
-  // old column j-1 of Hraw, rows 0..j-1 (contiguous)
-  if (j > base)
-    for (int k = tid; k < j; k += KWG) col[k] = Hraw[(size_t)(j - 1) * ld + k];
   __syncthreads();
   // |a|^2 and a^H b in parallel
   c64 p2 = {0, 0}, pab = {0, 0};
