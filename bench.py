@@ -452,6 +452,63 @@ def bench_matvec(args):
     return 0
 
 
+def bench_farfield(args):
+    """Scattered far field of a volume current on the n^3 cube of the sweep
+    (uniform lattice, edge 0.5 m): the reference's 974-direction scattered
+    power and a 1-degree pattern (181 x 360 directions), free space and over
+    a PEC plane 0.05 m below the cube.  Synthetic current.  CPU baseline: the
+    oracle's numpy restatement (one core) on a bounded sample of directions,
+    scaled per cell-direction pair."""
+    from nec2c_amd import vie, workloads as W
+    n = args.farfield_size
+    pitch = 0.5 / n
+    centers = W.cube_cells(n, pitch) + np.array([0.0, 0.0, 0.05])
+    sizes = np.full_like(centers, pitch)
+    rng = np.random.default_rng(11)
+    J = rng.standard_normal(3 * len(centers)) + 1j * rng.standard_normal(3 * len(centers))
+    k0 = W.k0_of(300e6)
+    th, ph = np.meshgrid(np.radians(np.arange(181.0)), np.radians(np.arange(360.0)), indexing="ij")
+    pattern = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1).reshape(-1, 3)
+    vie.scattered_power(centers, sizes, J, k0)  # warm-up (device buffers, code objects)
+    res = {}
+    for ground, tag in ((0, "free"), (1, "pec")):
+        t = time.perf_counter()
+        p = vie.scattered_power(centers, sizes, J, k0, ground=ground)
+        t_pow = time.perf_counter() - t
+        dirs = pattern if ground == 0 else pattern[pattern[:, 2] >= 0]
+        t = time.perf_counter()
+        vie.radiation_amplitudes(centers, sizes, J, k0, dirs, ground=ground)
+        t_pat = time.perf_counter() - t
+        res[tag] = {"power_w": p, "power_ms": t_pow * 1e3, "pattern_directions": len(dirs), "pattern_ms": t_pat * 1e3,
+                    "pairs_per_s": len(dirs) * len(centers) / t_pat}
+        log(f"farfield {tag}: power {p:.6e} W in {t_pow * 1e3:.2f} ms; {len(dirs)}-direction pattern in "
+            f"{t_pat * 1e3:.1f} ms ({res[tag]['pairs_per_s']:.3e} cell-direction pairs/s)")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import farfield as F
+        sample = pattern[:: max(1, len(pattern) // 16)][:16]
+        t = time.perf_counter()
+        F.radiation_amplitudes(centers, sizes, J, k0, sample)
+        t_cpu = time.perf_counter() - t
+        rate = len(sample) * len(centers) / t_cpu
+        cpu = {"value": rate / len(centers), "unit": "directions/s", "cores": 1, "kind": "port",
+               "sample": f"{len(sample)} pattern directions x {len(centers)} cells through oracle/farfield.py "
+                         f"(numpy, one core, free space) in {t_cpu:.2f} s"}
+        log(f"farfield CPU port: {rate:.3e} pairs/s")
+    free = res["free"]
+    print(json.dumps({
+        "metric": "far-field directions/s (radiation amplitude of a volume current)",
+        "value": free["pattern_directions"] / (free["pattern_ms"] * 1e-3), "unit": "directions/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": free["pattern_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128 (fp64 complex)", "data": "synthetic",
+        "config": {"workload": f"{n}^3 uniform cube (edge 0.5 m, 0.05 m above z = 0), 300 MHz, random current; "
+                               "1-degree pattern (181 x 360) and the 974-direction scattered power",
+                   "cells": len(centers)},
+        "farfield": res, "cpu_baseline": cpu, "roofline": None,
+    }))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,12 +524,13 @@ def main():
                     help="mixed: FP32 cycle stops at |g| <= max(target, this * beta); 0 = library default")
     ap.add_argument("--recycle", type=int, default=0,
                     help="mixed: recycled-subspace size of the GMRES cycles (0 = library default, -1 = off)")
-    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield"), default="sweep",
                     help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
                          "applies of one sweep point plus the copy calibration kernel (for counter runs)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="matvec: bytes of the calibration copy")
     ap.add_argument("--slab-size", type=int, default=128, help="slab: voxels per edge (128 = the 128^3 slab)")
+    ap.add_argument("--farfield-size", type=int, default=64, help="farfield: voxels per cube edge")
     ap.add_argument("--arnoldi-column", type=int, default=0,
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
@@ -500,6 +558,8 @@ def main():
         return bench_matvec(args)
     if args.workload == "slab":
         return bench_slab(args)
+    if args.workload == "farfield":
+        return bench_farfield(args)
 
     n = args.size
     pitch = 0.5 / n

@@ -115,6 +115,26 @@ int dielectric_fft_last_solve_escalated(const dielectric_fft_system *system);
    (0 when its cycles were plain: recycling off, one cycle was enough, or the
    first cycle was too short to supply one). */
 int dielectric_fft_last_solve_recycled(const dielectric_fft_system *system);
+/* Scattered far-field amplitudes of a solved volume current for many
+   directions at once on the device (the reference evaluates
+   dielectric_radiation_amplitude, dielectric_solver.c:124-158, one direction
+   at a time; dielectric.h:233-238).  cells: centers[3 cells] and sizes[3 cells]
+   in m; current[3 cells] cell-major, as that function takes it; k0 in 1/m;
+   directions[3 count] unit vectors.  ground 0: free space; 1: PEC plane z = 0
+   (mirrored cells, reflection -1 / +1 as dielectric_ground.c:133-144 with
+   gnd.iperf == 1); 2: a finite ground with the caller's per-direction Fresnel
+   coefficients rs[count], rp[count] (vie_halfspace_fresnel there).  Writes
+   amplitude[3 count] (E r e^{ikr}, SI).  1 on success, 0 with
+   vie_hip_last_error() on invalid input or a device failure. */
+int vie_hip_radiation_amplitudes(size_t cells, const double *centers, const double *sizes,
+                                 const VIE_CDOUBLE *current, double k0, int ground, const VIE_CDOUBLE *rs,
+                                 const VIE_CDOUBLE *rp, size_t count, const double *directions,
+                                 VIE_CDOUBLE *amplitude);
+/* Scattered power in W by the reference's 974-point golden-spiral rule over
+   the sphere (ground 0) or the upper hemisphere over a PEC plane (ground 1)
+   (scattered_power, dielectric_solver.c:160-217). */
+int vie_hip_scattered_power(size_t cells, const double *centers, const double *sizes, const VIE_CDOUBLE *current,
+                            double k0, int ground, double *power);
 /* Host-side step of the recycled subspace, exported for tests: the k harmonic
    Ritz vectors of smallest modulus (Morgan 2002) of the (m+1) x m Arnoldi
    Hessenberg hbar (column-major), as the m x k column-major p with unit

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib")
-SOURCES = ["vie_passes_f32.hip", "vie_passes_f64.hip", "spectrum.hip", "krylov.hip", "api.hip"]
+SOURCES = ["vie_passes_f32.hip", "vie_passes_f64.hip", "spectrum.hip", "krylov.hip", "farfield.hip", "api.hip"]
 ARCH = os.environ.get("VIE_HIP_ARCH", "gfx950")
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
          # packed-FP32 instructions are kept out of every kernel
@@ -43,7 +43,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(5, os.cpu_count() or 1)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     out = os.path.join(LIB, "libvie_hip.so")
     tmp = out + ".tmp"

@@ -2061,6 +2061,114 @@ extern "C" int dielectric_fft_last_solve_escalated(const dielectric_fft_system* 
   return s && s->sw.escalated ? 1 : 0;
 }
 
+// ------------------------------------------------------------------ scattered far field
+// The reference's post-processing of a solved volume current
+// (dielectric_solver.c:124-217) for many directions at once on the device.
+namespace {
+struct FarfieldWork {
+  std::mutex mu;
+  DevBuf centers, sizes, current, dirs, rs, rp, partial, amp;
+  hipStream_t stream = nullptr;
+};
+FarfieldWork& farfield_work() {
+  static FarfieldWork* w = new FarfieldWork();  // lives for the process (device buffers)
+  return *w;
+}
+}  // namespace
+
+static int farfield_run(size_t cells, const double* centers, const double* sizes, const double _Complex* current,
+                        double k0, int ground, const double _Complex* rs, const double _Complex* rp, size_t count,
+                        const double* directions, double _Complex* amplitude) {
+  g_last_error.clear();
+  if ((cells && (!centers || !sizes || !current)) || (count && (!directions || !amplitude)) || ground < 0 ||
+      ground > 2 || (ground == 2 && count && (!rs || !rp)) || !std::isfinite(k0) || !(k0 > 0)) {
+    set_error("radiation amplitudes: invalid argument");
+    return 0;
+  }
+  size_t b3c, b3d, bj, bpart;
+  if (!mul_ok(cells, 3 * sizeof(double), &b3c) || !mul_ok(count, 3 * sizeof(double), &b3d) ||
+      !mul_ok(cells, 3 * sizeof(c64), &bj)) {
+    set_error("radiation amplitudes: size overflow");
+    return 0;
+  }
+  if (count == 0) return 1;
+  if (!ensure_device()) return 0;
+  // one cell size for all (a lattice) lets the kernel keep the cell factor per direction
+  int uniform = 1;
+  for (size_t c = 1; c < cells && uniform; c++)
+    for (int a = 0; a < 3; a++)
+      if (sizes[3 * c + a] != sizes[a]) uniform = 0;
+  const unsigned slices = vie::farfield_slices(cells, count, ground);
+  if (!mul_ok(count, 6 * (size_t)slices * sizeof(c64), &bpart)) {
+    set_error("radiation amplitudes: size overflow");
+    return 0;
+  }
+  FarfieldWork& w = farfield_work();
+  std::lock_guard<std::mutex> lock(w.mu);
+  if (!w.stream) HIP_OK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+  const size_t bsz = uniform ? 3 * sizeof(double) : b3c;
+  if (!(w.centers.alloc(b3c ? b3c : 16) && w.sizes.alloc(bsz ? bsz : 16) && w.current.alloc(bj ? bj : 16) &&
+        w.dirs.alloc(b3d) && w.partial.alloc(bpart) && w.amp.alloc(count * 3 * sizeof(c64)) &&
+        (ground != 2 || (w.rs.alloc(count * sizeof(c64)) && w.rp.alloc(count * sizeof(c64))))))
+    return 0;
+  hipStream_t st = w.stream;
+  if (cells) {
+    HIP_OK(hipMemcpyAsync(w.centers.p, centers, b3c, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(w.sizes.p, sizes, bsz, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(w.current.p, current, bj, hipMemcpyHostToDevice, st));
+  }
+  HIP_OK(hipMemcpyAsync(w.dirs.p, directions, b3d, hipMemcpyHostToDevice, st));
+  if (ground == 2) {
+    HIP_OK(hipMemcpyAsync(w.rs.p, rs, count * sizeof(c64), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(w.rp.p, rp, count * sizeof(c64), hipMemcpyHostToDevice, st));
+  }
+  if (!vie::farfield_amplitudes(w.centers.as<double>(), w.sizes.as<double>(), uniform, w.current.as<c64>(), cells,
+                                w.dirs.as<double>(), count, k0, ground, w.rs.as<c64>(), w.rp.as<c64>(),
+                                w.partial.as<c64>(), slices, w.amp.as<c64>(), st)) {
+    set_error("far-field kernels failed to launch");
+    return 0;
+  }
+  HIP_OK(hipMemcpyAsync(amplitude, w.amp.p, count * 3 * sizeof(c64), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  return 1;
+}
+
+extern "C" int vie_hip_radiation_amplitudes(size_t cells, const double* centers, const double* sizes,
+                                            const double _Complex* current, double k0, int ground,
+                                            const double _Complex* rs, const double _Complex* rp, size_t count,
+                                            const double* directions, double _Complex* amplitude) {
+  return farfield_run(cells, centers, sizes, current, k0, ground, rs, rp, count, directions, amplitude);
+}
+
+extern "C" int vie_hip_scattered_power(size_t cells, const double* centers, const double* sizes,
+                                       const double _Complex* current, double k0, int ground, double* power) {
+  // the golden-spiral rule of scattered_power (dielectric_solver.c:160-217):
+  // the sphere in free space, the upper hemisphere over a PEC ground
+  if (!power || (ground != 0 && ground != 1)) {
+    set_error("scattered power: invalid argument");
+    return 0;
+  }
+  const size_t count = 974;
+  const double golden = kPi * (3.0 - sqrt(5.0));
+  std::vector<double> dirs(3 * count);
+  for (size_t i = 0; i < count; i++) {
+    const double z = 1.0 - (ground ? 1.0 : 2.0) * ((double)i + 0.5) / (double)count;
+    const double radial = sqrt(std::max(0.0, 1.0 - z * z)), az = golden * (double)i;
+    dirs[3 * i] = radial * cos(az);
+    dirs[3 * i + 1] = radial * sin(az);
+    dirs[3 * i + 2] = z;
+  }
+  std::vector<c64> amp(3 * count);
+  if (!farfield_run(cells, centers, sizes, current, k0, ground, nullptr, nullptr, count, dirs.data(),
+                    (double _Complex*)amp.data()))
+    return 0;
+  double sum = 0;
+  for (size_t i = 0; i < 3 * count; i++) sum += amp[i].re * amp[i].re + amp[i].im * amp[i].im;
+  const double eta0 = sqrt(1.25663706212e-6 / 8.8541878128e-12);  // dielectric_solver.c:20-23, 166
+  *power = ((ground ? 2.0 : 4.0) * kPi / (double)count) * sum / (2.0 * eta0);
+  return 1;
+}
+
 extern "C" int dielectric_fft_last_solve_recycled(const dielectric_fft_system* s) {
   return s && s->sw.rec_ready ? s->sw.rk : 0;
 }

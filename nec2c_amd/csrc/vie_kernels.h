@@ -46,6 +46,18 @@ struct SpectrumResult {
 int build_spectrum(const int kshape[3], const int padded[3], double h, double k, c64* spec64,
                    c32* spec32, SpectrumResult* out, hipStream_t s);
 
+// ---- scattered far field (farfield.hip) ----
+// Slices of the cells per direction tile (the partial sums per direction are
+// 2 x 3 x slices values).
+unsigned farfield_slices(size_t n, size_t ndir, int ground);
+// amplitude[3 d] for the unit directions dirs[3 d] (device arrays; centers and
+// sizes [3 n], sizes of one cell only when uniform, current [3 n] cell-major);
+// ground 0 free space, 1 PEC plane, 2 per-direction rs / rp.  partial holds
+// 6 ndir slices values.
+int farfield_amplitudes(const double* centers, const double* sizes, int uniform, const c64* current, size_t n,
+                        const double* dirs, size_t ndir, double k0, int ground, const c64* rs, const c64* rp,
+                        c64* partial, unsigned slices, c64* amplitude, hipStream_t s);
+
 // ---- vector kernels (krylov.hip) ----
 template <typename T> void vec_zero(cx<T>* x, size_t n, hipStream_t s);
 template <typename To, typename From> void vec_cast(cx<To>* y, const cx<From>* x, size_t n, hipStream_t s);

@@ -129,6 +129,7 @@ EXPORTS = (
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
     "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
     "dielectric_fft_last_solve_escalated", "dielectric_fft_last_solve_recycled", "vie_hip_harmonic_ritz",
+    "vie_hip_radiation_amplitudes", "vie_hip_scattered_power",
 )
 
 _LIB = None
@@ -214,6 +215,8 @@ def lib() -> ctypes.CDLL:
         "dielectric_fft_last_solve_escalated": (i, [vp]),
         "dielectric_fft_last_solve_recycled": (i, [vp]),
         "vie_hip_harmonic_ritz": (i, [vp, i, i, vp]),
+        "vie_hip_radiation_amplitudes": (i, [sz, vp, vp, vp, d, i, vp, vp, sz, vp, vp]),
+        "vie_hip_scattered_power": (i, [sz, vp, vp, vp, d, i, ctypes.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -234,6 +237,46 @@ def harmonic_ritz(hbar, k):
     if not lib().vie_hip_harmonic_ritz(hbar.ctypes.data, m, k, p.ctypes.data):
         raise RuntimeError("harmonic Ritz vectors failed")
     return p
+
+
+def _cells_arrays(centers, sizes, current):
+    c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+    s = np.ascontiguousarray(sizes, dtype=np.float64).reshape(-1, 3)
+    j = np.ascontiguousarray(current, dtype=np.complex128).reshape(-1)
+    if len(s) != len(c) or len(j) != 3 * len(c):
+        raise ValueError("centers, sizes and current must describe the same cells")
+    return c, s, j
+
+
+def radiation_amplitudes(centers, sizes, current, k0, directions, ground=0, rs=None, rp=None):
+    """vie_hip_radiation_amplitudes: [D, 3] scattered far-field amplitudes of a
+    volume current (free space, PEC plane, or per-direction rs / rp)."""
+    c, s, j = _cells_arrays(centers, sizes, current)
+    d = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(d), 3), dtype=np.complex128)
+    rsa = rpa = None
+    if ground == 2:
+        rsa = np.ascontiguousarray(rs, dtype=np.complex128).reshape(-1)
+        rpa = np.ascontiguousarray(rp, dtype=np.complex128).reshape(-1)
+        if len(rsa) != len(d) or len(rpa) != len(d):
+            raise ValueError("one rs and one rp per direction")
+    ok = lib().vie_hip_radiation_amplitudes(len(c), c.ctypes.data, s.ctypes.data, j.ctypes.data, float(k0),
+                                            int(ground), None if rsa is None else rsa.ctypes.data,
+                                            None if rpa is None else rpa.ctypes.data, len(d), d.ctypes.data,
+                                            out.ctypes.data)
+    if not ok:
+        raise RuntimeError(f"radiation amplitudes failed: {last_error()}")
+    return out
+
+
+def scattered_power(centers, sizes, current, k0, ground=0):
+    """vie_hip_scattered_power: the reference's 974-direction scattered power (W)."""
+    c, s, j = _cells_arrays(centers, sizes, current)
+    p = ctypes.c_double()
+    if not lib().vie_hip_scattered_power(len(c), c.ctypes.data, s.ctypes.data, j.ctypes.data, float(k0), int(ground),
+                                         ctypes.byref(p)):
+        raise RuntimeError(f"scattered power failed: {last_error()}")
+    return p.value
 
 
 def last_error() -> str:

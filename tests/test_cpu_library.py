@@ -278,3 +278,25 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_far_field_rejections_before_any_device_use():
+    L = vie.lib()
+    d = np.array([0.0, 0.0, 1.0])
+    out = np.zeros(3, complex)
+    c = np.zeros(3)
+    J = np.zeros(3, complex)
+    # unknown ground, missing Fresnel coefficients, bad k0, null cell arrays
+    assert not L.vie_hip_radiation_amplitudes(1, c.ctypes.data, c.ctypes.data, J.ctypes.data, 1.0, 5, None, None, 1,
+                                              d.ctypes.data, out.ctypes.data)
+    assert not L.vie_hip_radiation_amplitudes(1, c.ctypes.data, c.ctypes.data, J.ctypes.data, 1.0, 2, None, None, 1,
+                                              d.ctypes.data, out.ctypes.data)
+    assert not L.vie_hip_radiation_amplitudes(1, c.ctypes.data, c.ctypes.data, J.ctypes.data, float("nan"), 0, None,
+                                              None, 1, d.ctypes.data, out.ctypes.data)
+    assert not L.vie_hip_radiation_amplitudes(1, None, None, None, 1.0, 0, None, None, 1, d.ctypes.data,
+                                              out.ctypes.data)
+    p = ctypes.c_double()
+    assert not L.vie_hip_scattered_power(1, c.ctypes.data, c.ctypes.data, J.ctypes.data, 1.0, 2, ctypes.byref(p))
+    # no directions: nothing to do, no device needed
+    assert L.vie_hip_radiation_amplitudes(1, c.ctypes.data, c.ctypes.data, J.ctypes.data, 1.0, 0, None, None, 0,
+                                          None, None)
