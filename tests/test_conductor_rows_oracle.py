@@ -1,0 +1,114 @@
+"""Pins of the volume-to-conductor oracle (oracle/conductor_rows.py, a
+restatement of dielectric_wire.c:70-214 and :1164-1262) against what the
+mathematics fixes: numpy's Gauss-Legendre rule, finite differences of the
+scalar Green function for both dyadics, the point-source limit, image theory
+over a PEC plane, and a loop over the row assembly."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import conductor_rows as R
+
+
+def test_gauss_rule_is_gauss_legendre():
+    for order in (6, 8, 10):
+        x, w = R.gauss(order)
+        xr, wr = np.polynomial.legendre.leggauss(order)
+        assert np.allclose(x, xr, atol=2e-15) and np.allclose(w, wr, atol=2e-15)
+
+
+def green(p, k0):
+    r = np.linalg.norm(p)
+    return np.exp(-1j * k0 * r) / (4 * math.pi * r)
+
+
+def fd_hessian(p, k0, h=1e-4):
+    H = np.zeros((3, 3), complex)
+    e = np.eye(3) * h
+    for a in range(3):
+        for b in range(3):
+            H[a, b] = (green(p + e[a] + e[b], k0) - green(p + e[a] - e[b], k0) - green(p - e[a] + e[b], k0)
+                       + green(p - e[a] - e[b], k0)) / (4 * h * h)
+    return H
+
+
+def test_e_dyadic_is_k2_plus_grad_grad_of_g():
+    """A tiny cell far from the point: N ~ V (k^2 I + grad grad) g(r), with the
+    Hessian from finite differences (a flipped sign or a dropped term fails)."""
+    k0 = 7.0
+    obs = np.array([0.9, -0.4, 0.7])
+    c, s = np.array([0.05, 0.02, -0.01]), np.array([1e-3, 2e-3, 1.5e-3])
+    N = R.point_cuboid_n(obs, c, s, k0)
+    d = obs - c
+    expect = np.prod(s) * (k0 ** 2 * green(d, k0) * np.eye(3) + fd_hessian(d, k0))
+    # the cell's finite size enters at (s/r)^2 ~ 3e-6, the differences at ~1e-8
+    assert np.allclose(N, expect, rtol=2e-5, atol=0)
+    assert np.allclose(N, N.T, rtol=1e-13, atol=0)
+
+
+def test_h_dyadic_is_grad_g_cross():
+    k0 = 5.0
+    obs = np.array([-0.6, 0.8, 0.3])
+    c, s = np.array([0.0, 0.1, 0.0]), np.array([2e-3, 1e-3, 1e-3])
+    H = R.point_cuboid_h(obs, c, s, k0)
+    d = obs - c
+    h = 1e-6
+    # column q: (gradient of g with respect to the observation point) x e_q
+    grad = np.array([(green(d + h * e, k0) - green(d - h * e, k0)) / (2 * h) for e in np.eye(3)])
+    expect = np.prod(s) * np.stack([np.cross(grad, e) for e in np.eye(3)], axis=1)
+    assert np.allclose(H, expect, rtol=2e-5, atol=0)
+
+
+def test_point_source_limit_converges():
+    k0 = 10.0
+    obs = np.array([0.5, 0.5, 0.5])
+    errs = []
+    for size in (0.04, 0.02, 0.01):
+        s = np.full(3, size)
+        N = R.point_cuboid_n(obs, np.zeros(3), s, k0)
+        P = size ** 3 * (k0 ** 2 * green(obs, k0) * np.eye(3) + fd_hessian(obs, k0, 1e-3))
+        errs.append(np.abs(N - P).max() / np.abs(P).max())
+    # second order in the cell size
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-3
+    assert 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5
+
+
+@pytest.mark.parametrize("magnetic", [False, True])
+def test_pec_is_image_theory(magnetic):
+    """Over a PEC plane: the field of a current equals free space plus the image
+    current (horizontal components reversed, vertical kept) at the mirrored cell."""
+    k0 = 6.0
+    obs = np.array([0.3, -0.2, 0.4])
+    c, s = np.array([0.1, 0.05, 0.15]), np.array([0.02, 0.03, 0.02])
+    f = R.point_cuboid_h if magnetic else R.point_cuboid_n
+    got = R.point_background(obs, c, s, k0, magnetic, ground=1)
+    ci = c * np.array([1, 1, -1])
+    for q in range(3):
+        J = np.eye(3)[q]
+        Jimg = J * np.array([-1.0, -1.0, 1.0])
+        expect = f(obs, c, s, k0) @ J + f(obs, ci, s, k0) @ Jimg
+        assert np.allclose(got @ J, expect, rtol=1e-13, atol=0)
+
+
+def test_rows_follow_the_assembly():
+    rng = np.random.default_rng(3)
+    k0 = 2 * math.pi * 300e6 / R.C0
+    centers = rng.uniform(0.1, 0.3, (4, 3))
+    sizes = rng.uniform(0.01, 0.02, (4, 3))
+    wp = rng.uniform(-0.2, 0.0, (2, 3))
+    wt = rng.standard_normal((2, 3))
+    wt /= np.linalg.norm(wt, axis=1, keepdims=True)
+    pp = rng.uniform(-0.2, 0.0, (1, 3))
+    t1, t2 = np.eye(3)[0:1], np.eye(3)[1:2]
+    rows = R.conductor_rows(k0, centers, sizes, wp, wt, pp, t1, t2, [0.7], ground=0)
+    omega = k0 * R.C0
+    for w in range(2):
+        for cc in range(4):
+            block = R.point_cuboid_n(wp[w], centers[cc], sizes[cc], k0)
+            for q in range(3):
+                tested = sum(wt[w][p] * block[p][q] for p in range(3))
+                assert abs(rows[w, 3 * cc + q] - tested / (1j * omega * R.EPSILON_0)) <= 1e-14 * abs(rows[w]).max()
+    block = R.point_cuboid_h(pp[0], centers[1], sizes[1], k0)
+    assert np.allclose(rows[2, 3:6], -0.7 * (t2[0] @ block))
+    assert np.allclose(rows[3, 3:6], -0.7 * (t1[0] @ block))
