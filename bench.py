@@ -509,6 +509,72 @@ def bench_farfield(args):
     return 0
 
 
+def bench_rows(args):
+    """Volume-to-wire rows of the sweep's coupled system: the n^3 cube (edge
+    0.5 m) and the 21-segment dipole 0.1 m from one face, every (segment,
+    cell) pair by the reference's 10^3-point rule, free space and over a PEC
+    plane 0.05 m below the cube.  Kernel time with the geometry resident
+    (HIP events), plus the whole call with the rows copied to the host.  CPU
+    baseline: the C restatement oracle/conductor_rows.c with OpenMP over the
+    tasks, as the reference runs them, on a bounded sample of cells."""
+    from nec2c_amd import vie, workloads as W
+    n = args.rows_size
+    pitch = 0.5 / n
+    seg = 0.48 / 21
+    wp = np.array([[-0.1, 0.25, 0.01 + (i + 0.5) * seg] for i in range(21)])
+    wt = np.tile([0.0, 0.0, 1.0], (21, 1))
+    k0 = W.k0_of(250e6)
+    res = {}
+    for ground, tag in ((0, "free"), (1, "pec")):
+        centers = W.cube_cells(n, pitch) + np.array([0.0, 0.0, 0.05 if ground else 0.0])
+        sizes = np.full_like(centers, pitch)
+        pairs = len(wp) * len(centers)
+        ms = vie.time_volume_conductor_rows(k0, centers, sizes, wp, wt, ground=ground, repeats=args.steps)
+        t = time.perf_counter()
+        vie.volume_conductor_rows(k0, centers, sizes, wp, wt, ground=ground)
+        t_call = time.perf_counter() - t
+        res[tag] = {"kernel_ms": ms, "pairs": pairs, "pairs_per_s": pairs / (ms * 1e-3), "call_ms": t_call * 1e3,
+                    "row_bytes": pairs * 3 * 16}
+        log(f"rows {tag}: {pairs} pairs, kernel {ms:.3f} ms ({pairs / (ms * 1e-3):.3e} pairs/s), "
+            f"call with {pairs * 48 / 1e6:.0f} MB to the host {t_call * 1e3:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import conductor_rows as R
+        cores = min(16, len(os.sched_getaffinity(0)))
+        centers = W.cube_cells(n, pitch)
+        sample = centers[:: max(1, len(centers) // args.rows_cpu_cells)][: args.rows_cpu_cells]
+        sizes = np.full_like(sample, pitch)
+        t = time.perf_counter()
+        R.conductor_rows_c(k0, sample, sizes, wp, wt, threads=cores)
+        t_cpu = time.perf_counter() - t
+        rate = len(wp) * len(sample) / t_cpu
+        cpu = {"value": rate, "unit": "pairs/s", "cores": cores, "kind": "port",
+               "sample": f"21 segments x {len(sample)} cells (free space) through oracle/conductor_rows.c "
+                         f"(OpenMP, {cores} threads) in {t_cpu:.2f} s"}
+        log(f"rows CPU port: {rate:.3e} pairs/s on {cores} threads")
+    free = res["free"]
+    # FP64 VALU issue: the wire-row z loop of k_conductor_rows<kSeries4> is 108
+    # VALU instructions per two nodes in the AMD assembly of this build (54 a
+    # node, one a quarter-rate v_rsq_f64); peak = 256 CUs x 4 SIMDs x 16 lanes
+    # x 2.4 GHz lane-instructions/s (78.6 TFLOP/s FP64 counting an FMA as 2).
+    per_node, peak = 54, 256 * 4 * 16 * 2.4e9
+    achieved = free["pairs_per_s"] * 1000 * per_node
+    roof = {"bound": "valu-fp64", "achieved": achieved, "peak": peak, "unit": "FP64 VALU lane-instructions/s",
+            "frac": achieved / peak, "traffic": None, "kernel": "k_conductor_rows<kSeries4> (wire rows)",
+            "instructions_per_node": per_node}
+    log(f"rows roofline: {achieved:.3e} of {peak:.3e} FP64 VALU lane-instructions/s ({achieved / peak:.1%})")
+    print(json.dumps({
+        "metric": "volume-to-wire row pairs/s (10^3-point cuboid integral per segment-cell pair)",
+        "value": free["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": free["kernel_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 cube (edge 0.5 m), 21-segment dipole 0.1 m from a face, 250 MHz; "
+                               "the B rows of the coupled system", "cells": n ** 3, "segments": 21},
+        "rows": res, "cpu_baseline": cpu, "roofline": roof,
+    }))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -524,13 +590,16 @@ def main():
                     help="mixed: FP32 cycle stops at |g| <= max(target, this * beta); 0 = library default")
     ap.add_argument("--recycle", type=int, default=0,
                     help="mixed: recycled-subspace size of the GMRES cycles (0 = library default, -1 = off)")
-    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield", "rows"),
+                    default="sweep",
                     help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
                          "applies of one sweep point plus the copy calibration kernel (for counter runs)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="matvec: bytes of the calibration copy")
     ap.add_argument("--slab-size", type=int, default=128, help="slab: voxels per edge (128 = the 128^3 slab)")
     ap.add_argument("--farfield-size", type=int, default=64, help="farfield: voxels per cube edge")
+    ap.add_argument("--rows-size", type=int, default=64, help="rows: voxels per cube edge")
+    ap.add_argument("--rows-cpu-cells", type=int, default=16384, help="rows: cells in the CPU-baseline sample")
     ap.add_argument("--arnoldi-column", type=int, default=0,
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
@@ -560,6 +629,8 @@ def main():
         return bench_slab(args)
     if args.workload == "farfield":
         return bench_farfield(args)
+    if args.workload == "rows":
+        return bench_rows(args)
 
     n = args.size
     pitch = 0.5 / n

@@ -135,6 +135,34 @@ int vie_hip_radiation_amplitudes(size_t cells, const double *centers, const doub
    (scattered_power, dielectric_solver.c:160-217). */
 int vie_hip_scattered_power(size_t cells, const double *centers, const double *sizes, const VIE_CDOUBLE *current,
                             double k0, int ground, double *power);
+/* Volume-to-conductor rows of the coupled wire/patch + volume system on the
+   device: the two task loops of dielectric_wire.c:1164-1262 (wire rows from
+   dw_point_cuboid_n, patch rows from dw_point_cuboid_h, :101-214), every
+   (conductor, cell) pair at once.  Cells: centers[3 cells], sizes[3 cells] in
+   m; k0 in 1/m, omega = k0 c0.  Wire w: match point wire_points[3 w], unit
+   tangent wire_tangents[3 w] (the reference's x, y, z and cab, sab, salp);
+   patch p: patch_points[3 p], tangents patch_t1[3 p], patch_t2[3 p], scale
+   patch_scale[p] (psalp).  ground 0: free space; 1: PEC plane z = 0 (the
+   mirrored cell with column signs -1, -1, +1, gnd.iperf == 1); a finite
+   ground is rejected.  Writes row w = t.N/(i omega eps0) for each wire, rows
+   wires + 2p and wires + 2p + 1 = -psalp t2.H and -psalp t1.H for each patch,
+   entry (row, 3 cell + q) at rows[row ld + 3 cell + q], ld >= 3 cells (the
+   reference's system row with rows = &entry(0, conductor_unknowns), ld =
+   total).  1 on success; 0 with vie_hip_last_error() on invalid input, a
+   device failure, or a quadrature node on a match point (the reference's
+   message for its first failing pair: "wire W, volume cell C: ..."). */
+int vie_hip_volume_conductor_rows(double k0, size_t cells, const double *centers, const double *sizes, int wires,
+                                  const double *wire_points, const double *wire_tangents, int patches,
+                                  const double *patch_points, const double *patch_t1, const double *patch_t2,
+                                  const double *patch_scale, int ground, VIE_CDOUBLE *rows, size_t ld);
+/* The same rows computed on device buffers only, `repeats` times between HIP
+   events on the library's stream after one untimed pass; *ms receives the mean
+   kernel time per pass (new; for measurement). */
+int vie_hip_time_volume_conductor_rows(double k0, size_t cells, const double *centers, const double *sizes,
+                                       int wires, const double *wire_points, const double *wire_tangents,
+                                       int patches, const double *patch_points, const double *patch_t1,
+                                       const double *patch_t2, const double *patch_scale, int ground, int repeats,
+                                       float *ms);
 /* Host-side step of the recycled subspace, exported for tests: the k harmonic
    Ritz vectors of smallest modulus (Morgan 2002) of the (m+1) x m Arnoldi
    Hessenberg hbar (column-major), as the m x k column-major p with unit

@@ -2169,6 +2169,161 @@ extern "C" int vie_hip_scattered_power(size_t cells, const double* centers, cons
   return 1;
 }
 
+// ------------------------------------------------------------------ volume-to-conductor rows
+// The wire and patch rows of the coupled system over the volume unknowns
+// (dielectric_wire.c:1164-1262), every (conductor, cell) pair at once.
+namespace {
+struct RowsWork {
+  std::mutex mu;
+  DevBuf centers, sizes, cond, rows, failed;
+  hipStream_t stream = nullptr;
+};
+RowsWork& rows_work() {
+  static RowsWork* w = new RowsWork();  // lives for the process (device buffers)
+  return *w;
+}
+struct RowsArgs {
+  double k0;
+  size_t cells;
+  const double *centers, *sizes;
+  int wires;
+  const double *wire_points, *wire_tangents;
+  int patches;
+  const double *patch_points, *patch_t1, *patch_t2, *patch_scale;
+  int ground;
+};
+}  // namespace
+
+// Uploads the geometry and fills the device rows; with repeats > 0 the kernel
+// runs that many more times between HIP events and *ms gets the mean.  With
+// host_rows it downloads the rows into host_rows (leading dimension ld).
+static int rows_run(const RowsArgs& a, double _Complex* host_rows, size_t ld, int repeats, float* ms) {
+  g_last_error.clear();
+  const size_t nrows = (size_t)(a.wires > 0 ? a.wires : 0) + 2 * (size_t)(a.patches > 0 ? a.patches : 0);
+  if (a.wires < 0 || a.patches < 0 || a.ground < 0 || a.ground > 2 || !std::isfinite(a.k0) || !(a.k0 > 0) ||
+      (a.cells && (!a.centers || !a.sizes)) || (a.wires && (!a.wire_points || !a.wire_tangents)) ||
+      (a.patches && (!a.patch_points || !a.patch_t1 || !a.patch_t2 || !a.patch_scale)) ||
+      (nrows && a.cells && repeats <= 0 && (!host_rows || ld < 3 * a.cells))) {
+    set_error("volume-to-conductor rows: invalid argument");
+    return 0;
+  }
+  if (a.ground == 2) {
+    set_error("volume-to-conductor rows: a finite ground (Sommerfeld half space) is not supported on the device");
+    return 0;
+  }
+  size_t b3c, row_bytes, rows_bytes, pitch;
+  if (!mul_ok(a.cells, 3 * sizeof(double), &b3c) || !mul_ok(a.cells, 3 * sizeof(c64), &row_bytes) ||
+      !mul_ok(nrows, row_bytes, &rows_bytes) || !mul_ok(ld, sizeof(c64), &pitch) ||
+      (size_t)a.wires + 2 * (size_t)a.patches > (size_t)INT_MAX) {
+    set_error("volume-to-conductor rows: size overflow");
+    return 0;
+  }
+  if (nrows == 0 || a.cells == 0) {
+    if (ms) *ms = 0.0f;
+    return 1;
+  }
+  if (!ensure_device()) return 0;
+  const size_t nw = (size_t)a.wires, np = (size_t)a.patches;
+  const size_t ncond = 6 * nw + 10 * np;  // points, tangents; points, t1, t2, scale
+  RowsWork& w = rows_work();
+  std::lock_guard<std::mutex> lock(w.mu);
+  if (!w.stream) HIP_OK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+  if (!(w.centers.alloc(b3c) && w.sizes.alloc(b3c) && w.cond.alloc(ncond * sizeof(double)) &&
+        w.rows.alloc(rows_bytes) && w.failed.alloc(sizeof(unsigned long long))))
+    return 0;
+  hipStream_t st = w.stream;
+  double* cond = w.cond.as<double>();
+  double *wp = cond, *wt = wp + 3 * nw, *pp = wt + 3 * nw, *t1 = pp + 3 * np, *t2 = t1 + 3 * np, *ps = t2 + 3 * np;
+  HIP_OK(hipMemcpyAsync(w.centers.p, a.centers, b3c, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.sizes.p, a.sizes, b3c, hipMemcpyHostToDevice, st));
+  if (nw) {
+    HIP_OK(hipMemcpyAsync(wp, a.wire_points, 3 * nw * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(wt, a.wire_tangents, 3 * nw * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  if (np) {
+    HIP_OK(hipMemcpyAsync(pp, a.patch_points, 3 * np * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(t1, a.patch_t1, 3 * np * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(t2, a.patch_t2, 3 * np * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(ps, a.patch_scale, np * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  const unsigned long long none = ~0ull;
+  HIP_OK(hipMemcpyAsync(w.failed.p, &none, sizeof(none), hipMemcpyHostToDevice, st));
+  // largest cell diagonal (NaN when a size is NaN: the kernel then picks per cell)
+  double max_size = 0.0;
+  for (size_t c = 0; c < a.cells; c++) {
+    const double* z = a.sizes + 3 * c;
+    const double d = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+    max_size = std::isnan(d) || std::isnan(max_size) ? NAN : std::max(max_size, d);
+  }
+  auto launch = [&]() {
+    return vie::conductor_rows(a.k0, w.centers.as<double>(), w.sizes.as<double>(), a.cells, max_size, a.wires, wp, wt,
+                               a.patches, pp, t1, t2, ps, a.ground, w.rows.as<c64>(), 3 * a.cells,
+                               w.failed.as<unsigned long long>(), st);
+  };
+  if (!launch()) {
+    set_error("volume-to-conductor row kernel failed to launch");
+    return 0;
+  }
+  if (repeats > 0) {
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    int ok = hipEventRecord(e0, st) == hipSuccess;
+    for (int r = 0; r < repeats && ok; r++) ok = launch();
+    ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float t = 0;
+    if (ok) ok = hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) {
+      set_error("volume-to-conductor row timing failed");
+      return 0;
+    }
+    if (ms) *ms = t / (float)repeats;
+  }
+  unsigned long long failed = none;
+  HIP_OK(hipMemcpyAsync(&failed, w.failed.p, sizeof(failed), hipMemcpyDeviceToHost, st));
+  if (host_rows)
+    HIP_OK(hipMemcpy2DAsync(host_rows, pitch, w.rows.p, row_bytes, row_bytes, nrows,
+                            hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (failed != none) {
+    // the reference's message for its first failed task (dielectric_wire.c:1200-1207, :1249-1256)
+    const size_t task = (size_t)(failed / a.cells), cell = (size_t)(failed % a.cells);
+    if (task < nw)
+      set_error("wire %zu, volume cell %zu: volume-to-wire interaction failed", task + 1, cell + 1);
+    else
+      set_error("patch %zu, volume cell %zu: volume-to-patch interaction failed", task - nw + 1, cell + 1);
+    return 0;
+  }
+  return 1;
+}
+
+extern "C" int vie_hip_volume_conductor_rows(double k0, size_t cells, const double* centers, const double* sizes,
+                                             int wires, const double* wire_points, const double* wire_tangents,
+                                             int patches, const double* patch_points, const double* patch_t1,
+                                             const double* patch_t2, const double* patch_scale, int ground,
+                                             double _Complex* rows, size_t ld) {
+  const RowsArgs a{k0, cells, centers, sizes, wires, wire_points, wire_tangents,
+                   patches, patch_points, patch_t1, patch_t2, patch_scale, ground};
+  return rows_run(a, rows, ld, 0, nullptr);
+}
+
+extern "C" int vie_hip_time_volume_conductor_rows(double k0, size_t cells, const double* centers,
+                                                  const double* sizes, int wires, const double* wire_points,
+                                                  const double* wire_tangents, int patches,
+                                                  const double* patch_points, const double* patch_t1,
+                                                  const double* patch_t2, const double* patch_scale, int ground,
+                                                  int repeats, float* ms) {
+  const RowsArgs a{k0, cells, centers, sizes, wires, wire_points, wire_tangents,
+                   patches, patch_points, patch_t1, patch_t2, patch_scale, ground};
+  if (repeats < 1 || !ms) {
+    set_error("volume-to-conductor row timing: invalid argument");
+    return 0;
+  }
+  return rows_run(a, nullptr, 0, repeats, ms);
+}
+
 extern "C" int dielectric_fft_last_solve_recycled(const dielectric_fft_system* s) {
   return s && s->sw.rec_ready ? s->sw.rk : 0;
 }

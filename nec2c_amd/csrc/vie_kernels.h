@@ -58,6 +58,18 @@ int farfield_amplitudes(const double* centers, const double* sizes, int uniform,
                         const double* dirs, size_t ndir, double k0, int ground, const c64* rs, const c64* rp,
                         c64* partial, unsigned slices, c64* amplitude, hipStream_t s);
 
+// ---- volume-to-conductor rows (conductor_rows.hip) ----
+// Rows of the conductor equations over the volume unknowns (device arrays):
+// wire task w writes row w, patch task p rows nw + 2p and nw + 2p + 1; entry
+// (row, 3 cell + q) at rows[row ld + 3 cell + q].  ground 0 free space, 1 PEC
+// plane.  max_size: the largest |size| of a cell (the launch picks the phase
+// evaluation from it; NaN or inf selects the per-cell choice).  failed
+// (initialised to ~0ull by the caller) receives the smallest task * cells +
+// cell whose quadrature met the observation point.
+int conductor_rows(double k0, const double* centers, const double* sizes, size_t cells, double max_size, int nw,
+                   const double* wp, const double* wt, int np, const double* pp, const double* t1, const double* t2,
+                   const double* ps, int ground, c64* rows, size_t ld, unsigned long long* failed, hipStream_t s);
+
 // ---- vector kernels (krylov.hip) ----
 template <typename T> void vec_zero(cx<T>* x, size_t n, hipStream_t s);
 template <typename To, typename From> void vec_cast(cx<To>* y, const cx<From>* x, size_t n, hipStream_t s);

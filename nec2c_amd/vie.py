@@ -129,7 +129,8 @@ EXPORTS = (
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
     "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
     "dielectric_fft_last_solve_escalated", "dielectric_fft_last_solve_recycled", "vie_hip_harmonic_ritz",
-    "vie_hip_radiation_amplitudes", "vie_hip_scattered_power",
+    "vie_hip_radiation_amplitudes", "vie_hip_scattered_power", "vie_hip_volume_conductor_rows",
+    "vie_hip_time_volume_conductor_rows",
 )
 
 _LIB = None
@@ -217,6 +218,9 @@ def lib() -> ctypes.CDLL:
         "vie_hip_harmonic_ritz": (i, [vp, i, i, vp]),
         "vie_hip_radiation_amplitudes": (i, [sz, vp, vp, vp, d, i, vp, vp, sz, vp, vp]),
         "vie_hip_scattered_power": (i, [sz, vp, vp, vp, d, i, ctypes.POINTER(d)]),
+        "vie_hip_volume_conductor_rows": (i, [d, sz, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i, vp, sz]),
+        "vie_hip_time_volume_conductor_rows": (i, [d, sz, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i, i,
+                                                   ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -277,6 +281,49 @@ def scattered_power(centers, sizes, current, k0, ground=0):
                                          ctypes.byref(p)):
         raise RuntimeError(f"scattered power failed: {last_error()}")
     return p.value
+
+
+def _conductor_arrays(centers, sizes, wire_points, wire_tangents, patch_points, patch_t1, patch_t2, patch_scale):
+    def arr(x, width):
+        a = np.ascontiguousarray(np.zeros((0, width)) if x is None else x, dtype=np.float64)
+        return a.reshape(-1, width) if width > 1 else a.reshape(-1)
+    c, s = arr(centers, 3), arr(sizes, 3)
+    wp, wt = arr(wire_points, 3), arr(wire_tangents, 3)
+    pp, t1, t2, ps = arr(patch_points, 3), arr(patch_t1, 3), arr(patch_t2, 3), arr(patch_scale, 1)
+    if len(s) != len(c) or len(wt) != len(wp) or not (len(t1) == len(t2) == len(ps) == len(pp)):
+        raise ValueError("inconsistent cell, wire or patch arrays")
+    return c, s, wp, wt, pp, t1, t2, ps
+
+
+def volume_conductor_rows(k0, centers, sizes, wire_points=None, wire_tangents=None, patch_points=None,
+                          patch_t1=None, patch_t2=None, patch_scale=None, ground=0):
+    """vie_hip_volume_conductor_rows: the [wires + 2 patches, 3 cells] rows of
+    the conductor equations over the volume unknowns (dielectric_wire.c:1164-1262)."""
+    c, s, wp, wt, pp, t1, t2, ps = _conductor_arrays(centers, sizes, wire_points, wire_tangents, patch_points,
+                                                     patch_t1, patch_t2, patch_scale)
+    out = np.zeros((len(wp) + 2 * len(pp), 3 * len(c)), dtype=np.complex128)
+    ok = lib().vie_hip_volume_conductor_rows(float(k0), len(c), c.ctypes.data, s.ctypes.data, len(wp), wp.ctypes.data,
+                                             wt.ctypes.data, len(pp), pp.ctypes.data, t1.ctypes.data,
+                                             t2.ctypes.data, ps.ctypes.data, int(ground), out.ctypes.data,
+                                             out.shape[1])
+    if not ok:
+        raise RuntimeError(f"volume-to-conductor rows failed: {last_error()}")
+    return out
+
+
+def time_volume_conductor_rows(k0, centers, sizes, wire_points=None, wire_tangents=None, patch_points=None,
+                               patch_t1=None, patch_t2=None, patch_scale=None, ground=0, repeats=5):
+    """Mean device time (ms) of one pass of the row kernel over resident inputs."""
+    c, s, wp, wt, pp, t1, t2, ps = _conductor_arrays(centers, sizes, wire_points, wire_tangents, patch_points,
+                                                     patch_t1, patch_t2, patch_scale)
+    ms = ctypes.c_float()
+    ok = lib().vie_hip_time_volume_conductor_rows(float(k0), len(c), c.ctypes.data, s.ctypes.data, len(wp),
+                                                  wp.ctypes.data, wt.ctypes.data, len(pp), pp.ctypes.data,
+                                                  t1.ctypes.data, t2.ctypes.data, ps.ctypes.data, int(ground),
+                                                  int(repeats), ctypes.byref(ms))
+    if not ok:
+        raise RuntimeError(f"volume-to-conductor row timing failed: {last_error()}")
+    return ms.value
 
 
 def last_error() -> str:

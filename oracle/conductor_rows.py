@@ -15,12 +15,16 @@ lines it follows):
 * ``conductor_rows``        wire rows dielectric_wire.c:1164-1213 and patch
                             rows :1215-1262 (row order: wires, then two rows
                             per patch)
+* ``conductor_rows_c``      the same rows from the C restatement
+                            oracle/conductor_rows.c (OpenMP over the tasks, as
+                            the reference runs them; the CPU baseline)
 
 Parity status: pinned in tests/test_conductor_rows_oracle.py by numpy's
 Gauss-Legendre rule, finite differences of the scalar Green function for
 the E dyadic (k^2 + grad grad) g and for grad g x e_q, the point-source limit
 of a small cell far away, image theory over a PEC plane, and a loop that
-follows the row assembly statement by statement.
+follows the row assembly statement by statement; the C restatement entry for
+entry against this module.
 """
 from __future__ import annotations
 
@@ -136,3 +140,47 @@ def conductor_rows(k0, centers, sizes, wire_points=(), wire_tangents=(), patch_p
             rows[nw + 2 * p, 3 * c:3 * c + 3] = -patch_scale[p] * (t2 @ block)
             rows[nw + 2 * p + 1, 3 * c:3 * c + 3] = -patch_scale[p] * (t1 @ block)
     return rows
+
+
+_CLIB = None
+
+
+def _clib():
+    """oracle/_build/liboracle.so (built by oracle/Makefile), which holds the
+    C restatement oracle/conductor_rows.c."""
+    global _CLIB
+    if _CLIB is None:
+        import ctypes
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: run make -C {os.path.dirname(path)}")
+        L = ctypes.CDLL(path)
+        vp, d, i, sz = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_size_t
+        L.oracle_conductor_rows.restype = i
+        L.oracle_conductor_rows.argtypes = [d, sz, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i, vp, sz, i,
+                                            ctypes.POINTER(ctypes.c_longlong)]
+        _CLIB = L
+    return _CLIB
+
+
+def conductor_rows_c(k0, centers, sizes, wire_points=(), wire_tangents=(), patch_points=(), patch_t1=(),
+                     patch_t2=(), patch_scale=(), ground=0, threads=1):
+    """The same rows from the C restatement (oracle/conductor_rows.c), OpenMP
+    over the (conductor, cell) tasks with `threads` threads.  Returns (rows,
+    first failing task * cells + cell or -1)."""
+    import ctypes
+
+    def arr(x, width):
+        a = np.ascontiguousarray(np.zeros((0, width)) if len(x) == 0 else x, dtype=np.float64)
+        return a.reshape(-1, width) if width > 1 else a.reshape(-1)
+    c, s = arr(centers, 3), arr(sizes, 3)
+    wp, wt, pp = arr(wire_points, 3), arr(wire_tangents, 3), arr(patch_points, 3)
+    t1, t2, ps = arr(patch_t1, 3), arr(patch_t2, 3), arr(patch_scale, 1)
+    rows = np.zeros((len(wp) + 2 * len(pp), 3 * len(c)), dtype=np.complex128)
+    failed = ctypes.c_longlong(-1)
+    _clib().oracle_conductor_rows(float(k0), len(c), c.ctypes.data, s.ctypes.data, len(wp), wp.ctypes.data,
+                                  wt.ctypes.data, len(pp), pp.ctypes.data, t1.ctypes.data, t2.ctypes.data,
+                                  ps.ctypes.data, int(ground), rows.ctypes.data, rows.shape[1], int(threads),
+                                  ctypes.byref(failed))
+    return rows, failed.value

@@ -112,3 +112,29 @@ def test_rows_follow_the_assembly():
     block = R.point_cuboid_h(pp[0], centers[1], sizes[1], k0)
     assert np.allclose(rows[2, 3:6], -0.7 * (t2[0] @ block))
     assert np.allclose(rows[3, 3:6], -0.7 * (t1[0] @ block))
+
+
+@pytest.mark.parametrize("ground", [0, 1])
+def test_c_restatement_matches_numpy(ground):
+    """oracle/conductor_rows.c (the CPU baseline) entry for entry against the
+    numpy restatement, threaded, and its first-failure index."""
+    rng = np.random.default_rng(8 + ground)
+    k0 = 2 * math.pi * 250e6 / R.C0
+    centers = rng.uniform(0.0, 0.2, (6, 3)) + np.array([0, 0, 0.05])
+    sizes = rng.uniform(0.01, 0.02, (6, 3))
+    wp = rng.uniform(-0.3, -0.05, (3, 3))
+    wt = rng.standard_normal((3, 3))
+    wt /= np.linalg.norm(wt, axis=1, keepdims=True)
+    pp = rng.uniform(0.3, 0.5, (2, 3))
+    t1 = np.array([[1.0, 0, 0], [0, 0.6, 0.8]])
+    t2 = np.array([[0, 1.0, 0], [1.0, 0, 0]])
+    ps = [0.3, 0.9]
+    ref = R.conductor_rows(k0, centers, sizes, wp, wt, pp, t1, t2, ps, ground=ground)
+    got, failed = R.conductor_rows_c(k0, centers, sizes, wp, wt, pp, t1, t2, ps, ground=ground, threads=3)
+    assert failed == -1
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert np.max(np.abs(got - ref) / scale) < 1e-13
+    bad = pp.copy()
+    bad[1, 2] = math.nan
+    _, failed = R.conductor_rows_c(k0, centers, sizes, wp, wt, bad, t1, t2, ps, ground=ground, threads=3)
+    assert failed == (3 + 1) * 6 + 0

@@ -300,3 +300,37 @@ def test_far_field_rejections_before_any_device_use():
     # no directions: nothing to do, no device needed
     assert L.vie_hip_radiation_amplitudes(1, c.ctypes.data, c.ctypes.data, J.ctypes.data, 1.0, 0, None, None, 0,
                                           None, None)
+
+
+def test_conductor_row_rejections_before_any_device_use():
+    L = vie.lib()
+    c = np.zeros(3)
+    out = np.zeros(3, complex)
+    ms = ctypes.c_float()
+    # bad k0, negative count, null cell or wire arrays, ld below 3 cells, a finite ground
+    assert not L.vie_hip_volume_conductor_rows(float("nan"), 1, c.ctypes.data, c.ctypes.data, 1, c.ctypes.data,
+                                               c.ctypes.data, 0, None, None, None, None, 0, out.ctypes.data, 3)
+    assert not L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, -1, c.ctypes.data,
+                                               c.ctypes.data, 0, None, None, None, None, 0, out.ctypes.data, 3)
+    assert not L.vie_hip_volume_conductor_rows(1.0, 1, None, None, 1, c.ctypes.data, c.ctypes.data, 0, None, None,
+                                               None, None, 0, out.ctypes.data, 3)
+    assert not L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 1, None, None, 0, None, None,
+                                               None, None, 0, out.ctypes.data, 3)
+    assert not L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 0, None, None, 1,
+                                               c.ctypes.data, c.ctypes.data, None, c.ctypes.data, 0,
+                                               out.ctypes.data, 3)
+    assert not L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 1, c.ctypes.data,
+                                               c.ctypes.data, 0, None, None, None, None, 0, out.ctypes.data, 2)
+    assert not L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 1, c.ctypes.data,
+                                               c.ctypes.data, 0, None, None, None, None, 2, out.ctypes.data, 3)
+    assert "finite ground" in vie.last_error()
+    assert not L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 1, c.ctypes.data,
+                                               c.ctypes.data, 0, None, None, None, None, 0, None, 3)
+    assert not L.vie_hip_time_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 1, c.ctypes.data,
+                                                    c.ctypes.data, 0, None, None, None, None, 0, 0,
+                                                    ctypes.byref(ms))
+    # nothing to assemble: success without a device
+    assert L.vie_hip_volume_conductor_rows(1.0, 0, None, None, 1, c.ctypes.data, c.ctypes.data, 0, None, None,
+                                           None, None, 0, None, 0)
+    assert L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 0, None, None, 0, None, None, None,
+                                           None, 0, None, 0)
