@@ -575,6 +575,71 @@ def bench_rows(args):
     return 0
 
 
+def bench_columns(args):
+    """Volume rows of the conductor columns (wire -> volume) of the sweep's
+    coupled system: the n^3 cube (edge 0.5 m) and the 21-segment dipole 0.1 m
+    from one face at 250 MHz, one column per unknown with NEC's straight-wire
+    sparsity (segments j-1, j, j+1), NEC's near field at the 6^3 points of
+    every cell, free space and over a PEC plane 0.05 m below the cube.
+    Kernel time with the inputs resident (HIP events), plus the whole call
+    with the columns copied to the host.  CPU baseline: the C restatement
+    oracle/wire_volume.c with OpenMP over the (column, cell) tasks, as the
+    reference runs them, on a bounded sample of cells."""
+    from nec2c_amd import vie, workloads as W
+    n = args.columns_size
+    pitch = 0.5 / n
+    f_mhz = 250.0
+    wlam, k0 = 299.8 / f_mhz, W.k0_of(f_mhz * 1e6)
+    rng = np.random.default_rng(5)
+    src = np.zeros((21, 63), dtype=np.complex128)
+    for j in range(21):
+        for s in (j - 1, j, j + 1):
+            if 0 <= s < 21:
+                src[j, 3 * s:3 * s + 3] = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    res = {}
+    for ground, tag in ((0, "free"), (1, "pec")):
+        lift = 0.05 if ground else 0.0
+        centers = W.cube_cells(n, pitch) + np.array([0.0, 0.0, lift])
+        sizes = np.full_like(centers, pitch)
+        chi = np.full(len(centers), 3.0 - 0.1j)
+        seg = W.nec_straight_wire([-0.1, 0.25, 0.01 + lift], [-0.1, 0.25, 0.49 + lift], 21, 0.001, wlam)
+        pairs = 21 * len(centers)
+        ms = vie.time_conductor_volume_columns(k0, wlam, centers, sizes, chi, seg, src, ground=ground,
+                                               repeats=args.steps)
+        t = time.perf_counter()
+        vie.conductor_volume_columns(k0, wlam, centers, sizes, chi, seg, src, ground=ground)
+        t_call = time.perf_counter() - t
+        res[tag] = {"kernel_ms": ms, "pairs": pairs, "pairs_per_s": pairs / (ms * 1e-3), "call_ms": t_call * 1e3}
+        log(f"columns {tag}: {pairs} (column, cell) pairs, kernels {ms:.2f} ms ({pairs / (ms * 1e-3):.3e} pairs/s), "
+            f"call with {pairs * 48 / 1e6:.0f} MB to the host {t_call * 1e3:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, len(os.sched_getaffinity(0)))
+        centers = W.cube_cells(n, pitch)
+        sample = centers[:: max(1, len(centers) // args.columns_cpu_cells)][: args.columns_cpu_cells]
+        from oracle import wire_volume as V
+        dip = V.Conductors(*W.nec_straight_wire([-0.1, 0.25, 0.01], [-0.1, 0.25, 0.49], 21, 0.001, wlam))
+        t = time.perf_counter()
+        V.conductor_volume(k0, wlam, sample, np.full_like(sample, pitch), 3.0 - 0.1j, dip, src, threads=cores)
+        t_cpu = time.perf_counter() - t
+        rate = 21 * len(sample) / t_cpu
+        cpu = {"value": rate, "unit": "pairs/s", "cores": cores, "kind": "port",
+               "sample": f"21 columns x {len(sample)} cells (free space) through oracle/wire_volume.c "
+                         f"(OpenMP, {cores} threads) in {t_cpu:.2f} s"}
+        log(f"columns CPU port: {rate:.3e} pairs/s on {cores} threads")
+    free = res["free"]
+    print(json.dumps({
+        "metric": "conductor-to-volume column pairs/s (NEC near field averaged over 6^3 points per column-cell pair)",
+        "value": free["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": free["kernel_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 cube (edge 0.5 m), 21-segment dipole 0.1 m from a face, 250 MHz; "
+                               "the D columns of the coupled system", "cells": n ** 3, "columns": 21},
+        "columns": res, "cpu_baseline": cpu, "roofline": None,
+    }))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -590,7 +655,7 @@ def main():
                     help="mixed: FP32 cycle stops at |g| <= max(target, this * beta); 0 = library default")
     ap.add_argument("--recycle", type=int, default=0,
                     help="mixed: recycled-subspace size of the GMRES cycles (0 = library default, -1 = off)")
-    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield", "rows"),
+    ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield", "rows", "columns"),
                     default="sweep",
                     help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
@@ -600,6 +665,8 @@ def main():
     ap.add_argument("--farfield-size", type=int, default=64, help="farfield: voxels per cube edge")
     ap.add_argument("--rows-size", type=int, default=64, help="rows: voxels per cube edge")
     ap.add_argument("--rows-cpu-cells", type=int, default=16384, help="rows: cells in the CPU-baseline sample")
+    ap.add_argument("--columns-size", type=int, default=64, help="columns: voxels per cube edge")
+    ap.add_argument("--columns-cpu-cells", type=int, default=8192, help="columns: cells in the CPU-baseline sample")
     ap.add_argument("--arnoldi-column", type=int, default=0,
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
@@ -631,6 +698,8 @@ def main():
         return bench_farfield(args)
     if args.workload == "rows":
         return bench_rows(args)
+    if args.workload == "columns":
+        return bench_columns(args)
 
     n = args.size
     pitch = 0.5 / n

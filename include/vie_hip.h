@@ -155,6 +155,40 @@ int vie_hip_volume_conductor_rows(double k0, size_t cells, const double *centers
                                   const double *wire_points, const double *wire_tangents, int patches,
                                   const double *patch_points, const double *patch_t1, const double *patch_t2,
                                   const double *patch_scale, int ground, VIE_CDOUBLE *rows, size_t ld);
+/* Conductor-to-volume coupling on the device: the volume rows of every
+   conductor column, coupling[column * 3 cells + 3 cell + q] =
+   -i omega eps0 chi[cell] <E_q>, <E> NEC's near field of the column's frozen
+   basis coefficients averaged over the cell's 6^3 Gauss points (the task
+   loop dw_wire_to_volume_range / dw_average_wire_field,
+   dielectric_wire.c:316-420, over nefld, fields.c:1123-1370; the columns of
+   dw_assemble_wire_to_volume, :422-508).  omega = k0 c0.
+   NEC's classic engine: segment data in wavelengths as NEC holds it (x, y, z,
+   si, bi, cab, sab, salp [segments]); patches: patch_points[3 patches],
+   patch_area[patches] (pbi), patch_t1/patch_t2[3 patches], in wavelengths;
+   wlam NEC's wavelength in m (cells are in m: centers, sizes[3 cells]; chi[cells]
+   the contrast eps_r - 1 of each cell).  sources[column * (3 segments + 3
+   patches)]: (a, b, c) per segment, then the patch current (3 per patch), as
+   dw_prepare_conductor_sources freezes them.  rkh: NEC's lumped-element range
+   (wavelengths).  ground 0: free space; 1: PEC plane (gnd.iperf == 1); a
+   finite ground is refused.  1 on success; 0 with vie_hip_last_error(). */
+int vie_hip_conductor_volume_columns(double k0, double wlam, size_t cells, const double *centers,
+                                     const double *sizes, const VIE_CDOUBLE *chi, int segments, const double *x,
+                                     const double *y, const double *z, const double *si, const double *bi,
+                                     const double *cab, const double *sab, const double *salp, int patches,
+                                     const double *patch_points, const double *patch_area, const double *patch_t1,
+                                     const double *patch_t2, size_t columns, const VIE_CDOUBLE *sources,
+                                     double rkh, int ground, VIE_CDOUBLE *coupling);
+/* The same columns on device buffers only, `repeats` times between HIP
+   events after one untimed pass; *ms receives the mean time per pass (new;
+   for measurement). */
+int vie_hip_time_conductor_volume_columns(double k0, double wlam, size_t cells, const double *centers,
+                                          const double *sizes, const VIE_CDOUBLE *chi, int segments,
+                                          const double *x, const double *y, const double *z, const double *si,
+                                          const double *bi, const double *cab, const double *sab,
+                                          const double *salp, int patches, const double *patch_points,
+                                          const double *patch_area, const double *patch_t1,
+                                          const double *patch_t2, size_t columns, const VIE_CDOUBLE *sources,
+                                          double rkh, int ground, int repeats, float *ms);
 /* The same rows computed on device buffers only, `repeats` times between HIP
    events on the library's stream after one untimed pass; *ms receives the mean
    kernel time per pass (new; for measurement). */

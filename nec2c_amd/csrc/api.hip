@@ -2324,6 +2324,254 @@ extern "C" int vie_hip_time_volume_conductor_rows(double k0, size_t cells, const
   return rows_run(a, nullptr, 0, repeats, ms);
 }
 
+// ------------------------------------------------------------------ conductor-to-volume coupling
+// The volume rows of the conductor columns (dw_assemble_wire_to_volume,
+// dielectric_wire.c:316-508): NEC's near field of each column's frozen basis
+// coefficients averaged over every cell.
+namespace {
+struct ColumnsWork {
+  std::mutex mu;
+  DevBuf geom, cells, chi, active, jbase, term_seg, coef, skip, F, out;
+  hipStream_t stream = nullptr;
+};
+ColumnsWork& columns_work() {
+  static ColumnsWork* w = new ColumnsWork();  // lives for the process (device buffers)
+  return *w;
+}
+struct ColumnsArgs {
+  double k0, wlam;
+  size_t cells;
+  const double *centers, *sizes;
+  const double _Complex* chi;
+  int n;
+  const double *x, *y, *z, *si, *bi, *cab, *sab, *salp;
+  int m;
+  const double *pp, *pbi, *t1, *t2;
+  size_t columns;
+  const double _Complex* sources;
+  double rkh;
+  int ground;
+};
+constexpr size_t kColumnsFieldBudget = (size_t)1 << 30;  // bytes of averaged fields per cell chunk
+}  // namespace
+
+static int columns_run(const ColumnsArgs& a, double _Complex* coupling, int repeats, float* ms) {
+  g_last_error.clear();
+  const bool any = a.cells && a.columns && (a.n > 0 || a.m > 0);
+  if (a.n < 0 || a.m < 0 || !std::isfinite(a.k0) || !(a.k0 > 0) || !std::isfinite(a.wlam) || !(a.wlam > 0) ||
+      std::isnan(a.rkh) || a.ground < 0 || a.ground > 2 || (a.cells && (!a.centers || !a.sizes || !a.chi)) ||
+      (a.n && (!a.x || !a.y || !a.z || !a.si || !a.bi || !a.cab || !a.sab || !a.salp)) ||
+      (a.m && (!a.pp || !a.pbi || !a.t1 || !a.t2)) || (a.columns && (a.n || a.m) && !a.sources) ||
+      (any && repeats <= 0 && !coupling)) {
+    set_error("conductor-to-volume coupling: invalid argument");
+    return 0;
+  }
+  if (a.ground == 2) {
+    set_error("conductor-to-volume coupling: a finite ground (Sommerfeld) is not supported on the device");
+    return 0;
+  }
+  // finite geometry and positive segment lengths: NEC's Romberg integration
+  // advances by a fraction of the segment length, which must be finite
+  {
+    auto finite = [](const double* v, size_t count) {
+      for (size_t i = 0; i < count; i++)
+        if (!std::isfinite(v[i])) return false;
+      return true;
+    };
+    bool ok = finite(a.centers, 3 * a.cells) && finite(a.sizes, 3 * a.cells);
+    const double* seg[8] = {a.x, a.y, a.z, a.si, a.bi, a.cab, a.sab, a.salp};
+    for (int k = 0; k < 8 && a.n; k++) ok = ok && finite(seg[k], (size_t)a.n);
+    for (int i = 0; i < a.n && ok; i++) ok = a.si[i] > 0 && a.bi[i] >= 0;
+    if (a.m) ok = ok && finite(a.pp, 3 * (size_t)a.m) && finite(a.pbi, (size_t)a.m) &&
+                  finite(a.t1, 3 * (size_t)a.m) && finite(a.t2, 3 * (size_t)a.m);
+    if (!ok) {
+      set_error("conductor-to-volume coupling: non-finite geometry or a segment length that is not positive");
+      return 0;
+    }
+  }
+  const size_t n = (size_t)a.n, m = (size_t)a.m, stride = 3 * n + 3 * m;
+  size_t bcells, bout;
+  if (!mul_ok(a.cells, 3 * sizeof(double), &bcells) || !mul_ok(a.columns, 3 * a.cells, &bout) ||
+      !mul_ok(bout, sizeof(c64), &bout) || n + m > (size_t)INT_MAX / 4 || a.columns > (size_t)INT_MAX) {
+    set_error("conductor-to-volume coupling: size overflow");
+    return 0;
+  }
+  if (!any) {
+    if (coupling && a.columns && a.cells) memset(coupling, 0, bout);
+    if (ms) *ms = 0.0f;
+    return 1;
+  }
+  // radiating sources: segments with a nonzero coefficient, patches with a
+  // nonzero current, in some column; their basis terms and coefficients
+  const c64* src = (const c64*)a.sources;
+  const size_t cols = a.columns;
+  std::vector<int> active, jbase, term_seg;
+  std::vector<unsigned char> skip(cols * (n ? n : 1), 0);
+  for (size_t s = 0; s < n; s++) {
+    bool used = false;
+    for (size_t c = 0; c < cols; c++) {
+      const c64* v = src + c * stride + 3 * s;
+      const bool zero = v[0].re == 0 && v[0].im == 0 && v[1].re == 0 && v[1].im == 0 && v[2].re == 0 && v[2].im == 0;
+      skip[c * n + s] = zero;
+      used = used || !zero;
+    }
+    if (used) {
+      active.push_back((int)s);
+      jbase.push_back((int)term_seg.size());
+      for (int k = 0; k < 3; k++) term_seg.push_back((int)s);
+    }
+  }
+  for (size_t p = 0; p < m; p++) {
+    bool used = false;
+    for (size_t c = 0; c < cols && !used; c++) {
+      const c64* v = src + c * stride + 3 * n + 3 * p;
+      for (int k = 0; k < 3; k++) used = used || v[k].re != 0 || v[k].im != 0;
+    }
+    if (used) {
+      active.push_back((int)(n + p));
+      jbase.push_back((int)term_seg.size());
+      term_seg.push_back(-1);
+      term_seg.push_back(-1);
+    }
+  }
+  const size_t terms = term_seg.size();
+  if (terms == 0) {
+    if (coupling) memset(coupling, 0, bout);
+    if (ms) *ms = 0.0f;
+    return 1;
+  }
+  // coefficient of each term per column: (a, b, c) of a segment; t1.J and t2.J of a patch
+  std::vector<c64> coef(cols * terms);
+  for (size_t c = 0; c < cols; c++)
+    for (size_t q = 0; q < active.size(); q++) {
+      const size_t j = (size_t)jbase[q];
+      if ((size_t)active[q] < n) {
+        const c64* v = src + c * stride + 3 * (size_t)active[q];
+        for (int k = 0; k < 3; k++) coef[c * terms + j + k] = v[k];
+      } else {
+        const size_t p = (size_t)active[q] - n;
+        const c64* v = src + c * stride + 3 * n + 3 * p;
+        const double* u1 = a.t1 + 3 * p;
+        const double* u2 = a.t2 + 3 * p;
+        c64 s1 = {0, 0}, s2 = {0, 0};
+        for (int k = 0; k < 3; k++) {
+          s1 = s1 + u1[k] * v[k];
+          s2 = s2 + u2[k] * v[k];
+        }
+        coef[c * terms + j] = s1;
+        coef[c * terms + j + 1] = s2;
+      }
+    }
+  size_t chunk = kColumnsFieldBudget / (terms * 3 * sizeof(c64));
+  if (chunk < 1) chunk = 1;
+  if (chunk > a.cells) chunk = a.cells;
+  if (!ensure_device()) return 0;
+  ColumnsWork& w = columns_work();
+  std::lock_guard<std::mutex> lock(w.mu);
+  if (!w.stream) HIP_OK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+  const size_t ngeom = 8 * n + 10 * m;
+  if (!(w.geom.alloc((ngeom ? ngeom : 1) * sizeof(double)) && w.cells.alloc(2 * bcells) &&
+        w.chi.alloc(a.cells * sizeof(c64)) && w.active.alloc(active.size() * sizeof(int)) &&
+        w.jbase.alloc(jbase.size() * sizeof(int)) && w.term_seg.alloc(terms * sizeof(int)) &&
+        w.coef.alloc(coef.size() * sizeof(c64)) && w.skip.alloc(skip.size()) &&
+        w.F.alloc(terms * 3 * chunk * sizeof(c64)) && w.out.alloc(bout)))
+    return 0;
+  hipStream_t st = w.stream;
+  double* g = w.geom.as<double>();
+  const double* seg_src[8] = {a.x, a.y, a.z, a.si, a.bi, a.cab, a.sab, a.salp};
+  for (int k = 0; k < 8 && n; k++)
+    HIP_OK(hipMemcpyAsync(g + k * n, seg_src[k], n * sizeof(double), hipMemcpyHostToDevice, st));
+  double* pg = g + 8 * n;  // pp [3m], pbi [m], t1 [3m], t2 [3m]
+  if (m) {
+    HIP_OK(hipMemcpyAsync(pg, a.pp, 3 * m * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(pg + 3 * m, a.pbi, m * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(pg + 4 * m, a.t1, 3 * m * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(pg + 7 * m, a.t2, 3 * m * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  double* dcent = w.cells.as<double>();
+  double* dsize = dcent + 3 * a.cells;
+  HIP_OK(hipMemcpyAsync(dcent, a.centers, bcells, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(dsize, a.sizes, bcells, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.chi.p, a.chi, a.cells * sizeof(c64), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.active.p, active.data(), active.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.jbase.p, jbase.data(), jbase.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.term_seg.p, term_seg.data(), terms * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.coef.p, coef.data(), coef.size() * sizeof(c64), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.skip.p, skip.data(), skip.size(), hipMemcpyHostToDevice, st));
+  vie::ConductorSet cs{a.n, a.m, g, g + n, g + 2 * n, g + 3 * n, g + 4 * n, g + 5 * n, g + 6 * n, g + 7 * n,
+                       pg, pg + 3 * m, pg + 4 * m, pg + 7 * m, a.rkh};
+  auto pass = [&]() {
+    for (size_t c0 = 0; c0 < a.cells; c0 += chunk) {
+      const size_t nc = a.cells - c0 < chunk ? a.cells - c0 : chunk;
+      if (!vie::conductor_volume_fields(cs, w.active.as<int>(), w.jbase.as<int>(), (int)active.size(), dcent, dsize,
+                                        c0, nc, a.wlam, a.ground, w.F.as<c64>(), 3 * nc, st) ||
+          !vie::conductor_volume_columns(w.F.as<c64>(), 3 * nc, (int)terms, w.term_seg.as<int>(), w.coef.as<c64>(),
+                                         w.skip.as<unsigned char>(), a.n > 0 ? a.n : 1, (int)cols, w.chi.as<c64>(),
+                                         c0, nc, a.k0, w.out.as<c64>(), 3 * a.cells, st))
+        return false;
+    }
+    return true;
+  };
+  if (!pass()) {
+    set_error("conductor-to-volume kernels failed to launch");
+    return 0;
+  }
+  if (repeats > 0) {
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    bool ok = hipEventRecord(e0, st) == hipSuccess;
+    for (int r = 0; r < repeats && ok; r++) ok = pass();
+    ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float t = 0;
+    if (ok) ok = hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) {
+      set_error("conductor-to-volume timing failed");
+      return 0;
+    }
+    if (ms) *ms = t / (float)repeats;
+  }
+  if (coupling) HIP_OK(hipMemcpyAsync(coupling, w.out.p, bout, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  return 1;
+}
+
+extern "C" int vie_hip_conductor_volume_columns(double k0, double wlam, size_t cells, const double* centers,
+                                                const double* sizes, const double _Complex* chi, int segments,
+                                                const double* x, const double* y, const double* z, const double* si,
+                                                const double* bi, const double* cab, const double* sab,
+                                                const double* salp, int patches, const double* patch_points,
+                                                const double* patch_area, const double* patch_t1,
+                                                const double* patch_t2, size_t columns,
+                                                const double _Complex* sources, double rkh, int ground,
+                                                double _Complex* coupling) {
+  const ColumnsArgs a{k0,  wlam, cells, centers, sizes,    chi,          segments,   x,        y,
+                      z,   si,   bi,    cab,     sab,      salp,         patches,    patch_points, patch_area,
+                      patch_t1, patch_t2, columns, sources, rkh, ground};
+  return columns_run(a, coupling, 0, nullptr);
+}
+
+extern "C" int vie_hip_time_conductor_volume_columns(double k0, double wlam, size_t cells, const double* centers,
+                                                     const double* sizes, const double _Complex* chi, int segments,
+                                                     const double* x, const double* y, const double* z,
+                                                     const double* si, const double* bi, const double* cab,
+                                                     const double* sab, const double* salp, int patches,
+                                                     const double* patch_points, const double* patch_area,
+                                                     const double* patch_t1, const double* patch_t2, size_t columns,
+                                                     const double _Complex* sources, double rkh, int ground,
+                                                     int repeats, float* ms) {
+  const ColumnsArgs a{k0,  wlam, cells, centers, sizes,    chi,          segments,   x,        y,
+                      z,   si,   bi,    cab,     sab,      salp,         patches,    patch_points, patch_area,
+                      patch_t1, patch_t2, columns, sources, rkh, ground};
+  if (repeats < 1 || !ms) {
+    set_error("conductor-to-volume timing: invalid argument");
+    return 0;
+  }
+  return columns_run(a, nullptr, repeats, ms);
+}
+
 extern "C" int dielectric_fft_last_solve_recycled(const dielectric_fft_system* s) {
   return s && s->sw.rec_ready ? s->sw.rk : 0;
 }

@@ -70,6 +70,31 @@ int conductor_rows(double k0, const double* centers, const double* sizes, size_t
                    const double* wp, const double* wt, int np, const double* pp, const double* t1, const double* t2,
                    const double* ps, int ground, c64* rows, size_t ld, unsigned long long* failed, hipStream_t s);
 
+// ---- conductor-to-volume coupling (wire_volume.hip) ----
+// NEC's segment and patch arrays on the device, in wavelengths: segments
+// x y z si bi cab sab salp [n]; patches pp [3 m] (centres), pbi [m] (areas),
+// t1 t2 [3 m]; rkh the lumped-element range.
+struct ConductorSet {
+  int n, m;
+  const double *x, *y, *z, *si, *bi, *cab, *sab, *salp;
+  const double *pp, *pbi, *t1, *t2;
+  double rkh;
+};
+// F[(jbase[a] + k) * ldf + 3 (cell - cell0) + q]: the cell-averaged field of
+// basis term k of source active[a] (a segment index, or n + patch index:
+// segments have 3 terms, constant / sine / cosine current, patches 2, t1 /
+// t2 current) over cells [cell0, cell0 + cells); ground 0 free, 1 PEC.
+int conductor_volume_fields(const ConductorSet& cs, const int* active, const int* jbase, int nactive,
+                            const double* centers, const double* sizes, size_t cell0, size_t cells, double wlam,
+                            int ground, c64* F, size_t ldf, hipStream_t s);
+// coupling[col * ldc + 3 cell + q] = -i omega eps0 chi[cell] sum_j
+// coef[col * terms + j] F[j][3 (cell - cell0) + q] for the chunk's cells; a
+// term of segment term_segment[j] >= 0 is skipped where skip[col * nseg +
+// segment] is set.
+int conductor_volume_columns(const c64* F, size_t ldf, int terms, const int* term_segment, const c64* coef,
+                             const unsigned char* skip, int nseg, int columns, const c64* chi, size_t cell0,
+                             size_t cells, double k0, c64* coupling, size_t ldc, hipStream_t s);
+
 // ---- vector kernels (krylov.hip) ----
 template <typename T> void vec_zero(cx<T>* x, size_t n, hipStream_t s);
 template <typename To, typename From> void vec_cast(cx<To>* y, const cx<From>* x, size_t n, hipStream_t s);

@@ -130,7 +130,8 @@ EXPORTS = (
     "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
     "dielectric_fft_last_solve_escalated", "dielectric_fft_last_solve_recycled", "vie_hip_harmonic_ritz",
     "vie_hip_radiation_amplitudes", "vie_hip_scattered_power", "vie_hip_volume_conductor_rows",
-    "vie_hip_time_volume_conductor_rows",
+    "vie_hip_time_volume_conductor_rows", "vie_hip_conductor_volume_columns",
+    "vie_hip_time_conductor_volume_columns",
 )
 
 _LIB = None
@@ -220,6 +221,11 @@ def lib() -> ctypes.CDLL:
         "vie_hip_scattered_power": (i, [sz, vp, vp, vp, d, i, ctypes.POINTER(d)]),
         "vie_hip_volume_conductor_rows": (i, [d, sz, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i, vp, sz]),
         "vie_hip_time_volume_conductor_rows": (i, [d, sz, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i, i,
+                                                   ctypes.POINTER(ctypes.c_float)]),
+        "vie_hip_conductor_volume_columns": (i, [d, d, sz, vp, vp, vp, i] + [vp] * 8 + [i, vp, vp, vp, vp, sz, vp, d,
+                                                                                     i, vp]),
+        "vie_hip_time_conductor_volume_columns": (i, [d, d, sz, vp, vp, vp, i] + [vp] * 8 +
+                                                  [i, vp, vp, vp, vp, sz, vp, d, i, i,
                                                    ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
@@ -323,6 +329,55 @@ def time_volume_conductor_rows(k0, centers, sizes, wire_points=None, wire_tangen
                                                   int(repeats), ctypes.byref(ms))
     if not ok:
         raise RuntimeError(f"volume-to-conductor row timing failed: {last_error()}")
+    return ms.value
+
+
+def _columns_arrays(centers, sizes, chi, segments, patches, sources):
+    c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+    s = np.ascontiguousarray(sizes, dtype=np.float64).reshape(-1, 3)
+    ch = np.ascontiguousarray(np.broadcast_to(np.asarray(chi, dtype=np.complex128), (len(c),)))
+    seg = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in segments]
+    n = len(seg[0]) if seg else 0
+    if len(seg) != 8 or any(len(a) != n for a in seg) or len(s) != len(c):
+        raise ValueError("segments: x, y, z, si, bi, cab, sab, salp of equal length; one size per cell")
+    if patches is None:
+        pat = [np.zeros((0, 3)), np.zeros(0), np.zeros((0, 3)), np.zeros((0, 3))]
+    else:
+        pat = [np.ascontiguousarray(a, dtype=np.float64) for a in patches]
+    m = len(pat[1].reshape(-1))
+    stride = 3 * n + 3 * m
+    src = np.ascontiguousarray(sources, dtype=np.complex128).reshape(-1, stride) if stride else \
+        np.zeros((0, 0), dtype=np.complex128)
+    return c, s, ch, seg, n, pat, m, src
+
+
+def conductor_volume_columns(k0, wlam, centers, sizes, chi, segments, sources, patches=None, rkh=1.0, ground=0):
+    """vie_hip_conductor_volume_columns: [columns, 3 cells] volume rows of the
+    conductor columns (dielectric_wire.c:316-508).  segments = (x, y, z, si,
+    bi, cab, sab, salp) in wavelengths; patches = (points [m, 3], areas [m],
+    t1 [m, 3], t2 [m, 3]) in wavelengths or None; sources [columns, 3 n + 3 m]."""
+    c, s, ch, seg, n, pat, m, src = _columns_arrays(centers, sizes, chi, segments, patches, sources)
+    out = np.zeros((len(src), 3 * len(c)), dtype=np.complex128)
+    ok = lib().vie_hip_conductor_volume_columns(float(k0), float(wlam), len(c), c.ctypes.data, s.ctypes.data,
+                                                ch.ctypes.data, n, *[a.ctypes.data for a in seg], m,
+                                                *[a.ctypes.data for a in pat], len(src), src.ctypes.data,
+                                                float(rkh), int(ground), out.ctypes.data)
+    if not ok:
+        raise RuntimeError(f"conductor-to-volume coupling failed: {last_error()}")
+    return out
+
+
+def time_conductor_volume_columns(k0, wlam, centers, sizes, chi, segments, sources, patches=None, rkh=1.0, ground=0,
+                                  repeats=3):
+    """Mean device time (ms) of one pass of the coupling kernels over resident inputs."""
+    c, s, ch, seg, n, pat, m, src = _columns_arrays(centers, sizes, chi, segments, patches, sources)
+    ms = ctypes.c_float()
+    ok = lib().vie_hip_time_conductor_volume_columns(float(k0), float(wlam), len(c), c.ctypes.data, s.ctypes.data,
+                                                     ch.ctypes.data, n, *[a.ctypes.data for a in seg], m,
+                                                     *[a.ctypes.data for a in pat], len(src), src.ctypes.data,
+                                                     float(rkh), int(ground), int(repeats), ctypes.byref(ms))
+    if not ok:
+        raise RuntimeError(f"conductor-to-volume timing failed: {last_error()}")
     return ms.value
 
 

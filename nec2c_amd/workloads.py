@@ -292,3 +292,16 @@ def batch_cases(count, seed, lo=8, hi=48, max_ports=8, seg_lo=11, seg_hi=31, f_l
         ports = tuple(int(v) for v in rng.choice(segs, size=min(p, segs), replace=False))
         out.append(BatchCase(shape, er, f, h, dip, ports))
     return out
+
+
+def nec_straight_wire(start, end, segments, radius, wlam):
+    """NEC's arrays for a straight wire cut into equal segments (a GW card):
+    (x, y, z, si, bi, cab, sab, salp), lengths in wavelengths of wlam metres."""
+    start, end = np.asarray(start, float), np.asarray(end, float)
+    d = (end - start) / segments
+    centers = start + (np.arange(segments)[:, None] + 0.5) * d
+    length = float(np.linalg.norm(d))
+    t = d / length
+    full = lambda v: np.full(segments, v)
+    return (centers[:, 0] / wlam, centers[:, 1] / wlam, centers[:, 2] / wlam, full(length / wlam),
+            full(radius / wlam), full(t[0]), full(t[1]), full(t[2]))

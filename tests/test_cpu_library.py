@@ -334,3 +334,24 @@ def test_conductor_row_rejections_before_any_device_use():
                                            None, None, 0, None, 0)
     assert L.vie_hip_volume_conductor_rows(1.0, 1, c.ctypes.data, c.ctypes.data, 0, None, None, 0, None, None, None,
                                            None, 0, None, 0)
+
+
+def test_conductor_volume_rejections_before_any_device_use():
+    from nec2c_amd import vie as Vh
+    c = np.zeros(3)
+    sz = np.full(3, 0.01)
+    chi = np.ones(1, complex)
+    seg = [np.array([v]) for v in (0.0, 0.0, 0.0, 0.05, 0.001, 0.0, 0.0, 1.0)]
+    src = np.zeros((1, 3), complex)
+    src[0, 0] = 1.0
+    call = lambda **kw: Vh.conductor_volume_columns(kw.get("k0", 1.0), kw.get("wlam", 1.0), c, kw.get("sizes", sz),
+                                                    chi, kw.get("seg", seg), src, rkh=kw.get("rkh", 1.0),
+                                                    ground=kw.get("ground", 0))
+    for kw, msg in ((dict(k0=float("nan")), "invalid"), (dict(wlam=0.0), "invalid"), (dict(rkh=float("nan")), "invalid"),
+                    (dict(ground=3), "invalid"), (dict(ground=2), "finite ground"),
+                    (dict(sizes=np.array([0.01, np.inf, 0.01])), "non-finite"),
+                    (dict(seg=seg[:3] + [np.array([-0.05])] + seg[4:]), "segment length")):
+        with pytest.raises(RuntimeError, match=msg):
+            call(**kw)
+    # no columns: nothing to do, no device needed
+    assert Vh.conductor_volume_columns(1.0, 1.0, c, sz, chi, seg, np.zeros((0, 3), complex)).shape == (0, 3)
