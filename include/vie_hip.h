@@ -189,6 +189,33 @@ int vie_hip_time_conductor_volume_columns(double k0, double wlam, size_t cells, 
                                           const double *patch_area, const double *patch_t1,
                                           const double *patch_t2, size_t columns, const VIE_CDOUBLE *sources,
                                           double rkh, int ground, int repeats, float *ms);
+/* The conductor rows (B block) computed straight into a prepared system's
+   device storage (new): the volume-to-conductor rows of
+   vie_hip_volume_conductor_rows for the system's cells (centers, sizes: 3
+   values for each of the system's cells, in the mesh order given to
+   dielectric_fft_prepare; wires + 2 patches must equal the system's conductor
+   count), placed in the system's device layout without a
+   host round trip.  Replaces the whole B block; the host copy behind
+   dielectric_fft_entry is refreshed from the device at the next
+   dielectric_fft_entry call (write C through dielectric_fft_entry before
+   this call, not after, to avoid that download).  1 on success. */
+int dielectric_fft_assemble_conductor_rows(dielectric_fft_system *system, double k0, const double *centers,
+                                           const double *sizes, int wires, const double *wire_points,
+                                           const double *wire_tangents, int patches, const double *patch_points,
+                                           const double *patch_t1, const double *patch_t2,
+                                           const double *patch_scale, int ground);
+/* The conductor columns (D block) computed straight into a prepared system's
+   device storage (new), as vie_hip_conductor_volume_columns with one column
+   per conductor unknown (sources[c * (3 segments + 3 patches)]); same
+   storage rules as dielectric_fft_assemble_conductor_rows. */
+int dielectric_fft_assemble_conductor_columns(dielectric_fft_system *system, double k0, double wlam,
+                                              const double *centers, const double *sizes, const VIE_CDOUBLE *chi,
+                                              int segments, const double *x, const double *y, const double *z,
+                                              const double *si, const double *bi, const double *cab,
+                                              const double *sab, const double *salp, int patches,
+                                              const double *patch_points, const double *patch_area,
+                                              const double *patch_t1, const double *patch_t2,
+                                              const VIE_CDOUBLE *sources, double rkh, int ground);
 /* The same rows computed on device buffers only, `repeats` times between HIP
    events on the library's stream after one untimed pass; *ms receives the mean
    kernel time per pass (new; for measurement). */

@@ -12,6 +12,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <functional>
 #include <array>
 #include <chrono>
 #include <map>
@@ -606,6 +607,10 @@ struct dielectric_fft_system {
   std::vector<c64> blocks;  // reference layout (dielectric_fft.c:34-40)
   vie_fft_plan* plan = nullptr;
   bool blocks_dirty = true;
+  // B (rows) / D (columns) assembled on the device: the device copy is the
+  // newer one and the host block is stale until a dielectric_fft_entry call
+  // brings it back
+  bool rows_on_device = false, cols_on_device = false;
   hipStream_t stream = nullptr;  // the plan's stream, or an own one without cells
   bool own_stream = false;
   DevBuf C64, B64, D64, C32, B32, D32, bz, cdiag;
@@ -802,8 +807,33 @@ extern "C" void dielectric_fft_blocks_changed(dielectric_fft_system* s) {
   if (s) s->blocks_dirty = true;
 }
 
+// Brings device-assembled B / D blocks back into the host storage (reference
+// layout) and makes the host copy the one that uploads again.
+static int sync_cross_to_host(dielectric_fft_system* s) {
+  if (!s->rows_on_device && !s->cols_on_device) return 1;
+  const size_t c = s->c, v = s->v, nbox = s->nbox(), nv = 3 * nbox;
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (int which = 0; which < 2; which++) {
+    if (which == 0 ? !s->rows_on_device : !s->cols_on_device) continue;
+    std::vector<c64> dev(c * nv);
+    HIP_OK(hipMemcpy(dev.data(), which == 0 ? s->B64.p : s->D64.p, c * nv * 16, hipMemcpyDeviceToHost));
+    for (size_t cell = 0; cell < s->cells; cell++)
+      for (int comp = 0; comp < 3; comp++) {
+        const size_t bi = comp * nbox + s->plan->map[cell], ui = 3 * cell + comp;
+        for (size_t r = 0; r < c; r++) {
+          if (which == 0) s->blocks[r * (c + v) + c + ui] = dev[r * nv + bi];
+          else s->blocks[c * (c + v) + ui * c + r] = dev[r * nv + bi];
+        }
+      }
+  }
+  s->rows_on_device = s->cols_on_device = false;
+  return 1;
+}
+
 extern "C" double _Complex* dielectric_fft_entry(dielectric_fft_system* s, size_t row, size_t col) {
   size_t c = s->c, v = s->v;
+  // any host access may write through the returned pointer into B or D
+  if (!sync_cross_to_host(s)) return nullptr;
   s->blocks_dirty = true;
   if (row < c) return (double _Complex*)&s->blocks[row * (c + v) + col];
   if (col < c) return (double _Complex*)&s->blocks[c * (c + v) + (row - c) * c + col];
@@ -817,15 +847,17 @@ static int upload_blocks(dielectric_fft_system* s) {
   const size_t c = s->c, v = s->v, nbox = s->nbox(), nv = 3 * nbox;
   hipStream_t st = s->stream;
   if (c) {
-    std::vector<c64> C(c * c), B(c * nv, c64{0, 0}), D(c * nv, c64{0, 0});
+    // a block assembled on the device stays as it is there
+    const bool up_b = !s->rows_on_device, up_d = !s->cols_on_device;
+    std::vector<c64> C(c * c), B(up_b ? c * nv : 0, c64{0, 0}), D(up_d ? c * nv : 0, c64{0, 0});
     for (size_t r = 0; r < c; r++)
       for (size_t q = 0; q < c; q++) C[r * c + q] = s->blocks[r * (c + v) + q];
-    for (size_t cell = 0; cell < s->cells; cell++)
+    for (size_t cell = 0; cell < s->cells && (up_b || up_d); cell++)
       for (int comp = 0; comp < 3; comp++) {
         size_t bi = comp * nbox + s->plan->map[cell], ui = 3 * cell + comp;
         for (size_t r = 0; r < c; r++) {
-          B[r * nv + bi] = s->blocks[r * (c + v) + c + ui];
-          D[r * nv + bi] = s->blocks[c * (c + v) + ui * c + r];
+          if (up_b) B[r * nv + bi] = s->blocks[r * (c + v) + c + ui];
+          if (up_d) D[r * nv + bi] = s->blocks[c * (c + v) + ui * c + r];
         }
       }
     if (!(s->C64.alloc(c * c * 16) && s->C32.alloc(c * c * 8) &&
@@ -846,10 +878,14 @@ static int upload_blocks(dielectric_fft_system* s) {
       if (!(s->B64.alloc(c * nv * 16) && s->D64.alloc(c * nv * 16) && s->B32.alloc(c * nv * 8) &&
             s->D32.alloc(c * nv * 8)))
         return 0;
-      HIP_OK(hipMemcpyAsync(s->B64.p, B.data(), c * nv * 16, hipMemcpyHostToDevice, st));
-      HIP_OK(hipMemcpyAsync(s->D64.p, D.data(), c * nv * 16, hipMemcpyHostToDevice, st));
-      vie::vec_cast<float, double>(s->B32.as<c32>(), s->B64.as<c64>(), c * nv, st);
-      vie::vec_cast<float, double>(s->D32.as<c32>(), s->D64.as<c64>(), c * nv, st);
+      if (up_b) {
+        HIP_OK(hipMemcpyAsync(s->B64.p, B.data(), c * nv * 16, hipMemcpyHostToDevice, st));
+        vie::vec_cast<float, double>(s->B32.as<c32>(), s->B64.as<c64>(), c * nv, st);
+      }
+      if (up_d) {
+        HIP_OK(hipMemcpyAsync(s->D64.p, D.data(), c * nv * 16, hipMemcpyHostToDevice, st));
+        vie::vec_cast<float, double>(s->D32.as<c32>(), s->D64.as<c64>(), c * nv, st);
+      }
     }
     HIP_OK(hipStreamSynchronize(st));
   }
@@ -2197,13 +2233,18 @@ struct RowsArgs {
 // Uploads the geometry and fills the device rows; with repeats > 0 the kernel
 // runs that many more times between HIP events and *ms gets the mean.  With
 // host_rows it downloads the rows into host_rows (leading dimension ld).
-static int rows_run(const RowsArgs& a, double _Complex* host_rows, size_t ld, int repeats, float* ms) {
+// A consumer of the device result (rows [nrows][3 cells], or columns
+// [columns][3 cells]) on the computing stream, before the call returns.
+using DeviceSink = std::function<bool(const c64*, hipStream_t)>;
+
+static int rows_run(const RowsArgs& a, double _Complex* host_rows, size_t ld, int repeats, float* ms,
+                    const DeviceSink& sink = nullptr) {
   g_last_error.clear();
   const size_t nrows = (size_t)(a.wires > 0 ? a.wires : 0) + 2 * (size_t)(a.patches > 0 ? a.patches : 0);
   if (a.wires < 0 || a.patches < 0 || a.ground < 0 || a.ground > 2 || !std::isfinite(a.k0) || !(a.k0 > 0) ||
       (a.cells && (!a.centers || !a.sizes)) || (a.wires && (!a.wire_points || !a.wire_tangents)) ||
       (a.patches && (!a.patch_points || !a.patch_t1 || !a.patch_t2 || !a.patch_scale)) ||
-      (nrows && a.cells && repeats <= 0 && (!host_rows || ld < 3 * a.cells))) {
+      (nrows && a.cells && repeats <= 0 && !sink && (!host_rows || ld < 3 * a.cells))) {
     set_error("volume-to-conductor rows: invalid argument");
     return 0;
   }
@@ -2283,6 +2324,10 @@ static int rows_run(const RowsArgs& a, double _Complex* host_rows, size_t ld, in
   }
   unsigned long long failed = none;
   HIP_OK(hipMemcpyAsync(&failed, w.failed.p, sizeof(failed), hipMemcpyDeviceToHost, st));
+  if (sink && !sink(w.rows.as<c64>(), st)) {
+    set_error("volume-to-conductor rows: placing the rows failed");
+    return 0;
+  }
   if (host_rows)
     HIP_OK(hipMemcpy2DAsync(host_rows, pitch, w.rows.p, row_bytes, row_bytes, nrows,
                             hipMemcpyDeviceToHost, st));
@@ -2355,14 +2400,15 @@ struct ColumnsArgs {
 constexpr size_t kColumnsFieldBudget = (size_t)1 << 30;  // bytes of averaged fields per cell chunk
 }  // namespace
 
-static int columns_run(const ColumnsArgs& a, double _Complex* coupling, int repeats, float* ms) {
+static int columns_run(const ColumnsArgs& a, double _Complex* coupling, int repeats, float* ms,
+                       const DeviceSink& sink = nullptr) {
   g_last_error.clear();
   const bool any = a.cells && a.columns && (a.n > 0 || a.m > 0);
   if (a.n < 0 || a.m < 0 || !std::isfinite(a.k0) || !(a.k0 > 0) || !std::isfinite(a.wlam) || !(a.wlam > 0) ||
       std::isnan(a.rkh) || a.ground < 0 || a.ground > 2 || (a.cells && (!a.centers || !a.sizes || !a.chi)) ||
       (a.n && (!a.x || !a.y || !a.z || !a.si || !a.bi || !a.cab || !a.sab || !a.salp)) ||
       (a.m && (!a.pp || !a.pbi || !a.t1 || !a.t2)) || (a.columns && (a.n || a.m) && !a.sources) ||
-      (any && repeats <= 0 && !coupling)) {
+      (any && repeats <= 0 && !coupling && !sink)) {
     set_error("conductor-to-volume coupling: invalid argument");
     return 0;
   }
@@ -2533,6 +2579,10 @@ static int columns_run(const ColumnsArgs& a, double _Complex* coupling, int repe
     }
     if (ms) *ms = t / (float)repeats;
   }
+  if (sink && !sink(w.out.as<c64>(), st)) {
+    set_error("conductor-to-volume coupling: placing the columns failed");
+    return 0;
+  }
   if (coupling) HIP_OK(hipMemcpyAsync(coupling, w.out.p, bout, hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
   return 1;
@@ -2570,6 +2620,86 @@ extern "C" int vie_hip_time_conductor_volume_columns(double k0, double wlam, siz
     return 0;
   }
   return columns_run(a, nullptr, repeats, ms);
+}
+
+// ------------------------------------------------------------------ cross blocks straight into a system
+// Places [c][3 cells] results (reference cell order) into the system's device
+// block (box order) and its FP32 copy, the block zeroed first.
+static int system_block_target(dielectric_fft_system* s, size_t cells, const char* what, DevBuf*& b64,
+                               DevBuf*& b32, bool rows) {
+  if (!s || !s->plan || s->c == 0 || cells != s->cells) {
+    set_error("%s: the system has no conductors or volume cells, or the cell count differs", what);
+    return 0;
+  }
+  const size_t nv = 3 * s->nbox(), bytes = s->c * nv;
+  b64 = rows ? &s->B64 : &s->D64;
+  b32 = rows ? &s->B32 : &s->D32;
+  if (!(b64->alloc(bytes * 16) && b32->alloc(bytes * 8))) return 0;
+  HIP_OK(hipStreamSynchronize(s->stream));  // nothing of the system's still reads the block
+  HIP_OK(hipMemset(b64->p, 0, bytes * 16));
+  HIP_OK(hipMemset(b32->p, 0, bytes * 8));
+  return 1;
+}
+
+static DeviceSink system_block_sink(dielectric_fft_system* s, DevBuf* b64, DevBuf* b32) {
+  return [s, b64, b32](const c64* res, hipStream_t st) {
+    const size_t c = s->c, nbox = s->nbox(), nv = 3 * nbox, cells = s->cells;
+    for (size_t r = 0; r < c; r++)
+      vie::to_box<double>(res + r * 3 * cells, 0, cells, s->plan->map_dev.as<size_t>(), nbox,
+                          b64->as<c64>() + r * nv, st);
+    vie::vec_cast<float, double>(b32->as<c32>(), b64->as<c64>(), c * nv, st);
+    return hipGetLastError() == hipSuccess;
+  };
+}
+
+extern "C" int dielectric_fft_assemble_conductor_rows(dielectric_fft_system* s, double k0, const double* centers,
+                                                      const double* sizes, int wires, const double* wire_points,
+                                                      const double* wire_tangents, int patches,
+                                                      const double* patch_points, const double* patch_t1,
+                                                      const double* patch_t2, const double* patch_scale,
+                                                      int ground) {
+  g_last_error.clear();
+  DevBuf *b64 = nullptr, *b32 = nullptr;
+  if (!s || wires < 0 || patches < 0 || (size_t)wires + 2 * (size_t)patches != (s ? s->c : 0)) {
+    set_error("conductor rows into a system: wires + 2 patches must equal the system's conductor count");
+    return 0;
+  }
+  if (!system_block_target(s, s->cells, "conductor rows into a system", b64, b32, true)) return 0;
+  const RowsArgs a{k0, s->cells, centers, sizes, wires, wire_points, wire_tangents,
+                   patches, patch_points, patch_t1, patch_t2, patch_scale, ground};
+  // until the rows are in place the block is neither the host's nor the device's
+  s->rows_on_device = true;
+  if (!rows_run(a, nullptr, 0, 0, nullptr, system_block_sink(s, b64, b32))) {
+    s->rows_on_device = false;
+    s->blocks_dirty = true;  // the next upload restores the host block
+    return 0;
+  }
+  return 1;
+}
+
+extern "C" int dielectric_fft_assemble_conductor_columns(
+    dielectric_fft_system* s, double k0, double wlam, const double* centers, const double* sizes,
+    const double _Complex* chi, int segments, const double* x, const double* y, const double* z, const double* si,
+    const double* bi, const double* cab, const double* sab, const double* salp, int patches,
+    const double* patch_points, const double* patch_area, const double* patch_t1, const double* patch_t2,
+    const double _Complex* sources, double rkh, int ground) {
+  g_last_error.clear();
+  DevBuf *b64 = nullptr, *b32 = nullptr;
+  if (!s) {
+    set_error("conductor columns into a system: no system");
+    return 0;
+  }
+  if (!system_block_target(s, s->cells, "conductor columns into a system", b64, b32, false)) return 0;
+  const ColumnsArgs a{k0,  wlam, s->cells, centers, sizes,    chi,          segments,   x,        y,
+                      z,   si,   bi,       cab,     sab,      salp,         patches,    patch_points, patch_area,
+                      patch_t1, patch_t2, s->c, sources, rkh, ground};
+  s->cols_on_device = true;
+  if (!columns_run(a, nullptr, 0, nullptr, system_block_sink(s, b64, b32))) {
+    s->cols_on_device = false;
+    s->blocks_dirty = true;
+    return 0;
+  }
+  return 1;
 }
 
 extern "C" int dielectric_fft_last_solve_recycled(const dielectric_fft_system* s) {

@@ -131,7 +131,8 @@ EXPORTS = (
     "dielectric_fft_last_solve_escalated", "dielectric_fft_last_solve_recycled", "vie_hip_harmonic_ritz",
     "vie_hip_radiation_amplitudes", "vie_hip_scattered_power", "vie_hip_volume_conductor_rows",
     "vie_hip_time_volume_conductor_rows", "vie_hip_conductor_volume_columns",
-    "vie_hip_time_conductor_volume_columns",
+    "vie_hip_time_conductor_volume_columns", "dielectric_fft_assemble_conductor_rows",
+    "dielectric_fft_assemble_conductor_columns",
 )
 
 _LIB = None
@@ -224,6 +225,9 @@ def lib() -> ctypes.CDLL:
                                                    ctypes.POINTER(ctypes.c_float)]),
         "vie_hip_conductor_volume_columns": (i, [d, d, sz, vp, vp, vp, i] + [vp] * 8 + [i, vp, vp, vp, vp, sz, vp, d,
                                                                                      i, vp]),
+        "dielectric_fft_assemble_conductor_rows": (i, [vp, d, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i]),
+        "dielectric_fft_assemble_conductor_columns": (i, [vp, d, d, vp, vp, vp, i] + [vp] * 8 +
+                                                      [i, vp, vp, vp, vp, vp, d, i]),
         "vie_hip_time_conductor_volume_columns": (i, [d, d, sz, vp, vp, vp, i] + [vp] * 8 +
                                                   [i, vp, vp, vp, vp, sz, vp, d, i, i,
                                                    ctypes.POINTER(ctypes.c_float)]),
@@ -615,6 +619,31 @@ class System:
             dd = np.ctypeslib.as_array((ctypes.c_double * (2 * v * c)).from_address(base))
             dd.view(np.complex128).reshape(v, c)[:] = D
         lib().dielectric_fft_blocks_changed(self.s)
+
+    def assemble_rows(self, k0, centers, sizes, wire_points=None, wire_tangents=None, patch_points=None,
+                      patch_t1=None, patch_t2=None, patch_scale=None, ground=0):
+        """dielectric_fft_assemble_conductor_rows: the B block on the device."""
+        c, s, wp, wt, pp, t1, t2, ps = _conductor_arrays(centers, sizes, wire_points, wire_tangents, patch_points,
+                                                         patch_t1, patch_t2, patch_scale)
+        if len(c) != self.v // 3:
+            raise ValueError("the system's cell count differs from the cells given")
+        if not lib().dielectric_fft_assemble_conductor_rows(self.s, float(k0), c.ctypes.data, s.ctypes.data, len(wp),
+                                                            wp.ctypes.data, wt.ctypes.data, len(pp), pp.ctypes.data,
+                                                            t1.ctypes.data, t2.ctypes.data, ps.ctypes.data,
+                                                            int(ground)):
+            raise RuntimeError(f"conductor rows failed: {last_error()}")
+
+    def assemble_columns(self, k0, wlam, centers, sizes, chi, segments, sources, patches=None, rkh=1.0, ground=0):
+        """dielectric_fft_assemble_conductor_columns: the D block on the device."""
+        c, s, ch, seg, n, pat, m, src = _columns_arrays(centers, sizes, chi, segments, patches, sources)
+        if len(c) != self.v // 3 or len(src) != self.c:
+            raise ValueError("the system's cell count differs from the cells given, or not one column per conductor")
+        if not lib().dielectric_fft_assemble_conductor_columns(self.s, float(k0), float(wlam), c.ctypes.data,
+                                                               s.ctypes.data, ch.ctypes.data, n,
+                                                               *[a.ctypes.data for a in seg], m,
+                                                               *[a.ctypes.data for a in pat], src.ctypes.data,
+                                                               float(rkh), int(ground)):
+            raise RuntimeError(f"conductor columns failed: {last_error()}")
 
     def set_permittivities(self, eps):
         """Per-cell permittivity in the caller's cell order (vie_fft_set_permittivities)."""
