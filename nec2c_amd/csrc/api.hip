@@ -604,7 +604,9 @@ struct dielectric_fft_system {
   size_t c = 0, v = 0, cells = 0, bytes = 0;
   bool pec = false;
   bool owns_plan = true;
-  std::vector<c64> blocks;  // reference layout (dielectric_fft.c:34-40)
+  // reference layout (dielectric_fft.c:34-40), zero-initialised by calloc so
+  // that the pages of a block nobody writes from the host are never touched
+  c64* blocks = nullptr;
   vie_fft_plan* plan = nullptr;
   bool blocks_dirty = true;
   // B (rows) / D (columns) assembled on the device: the device copy is the
@@ -613,10 +615,11 @@ struct dielectric_fft_system {
   bool rows_on_device = false, cols_on_device = false;
   hipStream_t stream = nullptr;  // the plan's stream, or an own one without cells
   bool own_stream = false;
-  DevBuf C64, B64, D64, C32, B32, D32, bz, cdiag;
+  DevBuf C64, B64, D64, C32, B32, D32, bz, cdiag, stage;
   SolverWork sw;
   bool rhs_loaded = false;
   ~dielectric_fft_system() {
+    free(blocks);
     if (owns_plan) delete plan;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
   }
@@ -750,9 +753,8 @@ static int prepare(dielectric_fft_system** out, const dielectric_model* model, c
   s->v = v;
   s->cells = cells;
   s->pec = pec != 0;
-  try {
-    s->blocks.assign(blocks ? blocks : 1, c64{0, 0});
-  } catch (...) {
+  s->blocks = (c64*)calloc(blocks ? blocks : 1, sizeof(c64));
+  if (!s->blocks) {
     delete s;
     return diag_error(diag, "conductor block allocation failed");
   }
@@ -847,19 +849,9 @@ static int upload_blocks(dielectric_fft_system* s) {
   const size_t c = s->c, v = s->v, nbox = s->nbox(), nv = 3 * nbox;
   hipStream_t st = s->stream;
   if (c) {
-    // a block assembled on the device stays as it is there
-    const bool up_b = !s->rows_on_device, up_d = !s->cols_on_device;
-    std::vector<c64> C(c * c), B(up_b ? c * nv : 0, c64{0, 0}), D(up_d ? c * nv : 0, c64{0, 0});
+    std::vector<c64> C(c * c);
     for (size_t r = 0; r < c; r++)
       for (size_t q = 0; q < c; q++) C[r * c + q] = s->blocks[r * (c + v) + q];
-    for (size_t cell = 0; cell < s->cells && (up_b || up_d); cell++)
-      for (int comp = 0; comp < 3; comp++) {
-        size_t bi = comp * nbox + s->plan->map[cell], ui = 3 * cell + comp;
-        for (size_t r = 0; r < c; r++) {
-          if (up_b) B[r * nv + bi] = s->blocks[r * (c + v) + c + ui];
-          if (up_d) D[r * nv + bi] = s->blocks[c * (c + v) + ui * c + r];
-        }
-      }
     if (!(s->C64.alloc(c * c * 16) && s->C32.alloc(c * c * 8) &&
           s->bz.alloc(c * 16) && s->cdiag.alloc(c * 16)))
       return 0;
@@ -878,12 +870,25 @@ static int upload_blocks(dielectric_fft_system* s) {
       if (!(s->B64.alloc(c * nv * 16) && s->D64.alloc(c * nv * 16) && s->B32.alloc(c * nv * 8) &&
             s->D32.alloc(c * nv * 8)))
         return 0;
+      // a block assembled on the device stays as it is there; a host block
+      // goes up in the reference layout and is placed into the box order on
+      // the device
+      const bool up_b = !s->rows_on_device, up_d = !s->cols_on_device;
+      if ((up_b || up_d) && !s->stage.alloc(c * v * 16)) return 0;
+      const size_t* map = s->plan->map_dev.as<size_t>();
       if (up_b) {
-        HIP_OK(hipMemcpyAsync(s->B64.p, B.data(), c * nv * 16, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpy2DAsync(s->stage.p, v * 16, s->blocks + c, (c + v) * 16, v * 16, c, hipMemcpyHostToDevice,
+                                st));
+        HIP_OK(hipMemsetAsync(s->B64.p, 0, c * nv * 16, st));
+        for (size_t r = 0; r < c; r++)
+          vie::to_box<double>(s->stage.as<c64>() + r * v, 0, s->cells, map, nbox, s->B64.as<c64>() + r * nv, st);
         vie::vec_cast<float, double>(s->B32.as<c32>(), s->B64.as<c64>(), c * nv, st);
+        HIP_OK(hipStreamSynchronize(st));  // the staging buffer is read before it is refilled
       }
       if (up_d) {
-        HIP_OK(hipMemcpyAsync(s->D64.p, D.data(), c * nv * 16, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s->stage.p, s->blocks + c * (c + v), c * v * 16, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemsetAsync(s->D64.p, 0, c * nv * 16, st));
+        vie::columns_to_box(s->stage.as<c64>(), c, s->cells, map, nbox, s->D64.as<c64>(), st);
         vie::vec_cast<float, double>(s->D32.as<c32>(), s->D64.as<c64>(), c * nv, st);
       }
     }
@@ -1626,7 +1631,11 @@ static dielectric_fft_system* plan_system(vie_fft_plan* p) {
     s->plan = p;
     s->owns_plan = false;
     s->stream = p->stream;
-    s->blocks.assign(1, c64{0, 0});
+    s->blocks = (c64*)calloc(1, sizeof(c64));
+    if (!s->blocks) {
+      delete s;
+      return nullptr;
+    }
     s->blocks_dirty = false;
     p->wrap = s;
   }

@@ -153,6 +153,10 @@ template <typename T>
 void to_box(const c64* user, size_t c, size_t cells, const size_t* map, size_t nbox, cx<T>* sys, hipStream_t s);
 template <typename T>
 void from_box(const cx<T>* sys, size_t c, size_t cells, const size_t* map, size_t nbox, c64* user, hipStream_t s);
+// D[r][comp * nbox + map[cell]] = host_order[(3 cell + comp) * c + r] (the
+// reference's v x c conductor-column block into the device layout).
+void columns_to_box(const c64* host_order, size_t c, size_t cells, const size_t* map, size_t nbox, c64* D,
+                    hipStream_t s);
 void precondition_apply(const c64* x, c64* y, size_t n, const JacobiView& J, hipStream_t s);
 // di offsets vc, vv, yc, yv on the device and skips the whole product.
 template <typename T>

@@ -982,6 +982,16 @@ __global__ void k_from_box(const cx<T>* sys, size_t c, size_t cells, const size_
   }
 }
 
+// Conductor columns from the reference's row-major v x c block (row 3 cell +
+// comp, column r) into the device layout [r][comp * nbox + map[cell]].
+__global__ void k_columns_to_box(const c64* __restrict__ host_order, size_t c, size_t cells,
+                                 const size_t* __restrict__ map, size_t nbox, c64* __restrict__ D) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * cells * c) return;
+  const size_t ui = i / c, r = i - ui * c, cell = ui / 3, comp = ui - 3 * cell;
+  D[r * 3 * nbox + comp * nbox + map[cell]] = host_order[i];
+}
+
 // Jacobi preconditioner application on a system vector (compatibility path).
 __global__ void k_precondition(const c64* x, c64* y, size_t n, JacobiView J) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1189,6 +1199,11 @@ template <typename T>
 void from_box(const cx<T>* sys, size_t c, size_t cells, const size_t* map, size_t nbox, c64* user, hipStream_t s) {
   size_t n = c + 3 * cells;
   if (n) hipLaunchKernelGGL((k_from_box<T>), dim3(gblocks(n)), dim3(256), 0, s, sys, c, cells, map, nbox, user);
+}
+void columns_to_box(const c64* host_order, size_t c, size_t cells, const size_t* map, size_t nbox, c64* D,
+                    hipStream_t s) {
+  const size_t n = 3 * cells * c;
+  if (n) hipLaunchKernelGGL(k_columns_to_box, dim3(gblocks(n)), dim3(256), 0, s, host_order, c, cells, map, nbox, D);
 }
 void precondition_apply(const c64* x, c64* y, size_t n, const JacobiView& J, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_precondition, dim3(gblocks(n)), dim3(256), 0, s, x, y, n, J);
