@@ -224,6 +224,37 @@ int vie_hip_time_volume_conductor_rows(double k0, size_t cells, const double *ce
                                        int patches, const double *patch_points, const double *patch_t1,
                                        const double *patch_t2, const double *patch_scale, int ground, int repeats,
                                        float *ms);
+/* Far field of the coupled solve on the device (dw_far_field,
+   dielectric_wire.c:768-791, for many directions at once): the solved
+   wires' and patches' field by NEC's ffld / fflds (radiation.c:36-373,
+   classic engine) times wlam, plus the volume current's amplitude
+   (vie_hip_radiation_amplitudes) projected on theta and phi.  Segments as
+   NEC holds them (x, y, z, si in wavelengths; cab, sab, salp), their solved
+   current coefficients segment_current[3 segments] (constant, sine, cosine:
+   crnt.air + i crnt.aii, bir/bii, cir/cii, in NEC's amperes per wavelength);
+   patches patch_points[3 patches], patch_area[patches] (wavelengths) with
+   patch_current[3 patches] (crnt.cur[n + 3 j ..]); the volume cells and
+   current as vie_hip_radiation_amplitudes takes them; ground 0 free space,
+   1 PEC plane.  Writes e_theta[count], e_phi[count] (r E in V) for the
+   directions (theta[d], phi[d]).  1 on success. */
+int vie_hip_total_far_field(double k0, double wlam, int segments, const double *x, const double *y, const double *z,
+                            const double *si, const double *cab, const double *sab, const double *salp,
+                            const VIE_CDOUBLE *segment_current, int patches, const double *patch_points,
+                            const double *patch_area, const VIE_CDOUBLE *patch_current, size_t cells,
+                            const double *centers, const double *sizes, const VIE_CDOUBLE *volume_current,
+                            int ground, size_t count, const double *theta, const double *phi,
+                            VIE_CDOUBLE *e_theta, VIE_CDOUBLE *e_phi);
+/* Radiated power in W of the coupled solve by dw_radiated_power's golden
+   spiral of `count` directions (dielectric_wire.c:793-840; the reference
+   uses 3896 and 974): the sphere in free space, the upper hemisphere over a
+   PEC plane, NEC's eta = 376.73 in the denominator as there. */
+int vie_hip_total_radiated_power(double k0, double wlam, int segments, const double *x, const double *y,
+                                 const double *z, const double *si, const double *cab, const double *sab,
+                                 const double *salp, const VIE_CDOUBLE *segment_current, int patches,
+                                 const double *patch_points, const double *patch_area,
+                                 const VIE_CDOUBLE *patch_current, size_t cells, const double *centers,
+                                 const double *sizes, const VIE_CDOUBLE *volume_current, int ground, size_t count,
+                                 double *power);
 /* Host-side step of the recycled subspace, exported for tests: the k harmonic
    Ritz vectors of smallest modulus (Morgan 2002) of the (m+1) x m Arnoldi
    Hessenberg hbar (column-major), as the m x k column-major p with unit

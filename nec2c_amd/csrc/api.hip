@@ -2711,6 +2711,146 @@ extern "C" int dielectric_fft_assemble_conductor_columns(
   return 1;
 }
 
+// ------------------------------------------------------------------ far field of the coupled solve
+// dw_far_field (dielectric_wire.c:768-791): the wires' and patches' field
+// (ffld) times NEC's wavelength plus the volume amplitude projected on theta
+// and phi; dw_radiated_power (:793-840) over the golden spiral.
+namespace {
+struct TotalWork {
+  std::mutex mu;
+  DevBuf geom, cur, pcur, ang, e;
+  hipStream_t stream = nullptr;
+};
+TotalWork& total_work() {
+  static TotalWork* w = new TotalWork();  // lives for the process (device buffers)
+  return *w;
+}
+}  // namespace
+
+static int total_run(double k0, double wlam, int n, const double* x, const double* y, const double* z,
+                     const double* si, const double* cab, const double* sab, const double* salp,
+                     const double _Complex* seg_cur, int m, const double* pp, const double* pbi,
+                     const double _Complex* pat_cur, size_t cells, const double* centers, const double* sizes,
+                     const double _Complex* vol_cur, int ground, size_t count, const double* theta,
+                     const double* phi, double _Complex* et, double _Complex* ep) {
+  g_last_error.clear();
+  if (n < 0 || m < 0 || !std::isfinite(wlam) || !(wlam > 0) || (ground != 0 && ground != 1) ||
+      (n && (!x || !y || !z || !si || !cab || !sab || !salp || !seg_cur)) || (m && (!pp || !pbi || !pat_cur)) ||
+      (count && (!theta || !phi || !et || !ep))) {
+    set_error("total far field: invalid argument");
+    return 0;
+  }
+  if (count == 0) return 1;
+  const size_t N = (size_t)n, M = (size_t)m;
+  std::vector<c64> cet(count, c64{0, 0}), cep(count, c64{0, 0});
+  if (n || m) {
+    if (!ensure_device()) return 0;
+    TotalWork& w = total_work();
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (!w.stream) HIP_OK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+    if (!(w.geom.alloc((7 * N + 4 * M + 1) * sizeof(double)) && w.cur.alloc((3 * N + 1) * sizeof(c64)) &&
+          w.pcur.alloc((3 * M + 1) * sizeof(c64)) && w.ang.alloc(2 * count * sizeof(double)) &&
+          w.e.alloc(2 * count * sizeof(c64))))
+      return 0;
+    hipStream_t st = w.stream;
+    double* g = w.geom.as<double>();
+    const double* segs[7] = {x, y, z, si, cab, sab, salp};
+    for (int k = 0; k < 7 && n; k++) HIP_OK(hipMemcpyAsync(g + k * N, segs[k], N * 8, hipMemcpyHostToDevice, st));
+    if (m) {
+      HIP_OK(hipMemcpyAsync(g + 7 * N, pp, 3 * M * 8, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(g + 7 * N + 3 * M, pbi, M * 8, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(w.pcur.p, pat_cur, 3 * M * sizeof(c64), hipMemcpyHostToDevice, st));
+    }
+    if (n) HIP_OK(hipMemcpyAsync(w.cur.p, seg_cur, 3 * N * sizeof(c64), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(w.ang.p, theta, count * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(w.ang.as<double>() + count, phi, count * 8, hipMemcpyHostToDevice, st));
+    vie::ConductorFar cf{n,           m,          g,          g + N,   g + 2 * N,          g + 3 * N, g + 4 * N,
+                         g + 5 * N,   g + 6 * N,  w.cur.as<c64>(), g + 7 * N, g + 7 * N + 3 * M, w.pcur.as<c64>()};
+    if (!vie::conductor_far_field(cf, ground, count, w.ang.as<double>(), w.ang.as<double>() + count, w.e.as<c64>(),
+                                  w.e.as<c64>() + count, st)) {
+      set_error("conductor far-field kernel failed to launch");
+      return 0;
+    }
+    HIP_OK(hipMemcpyAsync(cet.data(), w.e.p, count * sizeof(c64), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(cep.data(), w.e.as<c64>() + count, count * sizeof(c64), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+  }
+  std::vector<c64> amp;
+  if (cells) {
+    std::vector<double> dirs(3 * count);
+    for (size_t d = 0; d < count; d++) {
+      dirs[3 * d] = sin(theta[d]) * cos(phi[d]);
+      dirs[3 * d + 1] = sin(theta[d]) * sin(phi[d]);
+      dirs[3 * d + 2] = cos(theta[d]);
+    }
+    amp.resize(3 * count);
+    if (!farfield_run(cells, centers, sizes, vol_cur, k0, ground, nullptr, nullptr, count, dirs.data(),
+                      (double _Complex*)amp.data()))
+      return 0;
+  }
+  c64* ot = (c64*)et;
+  c64* op = (c64*)ep;
+  for (size_t d = 0; d < count; d++) {
+    c64 t_ = wlam * cet[d], p_ = wlam * cep[d];
+    if (cells) {
+      const double th = theta[d], ph = phi[d];
+      const double tv[3] = {cos(th) * cos(ph), cos(th) * sin(ph), -sin(th)}, pv[3] = {-sin(ph), cos(ph), 0.0};
+      for (int a = 0; a < 3; a++) {
+        t_ = t_ + tv[a] * amp[3 * d + a];
+        p_ = p_ + pv[a] * amp[3 * d + a];
+      }
+    }
+    ot[d] = t_;
+    op[d] = p_;
+  }
+  return 1;
+}
+
+extern "C" int vie_hip_total_far_field(double k0, double wlam, int segments, const double* x, const double* y,
+                                       const double* z, const double* si, const double* cab, const double* sab,
+                                       const double* salp, const double _Complex* segment_current, int patches,
+                                       const double* patch_points, const double* patch_area,
+                                       const double _Complex* patch_current, size_t cells, const double* centers,
+                                       const double* sizes, const double _Complex* volume_current, int ground,
+                                       size_t count, const double* theta, const double* phi,
+                                       double _Complex* e_theta, double _Complex* e_phi) {
+  return total_run(k0, wlam, segments, x, y, z, si, cab, sab, salp, segment_current, patches, patch_points,
+                   patch_area, patch_current, cells, centers, sizes, volume_current, ground, count, theta, phi,
+                   e_theta, e_phi);
+}
+
+extern "C" int vie_hip_total_radiated_power(double k0, double wlam, int segments, const double* x, const double* y,
+                                            const double* z, const double* si, const double* cab,
+                                            const double* sab, const double* salp,
+                                            const double _Complex* segment_current, int patches,
+                                            const double* patch_points, const double* patch_area,
+                                            const double _Complex* patch_current, size_t cells,
+                                            const double* centers, const double* sizes,
+                                            const double _Complex* volume_current, int ground, size_t count,
+                                            double* power) {
+  if (!power || count == 0 || count > ((size_t)1 << 28)) {
+    set_error("total radiated power: invalid argument");
+    return 0;
+  }
+  // the golden-spiral rule of dw_radiated_power (dielectric_wire.c:793-840)
+  std::vector<double> th(count), ph(count);
+  for (size_t i = 0; i < count; i++) {
+    const double zc = 1.0 - (ground ? 1.0 : 2.0) * ((double)i + 0.5) / (double)count;
+    th[i] = acos(zc);
+    ph[i] = kPi * (3.0 - sqrt(5.0)) * (double)i;
+  }
+  std::vector<c64> et(count), ep(count);
+  if (!total_run(k0, wlam, segments, x, y, z, si, cab, sab, salp, segment_current, patches, patch_points,
+                 patch_area, patch_current, cells, centers, sizes, volume_current, ground, count, th.data(),
+                 ph.data(), (double _Complex*)et.data(), (double _Complex*)ep.data()))
+    return 0;
+  double sum = 0;
+  for (size_t i = 0; i < count; i++)
+    sum += et[i].re * et[i].re + et[i].im * et[i].im + ep[i].re * ep[i].re + ep[i].im * ep[i].im;
+  *power = sum * ((ground ? 2.0 : 4.0) * kPi / (double)count) / (2.0 * 376.73);  // NEC's eta (nec2c.h)
+  return 1;
+}
+
 extern "C" int dielectric_fft_last_solve_recycled(const dielectric_fft_system* s) {
   return s && s->sw.rec_ready ? s->sw.rk : 0;
 }

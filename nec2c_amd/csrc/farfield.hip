@@ -160,6 +160,99 @@ void launch_sum(const double* centers, const double* sizes, const c64* current, 
 
 }  // namespace
 
+// ---- far field of the wires and patches (ffld, radiation.c:36-338; fflds
+// :340-373), classic engine, free space or the PEC image (rrv = rrh = -1) ----
+namespace {
+constexpr double kNecPi = 3.141592654;  // NEC's rounded constants (nec2c.h)
+constexpr double kNecTp = 6.283185308;
+constexpr double kNecConst3 = -29.97922085;  // CONST3 = i (-29.979...)
+constexpr double kNecConst4 = 188.365;       // CONST4 = i 188.365
+
+// sum over segments of exp(i 2 pi r.d) (rr + i ri) t, the segment's current
+// a + b sin + c cos integrated in closed form (radiation.c:102-140)
+__device__ inline void segment_sum(const ConductorFar& cf, double rox, double roy, double roz, c64 ci[3]) {
+  ci[0] = ci[1] = ci[2] = c64{0, 0};
+  for (int i = 0; i < cf.n; i++) {
+    const double omega = -(rox * cf.cab[i] + roy * cf.sab[i] + roz * cf.salp[i]);
+    const double el = kNecPi * cf.si[i], sill = omega * el, top = el + sill, bot = el - sill;
+    const double a = fabs(omega) >= 1.0e-7 ? 2.0 * sin(sill) / omega : (2.0 - omega * omega * el * el / 3.0) * el;
+    const double too = fabs(top) >= 1.0e-7 ? sin(top) / top : 1.0 - top * top / 6.0;
+    const double boo = fabs(bot) >= 1.0e-7 ? sin(bot) / bot : 1.0 - bot * bot / 6.0;
+    const double b = el * (boo - too), c = el * (boo + too);
+    const c64 A = cf.cur[3 * i], B = cf.cur[3 * i + 1], C = cf.cur[3 * i + 2];
+    const double rr = a * A.re + b * B.im + c * C.re, ri = a * A.im - b * B.re + c * C.im;
+    double sn, cs;
+    sincos(kNecTp * (cf.x[i] * rox + cf.y[i] * roy + cf.z[i] * roz), &sn, &cs);
+    const c64 exa = c64{cs, sn} * c64{rr, ri};
+    ci[0] = ci[0] + cf.cab[i] * exa;
+    ci[1] = ci[1] + cf.sab[i] * exa;
+    ci[2] = ci[2] + cf.salp[i] * exa;
+  }
+}
+
+// fflds: CONST4 ((e.r) r - e), e = sum J area exp(i 2 pi r.p)
+__device__ inline void patch_sum(const ConductorFar& cf, double rox, double roy, double roz, c64 e[3]) {
+  e[0] = e[1] = e[2] = c64{0, 0};
+  for (int j = 0; j < cf.m; j++) {
+    double sn, cs;
+    sincos(kNecTp * (rox * cf.pp[3 * j] + roy * cf.pp[3 * j + 1] + roz * cf.pp[3 * j + 2]), &sn, &cs);
+    const c64 ct = {cs * cf.pbi[j], sn * cf.pbi[j]};
+    for (int q = 0; q < 3; q++) e[q] = e[q] + cf.pcur[3 * j + q] * ct;
+  }
+  const c64 dot = rox * e[0] + roy * e[1] + roz * e[2];
+  const double r[3] = {rox, roy, roz};
+  for (int q = 0; q < 3; q++) {
+    const c64 v = r[q] * dot - e[q];
+    e[q] = c64{-kNecConst4 * v.im, kNecConst4 * v.re};
+  }
+}
+
+__global__ void k_conductor_far(ConductorFar cf, int pec, size_t count, const double* __restrict__ theta,
+                                const double* __restrict__ phi, c64* __restrict__ et, c64* __restrict__ ep) {
+  const size_t d = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= count) return;
+  double sth, cth, sph, cph;
+  sincos(theta[d], &sth, &cth);
+  sincos(phi[d], &sph, &cph);
+  const double phx = -sph, phy = cph, roz = cth, thx = roz * phy, thy = -roz * phx, thz = -sth;
+  const double rox = -thz * phy, roy = thz * phx;
+  c64 ci[3] = {{0, 0}, {0, 0}, {0, 0}};
+  if (cf.n) {
+    segment_sum(cf, rox, roy, roz, ci);
+    if (pec) {
+      c64 im[3];
+      segment_sum(cf, rox, roy, -roz, im);
+      ci[0] = ci[0] - im[0];
+      ci[1] = ci[1] - im[1];
+      ci[2] = ci[2] + im[2];
+    }
+  }
+  c64 e[3] = {{0, 0}, {0, 0}, {0, 0}};
+  if (cf.m) {
+    patch_sum(cf, rox, roy, roz, e);
+    if (pec) {
+      c64 g[3];
+      patch_sum(cf, rox, roy, -roz, g);  // the image patch, negated over PEC
+      e[0] = e[0] - g[0];
+      e[1] = e[1] - g[1];
+      e[2] = e[2] + g[2];
+    }
+  }
+  // + CONST3 ci
+  for (int q = 0; q < 3; q++) e[q] = e[q] + c64{-kNecConst3 * ci[q].im, kNecConst3 * ci[q].re};
+  et[d] = thx * e[0] + thy * e[1] + thz * e[2];
+  ep[d] = phx * e[0] + phy * e[1];
+}
+}  // namespace
+
+int conductor_far_field(const ConductorFar& cf, int pec, size_t count, const double* theta, const double* phi,
+                        c64* et, c64* ep, hipStream_t s) {
+  if (count == 0) return 1;
+  hipLaunchKernelGGL(k_conductor_far, dim3((unsigned)((count + 127) / 128)), dim3(128), 0, s, cf, pec, count, theta,
+                     phi, et, ep);
+  return hipGetLastError() == hipSuccess;
+}
+
 unsigned farfield_slices(size_t n, size_t ndir, int ground) {
   const size_t dt = ground ? 2 : 4;  // directions per workgroup (launch_sum below)
   const size_t tiles = (ndir + dt - 1) / dt;

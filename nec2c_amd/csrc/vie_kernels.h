@@ -58,6 +58,21 @@ int farfield_amplitudes(const double* centers, const double* sizes, int uniform,
                         const double* dirs, size_t ndir, double k0, int ground, const c64* rs, const c64* rp,
                         c64* partial, unsigned slices, c64* amplitude, hipStream_t s);
 
+// Far field of the solved wires and patches (classic engine): segments
+// x y z si cab sab salp [n] in wavelengths with currents cur[3 n] (a, b, c
+// per segment, NEC's amperes per wavelength); patches pp[3 m], pbi[m] with
+// currents pcur[3 m].  et/ep: NEC's E_theta, E_phi (r in wavelengths) for
+// each (theta, phi); pec: the image over a PEC plane.
+struct ConductorFar {
+  int n, m;
+  const double *x, *y, *z, *si, *cab, *sab, *salp;
+  const c64* cur;
+  const double *pp, *pbi;
+  const c64* pcur;
+};
+int conductor_far_field(const ConductorFar& cf, int pec, size_t count, const double* theta, const double* phi,
+                        c64* et, c64* ep, hipStream_t s);
+
 // ---- volume-to-conductor rows (conductor_rows.hip) ----
 // Rows of the conductor equations over the volume unknowns (device arrays):
 // wire task w writes row w, patch task p rows nw + 2p and nw + 2p + 1; entry

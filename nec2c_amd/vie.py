@@ -132,7 +132,7 @@ EXPORTS = (
     "vie_hip_radiation_amplitudes", "vie_hip_scattered_power", "vie_hip_volume_conductor_rows",
     "vie_hip_time_volume_conductor_rows", "vie_hip_conductor_volume_columns",
     "vie_hip_time_conductor_volume_columns", "dielectric_fft_assemble_conductor_rows",
-    "dielectric_fft_assemble_conductor_columns",
+    "dielectric_fft_assemble_conductor_columns", "vie_hip_total_far_field", "vie_hip_total_radiated_power",
 )
 
 _LIB = None
@@ -226,6 +226,9 @@ def lib() -> ctypes.CDLL:
         "vie_hip_conductor_volume_columns": (i, [d, d, sz, vp, vp, vp, i] + [vp] * 8 + [i, vp, vp, vp, vp, sz, vp, d,
                                                                                      i, vp]),
         "dielectric_fft_assemble_conductor_rows": (i, [vp, d, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i]),
+        "vie_hip_total_far_field": (i, [d, d, i] + [vp] * 8 + [i, vp, vp, vp, sz, vp, vp, vp, i, sz, vp, vp, vp, vp]),
+        "vie_hip_total_radiated_power": (i, [d, d, i] + [vp] * 8 + [i, vp, vp, vp, sz, vp, vp, vp, i, sz,
+                                                                     ctypes.POINTER(d)]),
         "dielectric_fft_assemble_conductor_columns": (i, [vp, d, d, vp, vp, vp, i] + [vp] * 8 +
                                                       [i, vp, vp, vp, vp, vp, d, i]),
         "vie_hip_time_conductor_volume_columns": (i, [d, d, sz, vp, vp, vp, i] + [vp] * 8 +
@@ -383,6 +386,60 @@ def time_conductor_volume_columns(k0, wlam, centers, sizes, chi, segments, sourc
     if not ok:
         raise RuntimeError(f"conductor-to-volume timing failed: {last_error()}")
     return ms.value
+
+
+def _total_args(segments, segment_current, patches, patch_current, centers, sizes, volume_current):
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    c128 = lambda a: np.ascontiguousarray(a, dtype=np.complex128).reshape(-1)
+    seg = [f64(a) for a in segments] if segments is not None else [np.zeros(0)] * 7
+    n = len(seg[0])
+    if len(seg) != 7 or any(len(a) != n for a in seg):
+        raise ValueError("segments: x, y, z, si, cab, sab, salp of equal length")
+    scur = c128(segment_current) if n else np.zeros(0, complex)
+    pts, area = (f64(patches[0]), f64(patches[1])) if patches is not None else (np.zeros(0), np.zeros(0))
+    m = len(area)
+    pcur = c128(patch_current) if m else np.zeros(0, complex)
+    if len(scur) != 3 * n or len(pts) != 3 * m or len(pcur) != 3 * m:
+        raise ValueError("three current coefficients per segment and per patch, three coordinates per patch")
+    if centers is None:
+        c = s = np.zeros(0)
+        j = np.zeros(0, complex)
+    else:
+        c, s, j = _cells_arrays(centers, sizes, volume_current)
+    return seg, n, scur, pts, area, m, pcur, c, s, j
+
+
+def total_far_field(k0, wlam, theta, phi, segments=None, segment_current=None, patches=None, patch_current=None,
+                    centers=None, sizes=None, volume_current=None, ground=0):
+    """vie_hip_total_far_field: (r E_theta, r E_phi) of wires, patches and volume."""
+    seg, n, scur, pts, area, m, pcur, c, s, j = _total_args(segments, segment_current, patches, patch_current,
+                                                            centers, sizes, volume_current)
+    th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+    ph = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+    et = np.zeros(len(th), dtype=np.complex128)
+    ep = np.zeros(len(th), dtype=np.complex128)
+    ok = lib().vie_hip_total_far_field(float(k0), float(wlam), n, *[a.ctypes.data for a in seg], scur.ctypes.data, m,
+                                       pts.ctypes.data, area.ctypes.data, pcur.ctypes.data, len(c) // 3 if c.ndim == 1
+                                       else len(c), c.ctypes.data, s.ctypes.data, j.ctypes.data, int(ground), len(th),
+                                       th.ctypes.data, ph.ctypes.data, et.ctypes.data, ep.ctypes.data)
+    if not ok:
+        raise RuntimeError(f"total far field failed: {last_error()}")
+    return et, ep
+
+
+def total_radiated_power(k0, wlam, count=3896, segments=None, segment_current=None, patches=None,
+                         patch_current=None, centers=None, sizes=None, volume_current=None, ground=0):
+    """vie_hip_total_radiated_power: dw_radiated_power of the coupled solve (W)."""
+    seg, n, scur, pts, area, m, pcur, c, s, j = _total_args(segments, segment_current, patches, patch_current,
+                                                            centers, sizes, volume_current)
+    p = ctypes.c_double()
+    ok = lib().vie_hip_total_radiated_power(float(k0), float(wlam), n, *[a.ctypes.data for a in seg],
+                                            scur.ctypes.data, m, pts.ctypes.data, area.ctypes.data, pcur.ctypes.data,
+                                            len(c) // 3 if c.ndim == 1 else len(c), c.ctypes.data, s.ctypes.data,
+                                            j.ctypes.data, int(ground), int(count), ctypes.byref(p))
+    if not ok:
+        raise RuntimeError(f"total radiated power failed: {last_error()}")
+    return p.value
 
 
 def last_error() -> str:
