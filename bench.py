@@ -640,6 +640,72 @@ def bench_columns(args):
     return 0
 
 
+def bench_cross_slab(args):
+    """The cross blocks of the 128^3 slab (1 cm voxels) and its 4-element Yagi
+    (84 segments, the slab workload's geometry) at 200 MHz in free space: the
+    84 conductor rows (10^3-point rule per pair) and the 84 conductor columns
+    (NEC's near field at 216 points per cell, each element's unknowns with a
+    straight wire's basis sparsity), kernel time with the inputs resident.
+    CPU baseline: the C restatements on a bounded sample of cells, scaled per
+    pair."""
+    from nec2c_amd import vie, workloads as W
+    n, h, f = args.slab_size, 0.01, 200e6
+    k0, wlam = W.k0_of(f), 299.8 / (f / 1e6)
+    edge = n * h
+    yc, zc = 0.5 * edge, 0.5 * edge
+    lengths = (0.50, 0.47, 0.44, 0.43)
+    centers = W.cube_cells(n, h)
+    sizes = np.full_like(centers, h)
+    wires = [W.nec_straight_wire([-0.1, yc + (i - 1.5) * 0.2, zc - L / 2], [-0.1, yc + (i - 1.5) * 0.2, zc + L / 2],
+                                 21, 1e-3, wlam) for i, L in enumerate(lengths)]
+    seg = tuple(np.concatenate([w[q] for w in wires]) for q in range(8))
+    nseg = len(seg[0])
+    points = np.stack([seg[0], seg[1], seg[2]], -1) * wlam
+    tangents = np.stack([seg[5], seg[6], seg[7]], -1)
+    src = np.zeros((nseg, 3 * nseg), dtype=np.complex128)
+    for j in range(nseg):
+        for s_ in (j - 1, j, j + 1):
+            if 0 <= s_ < nseg and s_ // 21 == j // 21:
+                src[j, 3 * s_:3 * s_ + 3] = [1.0, 0.25j, -0.5]
+    pairs = nseg * len(centers)
+    log(f"cross blocks: {n}^3 cells x {nseg} segments = {pairs} pairs")
+    ms_rows = vie.time_volume_conductor_rows(k0, centers, sizes, points, tangents, repeats=args.steps)
+    log(f"rows: {ms_rows:.1f} ms ({pairs / (ms_rows * 1e-3):.3e} pairs/s)")
+    chi = np.full(len(centers), 5.0 - 0.9j)
+    ms_cols = vie.time_conductor_volume_columns(k0, wlam, centers, sizes, chi, seg, src, repeats=args.steps)
+    log(f"columns: {ms_cols:.1f} ms ({pairs / (ms_cols * 1e-3):.3e} pairs/s)")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import conductor_rows as R, wire_volume as V
+        cores = min(16, len(os.sched_getaffinity(0)))
+        sample = centers[:: len(centers) // 512][:512]
+        ss = np.full_like(sample, h)
+        t = time.perf_counter()
+        R.conductor_rows_c(k0, sample, ss, points, tangents, threads=cores)
+        t_rows = time.perf_counter() - t
+        t = time.perf_counter()
+        V.conductor_volume(k0, wlam, sample, ss, 5.0 - 0.9j, V.Conductors(*seg), src, threads=cores)
+        t_cols = time.perf_counter() - t
+        per_cell = (t_rows + t_cols) / len(sample)
+        cpu = {"value": 1.0 / (per_cell * len(centers)), "unit": "cross-block assemblies/s", "cores": cores,
+               "kind": "port", "sample": f"{len(sample)} cells x {nseg} conductors through oracle/conductor_rows.c "
+                                         f"({t_rows:.2f} s) and oracle/wire_volume.c ({t_cols:.2f} s), OpenMP "
+                                         f"{cores} threads, scaled to {len(centers)} cells"}
+        log(f"CPU port: rows {t_rows / len(sample) * len(centers):.0f} s + columns "
+            f"{t_cols / len(sample) * len(centers):.0f} s for the full slab on {cores} threads")
+    total = ms_rows + ms_cols
+    print(json.dumps({
+        "metric": "cross-block assemblies/s (conductor rows + columns of the coupled system)",
+        "value": 1e3 / total, "unit": "assemblies/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": total, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 slab (1 cm voxels), 4-element Yagi (84 segments) 0.1 m from a face, 200 MHz, "
+                               "free space", "cells": n ** 3, "conductors": nseg},
+        "rows_ms": ms_rows, "columns_ms": ms_cols, "pairs": pairs, "roofline": None, "cpu_baseline": cpu,
+    }))
+    return 0
+
+
 def bench_point(args):
     """One frequency point of the 64^3 coupled sweep from scratch, all on the
     device except the conductor block C: prepare (cell map, Galerkin spectrum,
@@ -726,7 +792,7 @@ def main():
     ap.add_argument("--recycle", type=int, default=0,
                     help="mixed: recycled-subspace size of the GMRES cycles (0 = library default, -1 = off)")
     ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield", "rows", "columns",
-                                           "point"),
+                                           "point", "cross_slab"),
                     default="sweep",
                     help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
@@ -773,6 +839,8 @@ def main():
         return bench_columns(args)
     if args.workload == "point":
         return bench_point(args)
+    if args.workload == "cross_slab":
+        return bench_cross_slab(args)
 
     n = args.size
     pitch = 0.5 / n
