@@ -136,6 +136,35 @@ def cpu_baseline(n, pitch, eps, case, iterations_needed, restart, sample_iters=1
     }, None
 
 
+def gain_difference_db(x_a, x_b, k0, centers, pitch, wire_points, wire_delta, floor_db=40.0, step_deg=5.0):
+    """Largest |gain difference| in dB between two solutions of the same
+    coupled system over a step_deg theta/phi grid: the total far field of the
+    wire (each segment's solved current as a constant current, NEC's
+    amperes-per-wavelength coefficient) and the volume current, through
+    vie_hip_total_far_field; directions more than floor_db below the pattern
+    maximum are left out (nulls)."""
+    from nec2c_amd import vie
+    wlam = 2 * math.pi / k0
+    th, ph = np.meshgrid(np.radians(np.arange(0.0, 180.0 + 1e-9, step_deg)),
+                         np.radians(np.arange(0.0, 360.0, step_deg)), indexing="ij")
+    th, ph = th.reshape(-1), ph.reshape(-1)
+    nseg = len(wire_points)
+    seg = (wire_points[:, 0] / wlam, wire_points[:, 1] / wlam, wire_points[:, 2] / wlam,
+           np.broadcast_to(np.asarray(wire_delta, dtype=np.float64) / wlam, (nseg,)).copy(), np.zeros(nseg),
+           np.zeros(nseg), np.ones(nseg))
+    sizes = np.full_like(centers, pitch)
+    gains = []
+    for x in (x_a, x_b):
+        cur = np.zeros(3 * nseg, dtype=np.complex128)
+        cur[0::3] = x[:nseg] / wlam
+        et, ep = vie.total_far_field(k0, wlam, th, ph, seg, cur, centers=centers, sizes=sizes,
+                                     volume_current=x[nseg:])
+        gains.append(np.abs(et) ** 2 + np.abs(ep) ** 2)
+    ga, gb = gains
+    keep = gb > gb.max() * 10 ** (-floor_db / 10)
+    return float(np.max(np.abs(10 * np.log10(ga[keep] / gb[keep]))))
+
+
 def _case_blocks(case):
     """Host-side inputs of one batched model (the caller's data)."""
     from nec2c_amd import workloads as W
@@ -377,6 +406,9 @@ def bench_slab(args):
     by64, _ = s.apply_bytes(vie.FP64)
     fi = yagi.feed_index()
     zm, zd = 1 / res["mixed"][3][fi], 1 / res["fp64"][3][fi]
+    gain_db = gain_difference_db(res["mixed"][3], res["fp64"][3], k0, W.cube_cells(n, h), h, yagi.points(),
+                                 yagi.deltas())
+    log(f"slab far-field gain, mixed vs FP64: max {gain_db:.2e} dB over a 5-degree grid")
     names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
     result = {
         "metric": METRIC,
@@ -403,6 +435,7 @@ def bench_slab(args):
         "converged": [bool(res["mixed"][0]), bool(res["fp64"][0])],
         "feed_impedance_ohm_fp64": str(complex(np.round(zd, 6))),
         "feed_impedance_rel_err_mixed_vs_fp64": float(abs(zm - zd) / abs(zd)),
+        "far_field_gain_max_db_mixed_vs_fp64": gain_db,
         "matvec": {"ms_mixed": ms32, "ms_fp64": ms64, "bytes_mixed": by32, "bytes_fp64": by64,
                    "effective_gbs_mixed": by32 / (ms32 * 1e-3) / 1e9, "effective_gbs_fp64": by64 / (ms64 * 1e-3) / 1e9,
                    "pass_ms_mixed": dict(zip(names, p32)), "pass_ms_fp64": dict(zip(names, p64)),
@@ -901,6 +934,9 @@ def main():
     z_err = max(abs(1 / xm[fi] - 1 / xd[fi]) / abs(1 / xd[fi]) for xm, xd in zip(x_mixed, x_fp64))
     i_err = max(float(np.max(np.abs(xm[:21] - xd[:21])) / np.max(np.abs(xd[:21])))
                 for xm, xd in zip(x_mixed, x_fp64))
+    gain_db = max(gain_difference_db(xm, xd, case[4], centers, pitch, dipole.points(), dipole.delta)
+                  for xm, xd, case in zip(x_mixed, x_fp64, cases[args.warmup:]))
+    log(f"far-field gain, mixed vs FP64: max {gain_db:.2e} dB over a 5-degree grid")
 
     # matvec timing and roofline of the dominant kernel
     s0 = timed[0]
@@ -966,6 +1002,7 @@ def main():
         "residual_max_fp64": max(r.relative_residual for r in rep_fp64),
         "feed_impedance_rel_err_mixed_vs_fp64": z_err,
         "conductor_current_rel_err_mixed_vs_fp64": i_err,
+        "far_field_gain_max_db_mixed_vs_fp64": gain_db,
         "feed_impedance_ohm_fp64": [str(complex(np.round(1 / x[fi], 6))) for x in x_fp64[:3]],
         "matvec": {
             "ms_mixed": mv_ms32, "ms_fp64": mv_ms64,
