@@ -255,6 +255,10 @@ int vie_hip_total_radiated_power(double k0, double wlam, int segments, const dou
                                  const VIE_CDOUBLE *patch_current, size_t cells, const double *centers,
                                  const double *sizes, const VIE_CDOUBLE *volume_current, int ground, size_t count,
                                  double *power);
+/* Frees the device buffers that the far-field, conductor-row, conductor-column
+   and coupled far-field entry points keep between calls (new; for long-running
+   servers; the next call allocates again).  1 on success. */
+int vie_hip_release_workspaces(void);
 /* Host-side step of the recycled subspace, exported for tests: the k harmonic
    Ritz vectors of smallest modulus (Morgan 2002) of the (m+1) x m Arnoldi
    Hessenberg hbar (column-major), as the m x k column-major p with unit

@@ -2851,6 +2851,38 @@ extern "C" int vie_hip_total_radiated_power(double k0, double wlam, int segments
   return 1;
 }
 
+// Frees the device buffers the far-field, conductor-row, conductor-column and
+// coupled far-field entry points keep between calls (they are reallocated on
+// the next call).
+extern "C" int vie_hip_release_workspaces(void) {
+  {
+    FarfieldWork& w = farfield_work();
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (w.stream) HIP_OK(hipStreamSynchronize(w.stream));
+    for (DevBuf* b : {&w.centers, &w.sizes, &w.current, &w.dirs, &w.rs, &w.rp, &w.partial, &w.amp}) b->release();
+  }
+  {
+    RowsWork& w = rows_work();
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (w.stream) HIP_OK(hipStreamSynchronize(w.stream));
+    for (DevBuf* b : {&w.centers, &w.sizes, &w.cond, &w.rows, &w.failed}) b->release();
+  }
+  {
+    ColumnsWork& w = columns_work();
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (w.stream) HIP_OK(hipStreamSynchronize(w.stream));
+    for (DevBuf* b : {&w.geom, &w.cells, &w.chi, &w.active, &w.jbase, &w.term_seg, &w.coef, &w.skip, &w.F, &w.out})
+      b->release();
+  }
+  {
+    TotalWork& w = total_work();
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (w.stream) HIP_OK(hipStreamSynchronize(w.stream));
+    for (DevBuf* b : {&w.geom, &w.cur, &w.pcur, &w.ang, &w.e}) b->release();
+  }
+  return 1;
+}
+
 extern "C" int dielectric_fft_last_solve_recycled(const dielectric_fft_system* s) {
   return s && s->sw.rec_ready ? s->sw.rk : 0;
 }

@@ -133,6 +133,7 @@ EXPORTS = (
     "vie_hip_time_volume_conductor_rows", "vie_hip_conductor_volume_columns",
     "vie_hip_time_conductor_volume_columns", "dielectric_fft_assemble_conductor_rows",
     "dielectric_fft_assemble_conductor_columns", "vie_hip_total_far_field", "vie_hip_total_radiated_power",
+    "vie_hip_release_workspaces",
 )
 
 _LIB = None
@@ -226,6 +227,7 @@ def lib() -> ctypes.CDLL:
         "vie_hip_conductor_volume_columns": (i, [d, d, sz, vp, vp, vp, i] + [vp] * 8 + [i, vp, vp, vp, vp, sz, vp, d,
                                                                                      i, vp]),
         "dielectric_fft_assemble_conductor_rows": (i, [vp, d, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i]),
+        "vie_hip_release_workspaces": (i, []),
         "vie_hip_total_far_field": (i, [d, d, i] + [vp] * 8 + [i, vp, vp, vp, sz, vp, vp, vp, i, sz, vp, vp, vp, vp]),
         "vie_hip_total_radiated_power": (i, [d, d, i] + [vp] * 8 + [i, vp, vp, vp, sz, vp, vp, vp, i, sz,
                                                                      ctypes.POINTER(d)]),

@@ -355,3 +355,7 @@ def test_conductor_volume_rejections_before_any_device_use():
             call(**kw)
     # no columns: nothing to do, no device needed
     assert Vh.conductor_volume_columns(1.0, 1.0, c, sz, chi, seg, np.zeros((0, 3), complex)).shape == (0, 3)
+
+
+def test_release_workspaces_without_device_work():
+    assert vie.lib().vie_hip_release_workspaces() == 1

@@ -164,3 +164,18 @@ def test_zero_columns_and_skipped_segments():
         vie.conductor_volume_columns(k0, wlam, centers, sizes, 2.0, bad, src)
     with pytest.raises(RuntimeError, match="finite ground"):
         vie.conductor_volume_columns(k0, wlam, centers, sizes, 2.0, segments(cond), src, ground=2)
+
+
+def test_workspaces_released_and_reallocated():
+    """vie_hip_release_workspaces frees the kept buffers; the next call
+    allocates again and gives the same result."""
+    f = 300.0
+    wlam, k0 = 299.8 / f, 2 * math.pi * f * 1e6 / V.C0
+    cond = V.scaled(V.straight_wire([0.0, 0.0, 0.36], [0.0, 0.0, 0.84], 11, 0.001), wlam)
+    centers = block((0.10, -0.02, 0.58), (2, 2, 2), 0.02)
+    sizes = np.full_like(centers, 0.02)
+    src = nec_like_sources(np.random.default_rng(1), 11)
+    a = vie.conductor_volume_columns(k0, wlam, centers, sizes, 2.0, segments(cond), src)
+    assert vie.lib().vie_hip_release_workspaces() == 1
+    b = vie.conductor_volume_columns(k0, wlam, centers, sizes, 2.0, segments(cond), src)
+    assert np.array_equal(a, b)
