@@ -99,6 +99,14 @@ __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int
   const size_t nchunks = (n + CH - 1) / CH;
   for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
     const size_t base = ch * CH;
+    cx<T> a[CPL], an[CPL];
+    auto load_key = [&](int k, cx<T>* dst) {
+      const cx<T>* vk = V + (size_t)k * ldv + base + lane;
+#pragma unroll
+      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? ld_stream(vk + e * 64) : cx<T>{0, 0};
+    };
+    // the wave's first key is in flight while the chunk of z is staged
+    if (wave < nk) load_key(wave, a);
     __syncthreads();
     for (int e = threadIdx.x; e < CH; e += KWG) {
       size_t i = base + e;
@@ -107,13 +115,6 @@ __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int
       zs[e] = c64{(double)v.re, (double)v.im};
     }
     __syncthreads();
-    cx<T> a[CPL], an[CPL];
-    auto load_key = [&](int k, cx<T>* dst) {
-      const cx<T>* vk = V + (size_t)k * ldv + base + lane;
-#pragma unroll
-      for (int e = 0; e < CPL; e++) dst[e] = base + lane + (size_t)e * 64 < n ? ld_stream(vk + e * 64) : cx<T>{0, 0};
-    };
-    if (wave < nk) load_key(wave, a);
     for (int k = wave; k < tot; k += WAVES) {
       c64 s = {0, 0};
       if (k < nk) {

@@ -317,7 +317,7 @@ __host__ __device__ constexpr int tile_lw(int l0, int v) {
   return v == 0 ? l0 : v == 2 ? (l0 / 4 > 0 ? l0 / 4 : 1) : (l0 / 2 > 0 ? l0 / 2 : 1);
 }
 __host__ __device__ constexpr int tile_wg(int v) { return v == 1 ? 128 : v == 2 ? 64 : 256; }
-constexpr int kZCap = 8, kYCap = 16, kXCap = 16;
+constexpr int kZCap = 8, kYCap = 16, kXCap = 32;
 
 template <typename T, int P> struct PassLauncher {
   static constexpr int E = PassShape<T>::E;
@@ -325,12 +325,14 @@ template <typename T, int P> struct PassLauncher {
     constexpr int WG = tile_wg(V);
     constexpr int LZ = tile_lw(tile_l0(E, P, kZCap), V);
     constexpr int LY = tile_lw(tile_l0(E, P, kYCap), V);
-    constexpr int LX = tile_lw(tile_l0(E, P, kXCap), V);
+    // the x pass holds three components per line: twice the elements and
+    // threads of the other passes, for wider HBM segments per line group
+    constexpr int LX = tile_lw(tile_l0(2 * E, P, kXCap), V);
     const long zl = (long)a.g.nx * a.g.ny, yl = 3L * a.g.nx * a.g.Pz, xl = (long)a.g.Py * a.g.Pz;
     switch (pass) {
       case 0: hipLaunchKernelGGL((k_zfwd<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
       case 1: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, false>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
-      case 2: hipLaunchKernelGGL((k_xpass<T, P, LX, WG>), dim3((xl + LX - 1) / LX), dim3(WG), 0, s, a); break;
+      case 2: hipLaunchKernelGGL((k_xpass<T, P, LX, 2 * WG>), dim3((xl + LX - 1) / LX), dim3(2 * WG), 0, s, a); break;
       case 3: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, true>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
       default: hipLaunchKernelGGL((k_zinv<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
     }
