@@ -79,11 +79,13 @@ def build_case(n, freq, eps, dipole, cells_xyz, centers, pitch, opts):
 
 def pass_kernel_name(pas, P, v, E=4096):
     """Kernel instantiation of FP32 volume pass `pas` at launch variant v
-    (lines per workgroup and workgroup size as the library's tile rules)."""
-    cap = (8, 16, 16, 16, 8)[pas]
-    l0 = min(E // P, cap)
+    (lines per workgroup and workgroup size as the library's tile rules; the
+    x pass takes twice the elements and threads)."""
+    cap = (8, 16, 32, 16, 8)[pas]
+    wide = 2 if pas == 2 else 1
+    l0 = min(wide * E // P, cap)
     lw = (l0, max(l0 // 2, 1), max(l0 // 4, 1), max(l0 // 2, 1))[v]
-    wg = (256, 128, 64, 256)[v]
+    wg = wide * (256, 128, 64, 256)[v]
     kern = ("k_zfwd<float,{P},{lw},{wg}>", "k_ypass<float,{P},{lw},{wg},false>",
             "k_xpass<float,{P},{lw},{wg}> (X FFT + 3x3 spectral multiply + inverse X)",
             "k_ypass<float,{P},{lw},{wg},true>", "k_zinv<float,{P},{lw},{wg}>")[pas]

@@ -607,11 +607,15 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
       const int c = j - 1;
       const int brk = alpha <= breakdown_eps * sd[0];
       st->breakdown = brk;
+      // earlier rotations in turn; the rotated entry k + 1 is carried in a
+      // register, so the chain waits on no LDS round trip
+      c64 f = col[base];
       for (int k = base; k < c; k++) {
-        c64 f = col[k], gg = col[k + 1];
+        const c64 gg = col[k + 1];
         col[k] = cs[k] * f + sn[k] * gg;
-        col[k + 1] = cs[k] * gg - conj(sn[k]) * f;
+        f = cs[k] * gg - conj(sn[k]) * f;
       }
+      col[c] = f;
       double cr;
       c64 z;
       givens(col[c], col[c + 1], &cr, &z);
