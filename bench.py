@@ -610,6 +610,83 @@ def bench_rows(args):
     return 0
 
 
+def bench_mesh(args):
+    """The mesher's relationship checks on the per-voxel form of the sweep's
+    model: one BOX record per voxel of the n^3 cube (edge 0.5 m) and the
+    21-segment dipole (radius 1 mm) 0.1 m from one face.  The box-interior
+    overlap scan of validate_box_relationships (dielectric_mesh.c:81-123)
+    tests all n^3 (n^3 - 1) / 2 pairs; the clearance of evaluate_clearance
+    (dielectric_mesh.c:215-316) every (box, segment) pair.  Kernel times with
+    the boxes resident (HIP events), plus the whole calls.  CPU baseline: the
+    C restatement oracle/mesh_check.c (the reference's pair loop without its
+    per-pair region lookups, so faster than the reference) on one core, on
+    the pairs of a bounded sample of left boxes, and the full clearance."""
+    from nec2c_amd import vie
+    n = args.mesh_size
+    pitch = 0.5 / n
+    idx = np.indices((n, n, n)).reshape(3, -1).T.astype(float)
+    lo = idx * pitch
+    hi = lo + pitch
+    cell = np.full(len(lo), pitch)
+    seg_len = 0.48 / 21
+    zc = 0.01 + (np.arange(21) + 0.5) * seg_len
+    seg = [np.full(21, -0.1), np.full(21, 0.25), zc, np.full(21, seg_len), np.zeros(21), np.zeros(21), np.ones(21),
+           np.full(21, 1e-3)]
+    B = len(lo)
+    pairs = B * (B - 1) // 2
+    pair, ms_overlap = vie.box_overlap(lo, hi, repeats=args.steps)
+    r, ms_clear = vie.conductor_clearance(lo, hi, cell, segments=seg, repeats=args.steps)
+    t = time.perf_counter()
+    vie.box_overlap(lo, hi)
+    vie.conductor_clearance(lo, hi, cell, segments=seg)
+    call_ms = (time.perf_counter() - t) * 1e3
+    rate = pairs / (ms_overlap * 1e-3)
+    log(f"mesh {n}^3 per-voxel: {B} boxes, {pairs:.3e} pairs in {ms_overlap:.2f} ms ({rate:.3e} pairs/s), "
+        f"first overlap {pair}; clearance {B * 21} pairs {ms_clear:.3f} ms, minimum gap {r['minimum_gap']:.4g} m; "
+        f"both calls from the host {call_ms:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import mesh_check as M
+        K = min(args.mesh_cpu_lefts, B)
+        t = time.perf_counter()
+        M.box_overlap(lo, hi, 0, K)
+        t_ov = time.perf_counter() - t
+        sampled = K * B - K * (K + 1) // 2
+        t = time.perf_counter()
+        M.clearance(lo, hi, cell, segments=seg)
+        t_cl = time.perf_counter() - t
+        cpu_rate = sampled / t_ov
+        cpu = {"value": cpu_rate, "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": f"the pairs of the first {K} left boxes ({sampled:.3e} pairs) through oracle/mesh_check.c "
+                         f"in {t_ov:.2f} s; the full scan would take {pairs / cpu_rate:.0f} s; the full clearance "
+                         f"({B * 21} pairs) {t_cl:.2f} s",
+               "clearance_s": t_cl}
+        log(f"mesh CPU port: {cpu_rate:.3e} pairs/s on one core (full scan {pairs / cpu_rate:.0f} s)")
+    # VALU issue: the pair loop of k_box_overlap is 20 VALU instructions per
+    # pair in the AMD assembly of this build (9 v_max_f64, 3 v_min_f64,
+    # 3 v_cmp_nlt_f64, a move and the loop's 4 integer ops) beside 3 LDS reads;
+    # peak = 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s.
+    per_pair, peak = 20, 256 * 4 * 16 * 2.4e9
+    achieved = rate * per_pair
+    roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "VALU lane-instructions/s",
+            "frac": achieved / peak, "traffic": None, "kernel": "k_box_overlap (all box pairs)",
+            "instructions_per_pair": per_pair}
+    log(f"mesh roofline: {achieved:.3e} of {peak:.3e} VALU lane-instructions/s ({achieved / peak:.1%})")
+    print(json.dumps({
+        "metric": "box pairs/s of the mesher's interior-overlap scan (validate_box_relationships), per-voxel model",
+        "value": rate, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": ms_overlap + ms_clear, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 per-voxel BOX records (edge 0.5 m) + 21-segment dipole 0.1 m from a face",
+                   "boxes": B, "pairs": pairs, "segments": 21},
+        "mesh": {"overlap_ms": ms_overlap, "clearance_ms": ms_clear, "calls_ms": call_ms, "first_overlap": pair,
+                 "minimum_gap_m": r["minimum_gap"], "required_gap_m": r["required_gap"],
+                 "clearance_insufficient": r["insufficient"]},
+        "cpu_baseline": cpu, "roofline": roof,
+    }))
+    return 0
+
+
 def bench_columns(args):
     """Volume rows of the conductor columns (wire -> volume) of the sweep's
     coupled system: the n^3 cube (edge 0.5 m) and the 21-segment dipole 0.1 m
@@ -827,7 +904,7 @@ def main():
     ap.add_argument("--recycle", type=int, default=0,
                     help="mixed: recycled-subspace size of the GMRES cycles (0 = library default, -1 = off)")
     ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield", "rows", "columns",
-                                           "point", "cross_slab"),
+                                           "point", "cross_slab", "mesh"),
                     default="sweep",
                     help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
@@ -839,6 +916,9 @@ def main():
     ap.add_argument("--rows-cpu-cells", type=int, default=16384, help="rows: cells in the CPU-baseline sample")
     ap.add_argument("--columns-size", type=int, default=64, help="columns: voxels per cube edge")
     ap.add_argument("--columns-cpu-cells", type=int, default=8192, help="columns: cells in the CPU-baseline sample")
+    ap.add_argument("--mesh-size", type=int, default=64, help="mesh: voxels per cube edge (one BOX per voxel)")
+    ap.add_argument("--mesh-cpu-lefts", type=int, default=2048,
+                    help="mesh: left boxes whose pairs the CPU baseline scans")
     ap.add_argument("--arnoldi-column", type=int, default=0,
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
@@ -870,6 +950,8 @@ def main():
         return bench_farfield(args)
     if args.workload == "rows":
         return bench_rows(args)
+    if args.workload == "mesh":
+        return bench_mesh(args)
     if args.workload == "columns":
         return bench_columns(args)
     if args.workload == "point":

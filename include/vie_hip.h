@@ -255,9 +255,55 @@ int vie_hip_total_radiated_power(double k0, double wlam, int segments, const dou
                                  const VIE_CDOUBLE *patch_current, size_t cells, const double *centers,
                                  const double *sizes, const VIE_CDOUBLE *volume_current, int ground, size_t count,
                                  double *power);
-/* Frees the device buffers that the far-field, conductor-row, conductor-column
-   and coupled far-field entry points keep between calls (new; for long-running
-   servers; the next call allocates again).  1 on success. */
+/* Box-interior overlap scan of the mesher's validate_box_relationships
+   (dielectric_mesh.c:81-123): boxes [minimum, maximum] (count x 3 each, the
+   model's BOX records in order).  *left, *right <- the 0-based indices of the
+   first pair (left < right, left the outer loop as there) whose interiors
+   overlap on all three axes, or -1, -1 when none does; the reference's
+   diagnostic reports that pair's region ids and the line of box `right`.
+   The region lookups that precede the scan stay with the caller.  All
+   count (count - 1) / 2 pairs are tested on the GPU.  1 on success. */
+int vie_hip_box_overlap(const double *minimum, const double *maximum, size_t count, long long *left,
+                        long long *right);
+/* vie_hip_box_overlap `repeats` times on resident corners; *ms <- average
+   milliseconds of one scan (HIP events).  Used by the benchmark. */
+int vie_hip_time_box_overlap(const double *minimum, const double *maximum, size_t count, int repeats,
+                             long long *left, long long *right, double *ms);
+typedef struct {
+  double worst_margin;    /* gap - required of the worst pair (DBL_MAX: none evaluated) */
+  double worst_gap;       /* its gap (m) and required clearance (m), as the diagnostic prints them */
+  double worst_required;
+  double minimum_gap;     /* mesh->minimum_conductor_gap and mesh->required_conductor_gap */
+  double required_gap;
+  long long worst_box;    /* 1-based box of the worst pair (0: none) */
+  long long worst_conductor; /* 1-based wire segment or patch of the worst pair */
+  int worst_is_patch;
+  int has_conductors;     /* mesh->has_conductors: cells and conductors exist */
+  int insufficient;       /* worst_margin < -1e-12: the reference fails with CLEARANCE */
+} vie_hip_clearance;
+/* Dielectric-to-conductor clearance of the mesher's evaluate_clearance
+   (dielectric_mesh.c:215-316): every box (corners as above; cell_longest =
+   the longest edge of the box's first cell, 0 for a box without cells, which
+   is skipped) against every wire segment (NEC's centre, length, direction
+   cosines and radius bi, in metres) and patch (centre patch_points[3 j ..],
+   area).  The worst pair and the smallest gap are the first in the
+   reference's loop order (boxes outer, then wires, then patches) and carry
+   the same values.  1 on success. */
+int vie_hip_conductor_clearance(const double *minimum, const double *maximum, const double *cell_longest,
+                                size_t boxes, int segments, const double *x, const double *y, const double *z,
+                                const double *si, const double *cab, const double *sab, const double *salp,
+                                const double *bi, int patches, const double *patch_points, const double *patch_area,
+                                vie_hip_clearance *out);
+/* vie_hip_conductor_clearance `repeats` times on resident inputs; *ms <-
+   average milliseconds of one evaluation.  Used by the benchmark. */
+int vie_hip_time_conductor_clearance(const double *minimum, const double *maximum, const double *cell_longest,
+                                     size_t boxes, int segments, const double *x, const double *y, const double *z,
+                                     const double *si, const double *cab, const double *sab, const double *salp,
+                                     const double *bi, int patches, const double *patch_points,
+                                     const double *patch_area, int repeats, vie_hip_clearance *out, double *ms);
+/* Frees the device buffers that the far-field, conductor-row, conductor-column,
+   coupled far-field and mesh-check entry points keep between calls (new; for
+   long-running servers; the next call allocates again).  1 on success. */
 int vie_hip_release_workspaces(void);
 /* Host-side step of the recycled subspace, exported for tests: the k harmonic
    Ritz vectors of smallest modulus (Morgan 2002) of the (m+1) x m Arnoldi

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib")
 SOURCES = ["vie_passes_f32.hip", "vie_passes_f64.hip", "spectrum.hip", "krylov.hip", "farfield.hip",
-           "conductor_rows.hip", "wire_volume.hip", "api.hip"]
+           "conductor_rows.hip", "wire_volume.hip", "mesh_check.hip", "api.hip"]
 ARCH = os.environ.get("VIE_HIP_ARCH", "gfx950")
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
          # packed-FP32 instructions are kept out of every kernel

@@ -2854,6 +2854,200 @@ extern "C" int vie_hip_total_radiated_power(double k0, double wlam, int segments
 // Frees the device buffers the far-field, conductor-row, conductor-column and
 // coupled far-field entry points keep between calls (they are reallocated on
 // the next call).
+// ------------------------------------------------------------------ mesh relationship checks
+namespace {
+struct MeshWork {
+  std::mutex mu;
+  DevBuf boxes, cond, part, first, res;
+  hipStream_t stream = nullptr;
+};
+MeshWork& mesh_work() {
+  static MeshWork* w = new MeshWork();  // lives for the process (device buffers)
+  return *w;
+}
+}  // namespace
+
+// The overlap scan of validate_box_relationships (dielectric_mesh.c:81-123)
+// over resident corners; repeats > 1 times the scan (average ms).
+static int overlap_run(const double* minimum, const double* maximum, size_t count, long long* left, long long* right,
+                       int repeats, double* ms) {
+  g_last_error.clear();
+  size_t bytes;
+  if ((count && (!minimum || !maximum)) || !left || !right || repeats < 1 ||
+      !mul_ok(count, 6 * sizeof(double), &bytes) || count > ((size_t)1 << 31)) {
+    set_error("box overlap: invalid argument");
+    return 0;
+  }
+  *left = *right = -1;
+  if (!ensure_device()) return 0;
+  MeshWork& w = mesh_work();
+  std::lock_guard<std::mutex> lock(w.mu);
+  if (!w.stream) HIP_OK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+  if (!(w.boxes.alloc(bytes) && w.first.alloc(sizeof(unsigned long long)))) return 0;
+  hipStream_t st = w.stream;
+  double* mn = w.boxes.as<double>();
+  double* mx = mn + 3 * count;
+  if (count) {
+    HIP_OK(hipMemcpyAsync(mn, minimum, 3 * count * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(mx, maximum, 3 * count * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  unsigned long long* first = w.first.as<unsigned long long>();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, st));
+  }
+  for (int r = 0; r < repeats; r++)
+    if (!vie::box_overlap(mn, mx, count, first, st)) {
+      set_error("box overlap kernel failed to launch");
+      return 0;
+    }
+  if (ms) HIP_OK(hipEventRecord(e1, st));
+  unsigned long long key = ~0ull;
+  HIP_OK(hipMemcpyAsync(&key, first, sizeof(key), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (ms) {
+    float t = 0;
+    HIP_OK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (key != ~0ull) {
+    *left = (long long)(key / count);
+    *right = (long long)(key % count);
+  }
+  return 1;
+}
+
+extern "C" int vie_hip_box_overlap(const double* minimum, const double* maximum, size_t count, long long* left,
+                                   long long* right) {
+  return overlap_run(minimum, maximum, count, left, right, 1, nullptr);
+}
+
+extern "C" int vie_hip_time_box_overlap(const double* minimum, const double* maximum, size_t count, int repeats,
+                                        long long* left, long long* right, double* ms) {
+  if (!ms) {
+    set_error("box overlap timing: invalid argument");
+    return 0;
+  }
+  return overlap_run(minimum, maximum, count, left, right, repeats, ms);
+}
+
+// evaluate_clearance (dielectric_mesh.c:215-316) over boxes and NEC's wire
+// segments and patches.
+static int clearance_run(const double* minimum, const double* maximum, const double* cell_longest, size_t boxes,
+                         int segments, const double* x, const double* y, const double* z, const double* si,
+                         const double* cab, const double* sab, const double* salp, const double* bi, int patches,
+                         const double* patch_points, const double* patch_area, vie_hip_clearance* out, int repeats,
+                         double* ms) {
+  g_last_error.clear();
+  size_t bytes;
+  if (!out || segments < 0 || patches < 0 || repeats < 1 ||
+      (boxes && (!minimum || !maximum || !cell_longest)) ||
+      (segments && (!x || !y || !z || !si || !cab || !sab || !salp || !bi)) ||
+      (patches && (!patch_points || !patch_area)) || !mul_ok(boxes, 7 * sizeof(double), &bytes) ||
+      boxes > ((size_t)1 << 40)) {
+    set_error("conductor clearance: invalid argument");
+    return 0;
+  }
+  vie::ClearanceResult r{DBL_MAX, 0.0, 0.0, DBL_MAX, 0.0, 0, 0, 0};
+  bool any_cells = false;
+  for (size_t b = 0; b < boxes && !any_cells; b++) any_cells = cell_longest[b] > 0.0;
+  out->has_conductors = any_cells && (segments > 0 || patches > 0);
+  if (out->has_conductors) {
+    if (!ensure_device()) return 0;
+    MeshWork& w = mesh_work();
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (!w.stream) HIP_OK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+    const size_t N = (size_t)segments, M = (size_t)patches;
+    const size_t pairs = boxes * (N + M);
+    if (!(w.boxes.alloc(bytes) && w.cond.alloc((8 * N + 4 * M + 1) * sizeof(double)) &&
+          w.part.alloc(vie::clearance_partials(pairs) * 32) && w.res.alloc(sizeof(vie::ClearanceResult))))
+      return 0;
+    hipStream_t st = w.stream;
+    double* bx = w.boxes.as<double>();
+    HIP_OK(hipMemcpyAsync(bx, minimum, 3 * boxes * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(bx + 3 * boxes, maximum, 3 * boxes * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(bx + 6 * boxes, cell_longest, boxes * 8, hipMemcpyHostToDevice, st));
+    double* g = w.cond.as<double>();
+    const double* segs[8] = {x, y, z, si, cab, sab, salp, bi};
+    for (int k = 0; k < 8 && N; k++) HIP_OK(hipMemcpyAsync(g + k * N, segs[k], N * 8, hipMemcpyHostToDevice, st));
+    // patch centres are given as M x 3 points; the kernel reads them per axis
+    std::vector<double> pt(3 * M);
+    for (size_t p = 0; p < M; p++)
+      for (int a = 0; a < 3; a++) pt[a * M + p] = patch_points[3 * p + a];
+    if (M) {
+      HIP_OK(hipMemcpyAsync(g + 8 * N, pt.data(), 3 * M * 8, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(g + 8 * N + 3 * M, patch_area, M * 8, hipMemcpyHostToDevice, st));
+    }
+    vie::ClearanceArgs A{bx,        bx + 3 * boxes, bx + 6 * boxes, boxes,     (long long)N,  (long long)M,
+                         g,         g + N,          g + 2 * N,      g + 3 * N, g + 4 * N,     g + 5 * N,
+                         g + 6 * N, g + 7 * N,      g + 8 * N,      g + 8 * N + M, g + 8 * N + 2 * M,
+                         g + 8 * N + 3 * M};
+    vie::ClearanceResult* dres = w.res.as<vie::ClearanceResult>();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+      HIP_OK(hipEventCreate(&e0));
+      HIP_OK(hipEventCreate(&e1));
+      HIP_OK(hipEventRecord(e0, st));
+    }
+    for (int k = 0; k < repeats; k++)
+      if (!vie::conductor_clearance(A, w.part.p, dres, st)) {
+        set_error("conductor clearance kernel failed to launch");
+        return 0;
+      }
+    if (ms) HIP_OK(hipEventRecord(e1, st));
+    HIP_OK(hipMemcpyAsync(&r, dres, sizeof(r), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (ms) {
+      float t = 0;
+      HIP_OK(hipEventElapsedTime(&t, e0, e1));
+      *ms = t / repeats;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  } else if (ms) {
+    *ms = 0.0;
+  }
+  out->worst_margin = r.worst_margin;
+  out->worst_gap = r.worst_gap;
+  out->worst_required = r.worst_required;
+  out->minimum_gap = r.minimum_gap;
+  out->required_gap = r.required_gap;
+  out->worst_box = r.worst_box;
+  out->worst_conductor = r.worst_conductor;
+  out->worst_is_patch = r.worst_is_patch;
+  out->insufficient = r.worst_margin < -1.0e-12;  // dielectric_mesh.c:305
+  return 1;
+}
+
+extern "C" int vie_hip_conductor_clearance(const double* minimum, const double* maximum, const double* cell_longest,
+                                           size_t boxes, int segments, const double* x, const double* y,
+                                           const double* z, const double* si, const double* cab, const double* sab,
+                                           const double* salp, const double* bi, int patches,
+                                           const double* patch_points, const double* patch_area,
+                                           vie_hip_clearance* out) {
+  return clearance_run(minimum, maximum, cell_longest, boxes, segments, x, y, z, si, cab, sab, salp, bi, patches,
+                       patch_points, patch_area, out, 1, nullptr);
+}
+
+extern "C" int vie_hip_time_conductor_clearance(const double* minimum, const double* maximum,
+                                                const double* cell_longest, size_t boxes, int segments,
+                                                const double* x, const double* y, const double* z, const double* si,
+                                                const double* cab, const double* sab, const double* salp,
+                                                const double* bi, int patches, const double* patch_points,
+                                                const double* patch_area, int repeats, vie_hip_clearance* out,
+                                                double* ms) {
+  if (!ms) {
+    set_error("conductor clearance timing: invalid argument");
+    return 0;
+  }
+  return clearance_run(minimum, maximum, cell_longest, boxes, segments, x, y, z, si, cab, sab, salp, bi, patches,
+                       patch_points, patch_area, out, repeats, ms);
+}
+
 extern "C" int vie_hip_release_workspaces(void) {
   {
     FarfieldWork& w = farfield_work();
@@ -2879,6 +3073,12 @@ extern "C" int vie_hip_release_workspaces(void) {
     std::lock_guard<std::mutex> lock(w.mu);
     if (w.stream) HIP_OK(hipStreamSynchronize(w.stream));
     for (DevBuf* b : {&w.geom, &w.cur, &w.pcur, &w.ang, &w.e}) b->release();
+  }
+  {
+    MeshWork& w = mesh_work();
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (w.stream) HIP_OK(hipStreamSynchronize(w.stream));
+    for (DevBuf* b : {&w.boxes, &w.cond, &w.part, &w.first, &w.res}) b->release();
   }
   return 1;
 }

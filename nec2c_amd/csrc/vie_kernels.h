@@ -114,4 +114,26 @@ int conductor_volume_columns(const c64* F, size_t ldf, int terms, const int* ter
 template <typename T> void vec_zero(cx<T>* x, size_t n, hipStream_t s);
 template <typename To, typename From> void vec_cast(cx<To>* y, const cx<From>* x, size_t n, hipStream_t s);
 
+// ---- mesh relationship checks (mesh_check.hip) ----
+// first <- i * count + j of the first pair (i < j, i outer) of boxes
+// [mn, mx] (count x 3 each) whose interiors overlap, or ~0 when none does.
+int box_overlap(const double* mn, const double* mx, size_t count, unsigned long long* first, hipStream_t s);
+// Boxes (count x 3 corners, cell_longest <= 0 for a box without cells) against
+// n wire segments (NEC's centre, length, direction cosines, radius) and m
+// patches (centre, area), all device pointers.
+struct ClearanceArgs {
+  const double *minimum, *maximum, *cell_longest;
+  size_t boxes;
+  long long n, m;
+  const double *x, *y, *z, *si, *cab, *sab, *salp, *bi;
+  const double *px, *py, *pz, *pbi;
+};
+struct ClearanceResult {
+  double worst_margin, worst_gap, worst_required, minimum_gap, required_gap;
+  long long worst_box, worst_conductor;  // 1-based, 0 when nothing was evaluated
+  int worst_is_patch;
+};
+size_t clearance_partials(size_t pairs);  // partial records (16 bytes, two per block) to provide
+int conductor_clearance(const ClearanceArgs& a, void* partials, ClearanceResult* out, hipStream_t s);
+
 }  // namespace vie

@@ -65,6 +65,14 @@ class SolveOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("inner_reduction", ctypes.c_double), ("recycle", ctypes.c_int)]
 
 
+class Clearance(ctypes.Structure):             # vie_hip_clearance (include/vie_hip.h)
+    _fields_ = [("worst_margin", ctypes.c_double), ("worst_gap", ctypes.c_double),
+                ("worst_required", ctypes.c_double), ("minimum_gap", ctypes.c_double),
+                ("required_gap", ctypes.c_double), ("worst_box", ctypes.c_longlong),
+                ("worst_conductor", ctypes.c_longlong), ("worst_is_patch", ctypes.c_int),
+                ("has_conductors", ctypes.c_int), ("insufficient", ctypes.c_int)]
+
+
 class Material(ctypes.Structure):              # dielectric.h:26-35
     _fields_ = [("id", ctypes.c_int), ("epsilon_r", ctypes.c_double), ("source_line", ctypes.c_size_t),
                 ("epsilon_loss", ctypes.c_double), ("conductivity", ctypes.c_double),
@@ -133,7 +141,8 @@ EXPORTS = (
     "vie_hip_time_volume_conductor_rows", "vie_hip_conductor_volume_columns",
     "vie_hip_time_conductor_volume_columns", "dielectric_fft_assemble_conductor_rows",
     "dielectric_fft_assemble_conductor_columns", "vie_hip_total_far_field", "vie_hip_total_radiated_power",
-    "vie_hip_release_workspaces",
+    "vie_hip_release_workspaces", "vie_hip_box_overlap", "vie_hip_time_box_overlap", "vie_hip_conductor_clearance",
+    "vie_hip_time_conductor_clearance",
 )
 
 _LIB = None
@@ -228,6 +237,12 @@ def lib() -> ctypes.CDLL:
                                                                                      i, vp]),
         "dielectric_fft_assemble_conductor_rows": (i, [vp, d, vp, vp, i, vp, vp, i, vp, vp, vp, vp, i]),
         "vie_hip_release_workspaces": (i, []),
+        "vie_hip_box_overlap": (i, [vp, vp, sz, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+        "vie_hip_time_box_overlap": (i, [vp, vp, sz, i, ctypes.POINTER(ctypes.c_longlong),
+                                         ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(d)]),
+        "vie_hip_conductor_clearance": (i, [vp, vp, vp, sz, i] + [vp] * 8 + [i, vp, vp, ctypes.POINTER(Clearance)]),
+        "vie_hip_time_conductor_clearance": (i, [vp, vp, vp, sz, i] + [vp] * 8 +
+                                             [i, vp, vp, i, ctypes.POINTER(Clearance), ctypes.POINTER(d)]),
         "vie_hip_total_far_field": (i, [d, d, i] + [vp] * 8 + [i, vp, vp, vp, sz, vp, vp, vp, i, sz, vp, vp, vp, vp]),
         "vie_hip_total_radiated_power": (i, [d, d, i] + [vp] * 8 + [i, vp, vp, vp, sz, vp, vp, vp, i, sz,
                                                                      ctypes.POINTER(d)]),
@@ -442,6 +457,63 @@ def total_radiated_power(k0, wlam, count=3896, segments=None, segment_current=No
     if not ok:
         raise RuntimeError(f"total radiated power failed: {last_error()}")
     return p.value
+
+
+def _corners(minimum, maximum):
+    mn = np.ascontiguousarray(minimum, dtype=np.float64).reshape(-1, 3)
+    mx = np.ascontiguousarray(maximum, dtype=np.float64).reshape(-1, 3)
+    if mn.shape != mx.shape:
+        raise ValueError("minimum and maximum corners differ in count")
+    return mn, mx
+
+
+def box_overlap(minimum, maximum, repeats=0):
+    """vie_hip_box_overlap: 0-based (left, right) of the first pair of boxes
+    whose interiors overlap in validate_box_relationships' loop order
+    (dielectric_mesh.c:81-123), or None.  repeats > 0 times the scan instead
+    and returns (pair, ms)."""
+    mn, mx = _corners(minimum, maximum)
+    a, b = ctypes.c_longlong(), ctypes.c_longlong()
+    if repeats:
+        ms = ctypes.c_double()
+        ok = lib().vie_hip_time_box_overlap(mn.ctypes.data, mx.ctypes.data, len(mn), int(repeats), ctypes.byref(a),
+                                            ctypes.byref(b), ctypes.byref(ms))
+    else:
+        ok = lib().vie_hip_box_overlap(mn.ctypes.data, mx.ctypes.data, len(mn), ctypes.byref(a), ctypes.byref(b))
+    if not ok:
+        raise RuntimeError(f"box overlap failed: {last_error()}")
+    pair = None if a.value < 0 else (a.value, b.value)
+    return (pair, ms.value) if repeats else pair
+
+
+def conductor_clearance(minimum, maximum, cell_longest, segments=None, patches=None, repeats=0):
+    """vie_hip_conductor_clearance: evaluate_clearance (dielectric_mesh.c:
+    215-316) of boxes against NEC wire segments (x, y, z, si, cab, sab, salp,
+    bi) and patches (points M x 3, areas).  Returns a dict of the Clearance
+    fields; repeats > 0 also returns the average ms."""
+    mn, mx = _corners(minimum, maximum)
+    cl = np.ascontiguousarray(cell_longest, dtype=np.float64).reshape(-1)
+    if len(cl) != len(mn):
+        raise ValueError("one cell_longest per box")
+    seg = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (segments or [[]] * 8)]
+    if len(seg) != 8 or len({len(a) for a in seg}) != 1:
+        raise ValueError("segments are eight arrays of one length")
+    pts = np.ascontiguousarray(patches[0] if patches else np.zeros((0, 3)), dtype=np.float64).reshape(-1, 3)
+    area = np.ascontiguousarray(patches[1] if patches else np.zeros(0), dtype=np.float64).reshape(-1)
+    if len(pts) != len(area):
+        raise ValueError("one area per patch point")
+    out = Clearance()
+    args = [mn.ctypes.data, mx.ctypes.data, cl.ctypes.data, len(mn), len(seg[0])] + [a.ctypes.data for a in seg] + \
+        [len(area), pts.ctypes.data, area.ctypes.data]
+    if repeats:
+        ms = ctypes.c_double()
+        ok = lib().vie_hip_time_conductor_clearance(*args, int(repeats), ctypes.byref(out), ctypes.byref(ms))
+    else:
+        ok = lib().vie_hip_conductor_clearance(*args, ctypes.byref(out))
+    if not ok:
+        raise RuntimeError(f"conductor clearance failed: {last_error()}")
+    r = {k: getattr(out, k) for k, _ in Clearance._fields_}
+    return (r, ms.value) if repeats else r
 
 
 def last_error() -> str:

@@ -224,6 +224,7 @@ def test_struct_layouts_match_c():
     assert ctypes.sizeof(vie.Material) == 64
     assert ctypes.sizeof(vie.GmresReport) == 48
     assert ctypes.sizeof(vie.SolveOptions) == 24
+    assert ctypes.sizeof(vie.Clearance) == 72
 
 
 def _arnoldi_hessenberg(A, b, m):
@@ -359,3 +360,20 @@ def test_conductor_volume_rejections_before_any_device_use():
 
 def test_release_workspaces_without_device_work():
     assert vie.lib().vie_hip_release_workspaces() == 1
+
+
+def test_mesh_check_rejections_before_any_device_use():
+    a, b = ctypes.c_longlong(7), ctypes.c_longlong(7)
+    assert vie.lib().vie_hip_box_overlap(None, None, 5, ctypes.byref(a), ctypes.byref(b)) == 0
+    assert "invalid" in vie.last_error()
+    lo, hi = np.zeros((2, 3)), np.ones((2, 3))
+    out = vie.Clearance()
+    assert vie.lib().vie_hip_conductor_clearance(lo.ctypes.data, hi.ctypes.data, None, 2, 0, *([None] * 8), 0,
+                                                 None, None, ctypes.byref(out)) == 0
+    with pytest.raises(ValueError):
+        vie.box_overlap(lo, hi[:1])
+    with pytest.raises(ValueError):
+        vie.conductor_clearance(lo, hi, [0.1])
+    # no conductors: nothing is evaluated and no device is needed (mesh->has_conductors = 0)
+    r = vie.conductor_clearance(lo, hi, [0.1, 0.1])
+    assert r["has_conductors"] == 0 and r["insufficient"] == 0 and r["worst_box"] == 0
