@@ -32,6 +32,21 @@ def test_first_pair_matches_oracle(count, seed):
     assert got == want
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_non_finite_corners_take_the_exact_scan(bad):
+    """A non-finite corner sends the scan to the IEEE build, whose fmax / fmin
+    treat NaN as the reference's C library does (the other operand)."""
+    lo, hi = lattice(8, 1.0)  # touching only: no pair overlaps until a corner goes bad
+    for corner, k, a in ((lo, 300, 0), (hi, 77, 1), (lo, 450, 2)):
+        c = corner.copy()
+        c[k, a] = bad
+        args = (c, hi) if corner is lo else (lo, c)
+        want = M.box_overlap(*args)
+        if np.isnan(bad):  # fmax / fmin take the other box's coordinate: new overlaps appear
+            assert want is not None
+        assert vie.box_overlap(*args) == want
+
+
 def test_no_overlap_in_a_voxel_lattice_and_planted_pairs():
     lo, hi = lattice(24)  # 13824 boxes, 9.6e7 pairs
     assert vie.box_overlap(lo, hi) is None
