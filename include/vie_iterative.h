@@ -1,13 +1,14 @@
 /*
  * Restarted GMRES contract of the VIE solver, C-ABI of libvie_hip.so.
  *
- * Replaces the reference's vie_iterative.h:11-62 symbol for symbol: the
- * callback operator type, the options / report structs, the status enum and
- * vie_gmres / vie_dense_gmres keep the reference's names, argument meaning
- * and error behaviour.  When vie_gmres is handed an operator produced by
- * vie_fft_operator or dielectric_fft_get_operator of this library, the whole
- * solve runs device-resident on the GPU (no per-iteration host round trip);
- * any other callback operator is iterated on the host exactly as before.
+ * Declares the reference's vie_iterative.h:11-62 types with its names, layouts
+ * and enum values (the callback operator, options, report and status), so the
+ * library's operators plug into nec2c's own vie_gmres, and this library's
+ * vie_gmres: the device-resident solve of an operator produced by
+ * vie_fft_operator or dielectric_fft_get_operator of this library, with the
+ * reference's argument meaning, report and return convention.  Other callback
+ * operators are refused (VIE_ITERATIVE_INVALID_INPUT): nec2c keeps its own
+ * vie_gmres and vie_dense_gmres (vie_iterative.c) for them.
  */
 #ifndef VIE_ITERATIVE_H
 #define VIE_ITERATIVE_H
@@ -63,14 +64,13 @@ typedef struct {
 void vie_gmres_default_options(vie_gmres_options *options);
 /* vie_iterative.c:35-48 */
 const char *vie_iterative_status_name(vie_iterative_status status);
-/* Right-preconditioned restarted GMRES (vie_iterative.c:80-199).  x is the
-   initial guess and is updated even on failure; success only on a recomputed
-   ||b - A x|| <= max(atol, rtol ||b||).  rhs and x must not alias. */
+/* Right-preconditioned restarted GMRES (vie_iterative.c:80-199) of this
+   library's operators, on the device.  x is the initial guess and is updated
+   even on failure; success only on a recomputed ||b - A x|| <=
+   max(atol, rtol ||b||).  rhs and x must not alias.  Returns 0 with status
+   VIE_ITERATIVE_INVALID_INPUT for an operator the library did not make. */
 int vie_gmres(const vie_linear_operator *op, const VIE_CDOUBLE *rhs, VIE_CDOUBLE *x,
               const vie_gmres_options *options, vie_gmres_report *report);
-/* Dense Jacobi-preconditioned adapter (vie_iterative.c:228-244). */
-int vie_dense_gmres(const VIE_CDOUBLE *matrix, size_t order, const VIE_CDOUBLE *rhs,
-                    VIE_CDOUBLE *x, const vie_gmres_options *options, vie_gmres_report *report);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,10 @@ def build(verbose: bool = True) -> str:
         objs = list(ex.map(_compile, SOURCES))
     out = os.path.join(LIB, "libvie_hip.so")
     tmp = out + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    # -Bsymbolic: the library's calls to its own exported names (vie_fft_create,
+    # dielectric_fft_prepare, ...) bind inside it, so a host program that also
+    # defines those names (nec2c with its CPU backends) never interposes them.
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

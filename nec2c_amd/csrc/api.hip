@@ -1805,177 +1805,6 @@ extern "C" const char* vie_iterative_status_name(vie_iterative_status s) {
   return "unknown status";
 }
 
-typedef std::complex<double> zc;
-
-// hypot-accumulated norm, NaN on a nonfinite entry (vie_iterative.c:9-19)
-static double hnorm(size_t n, const zc* x) {
-  double r = 0;
-  for (size_t i = 0; i < n; i++) {
-    if (!std::isfinite(x[i].real()) || !std::isfinite(x[i].imag())) return NAN;
-    r = hypot(r, std::abs(x[i]));
-  }
-  return r;
-}
-
-// Host iteration for callback operators that are not this library's (mock
-// operators, dense adapters): the reference's vie_gmres semantics.
-static int gmres_callbacks(const vie_linear_operator* op, const zc* b, zc* x, const vie_gmres_options* o,
-                           vie_gmres_report* rep) {
-  const size_t n = op->order;
-  size_t m = o->restart;
-  if (m > n) m = n;
-  if (m > o->maximum_iterations) m = o->maximum_iterations;
-  if (m == SIZE_MAX || n > SIZE_MAX / (m + 1) / sizeof(zc)) {
-    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
-    return 0;
-  }
-  std::vector<zc> V, Z, H, r, w, g, y, sn;
-  std::vector<double> cs;
-  try {
-    V.resize(n * (m + 1));
-    Z.resize(n * m);
-    H.resize((m + 1) * m);
-    r.resize(n);
-    w.resize(n);
-    g.resize(m + 1);
-    y.resize(m);
-    sn.resize(m);
-    cs.resize(m);
-  } catch (...) {
-    rep->status = VIE_ITERATIVE_ALLOCATION_FAILED;
-    return 0;
-  }
-  double bnorm = hnorm(n, b);
-  double target = std::max(o->absolute_tolerance, o->relative_tolerance * bnorm);
-  if (!std::isfinite(bnorm) || !std::isfinite(hnorm(n, x)) || !std::isfinite(target)) {
-    rep->status = VIE_ITERATIVE_NONFINITE;
-    return 0;
-  }
-  auto cb = [](vie_vector_map f, void* c, const zc* in, zc* out) {
-    return f(c, (const double _Complex*)in, (double _Complex*)out);
-  };
-  auto residual = [&]() -> int {
-    rep->operator_applications++;
-    if (!cb(op->apply, op->context, x, r.data())) {
-      rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-      return 0;
-    }
-    for (size_t i = 0; i < n; i++) r[i] = b[i] - r[i];
-    rep->residual_norm = hnorm(n, r.data());
-    if (!std::isfinite(rep->residual_norm)) {
-      rep->status = VIE_ITERATIVE_NONFINITE;
-      return 0;
-    }
-    return 1;
-  };
-  if (!residual()) return 0;
-  for (;;) {
-    size_t used = 0;
-    bool breakdown = false;
-    double beta = rep->residual_norm;
-    rep->relative_residual = bnorm > 0 ? beta / bnorm : beta;
-    if (beta <= target) {
-      rep->status = VIE_ITERATIVE_CONVERGED;
-      return 1;
-    }
-    if (rep->iterations >= o->maximum_iterations) {
-      rep->status = VIE_ITERATIVE_ITERATION_LIMIT;
-      return 0;
-    }
-    std::fill(H.begin(), H.end(), zc(0));
-    std::fill(g.begin(), g.end(), zc(0));
-    g[0] = beta;
-    for (size_t i = 0; i < n; i++) V[i] = r[i] / beta;
-    for (size_t j = 0; j < m && rep->iterations < o->maximum_iterations; j++) {
-      zc* vj = &V[j * n];
-      zc* zj = &Z[j * n];
-      if (op->precondition) {
-        if (!cb(op->precondition, op->precondition_context, vj, zj)) {
-          rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-          return 0;
-        }
-      } else
-        std::copy(vj, vj + n, zj);
-      if (!std::isfinite(hnorm(n, zj))) {
-        rep->status = VIE_ITERATIVE_NONFINITE;
-        return 0;
-      }
-      rep->operator_applications++;
-      if (!cb(op->apply, op->context, zj, w.data())) {
-        rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
-        return 0;
-      }
-      double before = hnorm(n, w.data());
-      if (!std::isfinite(before)) {
-        rep->status = VIE_ITERATIVE_NONFINITE;
-        return 0;
-      }
-      for (int pass = 0; pass < 2; pass++)
-        for (size_t k = 0; k <= j; k++) {
-          zc p = 0;
-          const zc* vk = &V[k * n];
-          for (size_t i = 0; i < n; i++) p += std::conj(vk[i]) * w[i];
-          H[k * m + j] += p;
-          for (size_t i = 0; i < n; i++) w[i] -= p * vk[i];
-        }
-      double next = hnorm(n, w.data());
-      if (!std::isfinite(next)) {
-        rep->status = VIE_ITERATIVE_NONFINITE;
-        return 0;
-      }
-      H[(j + 1) * m + j] = next;
-      breakdown = next <= 32 * DBL_EPSILON * before;
-      if (!breakdown)
-        for (size_t i = 0; i < n; i++) V[(j + 1) * n + i] = w[i] / next;
-      for (size_t k = 0; k < j; k++) {
-        zc f = H[k * m + j], sc = H[(k + 1) * m + j];
-        H[k * m + j] = cs[k] * f + sn[k] * sc;
-        H[(k + 1) * m + j] = -std::conj(sn[k]) * f + cs[k] * sc;
-      }
-      {
-        zc a = H[j * m + j], bb = H[(j + 1) * m + j];
-        double aa = std::abs(a), bv = std::abs(bb);
-        if (bv == 0.0) {
-          cs[j] = 1.0;
-          sn[j] = 0.0;
-        } else if (aa == 0.0) {
-          cs[j] = 0.0;
-          sn[j] = std::conj(bb) / bv;
-        } else {
-          double len = hypot(aa, bv);
-          cs[j] = aa / len;
-          sn[j] = (a / aa) * (std::conj(bb) / len);
-        }
-      }
-      H[j * m + j] = cs[j] * H[j * m + j] + sn[j] * H[(j + 1) * m + j];
-      H[(j + 1) * m + j] = 0.0;
-      g[j + 1] = -std::conj(sn[j]) * g[j];
-      g[j] *= cs[j];
-      rep->iterations++;
-      used = j + 1;
-      if (breakdown || std::abs(g[j + 1]) <= target) break;
-    }
-    for (size_t i = used; i-- > 0;) {
-      zc val = g[i];
-      for (size_t k = i + 1; k < used; k++) val -= H[i * m + k] * y[k];
-      if (std::abs(H[i * m + i]) == 0.0) {
-        rep->status = VIE_ITERATIVE_BREAKDOWN;
-        return 0;
-      }
-      y[i] = val / H[i * m + i];
-    }
-    for (size_t j = 0; j < used; j++)
-      for (size_t i = 0; i < n; i++) x[i] += Z[j * n + i] * y[j];
-    if (!residual()) return 0;
-    rep->relative_residual = bnorm > 0 ? rep->residual_norm / bnorm : rep->residual_norm;
-    if (breakdown && rep->residual_norm > target) {
-      rep->status = VIE_ITERATIVE_BREAKDOWN;
-      return 0;
-    }
-    if (rep->residual_norm > target && rep->iterations < o->maximum_iterations) rep->restarts++;
-  }
-}
-
 extern "C" int vie_gmres(const vie_linear_operator* op, const double _Complex* b, double _Complex* x,
                          const vie_gmres_options* options, vie_gmres_report* report) {
   vie_gmres_report local;
@@ -1995,45 +1824,13 @@ extern "C" int vie_gmres(const vie_linear_operator* op, const double _Complex* b
     if (!s) return 0;
     return solve_host(s, b, x, options, nullptr, report);
   }
-  return gmres_callbacks(op, (const zc*)b, (zc*)x, options, report);
-}
-
-// Dense Jacobi-preconditioned adapter (vie_iterative.c:201-244): host callbacks.
-struct DenseCtx {
-  const zc* a;
-  size_t n;
-};
-static int dense_apply(void* ctx, const double _Complex* x, double _Complex* y) {
-  const DenseCtx* d = (const DenseCtx*)ctx;
-  const zc* xv = (const zc*)x;
-  zc* yv = (zc*)y;
-  for (size_t i = 0; i < d->n; i++) {
-    zc s = 0;
-    for (size_t j = 0; j < d->n; j++) s += d->a[i * d->n + j] * xv[j];
-    yv[i] = s;
-  }
-  return 1;
-}
-static int dense_diag(void* ctx, const double _Complex* x, double _Complex* y) {
-  const DenseCtx* d = (const DenseCtx*)ctx;
-  const zc* xv = (const zc*)x;
-  zc* yv = (zc*)y;
-  for (size_t i = 0; i < d->n; i++) {
-    zc dg = d->a[i * d->n + i];
-    if (std::abs(dg) == 0.0) return 0;
-    yv[i] = xv[i] / dg;
-  }
-  return 1;
-}
-extern "C" int vie_dense_gmres(const double _Complex* a, size_t n, const double _Complex* b, double _Complex* x,
-                               const vie_gmres_options* options, vie_gmres_report* report) {
-  if (!a || n == 0 || n > SIZE_MAX / n || n * n > SIZE_MAX / sizeof(zc)) {
-    if (report) report_init(report);
-    return 0;
-  }
-  DenseCtx ctx = {(const zc*)a, n};
-  vie_linear_operator op = {n, dense_apply, &ctx, dense_diag, &ctx};
-  return vie_gmres(&op, b, x, options, report);
+  // Other callback operators are not this library's to iterate: nec2c keeps
+  // its own vie_gmres (vie_iterative.c:80-199) for them, and the dispatcher
+  // that adds the "hip" backend sends only this library's operators here.
+  report->status = VIE_ITERATIVE_INVALID_INPUT;
+  set_error("vie_gmres: the operator was not made by vie_fft_operator or dielectric_fft_get_operator "
+            "of this library; solve it with the host program's own vie_gmres");
+  return 0;
 }
 
 // ------------------------------------------------------------------ measurement hooks

@@ -107,12 +107,15 @@ class Server:
         sys_bytes = 0 if req.key in self.cache else system_device_bytes(req.shape, req.conductors)
         return sys_bytes + len(req.rhs) * self._job_bytes(req)
 
-    def _evict_for(self, need):
+    def _evict_for(self, need, keep=None):
+        """Closes idle cached systems, least recently used first, until `need`
+        more bytes fit.  The system of key `keep` (the request being admitted,
+        whose footprint counted it as cached) is never evicted."""
         for key in list(self.cache):
             if self.used + need <= self.budget:
                 return True
             s, nbytes, users = self.cache[key]
-            if users == 0:
+            if users == 0 and key != keep:
                 s.close()
                 del self.cache[key]
                 self.used -= nbytes
@@ -166,7 +169,7 @@ class Server:
         while self.queue:
             req = self.queue[0]
             need = self._footprint(req)
-            if self.used + need > self.budget and not self._evict_for(need):
+            if self.used + need > self.budget and not self._evict_for(need, keep=req.key):
                 if not self.inflight and need > self.budget:
                     self.rejected.append(self.queue.popleft())  # can never fit
                     continue
@@ -182,7 +185,7 @@ class Server:
     def run(self, requests: Sequence[Request]):
         """Serves `requests` (sorted by arrival) to completion in real time."""
         arrived = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        total = len(arrived)
+        total = len(arrived) + len(self.done) + len(self.rejected)
         self.t0 = self.clock()
         while len(self.done) + len(self.rejected) < total:
             if not self.step(arrived) and not self.inflight and not self.queue and arrived:

@@ -118,7 +118,7 @@ class MeshResult(ctypes.Structure):            # dielectric.h:147-178
 
 # every symbol the public headers declare (include/*.h)
 EXPORTS = (
-    "vie_gmres_default_options", "vie_iterative_status_name", "vie_gmres", "vie_dense_gmres",
+    "vie_gmres_default_options", "vie_iterative_status_name", "vie_gmres",
     "vie_fft_backend_set", "vie_fft_fftw_available", "vie_fft_backend_requested_name",
     "vie_fft_backend_name", "vie_fft_backend_last_used_name", "vie_fft_backend_clear_last_used",
     "vie_fft_create", "vie_fft_set_permittivity", "vie_fft_destroy", "vie_fft_bytes",
@@ -163,7 +163,6 @@ def lib() -> ctypes.CDLL:
         "vie_iterative_status_name": (ctypes.c_char_p, [i]),
         "vie_gmres": (i, [ctypes.POINTER(LinearOperator), vp, vp, ctypes.POINTER(GmresOptions),
                           ctypes.POINTER(GmresReport)]),
-        "vie_dense_gmres": (i, [vp, sz, vp, vp, ctypes.POINTER(GmresOptions), ctypes.POINTER(GmresReport)]),
         "vie_fft_backend_set": (i, [ctypes.c_char_p]),
         "vie_fft_fftw_available": (i, []),
         "vie_fft_backend_requested_name": (ctypes.c_char_p, []),

@@ -1,6 +1,8 @@
 """The drop-in boundary from C: a C caller written against include/*.h (the
 reference's vie_iterative.h / vie_fft.h / dielectric_fft.h names) compiles
-with gcc, links libvie_hip.so and runs the host-side API (no device)."""
+with gcc, links libvie_hip.so and runs the host-side API (no device): the
+reference's constants, backend names, a rejected plan, and a callback operator
+of the caller's own refused by vie_gmres (the host program keeps its own)."""
 import os
 import subprocess
 
@@ -38,14 +40,14 @@ int main(void)
   stencil(&n, e, b);
   vie_gmres_default_options(&o);
   o.restart = 3;
-  if(!vie_gmres(&op, b, x, &o, &r)) return 1;
-  for(i = 0; i < n; i++) if(cabs(x[i] - e[i]) > 1e-10) return 2;
+  /* a callback operator of the caller's own is the host program's to solve */
+  if(vie_gmres(&op, b, x, &o, &r) || r.status != VIE_ITERATIVE_INVALID_INPUT) return 1;
+  for(i = 0; i < n; i++) if(x[i] != 0) return 2;
   if(dielectric_gmres_settings.restart != 30) return 3;
   if(strcmp(dielectric_fft_backend(s), "none")) return 4;
   if(vie_fft_backend_set("fftw") || !vie_fft_backend_set("hip")) return 5;
   if(vie_fft_create(&p, shape, 1, xyz, .25, 5.0, 3.0, 1000000) || p) return 6;   /* k h > 1 */
-  printf("C ABI ok: %zu iterations, %zu restarts, status %s\n", r.iterations, r.restarts,
-         vie_iterative_status_name(r.status));
+  printf("C ABI ok: foreign operator refused (%s)\n", vie_iterative_status_name(r.status));
   return 0;
 }
 """

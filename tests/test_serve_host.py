@@ -124,3 +124,25 @@ def test_run_batch_keeps_inflight_cap(delay, monkeypatch):
     res, secs = serve.run_batch(jobs, inflight=3, batch=fb)
     assert len(res) == 10 and all(r[0] for r in res)
     assert fb.in_flight_max <= 3
+
+
+def test_returning_key_that_is_lru_is_not_evicted_for_itself():
+    """A request whose key is cached (and least recently used) must not evict
+    its own system to make room: the footprint counted that system as resident,
+    so evicting it would rebuild it and push the accounted bytes over budget."""
+    built = []
+    n = 16
+    sys_b = serve.system_device_bytes((n,) * 3, 21)
+    job_b = serve.job_device_bytes((n,) * 3, 21, 120)
+    srv = make_server(budget=2 * sys_b + 2 * job_b - 1)   # K1 + K2 cached fit, plus one job
+    rep = srv.run([request(0, "K1", n, 1, built), request(1, "K2", n, 1, built, arrival=0.01)])
+    assert rep["requests"] == 2 and built == ["K1", "K2"]
+    assert list(srv.cache) == ["K1", "K2"]           # K1 is the LRU entry now
+    used_max = []
+    step = srv.step
+    srv.step = lambda arrived: (step(arrived), used_max.append(srv.used))[0]
+    srv.run([request(2, "K1", n, 2, built)])         # two ports: K2 (idle) must go, K1 must stay
+    assert built == ["K1", "K2"]                     # one build each: K1 came from the cache
+    assert srv.done[-1].cache_hit
+    assert max(used_max) <= srv.budget
+    assert list(srv.cache) == ["K1"]
