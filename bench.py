@@ -1,21 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X VIE FFT-GMRES on the 64^3 random-permittivity sweep.
+"""Benchmark of the MI355X VIE FFT-GMRES.
 
-Workload (one "step" = one frequency point solved to a 1e-8 true residual):
-64 x 64 x 64 voxel block, edge 0.5 m, eps_r per voxel from a seeded Gaussian
-random field (correlation 4 voxels) mapped to [1.5, 10], loss tangent U[0, 0.05];
-a z-directed 0.48 m, 21-segment, 1 mm dipole with a 1 V centre delta gap, 0.1 m
-from the x = 0 face; frequencies from the 101-point 100-400 MHz sweep.  The
-operator is the reference's coupled [C B; D V] (c = 21, v = 786432); C/B/D are
+Default workload, the headline (one "step" = one mixed-precision solve to a
+1e-8 true residual from x0 = 0): a 128 x 128 x 128 voxel slab (1 cm voxels,
+eps_r 6, 0.01 S/m; 6.29 M volume unknowns, FFT grid padded to 256^3) driven by
+a 4-element wire Yagi (84 segments) 0.1 m from its x = 0 face, at 200 MHz, in
+free space.  The operator is the reference's coupled [C B; D V]; C/B/D are
 synthetic blocks of the reference's shape (nec2c_amd/workloads.py).
 
 Reported: mixed-precision (FP32 cycles, FP64 true residual) solves/s as the
-headline value, FP64 solves/s beside it, matvecs/s, p50/p99 solve latency, the
-roofline of the dominant kernel (X pass) from HIP events on the system stream,
-and the reference's own CPU FFT-GMRES (oracle/_ref) on a bounded sample.
+value, FP64 solves/s beside it (the reference's restarted GMRES, and the same
+recycled subspace as the mixed solve), matvecs/s, p50/p99 solve latency, the
+roofline of the fused matvec (algorithmic bytes of the pruned five-pass model
+plus the cross blocks, over HIP-event times of whole applies on the system
+stream), the roofline of the dominant Arnoldi kernel, each GPU solution's
+true residual on the reference's own operator (oracle/_ref), and the
+reference's CPU FFT-GMRES on the same system on a bounded sample.
+
+--workload sweep: the 64^3 random-permittivity block + 21-segment dipole over
+the 100-400 MHz sweep (one step = one frequency point); the other workloads
+(batch, serve, matvec, farfield, rows, columns, cross_slab, point, mesh) are
+listed by --help.
 
 All inputs (spectrum, permittivity, C/B/D, right-hand side) are resident in
-HBM before the timed region; each timed solve starts from x0 = 0.
+HBM before the timed region.
 """
 from __future__ import annotations
 
@@ -49,6 +57,20 @@ def profiled_traffic(kernel):
         if k.replace(" ", "") == name:
             return v["bytes_per_launch"]
     return None
+
+
+def profiled_traffic_sum(workload, precision):
+    """HBM bytes of one fused matvec (the sum over its kernels of FETCH_SIZE
+    and WRITE_SIZE per launch, each scaled by its reading on a copy of known
+    size) from the committed counter summary profiles/hbm_traffic_<workload>.json;
+    None when that workload was not profiled."""
+    try:
+        with open(os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")) as f:
+            return json.load(f)["matvec_bytes"][precision]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 METRIC = "VIE matvecs/s & solves/s at 1e-8 residual, mixed vs FP64 GPU; p50/p99 latency"
 
 
@@ -119,16 +141,18 @@ def cpu_baseline(n, pitch, eps, case, iterations_needed, restart, sample_iters=1
     t_iters = time.perf_counter() - t2
     del y
     per_it = t_iters / max(rep.iterations, 1)
-    total = t_build + per_it * iterations_needed
+    total = per_it * iterations_needed   # preparation excluded, as in the GPU value
     return {
         "value": 1.0 / total,
         "unit": "solves/s",
         "cores": cores,
         "kind": "reference",
         "sample": (f"1 frequency point ({k0 * 299792458.0 / 2 / math.pi / 1e6:.1f} MHz) of the same {n}^3 coupled "
-                   f"system: reference vie_fft_create (builtin FFT, OpenMP) {t_build:.2f} s + the first "
+                   f"system on the reference's FFT operator (builtin FFT, OpenMP): the first "
                    f"{rep.iterations} vie_gmres({restart}) iterations {t_iters:.2f} s, scaled to the "
-                   f"{iterations_needed} iterations the same algorithm needs there; one matvec {t_mv * 1e3:.1f} ms; "
+                   f"{iterations_needed} iterations the same algorithm needs there; like the GPU value it "
+                   f"excludes preparation (reference vie_fft_create {t_build:.2f} s, build_s); one matvec "
+                   f"{t_mv * 1e3:.1f} ms; "
                    "[C B; D] products restated around the reference volume operator (its prepare caps at 512 MiB)"),
         "matvec_ms": t_mv * 1e3,
         "build_s": t_build,
@@ -359,115 +383,291 @@ def bench_serve(args):
     return 0
 
 
-def bench_slab(args):
-    """The 128^3 slab: a homogeneous block (eps_r 6, sigma 0.01 S/m, 1 cm
-    voxels, 1.28 m edge) driven by a 4-element Yagi (reflector 0.50 m, driven
-    0.47 m, directors 0.44 m and 0.43 m, 21 segments each, 0.2 m spacing) 0.1 m
-    from its x = 0 face, at 200 MHz; one mixed and one FP64 solve to 1e-8
-    after a warm-up solve of each.  The reference's FFT path has no real-ground
-    kernel (dielectric_ground.c:28-29), so the slab is in free space here."""
-    from nec2c_amd import vie, workloads as W
-    n = args.slab_size
-    h = 0.01
-    f = 200e6
-    k0 = W.k0_of(f)
-    edge = n * h
-    yc, zc = 0.5 * edge, 0.5 * edge
+SLAB_H = 0.01          # voxel pitch (m): 1.28 m edge at 128^3
+SLAB_FREQ = 200e6
+
+
+def slab_geometry(n):
+    """The slab workload's inputs: 1 cm voxels, eps_r 6 with 0.01 S/m, and a
+    4-element Yagi (reflector 0.50 m, driven 0.47 m, directors 0.44 m and
+    0.43 m, 21 segments each, 1 mm radius, 0.2 m spacing) 0.1 m from the x = 0
+    face, at 200 MHz."""
+    from nec2c_amd import workloads as W
+    edge = n * SLAB_H
+    yc = zc = 0.5 * edge
     yagi = W.WireArray(tuple((-0.1, yc + (i - 1.5) * 0.2, zc, L) for i, L in enumerate((0.50, 0.47, 0.44, 0.43))),
                        segments_per_wire=21, radius=1e-3, feed_wire=1)
-    opts = dict(restart=args.restart, maximum_iterations=args.max_iterations, relative_tolerance=1e-8)
+    k0 = W.k0_of(SLAB_FREQ)
+    # eps_r - i sigma / (omega eps0) with the reference's constants (dielectric.h:55), as the
+    # library evaluates the material: the reference plan of the CPU leg must see the same value
+    eps_cell = 6.0 - 1j * 0.01 / (k0 * 299792458.0 * 8.8541878128e-12)
+    return yagi, k0, eps_cell
+
+
+def slab_system(n, opts, keep_blocks=False):
+    """Prepared GPU system of the slab (spectrum, C/B/D, rhs, reserved basis).
+    Returns (system, yagi, k0, eps_cell, blocks or None, prepare seconds)."""
+    from nec2c_amd import vie, workloads as W
+    yagi, k0, eps_cell = slab_geometry(n)
     t0 = time.perf_counter()
-    centers = W.cube_cells(n, h)
-    s = vie.System(centers, [vie.MaterialSpec(1, 6.0, 0.0, 0.01)], yagi.segments, k0, pitch=h)
+    centers = W.cube_cells(n, SLAB_H)
+    s = vie.System(centers, [vie.MaterialSpec(1, 6.0, 0.0, 0.01)], yagi.segments, k0, pitch=SLAB_H)
     t_prep = time.perf_counter() - t0
     log(f"slab: {n}^3 prepared (spectrum) in {t_prep:.1f} s; building Yagi blocks (c = {yagi.segments})")
-    eps_cell = 6.0 - 1j * 0.01 / (k0 * 299792458.0 * W.EPS0)
     C = yagi.impedance_matrix(k0)
-    B, D = W.coupling_blocks(yagi, centers, h, eps_cell, k0, threads=16)
+    B, D = W.coupling_blocks(yagi, centers, SLAB_H, eps_cell, k0, threads=16)
     s.set_blocks(C=C, B=B, D=D)
-    del B, D, centers
     b = np.zeros(s.order, dtype=np.complex128)
     b[: yagi.segments] = yagi.rhs()
     s.upload_rhs(b)
     s.reserve(**opts)
     log(f"slab: ready in {time.perf_counter() - t0:.1f} s")
-    res = {}
-    for prec, tag in ((vie.MIXED, "mixed"), (vie.FP64, "fp64")):
-        s.solve_resident(prec, args.inner_reduction, args.recycle, **opts)  # warm-up
+    blocks = (C, B, D, b) if keep_blocks else None
+    return s, yagi, k0, eps_cell, blocks, t_prep
+
+
+def matvec_bytes_bmv(n, P, c, S):
+    """Algorithmic bytes of one fused coupled matvec by the pruned five-pass
+    model: E_vec = 3 (2N + 4 nx ny Pz + 4 nx Py Pz) vector traffic, E_aux =
+    3N (x read again for the contrast) + N (permittivity), E_spec = the octant
+    spectrum 6 (Px/2+1)(Py/2+1)(Pz/2+1) that the multiply reads, E_cross =
+    c^2 + 2 c 3N (C, B and D streamed once); S bytes per complex value."""
+    N = n ** 3
+    e_vec = 3 * (2 * N + 4 * n * n * P + 4 * n * P * P)
+    e_aux = 3 * N + N
+    e_spec = 6 * (P // 2 + 1) ** 3
+    e_cross = c * c + 2 * c * 3 * N
+    return {"volume": (e_vec + e_aux + e_spec) * S, "cross": e_cross * S,
+            "total": (e_vec + e_aux + e_spec + e_cross) * S,
+            "spectrum_full_6G": 6 * P ** 3 * S}
+
+
+def slab_cpu_leg(n, k0, eps_cell, blocks, x_sols, iterations_needed, restart, sample_iters):
+    """The reference's own FFT operator (oracle/_ref: vie_fft.c, vie_kernel.c,
+    vie_iterative.c built from its sources with OpenMP) on the slab:
+    vie_fft_create at 128^3 (timed, reported beside the value), the true
+    relative residual of each GPU solution on that operator with the same
+    [C B; D] blocks, and the first ``sample_iters`` vie_gmres(restart)
+    iterations of the same system, scaled to ``iterations_needed``."""
+    from oracle import oracle as O
+    if not O.reference_available():
+        return None, {}
+    R = O.ReferenceLibrary()
+    cores = R.threads(0)
+    C, B, D, b = blocks
+    xyz = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    t0 = time.perf_counter()
+    plan = R.plan((n, n, n), xyz, SLAB_H, k0, eps_cell)
+    t_build = time.perf_counter() - t0
+    log(f"cpu: reference vie_fft_create at {n}^3 in {t_build:.1f} s on {cores} threads")
+    rc = plan.coupled(C, B, D)
+    bnorm = float(np.linalg.norm(b))
+    resid = {}
+    t_mv = None
+    for tag, x in x_sols.items():
+        t1 = time.perf_counter()
+        r = b - rc.apply(x)
+        t_mv = time.perf_counter() - t1
+        resid[tag] = float(np.linalg.norm(r) / bnorm)
+        log(f"cpu: {tag} solution, relative residual on the reference operator {resid[tag]:.3e}")
+    t2 = time.perf_counter()
+    ok, x, rep = rc.gmres(b, restart=restart, maxit=sample_iters, rtol=1e-8)
+    t_iters = time.perf_counter() - t2
+    per_it = t_iters / max(rep.iterations, 1)
+    cb = {
+        "value": 1.0 / (per_it * iterations_needed),
+        "unit": "solves/s",
+        "cores": cores,
+        "kind": "reference",
+        "sample": (f"the same {n}^3 slab + Yagi system on the reference's FFT operator (builtin FFT, OpenMP, "
+                   f"FP64): the first {rep.iterations} vie_gmres({restart}) iterations took {t_iters:.1f} s, "
+                   f"scaled to the {iterations_needed} iterations the same FP64 algorithm needs on the GPU "
+                   f"(optimistic for the CPU: later steps orthogonalise against more vectors); like the GPU "
+                   f"value it excludes preparation (reference vie_fft_create {t_build:.1f} s, reported as "
+                   f"build_s); [C B; D] products restated around the reference volume operator, whose own "
+                   f"prepare caps storage at 512 MiB"),
+        "build_s": t_build,
+        "matvec_s": t_mv,
+        "seconds_per_iteration": per_it,
+        "iterations_sampled": rep.iterations,
+        "iterations_scaled_to": iterations_needed,
+    }
+    return cb, resid
+
+
+def bench_slab(args):
+    """The 128^3 slab (the headline): eps_r 6, 0.01 S/m, 1 cm voxels, driven
+    by a 4-element Yagi 0.1 m from its x = 0 face, at 200 MHz.  One step is one
+    mixed-precision solve of that system to a 1e-8 true residual from x0 = 0.
+    FP64 solves (the reference's restarted GMRES, and the same recycled
+    subspace as the mixed solve) are timed beside it.  The reference's FFT
+    path rejects a real ground (dielectric_ground.c:28-29), so the slab is in
+    free space."""
+    from nec2c_amd import vie
+    n = args.slab_size
+    steps, warmup = args.steps, args.warmup
+    opts = dict(restart=args.restart, maximum_iterations=args.max_iterations, relative_tolerance=1e-8)
+    s, yagi, k0, eps_cell, blocks, t_prep = slab_system(n, opts, keep_blocks=not args.no_cpu)
+    c = yagi.segments
+
+    def timed(prec, recycle, count, warm):
+        for _ in range(warm):
+            s.solve_resident(prec, args.inner_reduction, recycle, **opts)
         vie.synchronize()
+        lat, reps = [], []
         t = time.perf_counter()
-        ok, rep = s.solve_resident(prec, args.inner_reduction, args.recycle, **opts)
+        for _ in range(count):
+            a = time.perf_counter()
+            ok, rep = s.solve_resident(prec, args.inner_reduction, recycle, **opts)
+            lat.append(time.perf_counter() - a)
+            if not ok:
+                raise RuntimeError(f"slab solve failed: {vie.STATUS[rep.status]} rel {rep.relative_residual}")
+            reps.append(rep)
         vie.synchronize()
-        dt = time.perf_counter() - t
-        res[tag] = (ok, rep, dt, s.download_solution())
-        log(f"slab {tag}: {'converged' if ok else 'NOT converged'} in {rep.iterations} iterations, {dt:.3f} s, "
-            f"relative residual {rep.relative_residual:.2e}")
+        return time.perf_counter() - t, lat, reps
+
+    log(f"slab: {warmup} warm-up + {steps} timed mixed solves")
+    s.solve_stats(reset=True)
+    t_mixed, lat_mixed, rep_mixed = timed(vie.MIXED, args.recycle, steps, warmup)
+    mean_col, _ = s.solve_stats(reset=True)
+    rec_k = s.recycled
+    x_mixed = s.download_solution()
+    log(f"slab mixed: {rep_mixed[0].iterations} iterations, {t_mixed / steps:.3f} s per solve")
+    fsteps = max(1, min(args.fp64_steps, steps))
+    t_fp64, lat_fp64, rep_fp64 = timed(vie.FP64, 0, fsteps, 1)
+    x_fp64 = s.download_solution()
+    log(f"slab fp64: {rep_fp64[0].iterations} iterations, {t_fp64 / fsteps:.3f} s per solve")
+    k_r = args.recycle if args.recycle > 0 else 20
+    t_fp64r, _, rep_fp64r = timed(vie.FP64, k_r, fsteps, 1)
+    log(f"slab fp64 recycled: {rep_fp64r[0].iterations} iterations, {t_fp64r / fsteps:.3f} s per solve")
+
+    # fused matvec: HIP events around whole applies on the system's stream
+    variants = s.tune()
     ms32, p32 = s.time_apply(vie.MIXED, args.matvec_repeats)
     ms64, p64 = s.time_apply(vie.FP64, args.matvec_repeats)
-    by32, pb32 = s.apply_bytes(vie.MIXED)
-    by64, _ = s.apply_bytes(vie.FP64)
+    lib32, pb32 = s.apply_bytes(vie.MIXED)
+    P = 2 * n
+    bmv32 = matvec_bytes_bmv(n, P, c, 8)
+    bmv64 = matvec_bytes_bmv(n, P, c, 16)
+    names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
+    gbs = bmv32["total"] / (ms32 * 1e-3) / 1e9
+    # the solve's streaming Arnoldi kernels at the mean basis column of the mixed solves
+    jbar = max(1, int(round(mean_col)))
+    arn = s.time_arnoldi(vie.MIXED, jbar, args.matvec_repeats)
+    arn["gbs_reduce"] = arn["bytes_reduce"] / (arn["ms_reduce"] * 1e-3) / 1e9
+    arn["gbs_update"] = arn["bytes_update"] / (arn["ms_update"] * 1e-3) / 1e9
+
     fi = yagi.feed_index()
-    zm, zd = 1 / res["mixed"][3][fi], 1 / res["fp64"][3][fi]
-    gain_db = gain_difference_db(res["mixed"][3], res["fp64"][3], k0, W.cube_cells(n, h), h, yagi.points(),
+    zm, zd = 1 / x_mixed[fi], 1 / x_fp64[fi]
+    i_err = float(np.max(np.abs(x_mixed[:c] - x_fp64[:c])) / np.max(np.abs(x_fp64[:c])))
+    from nec2c_amd import workloads as W
+    gain_db = gain_difference_db(x_mixed, x_fp64, k0, W.cube_cells(n, SLAB_H), SLAB_H, yagi.points(),
                                  yagi.deltas())
     log(f"slab far-field gain, mixed vs FP64: max {gain_db:.2e} dB over a 5-degree grid")
-    names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
+    traffic = profiled_traffic_sum(f"{n}slab_matvec", "fp32")
     result = {
         "metric": METRIC,
-        "value": 1.0 / res["mixed"][2],
+        "value": steps / t_mixed,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": 1,
-        "warmup": 1,
-        "ms_per_step": 1e3 * res["mixed"][2],
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * t_mixed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 inner (fp32 complex) / c128 outer (fp64 complex)",
         "data": "synthetic",
-        "config": {"workload": f"{n}^3 slab (eps_r 6, 0.01 S/m, h 1 cm) in free space + 4-element 84-segment Yagi "
-                               f"0.1 m away, 200 MHz, GMRES({args.restart}) to 1e-8",
-                   "unknowns": s.order, "conductors": yagi.segments, "padded": [2 * n, 2 * n, 2 * n]},
-        "fp64_solves_per_s": 1.0 / res["fp64"][2],
-        "speedup_mixed_vs_fp64": res["fp64"][2] / res["mixed"][2],
-        "iterations_mixed": res["mixed"][1].iterations,
-        "iterations_fp64": res["fp64"][1].iterations,
-        "residual_mixed": res["mixed"][1].relative_residual,
-        "residual_fp64": res["fp64"][1].relative_residual,
-        "converged": [bool(res["mixed"][0]), bool(res["fp64"][0])],
+        "config": {"workload": (f"{n}^3 voxel slab (eps_r 6, 0.01 S/m, 1 cm voxels, {s.order} unknowns, FFT grid "
+                                f"padded to {P}^3) + 4-element 84-segment wire Yagi 0.1 m away, 200 MHz, "
+                                f"GMRES({args.restart}) to 1e-8; in FREE SPACE, not over the real ground the "
+                                f"configuration names: the reference's FFT path rejects real ground "
+                                f"(dielectric_ground.c:28-29)"),
+                   "grid": [n, n, n], "padded": [P, P, P], "unknowns": s.order, "conductors": c},
+        "fp64_solves_per_s": fsteps / t_fp64,
+        "speedup_mixed_vs_fp64": (t_fp64 / fsteps) / (t_mixed / steps),
+        "fp64_recycled_solves_per_s": fsteps / t_fp64r,
+        "speedup_mixed_vs_fp64_recycled": (t_fp64r / fsteps) / (t_mixed / steps),
+        "recycled_subspace_mixed": rec_k,
+        "matvecs_per_s_mixed": 1e3 / ms32,
+        "matvecs_per_s_fp64": 1e3 / ms64,
+        "solve_matvecs_per_s_mixed": sum(r.operator_applications for r in rep_mixed) / t_mixed,
+        "p50_ms_mixed": 1e3 * float(np.percentile(lat_mixed, 50)),
+        "p99_ms_mixed": 1e3 * float(np.percentile(lat_mixed, 99)),
+        "p50_ms_fp64": 1e3 * float(np.percentile(lat_fp64, 50)),
+        "p99_ms_fp64": 1e3 * float(np.percentile(lat_fp64, 99)),
+        "iterations_mixed": rep_mixed[0].iterations,
+        "iterations_fp64": rep_fp64[0].iterations,
+        "iterations_fp64_recycled": rep_fp64r[0].iterations,
+        "residual_mixed": max(r.relative_residual for r in rep_mixed),
+        "residual_fp64": max(r.relative_residual for r in rep_fp64),
         "feed_impedance_ohm_fp64": str(complex(np.round(zd, 6))),
         "feed_impedance_rel_err_mixed_vs_fp64": float(abs(zm - zd) / abs(zd)),
+        "conductor_current_rel_err_mixed_vs_fp64": i_err,
         "far_field_gain_max_db_mixed_vs_fp64": gain_db,
-        "matvec": {"ms_mixed": ms32, "ms_fp64": ms64, "bytes_mixed": by32, "bytes_fp64": by64,
-                   "effective_gbs_mixed": by32 / (ms32 * 1e-3) / 1e9, "effective_gbs_fp64": by64 / (ms64 * 1e-3) / 1e9,
+        "matvec": {"ms_mixed": ms32, "ms_fp64": ms64,
+                   "bytes_bmv_mixed": bmv32, "bytes_bmv_fp64": bmv64,
+                   "gbs_bmv_fp64": bmv64["total"] / (ms64 * 1e-3) / 1e9,
                    "pass_ms_mixed": dict(zip(names, p32)), "pass_ms_fp64": dict(zip(names, p64)),
-                   "pass_bytes_mixed": dict(zip(names, pb32))},
-        "setup_seconds": {"prepare": t_prep},
-        "roofline": None,
+                   "pass_bytes_mixed": dict(zip(names, pb32)),
+                   "pass_gbs_mixed": {k: b / (t * 1e-3) / 1e9 for k, b, t in zip(names, pb32, p32) if t > 0},
+                   "tuned_variants": variants},
+        "setup_seconds": {"prepare_with_spectrum": t_prep},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": ("fused coupled matvec, FP32: five volume passes (k_zfwd, k_ypass, k_xpass, k_ypass "
+                       "inverse, k_zinv) + conductor coupling (k_dots, k_couple_out), HIP events around "
+                       f"{args.matvec_repeats} whole applies on the system stream"),
+            "achieved": gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": bmv32["total"],
+        },
+        "roofline_solve_kernel": {
+            "bound": "hbm",
+            "kernel": f"k_dots_dc<float> (delayed-CGS2 reduction at basis column {jbar}, the mean of the mixed "
+                      f"solves; ({jbar} + 2) x {s.order} complex float per launch)",
+            "achieved": arn["gbs_reduce"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": arn["gbs_reduce"] / HBM_PEAK_GBS, "update_gbs": arn["gbs_update"],
+            "ms_reduce": arn["ms_reduce"], "ms_update": arn["ms_update"],
+        },
         "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        log("CPU baseline: the reference's FFT operator at the same size")
+        cb, resid = slab_cpu_leg(n, k0, eps_cell, blocks, {"mixed": x_mixed, "fp64": x_fp64},
+                                 rep_fp64[0].iterations, args.restart, args.cpu_iterations)
+        result["cpu_baseline"] = cb
+        result["residual_on_reference_operator"] = resid
     print(json.dumps(result), flush=True)
     s.close()
     return 0
 
 
 def bench_matvec(args):
-    """Fused operator applies of the first timed sweep point (FP32 and FP64,
-    resident input, Jacobi fused) and a plain device copy of known size, the
-    calibration kernel for HBM byte counters.  Short by design: counter runs
-    replay every kernel."""
+    """Fused operator applies (FP32 and FP64, resident input, Jacobi fused) of
+    the first timed sweep point (--geometry sweep) or of the slab (--geometry
+    slab), and a plain device copy of known size, the calibration kernel for
+    HBM byte counters.  Short by design: counter runs replay every kernel."""
     from nec2c_amd import vie, workloads as W
-    n = args.size
-    pitch = 0.5 / n
-    eps = W.random_permittivity((n, n, n), seed=2)
-    centers = W.cube_cells(n, pitch)
-    dipole = W.Dipole(center=(-0.1, 0.25, 0.25))
-    f = sweep_frequencies(args.steps + args.warmup)[args.warmup]
     opts = dict(restart=4, maximum_iterations=4, relative_tolerance=1e-8)
-    s, _ = build_case(n, f, eps, dipole, None, centers, pitch, opts)
+    if args.geometry == "slab":
+        n = args.slab_size
+        s, _, _, _, _, _ = slab_system(n, opts)
+        what = f"{n}^3 slab + 84-segment Yagi, 200 MHz"
+    else:
+        n = args.size
+        pitch = 0.5 / n
+        eps = W.random_permittivity((n, n, n), seed=2)
+        centers = W.cube_cells(n, pitch)
+        dipole = W.Dipole(center=(-0.1, 0.25, 0.25))
+        f = sweep_frequencies(args.steps + args.warmup)[args.warmup]
+        s, _ = build_case(n, f, eps, dipole, None, centers, pitch, opts)
+        what = f"{n}^3 sweep point {f / 1e6:.1f} MHz"
     variants = s.tune()
-    out = {"workload": f"{n}^3 sweep point {f / 1e6:.1f} MHz, fused apply (Jacobi) x {args.matvec_repeats}",
-           "tuned_variants": variants}
+    out = {"workload": f"{what}, fused apply (Jacobi) x {args.matvec_repeats}", "tuned_variants": variants}
     names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
     for prec, tag in ((vie.MIXED, "fp32"), (vie.FP64, "fp64")):
         ms, passes = s.time_apply(prec, args.matvec_repeats)
@@ -891,8 +1091,16 @@ def bench_point(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8, help="timed frequency points (solves) per precision")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed solves (slab: mixed solves of the slab, default 5; sweep: frequency points, "
+                         "default 8)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed warm-up solves (default 1)")
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="slab: timed FP64 solves of each FP64 variant (after one warm-up)")
+    ap.add_argument("--cpu-iterations", type=int, default=2,
+                    help="slab: reference GMRES iterations timed for the CPU baseline")
+    ap.add_argument("--geometry", choices=("sweep", "slab"), default="sweep",
+                    help="matvec: the geometry whose fused apply is timed (counter runs)")
     ap.add_argument("--size", type=int, default=64, help="voxels per cube edge (64 = the 64^3 sweep)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--matvec-repeats", type=int, default=50)
@@ -905,8 +1113,9 @@ def main():
                     help="mixed: recycled-subspace size of the GMRES cycles (0 = library default, -1 = off)")
     ap.add_argument("--workload", choices=("sweep", "batch", "serve", "matvec", "slab", "farfield", "rows", "columns",
                                            "point", "cross_slab", "mesh"),
-                    default="sweep",
-                    help="sweep: the 64^3 frequency sweep (default); batch: 256 uneven-grid solves with "
+                    default="slab",
+                    help="slab: the 128^3 slab + Yagi (default, the headline); sweep: the 64^3 frequency "
+                         "sweep; batch: 256 uneven-grid solves with "
                          "1-8 ports each; serve: Poisson-arrival request stream; matvec: fused operator "
                          "applies of one sweep point plus the copy calibration kernel (for counter runs)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="matvec: bytes of the calibration copy")
@@ -928,6 +1137,10 @@ def main():
     ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean arrival rate (requests/s)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 5 if args.workload == "slab" else 8
+    if args.warmup is None:
+        args.warmup = 1
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -946,6 +1159,8 @@ def main():
         return bench_matvec(args)
     if args.workload == "slab":
         return bench_slab(args)
+    if args.workload == "sweep":
+        return bench_sweep(args)
     if args.workload == "farfield":
         return bench_farfield(args)
     if args.workload == "rows":
@@ -958,7 +1173,12 @@ def main():
         return bench_point(args)
     if args.workload == "cross_slab":
         return bench_cross_slab(args)
+    return 2
 
+
+def bench_sweep(args):
+    """The 64^3 random-permittivity sweep (module docstring)."""
+    from nec2c_amd import vie, workloads as W
     n = args.size
     pitch = 0.5 / n
     eps = W.random_permittivity((n, n, n), seed=2)
