@@ -76,6 +76,25 @@ int vie_fft_set_permittivities(vie_fft_plan *plan, const VIE_CDOUBLE *epsilon_r,
    device work and returns y widened to double. */
 int dielectric_fft_apply_ex(dielectric_fft_system *system, vie_hip_precision precision, int jacobi,
                             const VIE_CDOUBLE *x, VIE_CDOUBLE *y);
+/* Allocation-free variant of dielectric_fft_apply_ex (new; no counterpart in
+   the reference).  For callers that cannot allocate device memory: code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+   Same arguments and result as dielectric_fft_apply_ex, but every temporary
+   (staging, box-order vectors, FFT workspace, reduction partials) lives in a
+   96 MiB array of device memory that the library holds on each GPU it runs on
+   until the process ends; no device memory is allocated or freed.  Systems
+   needing more scratch than that (dielectric_fft_apply_noalloc_bytes) are
+   refused with an error (FP64 fits grids up to about 48^3, FP32 larger).  The
+   system's device copies must be current: call dielectric_fft_reserve, or
+   any apply or solve, after the last dielectric_fft_entry or permittivity
+   change.  All calls on one GPU share its array, so calls that overlap on one
+   GPU (from several threads) race; each call finishes its device work before
+   it returns.  1 on success, 0 with vie_hip_last_error(). */
+int dielectric_fft_apply_noalloc(dielectric_fft_system *system, vie_hip_precision precision, int jacobi,
+                                 const VIE_CDOUBLE *x, VIE_CDOUBLE *y);
+/* Scratch bytes one dielectric_fft_apply_noalloc of this system needs at this
+   precision (new). */
+size_t dielectric_fft_apply_noalloc_bytes(const dielectric_fft_system *system, vie_hip_precision precision);
 /* Pass-shape autotuner: times each launch-shape variant (lines per
    workgroup, workgroup size) of the five volume passes on this plan's
    geometry and keeps the fastest per pass and precision.  Choices are cached

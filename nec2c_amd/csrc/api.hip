@@ -945,6 +945,38 @@ static int sys_apply(dielectric_fft_system* s, const vie::cx<T>* x, vie::cx<T>* 
   return hipGetLastError() == hipSuccess;
 }
 
+// sys_apply with the volume passes on a caller-given FFT workspace
+// ([3][nx][ny][Pz] then [3][nx][Py][Pz]) instead of the plan's own.
+template <typename T>
+static int sys_apply_ws(dielectric_fft_system* s, const vie::cx<T>* x, vie::cx<T>* y, int precond,
+                        vie::ReduceScratch& rs, vie::cx<T>* work) {
+  const size_t c = s->c;
+  if (s->plan) {
+    vie::VolumeView<T> vv = s->plan->view<T>();
+    vv.A = work;
+    vv.B = work + 3 * (size_t)vv.g.nx * vv.g.ny * vv.g.Pz;
+    if (!vie::volume_apply<T>(vv, x + c, y + c, precond, s->stream)) {
+      set_error("volume kernels failed to launch");
+      return 0;
+    }
+  }
+  if (c) {
+    vie::CouplingView<T> cv;
+    cv.c = (int)c;
+    cv.nbox = s->nbox();
+    cv.nv = 3 * cv.nbox;
+    cv.C = Prec<T>::C(s).template as<vie::cx<T>>();
+    cv.B = Prec<T>::B(s).template as<vie::cx<T>>();
+    cv.D = Prec<T>::D(s).template as<vie::cx<T>>();
+    cv.bz = s->bz.as<c64>();
+    cv.eps = s->plan ? Prec<T>::eps(s->plan).template as<vie::cx<T>>() : nullptr;
+    for (int i = 0; i < 3; i++)
+      cv.self[i] = s->plan ? vie::cx<T>{(T)s->plan->self[i].re, (T)s->plan->self[i].im} : vie::cx<T>{0, 0};
+    vie::coupling_apply<T>(cv, x, x + c, y, y + c, precond, rs, s->stream, {});
+  }
+  return hipGetLastError() == hipSuccess;
+}
+
 static vie::JacobiView jacobi_view(dielectric_fft_system* s) {
   vie::JacobiView J;
   J.c = s->c;
@@ -1748,6 +1780,120 @@ extern "C" int dielectric_fft_apply_ex(dielectric_fft_system* s, vie_hip_precisi
     return 0;
   }
   return unstage_vector(s, s->sw.ax.as<c64>(), out);
+}
+
+// ------------------------------------------------------------------ allocation-free apply
+// Device scratch of dielectric_fft_apply_noalloc: one array per GPU for the
+// life of the process, so that entry point never allocates.  Host code reaches
+// it only through the per-device address hipGetSymbolAddress returns.
+constexpr size_t kNoAllocScratchBytes = (size_t)96 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevicesNoAlloc = 64;
+static std::mutex noalloc_mu;
+static void* noalloc_ptr[kMaxDevicesNoAlloc];
+
+// Address of the current device's scratch array (looked up once per device).
+static unsigned char* noalloc_scratch() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevicesNoAlloc) {
+    set_error("dielectric_fft_apply_noalloc: no usable current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("dielectric_fft_apply_noalloc: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    noalloc_ptr[dev] = ptr;
+  }
+  return (unsigned char*)noalloc_ptr[dev];
+}
+
+// Bytes of scratch one allocation-free apply of this system needs, and the
+// offsets of its parts (user staging, box-order input and output, FFT
+// workspace, reduction partials), each rounded up to 256 bytes.
+struct NoAllocLayout {
+  size_t user, xin, yout, work, partial, total;
+};
+static NoAllocLayout noalloc_layout(const dielectric_fft_system* s, bool mixed) {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t es = mixed ? 8 : 16;
+  const size_t nu = std::max<size_t>(s->c + s->v, 1), n = s->nsys();
+  size_t work = 0;
+  if (s->plan) {
+    const vie::Geom& g = s->plan->g;
+    work = (3 * (size_t)g.nx * g.ny * g.Pz + 3 * (size_t)g.nx * g.Py * g.Pz) * es;
+  }
+  NoAllocLayout L;
+  L.user = 0;
+  L.xin = L.user + up(nu * 16);
+  L.yout = L.xin + up(n * es);
+  L.work = L.yout + up(n * es);
+  L.partial = L.work + up(work);
+  L.total = L.partial + (s->c ? vie::kReduceScratchBytes : 0);
+  return L;
+}
+
+extern "C" size_t dielectric_fft_apply_noalloc_bytes(const dielectric_fft_system* s, vie_hip_precision prec) {
+  return s ? noalloc_layout(s, prec == VIE_HIP_MIXED).total : 0;
+}
+
+// dielectric_fft_apply_ex with every temporary in the per-device scratch
+// array: no device memory is allocated or freed.  The system's resident data
+// (spectrum, permittivity, C/B/D) must already be on the device.
+extern "C" int dielectric_fft_apply_noalloc(dielectric_fft_system* s, vie_hip_precision prec, int jacobi,
+                                            const double _Complex* in, double _Complex* out) {
+  if (!s || !in || !out || (prec != VIE_HIP_FP64 && prec != VIE_HIP_MIXED)) {
+    set_error("dielectric_fft_apply_noalloc: invalid arguments");
+    return 0;
+  }
+  const bool mixed = prec == VIE_HIP_MIXED;
+  const NoAllocLayout L = noalloc_layout(s, mixed);
+  if (L.total > kNoAllocScratchBytes) {
+    set_error("dielectric_fft_apply_noalloc: the system needs %zu bytes of scratch, more than the %zu-byte "
+              "maximum; use dielectric_fft_apply_ex",
+              L.total, kNoAllocScratchBytes);
+    return 0;
+  }
+  if (s->blocks_dirty || (s->plan && (s->plan->eps_dirty || !s->plan->map_dev.p))) {
+    set_error("dielectric_fft_apply_noalloc: the system's device copies are not current; call "
+              "dielectric_fft_reserve (or any apply or solve) after the last entry or permittivity change");
+    return 0;
+  }
+  unsigned char* base = noalloc_scratch();
+  if (!base) return 0;
+  const size_t nu = s->c + s->v, n = s->nsys();
+  const size_t* map = s->plan ? s->plan->map_dev.as<size_t>() : nullptr;
+  c64* user = (c64*)(base + L.user);
+  vie::ReduceScratch rs;
+  rs.partial = (c64*)(base + L.partial);
+  HIP_OK(hipMemcpyAsync(user, in, nu * 16, hipMemcpyHostToDevice, s->stream));
+  int ok;
+  if (mixed) {
+    c32* x = (c32*)(base + L.xin);
+    c32* y = (c32*)(base + L.yout);
+    vie::vec_zero<float>(x, n, s->stream);
+    vie::to_box<float>(user, s->c, s->cells, map, s->nbox(), x, s->stream);
+    ok = sys_apply_ws<float>(s, x, y, jacobi, rs, (c32*)(base + L.work));
+    if (ok) vie::from_box<float>(y, s->c, s->cells, map, s->nbox(), user, s->stream);
+  } else {
+    c64* x = (c64*)(base + L.xin);
+    c64* y = (c64*)(base + L.yout);
+    vie::vec_zero<double>(x, n, s->stream);
+    vie::to_box<double>(user, s->c, s->cells, map, s->nbox(), x, s->stream);
+    ok = sys_apply_ws<double>(s, x, y, jacobi, rs, (c64*)(base + L.work));
+    if (ok) vie::from_box<double>(y, s->c, s->cells, map, s->nbox(), user, s->stream);
+  }
+  if (!ok) {
+    (void)hipStreamSynchronize(s->stream);
+    return 0;
+  }
+  HIP_OK(hipMemcpyAsync(out, user, nu * 16, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return 1;
 }
 
 static int cb_plan_apply(void* ctx, const double _Complex* in, double _Complex* out) {

@@ -131,7 +131,8 @@ EXPORTS = (
     "dielectric_fft_plan", "vie_hip_last_error", "vie_hip_device_count",
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
     "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
-    "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_reserve",
+    "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_apply_noalloc",
+    "dielectric_fft_apply_noalloc_bytes", "dielectric_fft_reserve",
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
@@ -205,6 +206,8 @@ def lib() -> ctypes.CDLL:
         "vie_fft_set_permittivities": (i, [vp, vp, sz]),
         "vie_hip_synchronize": (i, []),
         "dielectric_fft_apply_ex": (i, [vp, i, i, vp, vp]),
+        "dielectric_fft_apply_noalloc": (i, [vp, i, i, vp, vp]),
+        "dielectric_fft_apply_noalloc_bytes": (sz, [vp, i]),
         "dielectric_fft_reserve": (i, [vp, ctypes.POINTER(GmresOptions)]),
         "vie_hip_copy_probe": (d, [sz, i]),
         "vie_hip_batch_create": (vp, []),
