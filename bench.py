@@ -383,25 +383,12 @@ def bench_serve(args):
     return 0
 
 
-SLAB_H = 0.01          # voxel pitch (m): 1.28 m edge at 128^3
-SLAB_FREQ = 200e6
+SLAB_H = 0.01          # voxel pitch (m) of the slab workload (nec2c_amd/workloads.py yagi_slab)
 
 
 def slab_geometry(n):
-    """The slab workload's inputs: 1 cm voxels, eps_r 6 with 0.01 S/m, and a
-    4-element Yagi (reflector 0.50 m, driven 0.47 m, directors 0.44 m and
-    0.43 m, 21 segments each, 1 mm radius, 0.2 m spacing) 0.1 m from the x = 0
-    face, at 200 MHz."""
     from nec2c_amd import workloads as W
-    edge = n * SLAB_H
-    yc = zc = 0.5 * edge
-    yagi = W.WireArray(tuple((-0.1, yc + (i - 1.5) * 0.2, zc, L) for i, L in enumerate((0.50, 0.47, 0.44, 0.43))),
-                       segments_per_wire=21, radius=1e-3, feed_wire=1)
-    k0 = W.k0_of(SLAB_FREQ)
-    # eps_r - i sigma / (omega eps0) with the reference's constants (dielectric.h:55), as the
-    # library evaluates the material: the reference plan of the CPU leg must see the same value
-    eps_cell = 6.0 - 1j * 0.01 / (k0 * 299792458.0 * 8.8541878128e-12)
-    return yagi, k0, eps_cell
+    return W.yagi_slab(n)
 
 
 def slab_system(n, opts, keep_blocks=False):

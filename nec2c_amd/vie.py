@@ -73,6 +73,12 @@ class Clearance(ctypes.Structure):             # vie_hip_clearance (include/vie_
                 ("has_conductors", ctypes.c_int), ("insufficient", ctypes.c_int)]
 
 
+class Sample(ctypes.Structure):                # dielectric.h:18-24
+    _fields_ = [("material_id", ctypes.c_int), ("conductivity_kind", ctypes.c_int),
+                ("frequency_mhz", ctypes.c_double), ("epsilon_r", ctypes.c_double), ("loss", ctypes.c_double),
+                ("source_line", ctypes.c_size_t)]
+
+
 class Material(ctypes.Structure):              # dielectric.h:26-35
     _fields_ = [("id", ctypes.c_int), ("epsilon_r", ctypes.c_double), ("source_line", ctypes.c_size_t),
                 ("epsilon_loss", ctypes.c_double), ("conductivity", ctypes.c_double),
@@ -671,10 +677,16 @@ class Cell:
 
 @dataclass
 class MaterialSpec:
+    """A material as the model holds it (dielectric.h:26-35): eps_r, a loss
+    (positive imaginary-part magnitude) and a conductivity in S/m; or, with
+    ``samples`` = ((frequency_mhz, eps_r, loss, conductivity_kind), ...) in
+    ascending frequency, a tabulated material interpolated linearly in
+    frequency (loss is a conductivity in S/m where conductivity_kind is 1)."""
     id: int
     epsilon_r: float
     epsilon_loss: float = 0.0
     conductivity: float = 0.0
+    samples: tuple = ()
 
 
 class System:
@@ -686,11 +698,21 @@ class System:
         with ``pitch`` (and optional per-cell ``material_ids``, default 1)."""
         L = lib()
         self._mats = (Material * max(1, len(materials)))()
+        self._samples = []
         for i, m in enumerate(materials):
             self._mats[i].id = m.id
             self._mats[i].epsilon_r = m.epsilon_r
             self._mats[i].epsilon_loss = m.epsilon_loss
             self._mats[i].conductivity = m.conductivity
+            if m.samples:
+                tab = (Sample * len(m.samples))()
+                for j, (f, er, loss, kind) in enumerate(m.samples):
+                    tab[j].material_id, tab[j].conductivity_kind = m.id, int(kind)
+                    tab[j].frequency_mhz, tab[j].epsilon_r, tab[j].loss = float(f), float(er), float(loss)
+                self._samples.append(tab)
+                self._mats[i].tabulated = 1
+                self._mats[i].samples = ctypes.cast(tab, ctypes.c_void_p)
+                self._mats[i].sample_count = len(m.samples)
         self._cells = (MeshCell * max(1, len(cells)))()
         if isinstance(cells, np.ndarray):
             arr = np.ctypeslib.as_array(self._cells)

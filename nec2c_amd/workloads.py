@@ -231,6 +231,27 @@ def coupling_blocks(wire, centers: np.ndarray, pitch: float, eps: np.ndarray, k0
     return B, D
 
 
+SLAB_PITCH = 0.01      # m: 1.28 m edge at 128^3
+SLAB_FREQ = 200e6
+EPS0_REF = 8.8541878128e-12   # the reference's eps0 in dielectric_material_epsilon (dielectric.h:55)
+
+
+def yagi_slab(n):
+    """The slab workload: an n^3 block of 1 cm voxels (eps_r 6, 0.01 S/m)
+    driven by a 4-element Yagi (reflector 0.50 m, driven 0.47 m, directors
+    0.44 m and 0.43 m, 21 segments each, 1 mm radius, 0.2 m spacing) 0.1 m
+    from its x = 0 face, at 200 MHz.  Returns (yagi, k0, eps_cell) with the
+    cell permittivity eps_r - i sigma / (omega eps0) evaluated as the
+    reference evaluates a conductive material."""
+    edge = n * SLAB_PITCH
+    yc = zc = 0.5 * edge
+    yagi = WireArray(tuple((-0.1, yc + (i - 1.5) * 0.2, zc, L) for i, L in enumerate((0.50, 0.47, 0.44, 0.43))),
+                     segments_per_wire=21, radius=1e-3, feed_wire=1)
+    k0 = k0_of(SLAB_FREQ)
+    eps_cell = 6.0 - 1j * 0.01 / (k0 * C0 * EPS0_REF)
+    return yagi, k0, eps_cell
+
+
 def cube_cells(n, pitch, origin=(0.0, 0.0, 0.0)):
     """Centres of an n^3 (or nx x ny x nz) block, x slowest, z fastest."""
     shape = (n, n, n) if np.isscalar(n) else tuple(n)

@@ -530,6 +530,12 @@ class ReferenceLibrary:
                                              ctypes.c_size_t, ctypes.c_double, ctypes.c_size_t,
                                              ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
         L.refshim_system_prepare.restype = vp
+        L.refshim_system_prepare_tab.argtypes = [ctypes.c_size_t, d, d, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.c_size_t, ctypes.POINTER(ctypes.c_int), d, sz, d,
+                                                 ctypes.c_size_t, ctypes.c_double, ctypes.c_size_t,
+                                                 ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        L.refshim_system_prepare_tab.restype = vp
+        L.refshim_system_set_blocks.argtypes = [vp, ctypes.c_size_t, ctypes.c_size_t, d, d, d]
         L.refshim_system_free.argtypes = [vp]
         L.refshim_system_set.argtypes = [vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double,
                                          ctypes.c_double]
@@ -687,9 +693,19 @@ class RefSystem:
                                      or np.zeros((1, 3)), dtype=np.float64)
         msg = ctypes.create_string_buffer(512)
         ip = ctypes.POINTER(ctypes.c_int)
-        self.s = R.L.refshim_system_prepare(n, _dp(centers), _dp(sizes), mids.ctypes.data_as(ip),
-                                            len(materials), matid.ctypes.data_as(ip), _dp(props),
-                                            conductors, float(k0), int(extra), int(pec), msg, 512)
+        tabs = [tuple(getattr(m, "samples", ()) or ()) for m in materials]
+        if any(tabs):
+            counts = np.ascontiguousarray([len(t) for t in tabs], dtype=np.uintp)
+            flat = np.ascontiguousarray([v for t in tabs for smp in t for v in smp] or [0.0], dtype=np.float64)
+            self.s = R.L.refshim_system_prepare_tab(n, _dp(centers), _dp(sizes), mids.ctypes.data_as(ip),
+                                                    len(materials), matid.ctypes.data_as(ip), _dp(props),
+                                                    counts.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)),
+                                                    _dp(flat), conductors, float(k0), int(extra), int(pec),
+                                                    msg, 512)
+        else:
+            self.s = R.L.refshim_system_prepare(n, _dp(centers), _dp(sizes), mids.ctypes.data_as(ip),
+                                                len(materials), matid.ctypes.data_as(ip), _dp(props),
+                                                conductors, float(k0), int(extra), int(pec), msg, 512)
         self.message = msg.value.decode()
         if not self.s:
             raise PrepareError(self.message)
@@ -699,6 +715,12 @@ class RefSystem:
     def set(self, row, col, value):
         value = complex(value)
         return bool(self.R.L.refshim_system_set(self.s, row, col, value.real, value.imag))
+
+    def set_blocks(self, C, B, D):
+        """C (c x c), B (c x v), D (v x c) through dielectric_fft_entry."""
+        C, B, D = [np.ascontiguousarray(a, dtype=np.complex128) for a in (C, B, D)]
+        self.R.L.refshim_system_set_blocks(self.s, self.c, self.v, _dp(C.view(np.float64)),
+                                           _dp(B.view(np.float64)), _dp(D.view(np.float64)))
 
     def storage(self):
         return self.R.L.refshim_system_storage(self.s)

@@ -108,7 +108,7 @@ int refshim_gmres(vie_fft_plan *p, const double *b, double *x, size_t restart,
    as centers/sizes/material ids; materials as (id, eps_r, eps_loss,
    conductivity) rows. */
 typedef struct { dielectric_fft_system *s; dielectric_model m; dielectric_mesh_result mesh;
-                 dielectric_material *mat; dielectric_cell *cells; } refshim_system;
+                 dielectric_material *mat; dielectric_cell *cells; dielectric_sample *samples; } refshim_system;
 
 refshim_system *refshim_system_prepare(size_t ncells, const double *centers, const double *sizes,
                                        const int *material_ids, size_t nmat, const int *mat_ids,
@@ -138,14 +138,72 @@ refshim_system *refshim_system_prepare(size_t ncells, const double *centers, con
   return r;
 }
 
+/* refshim_system_prepare with tabulated materials: material i is tabulated
+   when sample_counts[i] > 0, its samples 4 doubles each (frequency MHz,
+   eps_r, loss, conductivity_kind) at samples + 4 * (sum of earlier counts). */
+refshim_system *refshim_system_prepare_tab(size_t ncells, const double *centers, const double *sizes,
+                                           const int *material_ids, size_t nmat, const int *mat_ids,
+                                           const double *mat_props, const size_t *sample_counts,
+                                           const double *samples, size_t conductors, double k0, size_t extra,
+                                           int pec, char *message, size_t message_len)
+{
+  refshim_system *r = calloc(1, sizeof(*r)); dielectric_diagnostic d; size_t i, j, total = 0, at = 0; int ok;
+  if(!r) return NULL;
+  for(i = 0; i < nmat; i++) total += sample_counts[i];
+  r->mat = calloc(nmat ? nmat : 1, sizeof(*r->mat));
+  r->cells = calloc(ncells ? ncells : 1, sizeof(*r->cells));
+  r->samples = calloc(total ? total : 1, sizeof(*r->samples));
+  for(i = 0; i < nmat; i++) {
+    r->mat[i].id = mat_ids[i]; r->mat[i].epsilon_r = mat_props[3*i];
+    r->mat[i].epsilon_loss = mat_props[3*i+1]; r->mat[i].conductivity = mat_props[3*i+2];
+    if(sample_counts[i]) {
+      r->mat[i].tabulated = 1; r->mat[i].samples = r->samples + at; r->mat[i].sample_count = sample_counts[i];
+      for(j = 0; j < sample_counts[i]; j++, at++) {
+        r->samples[at].material_id = mat_ids[i];
+        r->samples[at].frequency_mhz = samples[4*at]; r->samples[at].epsilon_r = samples[4*at+1];
+        r->samples[at].loss = samples[4*at+2]; r->samples[at].conductivity_kind = (int)samples[4*at+3];
+      }
+    }
+  }
+  for(i = 0; i < ncells; i++) {
+    memcpy(r->cells[i].center, centers+3*i, 3*sizeof(double));
+    memcpy(r->cells[i].size, sizes+3*i, 3*sizeof(double));
+    r->cells[i].material_id = material_ids[i];
+  }
+  r->m.materials = r->mat; r->m.material_count = nmat;
+  r->mesh.cells = r->cells; r->mesh.cell_count = ncells;
+  memset(&d, 0, sizeof(d));
+  ok = (pec ? dielectric_fft_prepare_pec : dielectric_fft_prepare)(&r->s, &r->m, &r->mesh,
+        conductors, k0, extra, &d);
+  if(message && message_len) { strncpy(message, d.message, message_len-1); message[message_len-1] = 0; }
+  if(!ok) { free(r->mat); free(r->cells); free(r->samples); free(r); return NULL; }
+  return r;
+}
+
 void refshim_system_free(refshim_system *r)
-{ if(r) { dielectric_fft_free(r->s); free(r->mat); free(r->cells); free(r); } }
+{ if(r) { dielectric_fft_free(r->s); free(r->mat); free(r->cells); free(r->samples); free(r); } }
 
 int refshim_system_set(refshim_system *r, size_t row, size_t col, double re, double im)
 {
   complex double *e = dielectric_fft_entry(r->s, row, col);
   if(!e) return 0;
   *e = re+I*im; return 1;
+}
+
+/* All conductor blocks at once through dielectric_fft_entry: C c x c, B c x v,
+   D v x c, row-major (interleaved re, im). */
+int refshim_system_set_blocks(refshim_system *r, size_t c, size_t v, const double *C, const double *B,
+                              const double *D)
+{
+  size_t i, j;
+  const complex double *cc = (const complex double *)C, *bb = (const complex double *)B,
+                       *dd = (const complex double *)D;
+  for(i = 0; i < c; i++) {
+    for(j = 0; j < c; j++) *dielectric_fft_entry(r->s, i, j) = cc[i*c+j];
+    for(j = 0; j < v; j++) *dielectric_fft_entry(r->s, i, c+j) = bb[i*v+j];
+  }
+  for(i = 0; i < v; i++) for(j = 0; j < c; j++) *dielectric_fft_entry(r->s, c+i, j) = dd[i*c+j];
+  return 1;
 }
 
 size_t refshim_system_storage(refshim_system *r) { return dielectric_fft_storage(r->s); }
