@@ -2887,6 +2887,11 @@ static int clearance_run(const double* minimum, const double* maximum, const dou
                          double* ms) {
   g_last_error.clear();
   size_t bytes;
+  if (out) {  // the reference's values for "nothing evaluated", so a failed call leaves no garbage
+    memset(out, 0, sizeof(*out));
+    out->worst_margin = DBL_MAX;
+    out->minimum_gap = DBL_MAX;
+  }
   if (!out || segments < 0 || patches < 0 || repeats < 1 ||
       (boxes && (!minimum || !maximum || !cell_longest)) ||
       (segments && (!x || !y || !z || !si || !cab || !sab || !salp || !bi)) ||
