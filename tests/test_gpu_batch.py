@@ -112,3 +112,43 @@ def test_batch_rejects_bad_input(cases):
             batch.submit(s, np.ones(s.order), restart=vie.KMAX_RESTART + 1, maximum_iterations=1000)
     finally:
         batch.close()
+
+
+def test_batch_at_configuration_size(ref):
+    """The first 16 models of the batched workload (seed 4: grids 8-48 per
+    axis, cubes and cuboids, eps_r 2-12, 50-500 MHz, dipoles of 11-31
+    segments, 1-8 ports), every port in flight at once with restart 120:
+    each job equals its solo solve and meets 1e-8 on the reference operator."""
+    cs = W.batch_cases(256, seed=4, kh=0.1)[:16]
+    opts = dict(restart=120, maximum_iterations=3000, relative_tolerance=1e-8)
+    built = [build(c) for c in cs]
+    batch = vie.Batch()
+    jobs = []
+    try:
+        for ci, (case, (s, blocks, eps)) in enumerate(zip(cs, built)):
+            for port in range(len(case.ports)):
+                prec = vie.MIXED if (ci + port) % 2 else vie.FP64
+                b = full_rhs(case, s, port)
+                jobs.append((ci, port, prec, b, batch.submit(s, b, precision=prec, **opts)))
+        print(f"{len(jobs)} jobs of {len(cs)} models in flight: {batch.running}", flush=True)
+        batch.wait_all()
+        results = {jid: batch.result(jid) for *_, jid in jobs}
+    finally:
+        batch.close()
+    ops = {}
+    for ci, port, prec, b, jid in jobs:
+        case, (s, blocks, eps) = cs[ci], built[ci]
+        ok, x, rep, sec = results[jid]
+        ok1, x1, rep1 = s.solve(b, precision=prec, **opts)
+        if ci not in ops:
+            xyz = np.stack(np.meshgrid(*[np.arange(v) for v in case.shape], indexing="ij"), -1).reshape(-1, 3)
+            plan = ref.plan(case.shape, xyz, case.pitch, case.k0, eps[0])
+            ops[ci] = (plan, plan.coupled(*blocks))
+        r = float(np.linalg.norm(b - ops[ci][1].apply(x)) / np.linalg.norm(b))
+        print(f"model {ci} {case.shape} port {port} prec {prec}: it {rep.iterations} solo {rep1.iterations} "
+              f"reference residual {r:.2e}", flush=True)
+        assert ok and ok1 and rep.iterations == rep1.iterations
+        assert np.max(np.abs(x - x1)) <= 1e-12 * np.max(np.abs(x1))
+        assert r <= 1.05e-8
+    for s, _, _ in built:
+        s.close()
