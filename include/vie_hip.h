@@ -103,6 +103,23 @@ size_t dielectric_fft_apply_noalloc_bytes(const dielectric_fft_system *system, v
    environment keeps variant 0.  variants (optional, 10 ints) receives the
    FP64 then the FP32 choice of each pass. */
 int vie_fft_tune(vie_fft_plan *plan, int *variants);
+/* Timing phases of the library's hot-path work (new; the reference times its
+   own phases with performance_add, performance.h:8-17): wall seconds and calls,
+   cumulative over the process or since vie_hip_phases_reset, of "fft_build"
+   (prepare / vie_fft_create with the spectrum), "fft_upload" (C/B/D to the
+   device when called outside a solve), "fft_gmres" (device solves),
+   "fft_matvec" (operator applications on host vectors), "fft_conductor_rows",
+   "fft_conductor_columns", "fft_far_field" and "fft_mesh_checks".  Work done
+   inside another timed call counts once, in the outer phase.  Writes up to max
+   phases that have calls (names are static strings) and returns how many there
+   are. */
+typedef struct {
+  const char *name;
+  double seconds;
+  unsigned long long calls;
+} vie_hip_phase;
+size_t vie_hip_phases(vie_hip_phase *phases, size_t max);
+void vie_hip_phases_reset(void);
 /* Waits for all work on the current device. */
 int vie_hip_synchronize(void);
 /* Copies `bytes` (rounded down to 16) between two device buffers `repeats`

@@ -73,6 +73,54 @@ static void init_solve_defaults() {
   if (p && (!strcmp(p, "mixed") || !strcmp(p, "MIXED"))) g_solve.precision = VIE_HIP_MIXED;
 }
 
+// ------------------------------------------------------------------ timing phases
+// Wall seconds and calls of the library's hot-path work, cumulative over the
+// process (or since vie_hip_phases_reset), for timing reports such as nec2c's
+// performance_add phases.
+enum {
+  PHASE_BUILD, PHASE_UPLOAD, PHASE_GMRES, PHASE_MATVEC, PHASE_ROWS, PHASE_COLUMNS, PHASE_FAR_FIELD, PHASE_MESH,
+  PHASE_COUNT
+};
+static const char* const kPhaseNames[PHASE_COUNT] = {
+    "fft_build", "fft_upload", "fft_gmres", "fft_matvec", "fft_conductor_rows", "fft_conductor_columns",
+    "fft_far_field", "fft_mesh_checks"};
+static double g_phase_s[PHASE_COUNT];
+static unsigned long long g_phase_n[PHASE_COUNT];
+static std::mutex g_phase_mu;
+static thread_local int g_phase_depth = 0;  // an outer timed call owns nested work
+
+struct PhaseTimer {
+  int id;
+  bool own;
+  std::chrono::steady_clock::time_point t0;
+  explicit PhaseTimer(int i) : id(i), own(g_phase_depth == 0), t0(std::chrono::steady_clock::now()) {
+    g_phase_depth++;
+  }
+  ~PhaseTimer() {
+    g_phase_depth--;
+    if (!own) return;
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::lock_guard<std::mutex> g(g_phase_mu);
+    g_phase_s[id] += dt;
+    g_phase_n[id]++;
+  }
+};
+
+extern "C" size_t vie_hip_phases(vie_hip_phase* out, size_t max) {
+  std::lock_guard<std::mutex> g(g_phase_mu);
+  size_t k = 0;
+  for (int i = 0; i < PHASE_COUNT; i++) {
+    if (!g_phase_n[i]) continue;
+    if (out && k < max) out[k] = vie_hip_phase{kPhaseNames[i], g_phase_s[i], g_phase_n[i]};
+    k++;
+  }
+  return k;
+}
+extern "C" void vie_hip_phases_reset(void) {
+  std::lock_guard<std::mutex> g(g_phase_mu);
+  for (int i = 0; i < PHASE_COUNT; i++) g_phase_s[i] = 0, g_phase_n[i] = 0;
+}
+
 extern "C" int vie_hip_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -472,6 +520,7 @@ extern "C" void vie_fft_destroy(vie_fft_plan* p) { delete p; }
 
 extern "C" int vie_fft_create(vie_fft_plan** out, const size_t shape[3], size_t cells, const size_t* xyz, double h,
                               double k, double _Complex er, size_t limit) {
+  PhaseTimer phase_timer_(PHASE_BUILD);
   if (!out) return 0;
   *out = nullptr;
   g_last_error.clear();
@@ -681,6 +730,7 @@ static c64 material_epsilon(const dielectric_material* m, double k0) {
 
 static int prepare(dielectric_fft_system** out, const dielectric_model* model, const dielectric_mesh_result* mesh,
                    size_t conductors, double k0, size_t extra, dielectric_diagnostic* diag, int pec) {
+  PhaseTimer phase_timer_(PHASE_BUILD);
   g_last_error.clear();
   *out = nullptr;
   size_t cells = mesh->cell_count, v, n, bytes = sizeof(dielectric_fft_system), blocks = 0, t;
@@ -846,6 +896,7 @@ extern "C" double _Complex* dielectric_fft_entry(dielectric_fft_system* s, size_
 // and keeps FP64 and FP32 copies.
 static int upload_blocks(dielectric_fft_system* s) {
   if (!s->blocks_dirty) return 1;
+  PhaseTimer phase_timer_(PHASE_UPLOAD);
   const size_t c = s->c, v = s->v, nbox = s->nbox(), nv = 3 * nbox;
   hipStream_t st = s->stream;
   if (c) {
@@ -1588,6 +1639,7 @@ static int job_prepare(dielectric_fft_system* s, SolverWork& w, const vie_gmres_
 // holding the initial guess.
 static int solve_resident(dielectric_fft_system* s, const vie_gmres_options* options,
                           const vie_hip_solve_options* so, vie_gmres_report* rep) {
+  PhaseTimer phase_timer_(PHASE_GMRES);
   vie_gmres_options defaults = {30, 300, 1e-12, 0};
   if (!options) options = &defaults;
   if (!validate_options(options)) return 0;
@@ -1743,6 +1795,7 @@ extern "C" int dielectric_fft_download_solution(dielectric_fft_system* s, double
 // through device memory.  vie_gmres recognises these callbacks and runs the
 // whole solve on the device instead.
 static int apply_host(dielectric_fft_system* s, const double _Complex* in, double _Complex* out, bool precond) {
+  PhaseTimer phase_timer_(PHASE_MATVEC);
   if (!s || !in || !out) return 0;
   size_t n = s->nsys();
   if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
@@ -1761,6 +1814,7 @@ static int apply_host(dielectric_fft_system* s, const double _Complex* in, doubl
 
 extern "C" int dielectric_fft_apply_ex(dielectric_fft_system* s, vie_hip_precision prec, int jacobi,
                                        const double _Complex* in, double _Complex* out) {
+  PhaseTimer phase_timer_(PHASE_MATVEC);
   if (!s || !in || !out) return 0;
   size_t n = s->nsys();
   if (!s->sw.x.alloc(n * 16) || !s->sw.ax.alloc(n * 16) || !upload_blocks(s) ||
@@ -1846,6 +1900,7 @@ extern "C" size_t dielectric_fft_apply_noalloc_bytes(const dielectric_fft_system
 // (spectrum, permittivity, C/B/D) must already be on the device.
 extern "C" int dielectric_fft_apply_noalloc(dielectric_fft_system* s, vie_hip_precision prec, int jacobi,
                                             const double _Complex* in, double _Complex* out) {
+  PhaseTimer phase_timer_(PHASE_MATVEC);
   if (!s || !in || !out || (prec != VIE_HIP_FP64 && prec != VIE_HIP_MIXED)) {
     set_error("dielectric_fft_apply_noalloc: invalid arguments");
     return 0;
@@ -2125,11 +2180,13 @@ extern "C" int vie_hip_radiation_amplitudes(size_t cells, const double* centers,
                                             const double _Complex* current, double k0, int ground,
                                             const double _Complex* rs, const double _Complex* rp, size_t count,
                                             const double* directions, double _Complex* amplitude) {
+  PhaseTimer phase_timer_(PHASE_FAR_FIELD);
   return farfield_run(cells, centers, sizes, current, k0, ground, rs, rp, count, directions, amplitude);
 }
 
 extern "C" int vie_hip_scattered_power(size_t cells, const double* centers, const double* sizes,
                                        const double _Complex* current, double k0, int ground, double* power) {
+  PhaseTimer phase_timer_(PHASE_FAR_FIELD);
   // the golden-spiral rule of scattered_power (dielectric_solver.c:160-217):
   // the sphere in free space, the upper hemisphere over a PEC ground
   if (!power || (ground != 0 && ground != 1)) {
@@ -2301,6 +2358,7 @@ extern "C" int vie_hip_volume_conductor_rows(double k0, size_t cells, const doub
                                              int patches, const double* patch_points, const double* patch_t1,
                                              const double* patch_t2, const double* patch_scale, int ground,
                                              double _Complex* rows, size_t ld) {
+  PhaseTimer phase_timer_(PHASE_ROWS);
   const RowsArgs a{k0, cells, centers, sizes, wires, wire_points, wire_tangents,
                    patches, patch_points, patch_t1, patch_t2, patch_scale, ground};
   return rows_run(a, rows, ld, 0, nullptr);
@@ -2549,6 +2607,7 @@ extern "C" int vie_hip_conductor_volume_columns(double k0, double wlam, size_t c
                                                 const double* patch_t2, size_t columns,
                                                 const double _Complex* sources, double rkh, int ground,
                                                 double _Complex* coupling) {
+  PhaseTimer phase_timer_(PHASE_COLUMNS);
   const ColumnsArgs a{k0,  wlam, cells, centers, sizes,    chi,          segments,   x,        y,
                       z,   si,   bi,    cab,     sab,      salp,         patches,    patch_points, patch_area,
                       patch_t1, patch_t2, columns, sources, rkh, ground};
@@ -2610,6 +2669,7 @@ extern "C" int dielectric_fft_assemble_conductor_rows(dielectric_fft_system* s, 
                                                       const double* patch_points, const double* patch_t1,
                                                       const double* patch_t2, const double* patch_scale,
                                                       int ground) {
+  PhaseTimer phase_timer_(PHASE_ROWS);
   g_last_error.clear();
   DevBuf *b64 = nullptr, *b32 = nullptr;
   if (!s || wires < 0 || patches < 0 || (size_t)wires + 2 * (size_t)patches != (s ? s->c : 0)) {
@@ -2635,6 +2695,7 @@ extern "C" int dielectric_fft_assemble_conductor_columns(
     const double* bi, const double* cab, const double* sab, const double* salp, int patches,
     const double* patch_points, const double* patch_area, const double* patch_t1, const double* patch_t2,
     const double _Complex* sources, double rkh, int ground) {
+  PhaseTimer phase_timer_(PHASE_COLUMNS);
   g_last_error.clear();
   DevBuf *b64 = nullptr, *b32 = nullptr;
   if (!s) {
@@ -2757,6 +2818,7 @@ extern "C" int vie_hip_total_far_field(double k0, double wlam, int segments, con
                                        const double* sizes, const double _Complex* volume_current, int ground,
                                        size_t count, const double* theta, const double* phi,
                                        double _Complex* e_theta, double _Complex* e_phi) {
+  PhaseTimer phase_timer_(PHASE_FAR_FIELD);
   return total_run(k0, wlam, segments, x, y, z, si, cab, sab, salp, segment_current, patches, patch_points,
                    patch_area, patch_current, cells, centers, sizes, volume_current, ground, count, theta, phi,
                    e_theta, e_phi);
@@ -2771,6 +2833,7 @@ extern "C" int vie_hip_total_radiated_power(double k0, double wlam, int segments
                                             const double* centers, const double* sizes,
                                             const double _Complex* volume_current, int ground, size_t count,
                                             double* power) {
+  PhaseTimer phase_timer_(PHASE_FAR_FIELD);
   if (!power || count == 0 || count > ((size_t)1 << 28)) {
     set_error("total radiated power: invalid argument");
     return 0;
@@ -2866,6 +2929,7 @@ static int overlap_run(const double* minimum, const double* maximum, size_t coun
 
 extern "C" int vie_hip_box_overlap(const double* minimum, const double* maximum, size_t count, long long* left,
                                    long long* right) {
+  PhaseTimer phase_timer_(PHASE_MESH);
   return overlap_run(minimum, maximum, count, left, right, 1, nullptr);
 }
 
@@ -2977,6 +3041,7 @@ extern "C" int vie_hip_conductor_clearance(const double* minimum, const double* 
                                            const double* salp, const double* bi, int patches,
                                            const double* patch_points, const double* patch_area,
                                            vie_hip_clearance* out) {
+  PhaseTimer phase_timer_(PHASE_MESH);
   return clearance_run(minimum, maximum, cell_longest, boxes, segments, x, y, z, si, cab, sab, salp, bi, patches,
                        patch_points, patch_area, out, 1, nullptr);
 }

@@ -30,8 +30,14 @@
  * Loading the library does not start the GPU runtime: the device is first
  * used by a prepare.  The library is looked up as $VIE_HIP_LIBRARY, else as
  * "libvie_hip.so" on the loader's path.
+ *
+ * Timing: after every call it forwards, the dispatcher passes the library's
+ * own timing phases that advanced ("fft_build", "fft_upload", "fft_gmres",
+ * "fft_matvec", ...; vie_hip_phases) to nec2c's performance_add
+ * (performance.h:8-17), so --timing-json lists them beside nec2c's phases.
  */
 #include "dielectric_fft.h"
+#include "performance.h"
 #include "vie_fft.h"
 #include "vie_iterative.h"
 
@@ -67,6 +73,12 @@ int __real_vie_gmres(const vie_linear_operator *, const complex double *, comple
    diagnostic structs are layout-identical to nec2c's dielectric.h). */
 typedef int (*prepare_fn)(void **, const dielectric_model *, const dielectric_mesh_result *, size_t,
                           double, size_t, dielectric_diagnostic *);
+typedef struct {
+  const char *name;
+  double seconds;
+  unsigned long long calls;
+} hip_phase;                   /* vie_hip_phase */
+#define HIP_PHASES 16
 static struct {
   void *handle;
   prepare_fn prepare, prepare_pec;
@@ -77,6 +89,7 @@ static struct {
   int (*solve)(void *, const complex double *, complex double *, const vie_gmres_options *,
                vie_gmres_report *);
   const char *(*last_error)(void);
+  size_t (*phases)(hip_phase *, size_t);
 } lib;
 
 typedef struct hip_system {
@@ -88,6 +101,26 @@ typedef struct hip_system {
 static pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
 static hip_system *systems;    /* every live GPU system handed to nec2c */
 static int hip_requested, hip_last_used;
+static hip_phase reported[HIP_PHASES];
+static size_t reported_count;
+
+/* Hands the library's phases that advanced since the last report to
+   nec2c's timing document. */
+static void report_phases(void)
+{
+  hip_phase now[HIP_PHASES];
+  size_t n = lib.phases(now, HIP_PHASES), i, j;
+  if(n > HIP_PHASES) n = HIP_PHASES;
+  for(i = 0; i < n; i++) {
+    double before = 0;
+    unsigned long long calls = 0;
+    for(j = 0; j < reported_count; j++)
+      if(!strcmp(reported[j].name, now[i].name)) { before = reported[j].seconds; calls = reported[j].calls; }
+    if(now[i].calls > calls) performance_add(now[i].name, now[i].seconds - before);
+  }
+  memcpy(reported, now, n * sizeof(*now));
+  reported_count = n;
+}
 
 static int load_library(void)
 {
@@ -100,7 +133,8 @@ static int load_library(void)
   if(!TAKE(prepare, "dielectric_fft_prepare") || !TAKE(prepare_pec, "dielectric_fft_prepare_pec") ||
      !TAKE(release, "dielectric_fft_free") || !TAKE(entry, "dielectric_fft_entry") ||
      !TAKE(get_operator, "dielectric_fft_get_operator") || !TAKE(storage, "dielectric_fft_storage") ||
-     !TAKE(solve, "dielectric_fft_solve") || !TAKE(last_error, "vie_hip_last_error")) {
+     !TAKE(solve, "dielectric_fft_solve") || !TAKE(last_error, "vie_hip_last_error") ||
+     !TAKE(phases, "vie_hip_phases")) {
     dlclose(h);
     memset(&lib, 0, sizeof(lib));
     return 0;
@@ -152,8 +186,10 @@ static int prepare_hip(prepare_fn fn, dielectric_fft_system **out, const dielect
   }
   if(!fn(&h->system, model, mesh, conductors, k0, extra, diagnostic)) {
     free(h);
+    report_phases();
     return 0;
   }
+  report_phases();
   lib.get_operator(h->system, &h->op);
   pthread_mutex_lock(&lock);
   h->next = systems;
@@ -203,9 +239,19 @@ complex double *__wrap_dielectric_fft_entry(dielectric_fft_system *s, size_t row
    writes made through dielectric_fft_entry before this call reach the device
    with the next product or solve). */
 static int hip_apply(void *context, const complex double *x, complex double *y)
-{ hip_system *h = context; return h->op.apply(h->op.context, x, y); }
+{
+  hip_system *h = context;
+  int ok = h->op.apply(h->op.context, x, y);
+  report_phases();
+  return ok;
+}
 static int hip_precondition(void *context, const complex double *x, complex double *y)
-{ hip_system *h = context; return h->op.precondition(h->op.precondition_context, x, y); }
+{
+  hip_system *h = context;
+  int ok = h->op.precondition(h->op.precondition_context, x, y);
+  report_phases();
+  return ok;
+}
 
 void __wrap_dielectric_fft_get_operator(dielectric_fft_system *s, vie_linear_operator *op)
 {
@@ -236,7 +282,9 @@ int __wrap_vie_gmres(const vie_linear_operator *op, const complex double *rhs, c
 {
   if(op && op->apply == hip_apply && find(op->context)) {
     hip_system *h = op->context;
-    return lib.solve(h->system, rhs, x, options, report);
+    int ok = lib.solve(h->system, rhs, x, options, report);
+    report_phases();
+    return ok;
   }
   return __real_vie_gmres(op, rhs, x, options, report);
 }

@@ -223,6 +223,14 @@ class Server:
                          for r in self.done for p, res in enumerate(r.results) if not res[0]],
         }
 
+    def timing_document(self, deck="served request stream"):
+        """The run as a timing document (nec2c_amd/timing.py, schema version 3):
+        the library's phases and this server's report."""
+        from . import timing
+        rep = self.report()
+        return timing.document(deck, vie.phases(), rep["seconds"], serving=rep,
+                               success=not self.rejected and rep["converged"] == rep["solves"])
+
     def close(self):
         self.batch.close()
         for s, _, _ in self.cache.values():

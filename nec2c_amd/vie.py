@@ -79,6 +79,10 @@ class Sample(ctypes.Structure):                # dielectric.h:18-24
                 ("source_line", ctypes.c_size_t)]
 
 
+class Phase(ctypes.Structure):                 # vie_hip.h vie_hip_phase
+    _fields_ = [("name", ctypes.c_char_p), ("seconds", ctypes.c_double), ("calls", ctypes.c_ulonglong)]
+
+
 class Material(ctypes.Structure):              # dielectric.h:26-35
     _fields_ = [("id", ctypes.c_int), ("epsilon_r", ctypes.c_double), ("source_line", ctypes.c_size_t),
                 ("epsilon_loss", ctypes.c_double), ("conductivity", ctypes.c_double),
@@ -138,7 +142,7 @@ EXPORTS = (
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
     "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
     "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_apply_noalloc",
-    "dielectric_fft_apply_noalloc_bytes", "dielectric_fft_reserve",
+    "dielectric_fft_apply_noalloc_bytes", "vie_hip_phases", "vie_hip_phases_reset", "dielectric_fft_reserve",
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
@@ -214,6 +218,8 @@ def lib() -> ctypes.CDLL:
         "dielectric_fft_apply_ex": (i, [vp, i, i, vp, vp]),
         "dielectric_fft_apply_noalloc": (i, [vp, i, i, vp, vp]),
         "dielectric_fft_apply_noalloc_bytes": (sz, [vp, i]),
+        "vie_hip_phases": (sz, [vp, sz]),
+        "vie_hip_phases_reset": (None, []),
         "dielectric_fft_reserve": (i, [vp, ctypes.POINTER(GmresOptions)]),
         "vie_hip_copy_probe": (d, [sz, i]),
         "vie_hip_batch_create": (vp, []),
@@ -522,6 +528,18 @@ def conductor_clearance(minimum, maximum, cell_longest, segments=None, patches=N
         raise RuntimeError(f"conductor clearance failed: {last_error()}")
     r = {k: getattr(out, k) for k, _ in Clearance._fields_}
     return (r, ms.value) if repeats else r
+
+
+def phases(reset=False):
+    """The library's timing phases: [{"name", "seconds", "calls"}] (vie_hip_phases)."""
+    L = lib()
+    n = L.vie_hip_phases(None, 0)
+    arr = (Phase * max(n, 1))()
+    n = L.vie_hip_phases(arr, n)
+    out = [{"name": arr[i].name.decode(), "seconds": arr[i].seconds, "calls": int(arr[i].calls)} for i in range(n)]
+    if reset:
+        L.vie_hip_phases_reset()
+    return out
 
 
 def last_error() -> str:

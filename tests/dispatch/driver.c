@@ -12,7 +12,9 @@
  * the operator product of a fixed vector, and the vie_gmres solve of that
  * product (status, iterations, largest error against the fixed vector).
  * Exit status: 0 done, 3 backend refused, 4 prepare refused, 5 apply or solve
- * failed.  TEST INFRASTRUCTURE ONLY.
+ * failed.  nec2c's performance_add (performance.c, which needs nec2c's
+ * configure-generated header) is replaced here by a recorder that prints the
+ * timing phases the dispatcher hands it.  TEST INFRASTRUCTURE ONLY.
  */
 #include "dielectric_fft.h"
 #include "vie_fft.h"
@@ -21,6 +23,21 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+static struct { char name[64]; double seconds; size_t calls; } phases[32];
+static size_t phase_count;
+
+void performance_add(const char *phase, double elapsed_seconds)
+{
+  size_t i;
+  for(i = 0; i < phase_count && strcmp(phases[i].name, phase); i++) ;
+  if(i == phase_count) {
+    if(phase_count == 32) return;
+    snprintf(phases[phase_count++].name, sizeof(phases[0].name), "%s", phase);
+  }
+  phases[i].seconds += elapsed_seconds;
+  phases[i].calls++;
+}
 
 int main(int argc, char **argv)
 {
@@ -70,6 +87,8 @@ int main(int argc, char **argv)
          vie_iterative_status_name(rep.status), rep.iterations, rep.restarts, rep.relative_residual, err);
   printf("storage %zu threads %zu\n", dielectric_fft_storage(s), dielectric_fft_threads(s));
   dielectric_fft_free(s);
+  for(i = 0; i < phase_count; i++)
+    printf("phase %s %.9f %zu\n", phases[i].name, phases[i].seconds, phases[i].calls);
   free(x); free(b); free(sol);
   return ok ? 0 : 5;
 }

@@ -27,7 +27,7 @@ def run(backend, library=True):
     if library:
         env["VIE_HIP_LIBRARY"] = LIBRARY
     r = subprocess.run([DRIVER, backend], capture_output=True, text=True, timeout=120, env=env)
-    out = {"rc": r.returncode, "text": r.stdout + r.stderr, "y": {}}
+    out = {"rc": r.returncode, "text": r.stdout + r.stderr, "y": {}, "phases": {}}
     for line in r.stdout.splitlines():
         w = line.split()
         if w[0] == "y":
@@ -43,6 +43,8 @@ def run(backend, library=True):
                             "max_error": float(kv["max_error"])}
         elif w[0] == "storage":
             out["threads"] = int(w[3])
+        elif w[0] == "phase":
+            out["phases"][w[1]] = (float(w[2]), int(w[3]))
     if out["y"]:
         out["yv"] = np.array([out["y"][i] for i in range(len(out["y"]))])
     return out
@@ -53,6 +55,7 @@ def test_builtin_backend_is_nec2c_cpu_code(ref):
     assert b["rc"] == 0, b["text"]
     assert (b["requested"], b["backend"], b["last_used"]) == ("builtin", "builtin", "builtin")
     assert b["solve"]["ok"] and b["solve"]["status"] == "converged" and b["solve"]["max_error"] < 1e-10
+    assert b["phases"] == {}                 # nothing of the library's runs for builtin
     a = run("auto")
     assert a["rc"] == 0 and a["requested"] == "auto" and a["backend"] == "builtin"
     assert np.array_equal(a["yv"], b["yv"])
@@ -85,3 +88,8 @@ def test_hip_backend_matches_builtin_on_the_gpu():
     assert s["ok"] and s["status"] == "converged" and s["max_error"] < 1e-10
     assert s["relative_residual"] <= 1e-12
     assert abs(s["iterations"] - b["solve"]["iterations"]) <= 2
+    # the library's timing phases reach nec2c's performance_add
+    ph = h["phases"]
+    print("phases", ph, flush=True)
+    assert ph["fft_build"][1] == 1 and ph["fft_gmres"][1] == 1 and ph["fft_matvec"][1] == 1
+    assert all(sec > 0 for sec, _ in ph.values())

@@ -117,6 +117,12 @@ def test_served_results_equal_solo_solves_and_meet_the_reference(workload, alone
         assert rep["requests"] == len(spec) and rep["rejected"] == 0
         assert rep["converged"] == sum(len(p) for _, p in spec)
         assert rep["system_builds"] == len(keys) and rep["cache_hits"] == len(spec) - len(keys)
+        from nec2c_amd import timing
+        from schema_check import check
+        doc = srv.timing_document()
+        assert check(doc, timing.schema()) == []
+        assert {"fft_build", "fft_upload"} <= {p["name"] for p in doc["phases"]}
+        assert doc["serving"]["solves"] == rep["solves"]
         done = sorted(srv.done, key=lambda r: r.admitted)
         assert any(b.admitted < a.finished for a, b in zip(done, done[1:])), "no request overlapped another"
         check_results(srv, spec, alone)
