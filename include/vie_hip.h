@@ -120,6 +120,31 @@ typedef struct {
 } vie_hip_phase;
 size_t vie_hip_phases(vie_hip_phase *phases, size_t max);
 void vie_hip_phases_reset(void);
+/* ---- compact per-voxel resource (new: the reference's format writes a
+   per-voxel model as one BOX record per voxel, dielectric_io.c:472-512, whose
+   mesher then tests every pair of boxes, dielectric_mesh.c:81-123).  A
+   resource "NEC2C-DIELECTRIC 4" holds MATERIAL records (ER [EI|SIGMA], or
+   TABLE with SAMPLE records, as versions 2 and 3), one
+     VOXELS <nx> <ny> <nz> PITCH <m> ORIGIN <x0> <y0> <z0> MATERIALS|PERMITTIVITY <file>
+   record and END.  The side file holds nx*ny*nz little-endian uint32 material
+   ids (0: no dielectric) or (Re, Im) float64 permittivities ((1, 0): vacuum),
+   x slowest and z fastest; voxel (i, j, k) spans ORIGIN + PITCH (i, j, k) to
+   ORIGIN + PITCH (i+1, j+1, k+1).  The lattice is one by construction, so no
+   overlap check is needed.  Diagnostics name the record, value and line as
+   the reference's parser does. */
+typedef struct vie_hip_voxels vie_hip_voxels;
+int vie_hip_voxels_load(vie_hip_voxels **out, const char *path, dielectric_diagnostic *diagnostic);
+void vie_hip_voxels_free(vie_hip_voxels *voxels);
+/* Lattice and the number of dielectric voxels (the cells of a system). */
+int vie_hip_voxels_info(const vie_hip_voxels *voxels, size_t shape[3], double *pitch, double origin[3],
+                        size_t *cells);
+/* Per cell (the dielectric voxels in storage order): centre and size (3 each,
+   m) and material id (PERMITTIVITY: the cell index + 1); any may be NULL. */
+int vie_hip_voxels_cells(const vie_hip_voxels *voxels, double *centers, double *sizes, int *material_ids);
+/* dielectric_fft_prepare (pec = 0) or dielectric_fft_prepare_pec (pec = 1) of
+   the voxel model: the same system the equivalent cell list would give. */
+int dielectric_fft_prepare_voxels(dielectric_fft_system **out, const vie_hip_voxels *voxels, size_t conductors,
+                                  double k0, size_t extra_bytes, int pec, dielectric_diagnostic *diagnostic);
 /* Waits for all work on the current device. */
 int vie_hip_synchronize(void);
 /* Copies `bytes` (rounded down to 16) between two device buffers `repeats`

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib")
 SOURCES = ["vie_passes_f32.hip", "vie_passes_f64.hip", "spectrum.hip", "krylov.hip", "farfield.hip",
-           "conductor_rows.hip", "wire_volume.hip", "mesh_check.hip", "api.hip"]
+           "conductor_rows.hip", "wire_volume.hip", "mesh_check.hip", "voxels.cpp", "api.hip"]
 ARCH = os.environ.get("VIE_HIP_ARCH", "gfx950")
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
          # packed-FP32 instructions are kept out of every kernel
@@ -33,7 +33,7 @@ def hipcc() -> str:
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     cmd = [hipcc(), *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

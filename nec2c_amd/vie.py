@@ -142,7 +142,8 @@ EXPORTS = (
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
     "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
     "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_apply_noalloc",
-    "dielectric_fft_apply_noalloc_bytes", "vie_hip_phases", "vie_hip_phases_reset", "dielectric_fft_reserve",
+    "dielectric_fft_apply_noalloc_bytes", "vie_hip_phases", "vie_hip_phases_reset", "vie_hip_voxels_load", "vie_hip_voxels_free",
+    "vie_hip_voxels_info", "vie_hip_voxels_cells", "dielectric_fft_prepare_voxels", "dielectric_fft_reserve",
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
@@ -220,6 +221,11 @@ def lib() -> ctypes.CDLL:
         "dielectric_fft_apply_noalloc_bytes": (sz, [vp, i]),
         "vie_hip_phases": (sz, [vp, sz]),
         "vie_hip_phases_reset": (None, []),
+        "vie_hip_voxels_load": (i, [vp, ctypes.c_char_p, vp]),
+        "vie_hip_voxels_free": (None, [vp]),
+        "vie_hip_voxels_info": (i, [vp, vp, vp, vp, vp]),
+        "vie_hip_voxels_cells": (i, [vp, vp, vp, vp]),
+        "dielectric_fft_prepare_voxels": (i, [vp, vp, sz, ctypes.c_double, sz, i, vp]),
         "dielectric_fft_reserve": (i, [vp, ctypes.POINTER(GmresOptions)]),
         "vie_hip_copy_probe": (d, [sz, i]),
         "vie_hip_batch_create": (vp, []),
@@ -707,6 +713,41 @@ class MaterialSpec:
     samples: tuple = ()
 
 
+class Voxels:
+    """A compact per-voxel resource (vie_hip_voxels_load): NEC2C-DIELECTRIC 4
+    with one VOXELS record and a binary side file."""
+
+    def __init__(self, path):
+        self.v = ctypes.c_void_p()
+        self.diag = Diagnostic()
+        if not lib().vie_hip_voxels_load(ctypes.byref(self.v), str(path).encode(), ctypes.byref(self.diag)):
+            self.v = None
+            d = self.diag
+            raise ValueError(f"{d.resource.decode()}:{d.line}: {d.record.decode()} {d.value.decode()}: "
+                             f"{d.message.decode()}")
+        shape = (ctypes.c_size_t * 3)()
+        origin = (ctypes.c_double * 3)()
+        pitch, cells = ctypes.c_double(), ctypes.c_size_t()
+        lib().vie_hip_voxels_info(self.v, shape, ctypes.byref(pitch), origin, ctypes.byref(cells))
+        self.shape, self.pitch, self.origin, self.cells = tuple(shape), pitch.value, tuple(origin), cells.value
+
+    def cell_arrays(self):
+        """(centers [cells, 3], sizes [cells, 3], material ids [cells])."""
+        c = np.zeros((self.cells, 3))
+        s = np.zeros((self.cells, 3))
+        m = np.zeros(self.cells, dtype=np.int32)
+        lib().vie_hip_voxels_cells(self.v, c.ctypes.data, s.ctypes.data, m.ctypes.data)
+        return c, s, m
+
+    def close(self):
+        if getattr(self, "v", None):
+            lib().vie_hip_voxels_free(self.v)
+            self.v = None
+
+    def __del__(self):
+        self.close()
+
+
 class System:
     """dielectric_fft_system (dielectric_fft.h) on the GPU."""
 
@@ -765,6 +806,24 @@ class System:
     @property
     def order(self):
         return self.c + self.v
+
+    @classmethod
+    def from_voxels(cls, voxels: "Voxels", conductors, k0, pec=False, extra_bytes=0):
+        """dielectric_fft_prepare_voxels: the system of a compact voxel resource."""
+        self = cls.__new__(cls)
+        self.diag = Diagnostic()
+        s = ctypes.c_void_p()
+        ok = lib().dielectric_fft_prepare_voxels(ctypes.byref(s), voxels.v, int(conductors), float(k0),
+                                                 int(extra_bytes), int(bool(pec)), ctypes.byref(self.diag))
+        self.message = self.diag.message.decode()
+        if not ok:
+            self.s = None
+            raise ValueError(self.message)
+        self.s = s
+        self.c = int(conductors)
+        self.v = 3 * voxels.cells
+        self.op = LinearOperator()
+        return self
 
     def set(self, row, col, value) -> bool:
         p = lib().dielectric_fft_entry(self.s, int(row), int(col))

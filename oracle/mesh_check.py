@@ -52,6 +52,8 @@ def _ref():
         R.ref_mesh_check.argtypes = [sz, vp, vp, d, d, sz, i] + [vp] * 8 + [i] + [vp] * 4 + \
             [vp, vp, ctypes.POINTER(sz), vp, ctypes.POINTER(i), vp]
         R.ref_mesh_check.restype = i
+        R.ref_mesh_cells.argtypes = [sz, vp, vp, d, d, sz, vp, vp, vp, sz, ctypes.POINTER(sz), vp, vp]
+        R.ref_mesh_cells.restype = i
         _R = R
     return _R
 
@@ -112,6 +114,20 @@ def reference_mesh(minimum, maximum, frequency_mhz=1.0, max_kh=0.5, max_cells=1_
                                py.ctypes.data, pz.ctypes.data, area.ctypes.data, rec, val, ctypes.byref(line),
                                gaps.ctypes.data, ctypes.byref(has), cl.ctypes.data)
     return (bool(ok), rec.value.decode(), val.value.decode(), line.value, gaps[0], gaps[1], has.value, cl)
+
+
+def reference_cells(minimum, maximum, frequency_mhz=1.0, max_kh=0.5, max_cells=1_000_000):
+    """Cells of the reference's dielectric_mesh_build for one region per box:
+    (ok, record, value, centers [n, 3], sizes [n, 3], box index [n])."""
+    mn, mx = _f64(minimum, (-1, 3)), _f64(maximum, (-1, 3))
+    cap = max_cells
+    c, s, b = np.zeros((cap, 3)), np.zeros((cap, 3)), np.zeros(cap, dtype=np.uintp)
+    n = ctypes.c_size_t()
+    rec, val = ctypes.create_string_buffer(32), ctypes.create_string_buffer(128)
+    ok = _ref().ref_mesh_cells(len(mn), mn.ctypes.data, mx.ctypes.data, frequency_mhz, max_kh, max_cells,
+                               c.ctypes.data, s.ctypes.data, b.ctypes.data, cap, ctypes.byref(n), rec, val)
+    k = min(n.value, cap)
+    return bool(ok), rec.value.decode(), val.value.decode(), c[:k], s[:k], b[:k]
 
 
 def wire(start, end, radius, segments=1):
