@@ -106,12 +106,13 @@ def pass_kernel_name(pas, P, v, E=4096):
     cap = (8, 16, 32, 16, 8)[pas]
     wide = 2 if pas == 2 else 1
     l0 = min(wide * E // P, cap)
-    lw = (l0, max(l0 // 2, 1), max(l0 // 4, 1), max(l0 // 2, 1))[v]
-    wg = wide * (256, 128, 64, 256)[v]
+    lw = (l0, max(l0 // 2, 1), max(l0 // 4, 1), max(l0 // 2, 1))[v & 3]
+    wg = wide * (256, 128, 64, 256)[v & 3]
+    pair = ",true" if pas == 2 and v >= 4 else ""
     kern = ("k_zfwd<float,{P},{lw},{wg}>", "k_ypass<float,{P},{lw},{wg},false>",
-            "k_xpass<float,{P},{lw},{wg}> (X FFT + 3x3 spectral multiply + inverse X)",
+            "k_xpass<float,{P},{lw},{wg}{pair}> (X FFT + 3x3 spectral multiply + inverse X)",
             "k_ypass<float,{P},{lw},{wg},true>", "k_zinv<float,{P},{lw},{wg}>")[pas]
-    return kern.format(P=P, lw=lw, wg=wg)
+    return kern.format(P=P, lw=lw, wg=wg, pair=pair)
 
 
 def cpu_baseline(n, pitch, eps, case, iterations_needed, restart, sample_iters=10):

@@ -145,6 +145,11 @@ int vie_hip_voxels_cells(const vie_hip_voxels *voxels, double *centers, double *
    the voxel model: the same system the equivalent cell list would give. */
 int dielectric_fft_prepare_voxels(dielectric_fft_system **out, const vie_hip_voxels *voxels, size_t conductors,
                                   double k0, size_t extra_bytes, int pec, dielectric_diagnostic *diagnostic);
+/* Sets the launch variant of each pass (10 ints: FP64 then FP32, passes
+   z-forward, y-forward, x, y-inverse, z-inverse) instead of the tuner's choice,
+   for tests and measurements (new).  Every variant computes the same
+   arithmetic.  0 when a variant does not exist on this geometry. */
+int vie_fft_set_variants(vie_fft_plan *plan, const int *variants);
 /* Waits for all work on the current device. */
 int vie_hip_synchronize(void);
 /* Copies `bytes` (rounded down to 16) between two device buffers `repeats`

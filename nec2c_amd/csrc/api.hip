@@ -341,8 +341,7 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
   const int reps = 10;
   bool ok = true;
   for (int pass = 0; pass < 5 && ok; pass++) {
-    int P = pass == 2 ? p->g.Px : (pass == 1 || pass == 3) ? p->g.Py : p->g.Pz;
-    int nv = vie::pass_variants(P);
+    int nv = vie::pass_variant_count<T>(p->g, pass);
     float best = 0;
     int best_v = 0;
     for (int var = 0; var < nv && ok; var++) {
@@ -389,6 +388,24 @@ extern "C" int vie_fft_tune(vie_fft_plan* p, int* variants) {
       variants[i] = p->var[0][i];
       variants[5 + i] = p->var[1][i];
     }
+  return 1;
+}
+
+extern "C" int vie_fft_set_variants(vie_fft_plan* p, const int* variants) {
+  if (!p || !variants) return 0;
+  for (int i = 0; i < 10; i++) {
+    const int pass = i % 5;
+    const int count = i < 5 ? vie::pass_variant_count<double>(p->g, pass) : vie::pass_variant_count<float>(p->g, pass);
+    if (variants[i] < 0 || variants[i] >= count) {
+      set_error("vie_fft_set_variants: pass %d has %d launch variants in %s", pass, count, i < 5 ? "FP64" : "FP32");
+      return 0;
+    }
+  }
+  for (int i = 0; i < 5; i++) {
+    p->var[0][i] = (unsigned char)variants[i];
+    p->var[1][i] = (unsigned char)variants[5 + i];
+  }
+  p->tuned[0] = p->tuned[1] = true;
   return 1;
 }
 

@@ -26,6 +26,9 @@ inline int pass_variants(int P) { return P >= 16 ? kPassVariants : 1; }
 template <typename T>
 int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin, cx<T>* yout, int precond,
                 hipStream_t s);
+// Launch variants of a pass on this geometry: the four shapes, and for the x
+// pass also those shapes with kz-mirror pairing where Pz allows it.
+template <typename T> int pass_variant_count(const Geom& g, int pass);
 
 // y = eps z - (eps-1) conv(z), z = precond ? D^{-1} xin : xin.  Volume part only.
 // ev (optional, 6 events): recorded before each of the five passes and after the last.

@@ -15,6 +15,8 @@
 // pass touches the minimum the pruned pencil pipeline allows.  The 1/(PxPyPz)
 // inverse scaling lives in the spectrum.
 #pragma once
+#include <algorithm>
+
 #include "fft_core.h"
 #include "vie_kernels.h"
 
@@ -160,7 +162,26 @@ __global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
 // FP32 at LW = 16).  The octant spectrum (13 MB at 128^3 in FP32) is read by
 // the eight mirror points of each entry from different workgroups; it stays
 // resident in the last-level cache, so those re-reads cost no HBM traffic.
-template <typename T, int PX, int LW, int WG>
+// Line (ky Pz + kz) of local line l of workgroup blk.  Unpaired: LW
+// consecutive lines.  Paired (kz mirrors): lines l < LW/2 are kz = gz LW/2 + l
+// of one ky and lines LW/2 + l their mirrors Pz - kz (kz = 0 pairs with Pz/2),
+// so both halves multiply by the same octant spectrum entries, which a wave
+// then loads once for both (needs Pz % LW == 0; the same workgroup count).
+template <bool PAIR, int LW>
+__device__ __forceinline__ long xline(const Geom& g, long blk, int l) {
+  if constexpr (!PAIR) {
+    return blk * LW + l;
+  } else {
+    constexpr int H = LW / 2;
+    const long groups = g.Pz / LW;
+    const long ky = blk / groups;
+    int kz = (int)(blk % groups) * H + (l < H ? l : l - H);
+    if (l >= H) kz = kz ? g.Pz - kz : g.Pz / 2;
+    return ky * g.Pz + kz;
+  }
+}
+
+template <typename T, int PX, int LW, int WG, bool PAIR = false>
 __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   constexpr int E = LW * PX, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
@@ -169,7 +190,7 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.Py * g.Pz;
-  const long line0 = (long)blockIdx.x * LW;
+  const long blk = blockIdx.x;
   const size_t plane = (size_t)g.Py * g.Pz;
   for (int i = tid; i < PX; i += WG) tw[i] = a.tw[i];
   cx<T> hold[3][EPT], next[EPT];
@@ -177,7 +198,7 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
 #define VIE_XLOAD(comp, i)                                                                   \
   do {                                                                                       \
     const int f_ = tid + (i) * WG, e_ = f_ / LW, l_ = f_ % LW;                               \
-    const long line_ = line0 + l_;                                                           \
+    const long line_ = xline<PAIR, LW>(g, blk, l_);                                          \
     next[i] = (E % WG == 0 || f_ < E) && line_ < lines && e_ < g.nx                          \
                   ? a.B[((size_t)(comp) * g.nx + e_) * plane + line_]                         \
                   : cx<T>{(T)0, (T)0};                                                       \
@@ -213,7 +234,7 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
     int f = tid + i * WG;
     if (E % WG == 0 || f < E) {
       int kx = f / LW, l = f % LW;
-      long line = line0 + l;
+      long line = xline<PAIR, LW>(g, blk, l);
       if (line < lines) {
         int ky = (int)(line / g.Pz), kz = (int)(line % g.Pz);
         int ox = kx <= g.Px / 2 ? kx : g.Px - kx;
@@ -245,7 +266,7 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
     fft_lines<T, PX, LW, WG, LW, 1, true>(lds, tw, tid);
     for (int f = tid; f < E; f += WG) {
       int e = f / LW, l = f % LW;
-      long line = line0 + l;
+      long line = xline<PAIR, LW>(g, blk, l);
       if (line < lines && e < g.nx) a.B[((size_t)comp * g.nx + e) * plane + line] = conj(lds[e * LW + l]);
     }
     __syncthreads();
@@ -321,7 +342,7 @@ constexpr int kZCap = 8, kYCap = 16, kXCap = 32;
 
 template <typename T, int P> struct PassLauncher {
   static constexpr int E = PassShape<T>::E;
-  template <int V> static void run(int pass, const OpArgs<T>& a, hipStream_t s) {
+  template <int V, bool PAIR = false> static void run(int pass, const OpArgs<T>& a, hipStream_t s) {
     constexpr int WG = tile_wg(V);
     constexpr int LZ = tile_lw(tile_l0(E, P, kZCap), V);
     constexpr int LY = tile_lw(tile_l0(E, P, kYCap), V);
@@ -332,18 +353,22 @@ template <typename T, int P> struct PassLauncher {
     switch (pass) {
       case 0: hipLaunchKernelGGL((k_zfwd<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
       case 1: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, false>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
-      case 2: hipLaunchKernelGGL((k_xpass<T, P, LX, 2 * WG>), dim3((xl + LX - 1) / LX), dim3(2 * WG), 0, s, a); break;
+      case 2: hipLaunchKernelGGL((k_xpass<T, P, LX, 2 * WG, PAIR>), dim3((xl + LX - 1) / LX), dim3(2 * WG), 0, s, a); break;
       case 3: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, true>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
       default: hipLaunchKernelGGL((k_zinv<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
     }
   }
+  // x-pass line group of variant v (for the paired mapping's Pz % LX test)
+  static constexpr int x_lines(int v) { return tile_lw(tile_l0(2 * E, P, kXCap), v & 3); }
   static void launch(int pass, int v, const OpArgs<T>& a, hipStream_t s) {
     if constexpr (P >= 16) {
-      switch (v) {
-        case 1: run<1>(pass, a, s); return;
-        case 2: run<2>(pass, a, s); return;
-        case 3: run<3>(pass, a, s); return;
-        default: run<0>(pass, a, s); return;
+      // variants 4-7 of the x pass: the shapes of 0-3 with kz-mirror pairing
+      const bool pair = pass == 2 && v >= 4;
+      switch (v & 3) {
+        case 1: pair ? run<1, true>(pass, a, s) : run<1>(pass, a, s); return;
+        case 2: pair ? run<2, true>(pass, a, s) : run<2>(pass, a, s); return;
+        case 3: pair ? run<3, true>(pass, a, s) : run<3>(pass, a, s); return;
+        default: pair ? run<0, true>(pass, a, s) : run<0>(pass, a, s); return;
       }
     } else {
       run<0>(pass, a, s);
@@ -393,6 +418,21 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
   OpArgs<T> a = op_args(v, xin, yout, precond);
   bool ok = dispatch_p<T>(pass_extent(v.g, pass), [&](auto L) { L.launch(pass, variant, a, s); });
   return ok && hipGetLastError() == hipSuccess;
+}
+
+template <typename T> int pass_variant_count(const Geom& g, int pass) {
+  const int P = pass_extent(g, pass);
+  if (P < 16) return 1;
+  if (pass != 2) return kPassVariants;
+  // paired variants need every shape's line group to divide Pz
+  int lx_min = 1 << 30, lx_max = 0;
+  dispatch_p<T>(P, [&](auto L) {
+    for (int v = 0; v < kPassVariants; v++) {
+      lx_min = std::min(lx_min, L.x_lines(v));
+      lx_max = std::max(lx_max, L.x_lines(v));
+    }
+  });
+  return lx_min >= 2 && g.Pz % lx_max == 0 && g.Pz >= lx_max ? 2 * kPassVariants : kPassVariants;
 }
 
 template <typename T>

@@ -4,5 +4,6 @@
 namespace vie {
 template int volume_apply<double>(const VolumeView<double>&, const c64*, c64*, int, hipStream_t, hipEvent_t*,
                                    DevIndex);
+template int pass_variant_count<double>(const Geom&, int);
 template int pass_launch<double>(const VolumeView<double>&, int, int, const c64*, c64*, int, hipStream_t);
 }  // namespace vie

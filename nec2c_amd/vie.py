@@ -142,7 +142,7 @@ EXPORTS = (
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
     "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
     "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_apply_noalloc",
-    "dielectric_fft_apply_noalloc_bytes", "vie_hip_phases", "vie_hip_phases_reset", "vie_hip_voxels_load", "vie_hip_voxels_free",
+    "dielectric_fft_apply_noalloc_bytes", "vie_hip_phases", "vie_hip_phases_reset", "vie_fft_set_variants", "vie_hip_voxels_load", "vie_hip_voxels_free",
     "vie_hip_voxels_info", "vie_hip_voxels_cells", "dielectric_fft_prepare_voxels", "dielectric_fft_reserve",
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
@@ -221,6 +221,7 @@ def lib() -> ctypes.CDLL:
         "dielectric_fft_apply_noalloc_bytes": (sz, [vp, i]),
         "vie_hip_phases": (sz, [vp, sz]),
         "vie_hip_phases_reset": (None, []),
+        "vie_fft_set_variants": (i, [vp, vp]),
         "vie_hip_voxels_load": (i, [vp, ctypes.c_char_p, vp]),
         "vie_hip_voxels_free": (None, [vp]),
         "vie_hip_voxels_info": (i, [vp, vp, vp, vp, vp]),
@@ -974,6 +975,12 @@ class System:
         if not lib().vie_fft_tune(plan, v):
             raise RuntimeError(f"tune failed: {last_error()}")
         return {"fp64": list(v[:5]), "fp32": list(v[5:])}
+
+    def set_variants(self, fp64, fp32):
+        """vie_fft_set_variants: five pass variants per precision; False when one does not exist."""
+        plan = lib().dielectric_fft_plan(self.s)
+        v = (ctypes.c_int * 10)(*(list(fp64) + list(fp32)))
+        return bool(plan) and bool(lib().vie_fft_set_variants(plan, v))
 
     def time_apply(self, precision=FP64, repeats=20):
         ms = ctypes.c_double()
