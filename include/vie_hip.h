@@ -35,6 +35,20 @@ typedef struct {
   int recycle;
 } vie_hip_solve_options;
 
+/* Hybrid system over a real (lossy dielectric) ground z < 0 of relative
+   permittivity ground_re + i ground_im (new: the reference's FFT path rejects
+   a real ground, dielectric_ground.c:28-29; its dense and GMRES paths add the
+   reflected tensor vie_halfspace_cuboids to every pair, dielectric_ground.c:
+   116-131).  The volume operator adds the reflected tensor, a convolution in
+   x, y and a correlation in z, through a second spectrum.  Every cell must
+   lie strictly above z = 0 (dielectric_ground.c:41-45); the Jacobi
+   preconditioner keeps the free-space self term, as the PEC variant does.
+   Pass nec2c's dielectric_ground_epsilon(frequency) (dielectric_ground.c:97-100).
+   Otherwise as dielectric_fft_prepare. */
+int dielectric_fft_prepare_ground(dielectric_fft_system **out, const dielectric_model *model,
+                                  const dielectric_mesh_result *mesh, size_t conductors, double k0,
+                                  size_t extra_bytes, double ground_re, double ground_im,
+                                  dielectric_diagnostic *diagnostic);
 /* Device-resident solve of a system prepared by dielectric_fft_prepare[_pec]
    (new; the drop-in for vie_gmres at dielectric_wire.c:1303 and
    dielectric_solver.c:526).  Same contract and report as vie_gmres. */
@@ -150,6 +164,18 @@ int dielectric_fft_prepare_voxels(dielectric_fft_system **out, const vie_hip_vox
    for tests and measurements (new).  Every variant computes the same
    arithmetic.  0 when a variant does not exist on this geometry. */
 int vie_fft_set_variants(vie_fft_plan *plan, const int *variants);
+/* The reflected tensor of a passive lower half-space z < 0 (permittivity
+   eps_re + i eps_im, eps_im <= 0) between the cells of an nx x ny x nz lattice
+   of cubes of edge h whose lowest layer is centred at height z0 > h / 2, on
+   the device (new; the reference evaluates it one pair at a time,
+   vie_halfspace_cuboids, vie_halfspace.c:230, for its dense GN 2 path):
+   out[((c nx + dx) ny + dy)(2 nz - 1) + s] for components c = xx, yy, zz, xy,
+   xz, yz (zx = -xz, zy = -yz), lateral offsets dx, dy >= 0 (in cells, test
+   minus source) and layer sums s = iz_test + iz_source; Gauss-Legendre order
+   `order` per transverse-wavenumber segment.  The field it maps a source
+   current to is the reference's N (dimensionless).  1 on success. */
+int vie_hip_ground_table(int nx, int ny, int nz, double h, double z0, double k0, double eps_re, double eps_im,
+                         int order, VIE_CDOUBLE *out);
 /* Waits for all work on the current device. */
 int vie_hip_synchronize(void);
 /* Copies `bytes` (rounded down to 16) between two device buffers `repeats`

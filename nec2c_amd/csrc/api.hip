@@ -195,6 +195,9 @@ struct vie_fft_plan {
   hipStream_t stream = nullptr;
   // device
   DevBuf spec64, spec32, work, tw64, tw32, eps64, eps32, map_dev;
+  DevBuf rspec64, rspec32;                // real ground: reflected spectrum [6][Px/2+1][Py/2+1][Pz]
+  c64 ground_eps = {1, 0};
+  double ground_z0 = 0;                   // centre height of the lowest cell layer
   DevBuf io_user, io_sys_in, io_sys_out;  // host-pointer apply staging
   dielectric_fft_system* wrap = nullptr;  // c = 0 system around this plan, for solves
   unsigned char var[2][5] = {};          // pass launch variants: [0] FP64, [1] FP32
@@ -212,6 +215,7 @@ template <> vie::VolumeView<double> vie_fft_plan::view<double>() {
   v.A = work.as<c64>();
   v.B = work.as<c64>() + a_elems;
   v.spec = spec64.as<c64>();
+  v.rspec = g.ground ? rspec64.as<c64>() : nullptr;
   v.eps = eps64.as<c64>();
   v.tw = tw64.as<c64>();
   for (int i = 0; i < 3; i++) v.self[i] = self[i];
@@ -225,6 +229,7 @@ template <> vie::VolumeView<float> vie_fft_plan::view<float>() {
   v.A = work.as<c32>();
   v.B = work.as<c32>() + a_elems;
   v.spec = spec32.as<c32>();
+  v.rspec = g.ground ? rspec32.as<c32>() : nullptr;
   v.eps = eps32.as<c32>();
   v.tw = tw32.as<c32>();
   for (int i = 0; i < 3; i++) v.self[i] = c32{(float)self[i].re, (float)self[i].im};
@@ -316,7 +321,7 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
   }
   TuneKey key;
   memset(&key, 0, sizeof(key));
-  key = TuneKey{slot, p->g.nx, p->g.ny, p->g.nz, p->g.Px, p->g.Py, p->g.Pz, p->g.pec};
+  key = TuneKey{slot, p->g.nx, p->g.ny, p->g.nz, p->g.Px, p->g.Py, p->g.Pz, p->g.pec + 2 * p->g.ground};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -331,7 +336,11 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
   if (!xin.alloc(n * sizeof(vie::cx<T>)) || !yout.alloc(n * sizeof(vie::cx<T>))) return 0;
   hipStream_t st = p->stream;
   HIP_OK(hipMemsetAsync(xin.p, 0, n * sizeof(vie::cx<T>), st));
-  memset(p->var[slot], 0, 5);
+  for (int pass = 0; pass < 5; pass++) {  // a valid start (the x pass over a ground needs paired groups)
+    int v0 = 0;
+    while (!vie::pass_variant_ok<T>(p->g, pass, v0)) v0++;
+    p->var[slot][pass] = (unsigned char)v0;
+  }
   vie::VolumeView<T> v = p->view<T>();
   // a zero input leaves zeros in the workspace, so every pass maps zeros to zeros
   if (!vie::volume_apply<T>(v, xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st)) return 0;
@@ -343,8 +352,9 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
   for (int pass = 0; pass < 5 && ok; pass++) {
     int nv = vie::pass_variant_count<T>(p->g, pass);
     float best = 0;
-    int best_v = 0;
+    int best_v = -1;
     for (int var = 0; var < nv && ok; var++) {
+      if (!vie::pass_variant_ok<T>(p->g, pass, var)) continue;
       for (int r = 0; r < 2; r++)
         ok = ok && vie::pass_launch<T>(v, pass, var, xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st);
       ok = ok && hipEventRecord(e0, st) == hipSuccess;
@@ -354,7 +364,7 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
       float ms = 0;
       ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
       // a later variant must win by 2% to displace an earlier one
-      if (ok && (var == 0 || ms < 0.98f * best)) {
+      if (ok && (best_v < 0 || ms < 0.98f * best)) {
         best = ms;
         best_v = var;
       }
@@ -364,7 +374,6 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (!ok) {
-    memset(p->var[slot], 0, 5);
     set_error("pass tuning failed");
     return 0;
   }
@@ -391,13 +400,38 @@ extern "C" int vie_fft_tune(vie_fft_plan* p, int* variants) {
   return 1;
 }
 
+extern "C" int vie_hip_ground_table(int nx, int ny, int nz, double h, double z0, double k0, double eps_re,
+                                    double eps_im, int order, double _Complex* out) {
+  g_last_error.clear();
+  if (nx < 1 || ny < 1 || nz < 1 || !(h > 0) || !(z0 - 0.5 * h > 0) || !(k0 > 0) || !std::isfinite(eps_re) ||
+      !std::isfinite(eps_im) || eps_re < 1 || eps_im > 0 || order < 8 || order > 256 || !out ||
+      (size_t)nx * ny * nz > ((size_t)1 << 26)) {
+    set_error("vie_hip_ground_table: invalid arguments");
+    return 0;
+  }
+  if (!ensure_device()) return 0;
+  const size_t n = 6 * (size_t)nx * ny * (2 * nz - 1);
+  DevBuf t;
+  hipStream_t st;
+  HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  int ok = t.alloc(n * sizeof(c64)) &&
+           vie::ground_table(nx, ny, nz, h, z0, k0, c64{eps_re, eps_im}, order, t.as<c64>(), st) &&
+           hipMemcpyAsync(out, t.p, n * sizeof(c64), hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipStreamSynchronize(st) == hipSuccess;
+  (void)hipStreamDestroy(st);
+  if (!ok && g_last_error.empty()) set_error("vie_hip_ground_table: quadrature or device failure");
+  return ok;
+}
+
 extern "C" int vie_fft_set_variants(vie_fft_plan* p, const int* variants) {
   if (!p || !variants) return 0;
   for (int i = 0; i < 10; i++) {
     const int pass = i % 5;
-    const int count = i < 5 ? vie::pass_variant_count<double>(p->g, pass) : vie::pass_variant_count<float>(p->g, pass);
-    if (variants[i] < 0 || variants[i] >= count) {
-      set_error("vie_fft_set_variants: pass %d has %d launch variants in %s", pass, count, i < 5 ? "FP64" : "FP32");
+    const bool valid = i < 5 ? vie::pass_variant_ok<double>(p->g, pass, variants[i])
+                             : vie::pass_variant_ok<float>(p->g, pass, variants[i]);
+    if (!valid) {
+      set_error("vie_fft_set_variants: variant %d of pass %d does not apply to this %s plan", variants[i], pass,
+                i < 5 ? "FP64" : "FP32");
       return 0;
     }
   }
@@ -411,8 +445,17 @@ extern "C" int vie_fft_set_variants(vie_fft_plan* p, const int* variants) {
 
 // Internal creation shared by vie_fft_create (pec = 0) and the PEC system.
 // xyz: box coordinates of the unknown cells; kshape_z: kernel z extent.
+// Real ground of a plan: the lower half-space's permittivity and the centre
+// height of the lattice's lowest cell layer above it.
+struct GroundSpec {
+  c64 eps;
+  double z0;
+};
+static const int kGroundOrder = 48;  // Gauss-Legendre points per transverse-wavenumber segment
+
 static vie_fft_plan* plan_create(const size_t box[3], size_t kshape_z, int pec, size_t zoff, size_t cells,
-                                 const size_t* xyz, double h, double k, c64 er, size_t limit) {
+                                 const size_t* xyz, double h, double k, c64 er, size_t limit,
+                                 const GroundSpec* ground = nullptr) {
   if (!cells || cells > SIZE_MAX / 3 || !std::isfinite(h) || h <= 0 || !std::isfinite(k) || k < 0 || !passive(er) ||
       !std::isfinite(k * h) || k * h > 1) {
     set_error("invalid plan arguments");
@@ -441,8 +484,14 @@ static vie_fft_plan* plan_create(const size_t box[3], size_t kshape_z, int pec, 
   g.kz = (int)kshape_z;
   g.pec = pec;
   g.zoff = (int)zoff;
+  g.ground = ground ? 1 : 0;
+  if (ground && (pec || g.Pz < 2 * g.nz - 1)) {
+    set_error("a real ground needs a free-space plan");
+    return nullptr;
+  }
   size_t bytes;
-  if (!plan_bytes(g, cells, &bytes) || bytes > limit) {
+  const size_t rspec_n = ground ? 6 * (size_t)(g.Px / 2 + 1) * (g.Py / 2 + 1) * g.Pz : 0;
+  if (!plan_bytes(g, cells, &bytes) || bytes + rspec_n * 24 > limit) {
     set_error("plan storage exceeds the byte limit");
     return nullptr;
   }
@@ -502,6 +551,26 @@ static vie_fft_plan* plan_create(const size_t box[3], size_t kshape_z, int pec, 
     int ks[3] = {g.nx, g.ny, g.kz}, P[3] = {g.Px, g.Py, g.Pz};
     ok = vie::build_spectrum(ks, P, h, k, p->spec64.as<c64>(), p->spec32.as<c32>(), &p->spec_info, p->stream) != 0;
     if (!ok && g_last_error.empty()) set_error("kernel spectrum quadrature failed");
+  }
+  if (ok && ground) {
+    p->ground_eps = ground->eps;
+    p->ground_z0 = ground->z0;
+    p->bytes += rspec_n * 24;
+    int P[3] = {g.Px, g.Py, g.Pz};
+    ok = p->rspec64.alloc(rspec_n * sizeof(c64)) && p->rspec32.alloc(rspec_n * sizeof(c32)) &&
+         vie::build_ground_spectrum(g.nx, g.ny, g.nz, P, h, ground->z0, k, ground->eps, kGroundOrder,
+                                    p->rspec64.as<c64>(), p->rspec32.as<c32>(), p->stream) != 0;
+    if (!ok && g_last_error.empty()) set_error("reflected (real-ground) spectrum failed");
+    // the x pass must keep each line's kz mirror in its workgroup
+    for (int slot = 0; slot < 2 && ok; slot++) {
+      int v = 0;
+      while (v < 8 && !(slot ? vie::pass_variant_ok<float>(g, 2, v) : vie::pass_variant_ok<double>(g, 2, v))) v++;
+      if (v == 8) {
+        set_error("no x-pass launch shape keeps the kz mirrors of this grid together (Pz = %d)", g.Pz);
+        ok = false;
+      }
+      p->var[slot][2] = (unsigned char)v;
+    }
   }
   if (ok) {
     for (int i = 0; i < 3; i++) p->self[i] = p->spec_info.self[i];
@@ -746,7 +815,8 @@ static c64 material_epsilon(const dielectric_material* m, double k0) {
 }
 
 static int prepare(dielectric_fft_system** out, const dielectric_model* model, const dielectric_mesh_result* mesh,
-                   size_t conductors, double k0, size_t extra, dielectric_diagnostic* diag, int pec) {
+                   size_t conductors, double k0, size_t extra, dielectric_diagnostic* diag, int pec,
+                   const c64* ground = nullptr) {
   PhaseTimer phase_timer_(PHASE_BUILD);
   g_last_error.clear();
   *out = nullptr;
@@ -776,6 +846,8 @@ static int prepare(dielectric_fft_system** out, const dielectric_model* model, c
       bool found = false;
       if (pec && mesh->cells[i].center[2] - .5 * mesh->cells[i].size[2] < 0)
         return diag_error(diag, "PEC FFT cells must lie in z >= 0");
+      if (ground && !(mesh->cells[i].center[2] - .5 * mesh->cells[i].size[2] > 0))
+        return diag_error(diag, "real ground requires strictly positive clearance for every volume cell");
       auto it = by_id.find(mesh->cells[i].material_id);
       if (it != by_id.end()) {
         value = material_epsilon(&model->materials[it->second], k0);
@@ -827,7 +899,8 @@ static int prepare(dielectric_fft_system** out, const dielectric_model* model, c
   }
   if (cells) {
     size_t limit = dielectric_fft_budget() - bytes;
-    s->plan = plan_create(shape, kz, pec, zoff, cells, xyz.data(), h, k0, er, limit);
+    GroundSpec gs{ground ? *ground : c64{1, 0}, origin[2]};
+    s->plan = plan_create(shape, kz, pec, zoff, cells, xyz.data(), h, k0, er, limit, ground ? &gs : nullptr);
     if (!s->plan) {
       std::string why = g_last_error;
       delete s;
@@ -866,6 +939,17 @@ extern "C" int dielectric_fft_prepare_pec(dielectric_fft_system** out, const die
                                           const dielectric_mesh_result* mesh, size_t conductors, double k0,
                                           size_t extra, dielectric_diagnostic* diag) {
   return prepare(out, model, mesh, conductors, k0, extra, diag, 1);
+}
+extern "C" int dielectric_fft_prepare_ground(dielectric_fft_system** out, const dielectric_model* model,
+                                             const dielectric_mesh_result* mesh, size_t conductors, double k0,
+                                             size_t extra, double ground_re, double ground_im,
+                                             dielectric_diagnostic* diag) {
+  PhaseTimer phase_timer_(PHASE_BUILD);
+  if (out) *out = nullptr;
+  if (!std::isfinite(ground_re) || !std::isfinite(ground_im) || ground_re < 1 || ground_im > 0)
+    return diag_error(diag, "real ground requires a passive finite permittivity (Re >= 1, Im <= 0)");
+  const c64 e{ground_re, ground_im};
+  return prepare(out, model, mesh, conductors, k0, extra, diag, 0, &e);
 }
 extern "C" void dielectric_fft_free(dielectric_fft_system* s) { delete s; }
 extern "C" size_t dielectric_fft_storage(const dielectric_fft_system* s) { return s ? s->bytes : 0; }
@@ -3194,7 +3278,7 @@ extern "C" int dielectric_fft_apply_bytes(const dielectric_fft_system* s, vie_hi
     double oct = 6.0 * (g.Px / 2 + 1) * (g.Py / 2 + 1) * (g.Pz / 2 + 1);
     p[0] = (3 * N + N + a) * S;
     p[1] = (a + b) * S;
-    p[2] = (2 * b + oct) * S;
+    p[2] = (2 * b + oct + (g.ground ? 6.0 * (g.Px / 2 + 1) * (g.Py / 2 + 1) * g.Pz : 0.0)) * S;
     p[3] = (b + a) * S;
     p[4] = (a + 3 * N + N + 3 * N) * S;
   }

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <vector>
 #include "vie_kernels.h"
 
@@ -764,6 +765,332 @@ int build_spectrum(const int ks[3], const int P[3], double h, double k, c64* spe
   (void)hipFree(w1);
   (void)hipFree(d_t);
   (void)hipFree(d_o);
+  (void)hipFree(d_trig);
+  return ok ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ real ground
+// Reflected tensor of a passive lower half-space z < 0 (the reference's
+// vie_halfspace_cuboids, vie_halfspace.c:73-203, which its dense and GMRES
+// paths add to the free-space tensor for GN 2, dielectric_ground.c:116-131),
+// for every pair of cells of a lattice of cubes of edge h above the ground.
+// The spectral integral over the transverse wavenumber u (propagating: u =
+// sin t, w = cos t, t in [0, pi/2]; evanescent: u = hypot(1, ev), w = -i ev)
+// and the azimuth phi has an integrand that factors into
+//   (terms in u, phi) x exp(-i k u (cos phi dx + sin phi dy) h) x exp(-i k w S) sinc^2(k w h / 2),
+// dx, dy the lateral offset in cells and S = z_test + z_source.  So one set
+// of nodes serves every pair: per node u the five azimuthal integrals
+// J(dx, dy) of {sin^2, cos^2, sin cos, cos, sin} (trapezoidal in phi, enough
+// points for the lattice's widest lateral phase), then the tensor for every
+// S as a sum over u.  The u nodes are Gauss-Legendre on the reference's
+// segments (t in [0, pi/2], then ev in [0, 1, 4, 16, 64] / scale and the
+// substrate branch point sqrt(Re eps - 1)), with scale = k x (the smallest
+// gap S - h of the lattice): the integrand of every larger S decays faster
+// on the same nodes.  Components xx, yy, zz, xy, xz, yz; zx = -xz and zy = -yz
+// (the reflected tensor is not symmetric).
+
+struct GroundNode {
+  double u;
+  c64 w, rs, rp;
+  c64 weight;   // -i k^3 / (8 pi^2) h^3 x (radial Jacobian x Gauss weight)
+  int nphi, phi0;  // azimuth points and their offset in the per-node tables
+};
+
+static std::complex<double> to_std(c64 a) { return {a.re, a.im}; }
+static c64 from_std(std::complex<double> a) { return c64{a.real(), a.imag()}; }
+
+// Fresnel coefficients of the lower half-space for transverse wavenumber u/k
+// (vie_halfspace.c:14-28 of the reference: outgoing/decaying branch of the
+// substrate's normal wavenumber).
+static bool ground_fresnel(std::complex<double> e, double u, std::complex<double>* rs, std::complex<double>* rp) {
+  using C = std::complex<double>;
+  if (e == C(1.0, 0.0)) {
+    *rs = *rp = 0.0;
+    return true;
+  }
+  C w = u <= 1 ? C(sqrt((1 - u) * (1 + u)), 0) : C(0, -sqrt((u - 1) * (u + 1)));
+  C v = std::sqrt(e - u * u);
+  if (v.imag() > 0) v = -v;
+  C a = w + v, b = e * w + v;
+  if (std::abs(a) == 0 || std::abs(b) == 0) return false;
+  *rs = (w - v) / a;
+  *rp = (e * w - v) / b;
+  return std::isfinite(rs->real()) && std::isfinite(rs->imag()) && std::isfinite(rp->real()) &&
+         std::isfinite(rp->imag());
+}
+
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton on the Legendre recurrence).
+static void gauss_nodes(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0);
+  w.assign(n, 0);
+  for (int j = 0; j < (n + 1) / 2; j++) {
+    double z = cos(kPi * (j + 0.75) / (n + 0.5)), d = 0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1, p1 = z;
+      for (int m = 2; m <= n; m++) {
+        double p2 = ((2 * m - 1) * z * p1 - (m - 1) * p0) / m;
+        p0 = p1;
+        p1 = p2;
+      }
+      d = n * (z * p1 - p0) / (z * z - 1);
+      double dz = p1 / d;
+      z -= dz;
+      if (fabs(dz) < 1e-16) break;
+    }
+    x[j] = -z;
+    x[n - 1 - j] = z;
+    w[j] = w[n - 1 - j] = 2 / ((1 - z * z) * d * d);
+  }
+}
+
+// J_f(node, dx, dy) for the five azimuthal weights, combined into the six
+// tensor components (times the node's weight):
+//   xx = rs J_ss - rp w^2 J_cc, yy = rs J_cc - rp w^2 J_ss, zz = rp u^2 (J_ss + J_cc),
+//   xy = -(rs + rp w^2) J_sc, xz = -rp u w J_c, yz = -rp u w J_s.
+// One thread per (node, dx, dy); phi tables hold (k u h cos, k u h sin) and
+// dphi shape(u, phi) x {s^2, c^2, s c, c, s} per azimuth point.
+__global__ __launch_bounds__(256) void k_ground_angular(const GroundNode* nodes, int U, const double* phase_tab,
+                                                        const double* wt_tab, int nx, int ny, c64* G) {
+  const size_t lat = (size_t)nx * ny;
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)U * lat) return;
+  const int q = (int)(id / lat);
+  const int l = (int)(id % lat);
+  const int dx = l / ny, dy = l % ny;
+  const GroundNode nd = nodes[q];
+  double jr[5] = {0, 0, 0, 0, 0}, ji[5] = {0, 0, 0, 0, 0};
+  for (int j = 0; j < nd.nphi; j++) {
+    const double* ph = phase_tab + 2 * (size_t)(nd.phi0 + j);
+    const double* wt = wt_tab + 5 * (size_t)(nd.phi0 + j);
+    double sn, cs;
+    sincos(ph[0] * dx + ph[1] * dy, &sn, &cs);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      jr[f] += wt[f] * cs;
+      ji[f] -= wt[f] * sn;
+    }
+  }
+  c64 J[5];
+  for (int f = 0; f < 5; f++) J[f] = c64{jr[f], ji[f]};
+  const c64 w2 = nd.w * nd.w, uw = nd.u * nd.w, rpw2 = nd.rp * w2, rpuw = nd.rp * uw;
+  c64 out[6];
+  out[0] = nd.rs * J[0] - rpw2 * J[1];
+  out[1] = nd.rs * J[1] - rpw2 * J[0];
+  out[2] = (nd.u * nd.u) * (nd.rp * (J[0] + J[1]));
+  out[3] = c64{0, 0} - (nd.rs + rpw2) * J[2];
+  out[4] = c64{0, 0} - rpuw * J[3];
+  out[5] = c64{0, 0} - rpuw * J[4];
+  for (int c = 0; c < 6; c++) G[((size_t)q * 6 + c) * lat + l] = nd.weight * out[c];
+}
+
+// T[c][dx][dy][s] = sum_q G[q][c][dx][dy] Z[q][s]; one thread per output, s fastest.
+__global__ __launch_bounds__(256) void k_ground_sum(const c64* G, const c64* Z, int U, int lat, int L, c64* T) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)6 * lat * L) return;
+  const int sidx = (int)(id % L);
+  const size_t cl = id / L;  // c * lat + l
+  const int c = (int)(cl / lat), l = (int)(cl % lat);
+  double ar = 0, ai = 0;
+  for (int q = 0; q < U; q++) {
+    const c64 g = G[((size_t)q * 6 + c) * lat + l], z = Z[(size_t)q * L + sidx];
+    ar += g.re * z.re - g.im * z.im;
+    ai += g.re * z.im + g.im * z.re;
+  }
+  T[id] = c64{ar, ai};
+}
+
+// Full DFT along the last axis: out[r][k] = sum_{s < L} in[r][s] exp(-2 pi i k s / P), k < P.
+__global__ __launch_bounds__(256) void k_dft_last(const c64* in, size_t rows, int L, int P, const double* cosT,
+                                                  const double* sinT, c64* out) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= rows * P) return;
+  const int k = (int)(id % P);
+  const size_t r = id / P;
+  double ar = 0, ai = 0;
+  for (int s = 0; s < L; s++) {
+    const c64 v = in[r * L + s];
+    const int m = (int)(((long)k * s) % P);
+    const double cw = cosT[m], sw = sinT[m];  // exp(-i theta) = cw - i sw
+    ar += v.re * cw + v.im * sw;
+    ai += v.im * cw - v.re * sw;
+  }
+  out[id] = c64{ar, ai};
+}
+
+// Host: the quadrature nodes of the reflected tensor for a lattice of nx x ny
+// lateral cells of edge h whose lowest cell centre is at height z0.
+static bool ground_nodes(int nx, int ny, double h, double z0, double k, c64 eps, int order,
+                         std::vector<GroundNode>& nodes, std::vector<double>& phase_tab, std::vector<double>& wt_tab) {
+  using C = std::complex<double>;
+  const C e = to_std(eps);
+  const double gap = 2 * z0 - h, scale = k * gap;
+  if (!(gap > 0) || !std::isfinite(scale) || !(scale > 0) || order < 8) return false;
+  std::vector<double> cut = {0, 1, 4, 16, 64};
+  const double branch = scale * sqrt(std::max(e.real() - 1, 0.0));
+  if (branch > 0 && branch < 64) cut.push_back(branch);
+  std::sort(cut.begin(), cut.end());
+  std::vector<double> gx, gw;
+  gauss_nodes(order, gx, gw);
+  const double rho = h * hypot((double)std::max(nx - 1, 0), (double)std::max(ny - 1, 0));
+  const C pref = C(0, -1) * (k * k * k / (8 * kPi * kPi)) * (h * h * h);
+  nodes.clear();
+  phase_tab.clear();
+  wt_tab.clear();
+  for (int seg = -1; seg < (int)cut.size() - 1; seg++) {
+    const double lo = seg < 0 ? 0 : cut[seg], hi = seg < 0 ? kPi / 2 : cut[seg + 1];
+    if (hi == lo) continue;
+    for (int j = 0; j < order; j++) {
+      const double t = 0.5 * ((hi - lo) * gx[j] + hi + lo), rw = 0.5 * (hi - lo) * gw[j];
+      double u;
+      C w, radial;
+      if (seg < 0) {
+        u = sin(t);
+        w = cos(t);
+        radial = sin(t) * rw;
+      } else {
+        const double ev = t / scale;
+        u = hypot(1.0, ev);
+        w = C(0, -ev);
+        radial = C(0, 1) * rw / scale;
+      }
+      C rs, rp;
+      if (!ground_fresnel(e, u, &rs, &rp)) return false;
+      GroundNode nd;
+      nd.u = u;
+      nd.w = from_std(w);
+      nd.rs = from_std(rs);
+      nd.rp = from_std(rp);
+      nd.weight = from_std(pref * radial);
+      // azimuth: periodic trapezoid, enough points for the widest lateral phase k u rho
+      int np = (int)ceil(1.25 * k * u * rho + 48.0);
+      np = std::max(np, 2 * order);
+      np = (np + 3) / 4 * 4;
+      nd.nphi = np;
+      nd.phi0 = (int)(phase_tab.size() / 2);
+      const double dphi = 2 * kPi / np;
+      for (int a = 0; a < np; a++) {
+        // quadrant points exact so the parity of the lattice sums is exact
+        double c, sn;
+        if ((4 * a) % np == 0) {
+          int qd = 4 * a / np;
+          c = qd == 0 ? 1 : qd == 2 ? -1 : 0;
+          sn = qd == 1 ? 1 : qd == 3 ? -1 : 0;
+        } else {
+          c = cos(dphi * a);
+          sn = sin(dphi * a);
+        }
+        auto sinc = [](double v) { return fabs(v) < 1e-5 ? 1 - v * v / 6 + v * v * v * v / 120 : sin(v) / v; };
+        const double sx = sinc(0.5 * k * u * c * h), sy = sinc(0.5 * k * u * sn * h);
+        const double shape = sx * sx * sy * sy * dphi;
+        phase_tab.push_back(k * u * h * c);
+        phase_tab.push_back(k * u * h * sn);
+        wt_tab.push_back(shape * sn * sn);
+        wt_tab.push_back(shape * c * c);
+        wt_tab.push_back(shape * sn * c);
+        wt_tab.push_back(shape * c);
+        wt_tab.push_back(shape * sn);
+      }
+      nodes.push_back(nd);
+    }
+  }
+  return true;
+}
+
+// exp(-i k w S) sinc^2(k w h / 2) for the S of every layer sum s (the product
+// of the two cells' z averages, vie_halfspace.c:61-69).
+static void ground_z(const std::vector<GroundNode>& nodes, double k, double h, double z0, int L, std::vector<c64>& Z) {
+  using C = std::complex<double>;
+  Z.resize(nodes.size() * (size_t)L);
+  for (size_t q = 0; q < nodes.size(); q++) {
+    const C kw = k * to_std(nodes[q].w);
+    const C a = 0.5 * kw * h;
+    const C sc = std::abs(a) < 1e-4 ? 1.0 - a * a / 6.0 + a * a * a * a / 120.0 : std::sin(a) / a;
+    for (int s = 0; s < L; s++) {
+      const double S = 2 * z0 + s * h;
+      Z[q * L + s] = from_std(std::exp(C(0, -1) * kw * S) * sc * sc);
+    }
+  }
+}
+
+int ground_table(int nx, int ny, int nz, double h, double z0, double k, c64 eps, int order, c64* T_dev,
+                 hipStream_t s) {
+  std::vector<GroundNode> nodes;
+  std::vector<double> ph, wt;
+  if (!ground_nodes(nx, ny, h, z0, k, eps, order, nodes, ph, wt)) return 0;
+  const int U = (int)nodes.size(), L = 2 * nz - 1;
+  const size_t lat = (size_t)nx * ny;
+  std::vector<c64> Z;
+  ground_z(nodes, k, h, z0, L, Z);
+  GroundNode* d_nodes = nullptr;
+  double *d_ph = nullptr, *d_wt = nullptr;
+  c64 *d_G = nullptr, *d_Z = nullptr;
+  bool ok = hipMalloc(&d_nodes, nodes.size() * sizeof(GroundNode)) == hipSuccess &&
+            hipMalloc(&d_ph, ph.size() * sizeof(double)) == hipSuccess &&
+            hipMalloc(&d_wt, wt.size() * sizeof(double)) == hipSuccess &&
+            hipMalloc(&d_G, (size_t)U * 6 * lat * sizeof(c64)) == hipSuccess &&
+            hipMalloc(&d_Z, Z.size() * sizeof(c64)) == hipSuccess;
+  ok = ok && hipMemcpyAsync(d_nodes, nodes.data(), nodes.size() * sizeof(GroundNode), hipMemcpyHostToDevice, s) ==
+                 hipSuccess &&
+       hipMemcpyAsync(d_ph, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
+       hipMemcpyAsync(d_wt, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess &&
+       hipMemcpyAsync(d_Z, Z.data(), Z.size() * sizeof(c64), hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok) {
+    const size_t n1 = (size_t)U * lat, n2 = 6 * lat * (size_t)L;
+    hipLaunchKernelGGL(k_ground_angular, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, s, d_nodes, U, d_ph, d_wt,
+                       nx, ny, d_G);
+    hipLaunchKernelGGL(k_ground_sum, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, d_G, d_Z, U, (int)lat, L,
+                       T_dev);
+    ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+  }
+  (void)hipFree(d_nodes);
+  (void)hipFree(d_ph);
+  (void)hipFree(d_wt);
+  (void)hipFree(d_G);
+  (void)hipFree(d_Z);
+  return ok ? 1 : 0;
+}
+
+int build_ground_spectrum(int nx, int ny, int nz, const int padded[3], double h, double z0, double k, c64 eps,
+                          int order, c64* rspec64, c32* rspec32, hipStream_t s) {
+  const int L = 2 * nz - 1, Px = padded[0], Py = padded[1], Pz = padded[2];
+  if (L > Pz) return 0;
+  const int Hx = Px / 2 + 1, Hy = Py / 2 + 1;
+  const size_t s0 = 6 * (size_t)nx * ny * L, s1 = 6 * (size_t)nx * ny * Pz, s2 = 6 * (size_t)Hx * ny * Pz,
+               s3 = 6 * (size_t)Hx * Hy * Pz;
+  const size_t big = std::max(std::max(s0, s1), std::max(s2, s3));
+  std::vector<double> trig;
+  size_t off[3];
+  const int P[3] = {Px, Py, Pz};
+  for (int a = 0; a < 3; a++) {
+    std::vector<double> c, sn;
+    trig_table(P[a], c, sn);
+    off[a] = trig.size();
+    trig.insert(trig.end(), c.begin(), c.end());
+    trig.insert(trig.end(), sn.begin(), sn.end());
+  }
+  c64 *w0 = nullptr, *w1 = nullptr;
+  double* d_trig = nullptr;
+  bool ok = hipMalloc(&w0, big * sizeof(c64)) == hipSuccess && hipMalloc(&w1, big * sizeof(c64)) == hipSuccess &&
+            hipMalloc(&d_trig, trig.size() * sizeof(double)) == hipSuccess &&
+            hipMemcpyAsync(d_trig, trig.data(), trig.size() * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess;
+  ok = ok && ground_table(nx, ny, nz, h, z0, k, eps, order, w0, s);
+  if (ok) {
+    // z: full DFT of the layer sums s = 0 .. 2 nz - 2 (no parity)
+    hipLaunchKernelGGL(k_dft_last, dim3((unsigned)((s1 + 255) / 256)), dim3(256), 0, s, w0, 6 * (size_t)nx * ny, L, Pz,
+                       d_trig + off[2], d_trig + off[2] + Pz, w1);
+    // x, y: even / odd lateral transforms (xy odd in x and y, xz odd in x, yz odd in y)
+    const int odd_x = (1 << 3) | (1 << 4), odd_y = (1 << 3) | (1 << 5);
+    hipLaunchKernelGGL(k_axis_transform, dim3((unsigned)((s2 + 255) / 256)), dim3(256), 0, s, w1, w0, nx, ny, Pz, 0, Px,
+                       odd_x, d_trig + off[0], d_trig + off[0] + Px);
+    hipLaunchKernelGGL(k_axis_transform, dim3((unsigned)((s3 + 255) / 256)), dim3(256), 0, s, w0, w1, Hx, ny, Pz, 1, Py,
+                       odd_y, d_trig + off[1], d_trig + off[1] + Py);
+    const double scale = 1.0 / ((double)Px * Py * Pz);
+    hipLaunchKernelGGL(k_finish_spectrum, dim3((unsigned)((s3 + 255) / 256)), dim3(256), 0, s, w1, s3, scale, rspec64,
+                       rspec32);
+    ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+  }
+  (void)hipFree(w0);
+  (void)hipFree(w1);
   (void)hipFree(d_trig);
   return ok ? 1 : 0;
 }

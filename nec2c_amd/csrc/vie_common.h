@@ -96,6 +96,7 @@ struct Geom {
   int kz;                // kernel z extent (nz, or 2 nz + gap for the PEC image)
   int pec;               // 1: image cells mirrored below the physical box
   int zoff;              // z index of the first physical plane inside the FFT grid
+  int ground = 0;        // 1: a reflected (real-ground) spectrum is added in the x pass
   __host__ __device__ size_t nbox() const { return (size_t)nx * ny * nz; }
 };
 

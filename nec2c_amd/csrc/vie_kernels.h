@@ -11,6 +11,7 @@ template <typename T> struct VolumeView {
   cx<T>* A;           // [3][nx][ny][Pz] workspace
   cx<T>* B;           // [3][nx][Py][Pz] workspace
   const cx<T>* spec;  // [6][Px/2+1][Py/2+1][Pz/2+1], scaled by 1/(Px Py Pz)
+  const cx<T>* rspec = nullptr;  // real ground: [6][Px/2+1][Py/2+1][Pz] reflected spectrum
   const cx<T>* eps;   // [nbox]
   const cx<T>* tw;    // twiddle tables, x then y then z
   cx<T> self[3];
@@ -26,9 +27,13 @@ inline int pass_variants(int P) { return P >= 16 ? kPassVariants : 1; }
 template <typename T>
 int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin, cx<T>* yout, int precond,
                 hipStream_t s);
-// Launch variants of a pass on this geometry: the four shapes, and for the x
-// pass also those shapes with kz-mirror pairing where Pz allows it.
+// Launch variants of a pass: the four shapes (one below 16 points), and for
+// the x pass also those shapes with kz-mirror pairing (variants 4-7).  A
+// variant applies to a geometry when pass_variant_ok: paired groups must
+// divide Pz, and over a real ground every x-pass group must hold the kz
+// mirror of each of its lines (paired, or whole ky rows).
 template <typename T> int pass_variant_count(const Geom& g, int pass);
+template <typename T> bool pass_variant_ok(const Geom& g, int pass, int variant);
 
 // y = eps z - (eps-1) conv(z), z = precond ? D^{-1} xin : xin.  Volume part only.
 // ev (optional, 6 events): recorded before each of the five passes and after the last.
@@ -48,6 +53,18 @@ struct SpectrumResult {
 // kshape: lattice extent of the kernel (z includes the PEC image).
 int build_spectrum(const int kshape[3], const int padded[3], double h, double k, c64* spec64,
                    c32* spec32, SpectrumResult* out, hipStream_t s);
+
+// ---- reflected (real-ground) tensor (FP64) ----
+// T[6][nx][ny][2 nz - 1] on the device: the reflected tensor of a lower
+// half-space of permittivity eps between cells (dx, dy) apart laterally whose
+// heights add to 2 z0 + s h (z0: centre of the lowest cell layer), at
+// Gauss-Legendre order `order` per u segment.
+int ground_table(int nx, int ny, int nz, double h, double z0, double k, c64 eps, int order, c64* T_dev,
+                 hipStream_t s);
+// Its spectrum [6][Px/2+1][Py/2+1][Pz] (lateral octant, full kz), scaled by
+// 1/(Px Py Pz), into rspec64 and its FP32 rounding into rspec32.
+int build_ground_spectrum(int nx, int ny, int nz, const int padded[3], double h, double z0, double k, c64 eps,
+                          int order, c64* rspec64, c32* rspec32, hipStream_t s);
 
 // ---- scattered far field (farfield.hip) ----
 // Slices of the cells per direction tile (the partial sums per direction are

@@ -29,6 +29,7 @@ template <typename T> struct OpArgs {
   cx<T>* A;           // [3][nx][ny][Pz]
   cx<T>* B;           // [3][nx][Py][Pz]
   const cx<T>* spec;  // [6][Px/2+1][Py/2+1][Pz/2+1] octant spectrum / G
+  const cx<T>* rspec; // real ground: [6][Px/2+1][Py/2+1][Pz] reflected spectrum / G
   const cx<T>* eps;   // [nbox] relative permittivity (1 in holes)
   const cx<T>* tw;    // twiddles: x table (Px), then y (Py), then z (Pz)
   cx<T> self[3];      // self-term diagonal N_ii(0)
@@ -229,9 +230,33 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   // spectral multiply: dst_i = sum_j T^_ij src_j   (vie_fft.c:253-256)
   const int Hx = g.Px / 2 + 1, Hy = g.Py / 2 + 1, Hz = g.Pz / 2 + 1;
   const size_t oct = (size_t)Hx * Hy * Hz;
+  // real ground: + R^(k) X(kx, ky, -kz), the reflected tensor being a
+  // correlation in z (the z-flipped source's transform is X at -kz).  The
+  // line of -kz is in this group (paired halves, or whole ky rows); its
+  // element of the same kx sits in the lane l' - l away.
+  int mlane = 0;
+  if (g.ground) {
+    const int l = tid % LW;
+    int lm;
+    if constexpr (PAIR) {
+      constexpr int H = LW / 2;
+      lm = ((blk % (g.Pz / LW)) == 0 && (l % H) == 0) ? l : (l ^ H);
+    } else {
+      const int kz = l % g.Pz;
+      lm = l - kz + ((g.Pz - kz) & (g.Pz - 1));
+    }
+    mlane = (tid - l + lm) & 63;
+  }
+  const size_t rstride = (size_t)Hx * Hy * g.Pz;
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     int f = tid + i * WG;
+    cx<T> m0 = {(T)0, (T)0}, m1 = m0, m2 = m0;
+    if (g.ground) {  // uniform branch: every lane takes part in the shuffles
+      m0 = cx<T>{__shfl(hold[0][i].re, mlane), __shfl(hold[0][i].im, mlane)};
+      m1 = cx<T>{__shfl(hold[1][i].re, mlane), __shfl(hold[1][i].im, mlane)};
+      m2 = cx<T>{__shfl(hold[2][i].re, mlane), __shfl(hold[2][i].im, mlane)};
+    }
     if (E % WG == 0 || f < E) {
       int kx = f / LW, l = f % LW;
       long line = xline<PAIR, LW>(g, blk, l);
@@ -249,9 +274,24 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
         cx<T> txz = (sx * sz) * a.spec[4 * oct + o];
         cx<T> tyz = (sy * sz) * a.spec[5 * oct + o];
         cx<T> s0 = hold[0][i], s1 = hold[1][i], s2 = hold[2][i];
-        hold[0][i] = txx * s0 + txy * s1 + txz * s2;
-        hold[1][i] = txy * s0 + tyy * s1 + tyz * s2;
-        hold[2][i] = txz * s0 + tyz * s1 + tzz * s2;
+        cx<T> d0 = txx * s0 + txy * s1 + txz * s2;
+        cx<T> d1 = txy * s0 + tyy * s1 + tyz * s2;
+        cx<T> d2 = txz * s0 + tyz * s1 + tzz * s2;
+        if (g.ground) {
+          // reflected tensor: xy odd in x and y, xz odd in x, yz odd in y;
+          // zx = -xz, zy = -yz
+          const size_t r = ((size_t)ox * Hy + oy) * g.Pz + kz;
+          cx<T> rxx = a.rspec[r], ryy = a.rspec[rstride + r], rzz = a.rspec[2 * rstride + r];
+          cx<T> rxy = (sx * sy) * a.rspec[3 * rstride + r];
+          cx<T> rxz = sx * a.rspec[4 * rstride + r];
+          cx<T> ryz = sy * a.rspec[5 * rstride + r];
+          d0 = d0 + rxx * m0 + rxy * m1 + rxz * m2;
+          d1 = d1 + rxy * m0 + ryy * m1 + ryz * m2;
+          d2 = d2 - rxz * m0 - ryz * m1 + rzz * m2;
+        }
+        hold[0][i] = d0;
+        hold[1][i] = d1;
+        hold[2][i] = d2;
       }
     }
   }
@@ -371,7 +411,7 @@ template <typename T, int P> struct PassLauncher {
         default: pair ? run<0, true>(pass, a, s) : run<0>(pass, a, s); return;
       }
     } else {
-      run<0>(pass, a, s);
+      pass == 2 && v >= 4 ? run<0, true>(pass, a, s) : run<0>(pass, a, s);
     }
   }
 };
@@ -403,6 +443,7 @@ static OpArgs<T> op_args(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, 
   a.A = v.A;
   a.B = v.B;
   a.spec = v.spec;
+  a.rspec = v.rspec;
   a.eps = v.eps;
   a.tw = v.tw;
   for (int i = 0; i < 3; i++) a.self[i] = v.self[i];
@@ -422,17 +463,21 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
 
 template <typename T> int pass_variant_count(const Geom& g, int pass) {
   const int P = pass_extent(g, pass);
-  if (P < 16) return 1;
-  if (pass != 2) return kPassVariants;
-  // paired variants need every shape's line group to divide Pz
-  int lx_min = 1 << 30, lx_max = 0;
-  dispatch_p<T>(P, [&](auto L) {
-    for (int v = 0; v < kPassVariants; v++) {
-      lx_min = std::min(lx_min, L.x_lines(v));
-      lx_max = std::max(lx_max, L.x_lines(v));
-    }
-  });
-  return lx_min >= 2 && g.Pz % lx_max == 0 && g.Pz >= lx_max ? 2 * kPassVariants : kPassVariants;
+  if (pass == 2) return 2 * kPassVariants;
+  return P >= 16 ? kPassVariants : 1;
+}
+
+template <typename T> bool pass_variant_ok(const Geom& g, int pass, int v) {
+  const int P = pass_extent(g, pass);
+  if (v < 0 || v >= pass_variant_count<T>(g, pass)) return false;
+  if (pass != 2) return true;
+  if (P < 16 && (v & 3) != 0) return false;
+  int lx = 0;
+  dispatch_p<T>(P, [&](auto L) { lx = L.x_lines(v & 3); });
+  const bool paired = v >= 4;
+  if (paired && !(lx >= 2 && g.Pz >= lx && g.Pz % lx == 0)) return false;
+  if (g.ground && !paired && lx % g.Pz != 0) return false;
+  return true;
 }
 
 template <typename T>

@@ -5,5 +5,6 @@ namespace vie {
 template int volume_apply<double>(const VolumeView<double>&, const c64*, c64*, int, hipStream_t, hipEvent_t*,
                                    DevIndex);
 template int pass_variant_count<double>(const Geom&, int);
+template bool pass_variant_ok<double>(const Geom&, int, int);
 template int pass_launch<double>(const VolumeView<double>&, int, int, const c64*, c64*, int, hipStream_t);
 }  // namespace vie

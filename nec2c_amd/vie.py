@@ -142,7 +142,7 @@ EXPORTS = (
     "dielectric_fft_upload_rhs", "dielectric_fft_solve_resident", "dielectric_fft_download_solution",
     "dielectric_fft_time_apply", "dielectric_fft_apply_bytes", "vie_fft_set_permittivities",
     "vie_hip_synchronize", "vie_hip_copy_probe", "dielectric_fft_apply_ex", "dielectric_fft_apply_noalloc",
-    "dielectric_fft_apply_noalloc_bytes", "vie_hip_phases", "vie_hip_phases_reset", "vie_fft_set_variants", "vie_hip_voxels_load", "vie_hip_voxels_free",
+    "dielectric_fft_apply_noalloc_bytes", "vie_hip_phases", "vie_hip_phases_reset", "vie_fft_set_variants", "vie_hip_ground_table", "dielectric_fft_prepare_ground", "vie_hip_voxels_load", "vie_hip_voxels_free",
     "vie_hip_voxels_info", "vie_hip_voxels_cells", "dielectric_fft_prepare_voxels", "dielectric_fft_reserve",
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
@@ -222,6 +222,10 @@ def lib() -> ctypes.CDLL:
         "vie_hip_phases": (sz, [vp, sz]),
         "vie_hip_phases_reset": (None, []),
         "vie_fft_set_variants": (i, [vp, vp]),
+        "dielectric_fft_prepare_ground": (i, [vp, vp, vp, sz, ctypes.c_double, sz, ctypes.c_double,
+                                              ctypes.c_double, vp]),
+        "vie_hip_ground_table": (i, [i, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, i, vp]),
         "vie_hip_voxels_load": (i, [vp, ctypes.c_char_p, vp]),
         "vie_hip_voxels_free": (None, [vp]),
         "vie_hip_voxels_info": (i, [vp, vp, vp, vp, vp]),
@@ -549,6 +553,17 @@ def phases(reset=False):
     return out
 
 
+def ground_table(shape, h, z0, k0, eps, order=48):
+    """vie_hip_ground_table: reflected tensor [6, nx, ny, 2 nz - 1] (xx, yy, zz, xy, xz, yz)."""
+    nx, ny, nz = shape
+    out = np.zeros(6 * nx * ny * (2 * nz - 1), dtype=np.complex128)
+    eps = complex(eps)
+    if not lib().vie_hip_ground_table(nx, ny, nz, float(h), float(z0), float(k0), eps.real, eps.imag, int(order),
+                                      out.ctypes.data):
+        raise RuntimeError(f"ground table failed: {last_error()}")
+    return out.reshape(6, nx, ny, 2 * nz - 1)
+
+
 def last_error() -> str:
     return lib().vie_hip_last_error().decode()
 
@@ -753,9 +768,11 @@ class System:
     """dielectric_fft_system (dielectric_fft.h) on the GPU."""
 
     def __init__(self, cells, materials, conductors, k0, pec=False, extra_bytes=0, pitch=None,
-                 material_ids=None):
+                 material_ids=None, ground=None):
         """cells: a list of Cell, or an (N, 3) array of cell centres together
-        with ``pitch`` (and optional per-cell ``material_ids``, default 1)."""
+        with ``pitch`` (and optional per-cell ``material_ids``, default 1).
+        ground: the complex permittivity of a real ground below z = 0
+        (dielectric_fft_prepare_ground), or None."""
         L = lib()
         self._mats = (Material * max(1, len(materials)))()
         self._samples = []
@@ -792,9 +809,15 @@ class System:
         self.mesh.cell_count = len(cells)
         self.diag = Diagnostic()
         s = ctypes.c_void_p()
-        fn = L.dielectric_fft_prepare_pec if pec else L.dielectric_fft_prepare
-        ok = fn(ctypes.byref(s), ctypes.byref(self.model), ctypes.byref(self.mesh), int(conductors),
-                float(k0), int(extra_bytes), ctypes.byref(self.diag))
+        if ground is not None:
+            g = complex(ground)
+            ok = L.dielectric_fft_prepare_ground(ctypes.byref(s), ctypes.byref(self.model), ctypes.byref(self.mesh),
+                                                 int(conductors), float(k0), int(extra_bytes), g.real, g.imag,
+                                                 ctypes.byref(self.diag))
+        else:
+            fn = L.dielectric_fft_prepare_pec if pec else L.dielectric_fft_prepare
+            ok = fn(ctypes.byref(s), ctypes.byref(self.model), ctypes.byref(self.mesh), int(conductors),
+                    float(k0), int(extra_bytes), ctypes.byref(self.diag))
         self.message = self.diag.message.decode()
         if not ok:
             self.s = None

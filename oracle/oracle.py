@@ -514,6 +514,9 @@ class ReferenceLibrary:
                                        ctypes.c_int, ctypes.c_int, d, ctypes.POINTER(ctypes.c_int)]
         L.refshim_cuboid_pec_n.argtypes = [d, d, d, d, ctypes.c_double, d]
         L.refshim_backend_set.argtypes = [ctypes.c_char_p]
+        L.refshim_halfspace_cuboids.argtypes = [d, d, d, d, ctypes.c_double, ctypes.c_double, ctypes.c_double, d,
+                                                ctypes.POINTER(ctypes.c_int)]
+        L.refshim_halfspace_fresnel.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, d]
         L.refshim_create.argtypes = [sz, ctypes.c_size_t, sz, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_size_t]
         L.refshim_create.restype = vp
@@ -565,6 +568,22 @@ class ReferenceLibrary:
         ok = self.L.refshim_cuboid_n(*[_dp(a) for a in args], float(k), rtol, atol, max_order,
                                      far_order, _dp(out), ctypes.byref(order))
         return out.view(np.complex128).reshape(3, 3).copy() if ok else None
+
+    def halfspace_cuboids(self, tc, tsz, sc, ssz, k, eps):
+        """Reflected tensor of a lower half-space of permittivity eps (vie_halfspace_cuboids)."""
+        args = [np.ascontiguousarray(v, dtype=np.float64) for v in (tc, tsz, sc, ssz)]
+        out = np.zeros(18)
+        order = ctypes.c_int(0)
+        eps = complex(eps)
+        ok = self.L.refshim_halfspace_cuboids(*[_dp(a) for a in args], float(k), eps.real, eps.imag, _dp(out),
+                                              ctypes.byref(order))
+        return (out.view(np.complex128).reshape(3, 3).copy(), order.value) if ok else (None, order.value)
+
+    def halfspace_fresnel(self, eps, u):
+        out = np.zeros(4)
+        eps = complex(eps)
+        ok = self.L.refshim_halfspace_fresnel(eps.real, eps.imag, float(u), _dp(out))
+        return (complex(out[0], out[1]), complex(out[2], out[3])) if ok else None
 
     def cuboid_pec_n(self, tc, tsz, sc, ssz, k):
         args = [np.ascontiguousarray(v, dtype=np.float64) for v in (tc, tsz, sc, ssz)]

@@ -9,6 +9,7 @@
 #include "dielectric_fft.h"
 #include "vie_fft.h"
 #include "vie_ground.h"
+#include "vie_halfspace.h"
 #include "vie_iterative.h"
 #include "vie_kernel.h"
 #include <stdlib.h>
@@ -54,6 +55,31 @@ int refshim_cuboid_pec_n(const double tc[3], const double tsz[3], const double s
   memcpy(s.center, sc, 3*sizeof(double)); memcpy(s.size, ssz, 3*sizeof(double));
   ok = vie_cuboid_pec_n(&t, &s, k, 0, NULL, m, NULL);
   if(ok) memcpy(out, m, sizeof(m));
+  return ok;
+}
+
+/* vie_halfspace_cuboids (vie_halfspace.c:230) with default options: the
+   reflected tensor of a passive lower half-space eps = er + i ei (ei <= 0)
+   between testing cuboid (tc, tsz) and source cuboid (sc, ssz); out = 18
+   doubles (N), order_used the converged quadrature order. */
+int refshim_halfspace_cuboids(const double tc[3], const double tsz[3], const double sc[3],
+                              const double ssz[3], double k, double er, double ei, double *out,
+                              int *order_used)
+{
+  vie_cuboid t, s; complex double n[3][3], h[3][3]; vie_halfspace_report r; int ok;
+  memcpy(t.center, tc, 3*sizeof(double)); memcpy(t.size, tsz, 3*sizeof(double));
+  memcpy(s.center, sc, 3*sizeof(double)); memcpy(s.size, ssz, 3*sizeof(double));
+  ok = vie_halfspace_cuboids(&t, &s, k, er + I*ei, NULL, n, h, &r);
+  if(ok) memcpy(out, n, sizeof(n));
+  if(order_used) *order_used = r.order;
+  return ok;
+}
+
+/* vie_halfspace_fresnel (vie_halfspace.c:14): out = rs (2 doubles), rp (2). */
+int refshim_halfspace_fresnel(double er, double ei, double u, double *out)
+{
+  complex double rs, rp; int ok = vie_halfspace_fresnel(er + I*ei, u, &rs, &rp);
+  if(ok) { out[0] = creal(rs); out[1] = cimag(rs); out[2] = creal(rp); out[3] = cimag(rp); }
   return ok;
 }
 
