@@ -387,21 +387,30 @@ def bench_serve(args):
 SLAB_H = 0.01          # voxel pitch (m) of the slab workload (nec2c_amd/workloads.py yagi_slab)
 
 
-def slab_geometry(n):
+def slab_geometry(n, ground=False):
     from nec2c_amd import workloads as W
-    return W.yagi_slab(n)
+    return W.yagi_slab(n, ground=ground)
 
 
-def slab_system(n, opts, keep_blocks=False):
-    """Prepared GPU system of the slab (spectrum, C/B/D, rhs, reserved basis).
-    Returns (system, yagi, k0, eps_cell, blocks or None, prepare seconds)."""
+def slab_ground_epsilon():
+    from nec2c_amd import workloads as W
+    er, sigma = W.SLAB_GROUND
+    return W.nec_ground_epsilon(er, sigma, W.SLAB_FREQ / 1e6)
+
+
+def slab_system(n, opts, keep_blocks=False, ground=False):
+    """Prepared GPU system of the slab (spectrum, C/B/D, rhs, reserved basis),
+    over the real ground when ``ground``.  Returns (system, yagi, k0,
+    eps_cell, blocks or None, prepare seconds)."""
     from nec2c_amd import vie, workloads as W
-    yagi, k0, eps_cell = slab_geometry(n)
+    yagi, k0, eps_cell = slab_geometry(n, ground)
     t0 = time.perf_counter()
-    centers = W.cube_cells(n, SLAB_H)
-    s = vie.System(centers, [vie.MaterialSpec(1, 6.0, 0.0, 0.01)], yagi.segments, k0, pitch=SLAB_H)
+    centers = W.cube_cells(n, SLAB_H, origin=W.slab_origin(ground))
+    s = vie.System(centers, [vie.MaterialSpec(1, 6.0, 0.0, 0.01)], yagi.segments, k0, pitch=SLAB_H,
+                   ground=slab_ground_epsilon() if ground else None)
     t_prep = time.perf_counter() - t0
-    log(f"slab: {n}^3 prepared (spectrum) in {t_prep:.1f} s; building Yagi blocks (c = {yagi.segments})")
+    log(f"slab: {n}^3 prepared (spectrum{' + reflected spectrum' if ground else ''}) in {t_prep:.1f} s; "
+        f"building Yagi blocks (c = {yagi.segments})")
     C = yagi.impedance_matrix(k0)
     B, D = W.coupling_blocks(yagi, centers, SLAB_H, eps_cell, k0, threads=16)
     s.set_blocks(C=C, B=B, D=D)
@@ -414,29 +423,57 @@ def slab_system(n, opts, keep_blocks=False):
     return s, yagi, k0, eps_cell, blocks, t_prep
 
 
-def matvec_bytes_bmv(n, P, c, S):
+def matvec_bytes_bmv(n, P, c, S, ground=False):
     """Algorithmic bytes of one fused coupled matvec by the pruned five-pass
     model: E_vec = 3 (2N + 4 nx ny Pz + 4 nx Py Pz) vector traffic, E_aux =
     3N (x read again for the contrast) + N (permittivity), E_spec = the octant
-    spectrum 6 (Px/2+1)(Py/2+1)(Pz/2+1) that the multiply reads, E_cross =
+    spectrum 6 (Px/2+1)(Py/2+1)(Pz/2+1) that the multiply reads (over a real
+    ground also the reflected spectrum 6 (Px/2+1)(Py/2+1) Pz), E_cross =
     c^2 + 2 c 3N (C, B and D streamed once); S bytes per complex value."""
     N = n ** 3
     e_vec = 3 * (2 * N + 4 * n * n * P + 4 * n * P * P)
     e_aux = 3 * N + N
-    e_spec = 6 * (P // 2 + 1) ** 3
+    e_spec = 6 * (P // 2 + 1) ** 3 + (6 * (P // 2 + 1) ** 2 * P if ground else 0)
     e_cross = c * c + 2 * c * 3 * N
     return {"volume": (e_vec + e_aux + e_spec) * S, "cross": e_cross * S,
             "total": (e_vec + e_aux + e_spec + e_cross) * S,
             "spectrum_full_6G": 6 * P ** 3 * S}
 
 
-def slab_cpu_leg(n, k0, eps_cell, blocks, x_sols, iterations_needed, restart, sample_iters):
+def slab_ground_table_check(n, k0):
+    """The device's reflected-tensor table at the slab's full size against the
+    oracle (oracle/ground.py, pinned to the reference's vie_halfspace_cuboids)
+    on sampled lateral offsets, every layer sum: the largest difference over
+    the table's largest entry."""
+    from nec2c_amd import vie, workloads as W
+    from oracle import ground as G
+    z0 = W.SLAB_GROUND_HEIGHT + 0.5 * SLAB_H
+    t0 = time.perf_counter()
+    T = vie.ground_table((n, n, n), SLAB_H, z0, k0, slab_ground_epsilon())
+    t_dev = time.perf_counter() - t0
+    dx = np.array(sorted({0, 1, 2, 7, n // 3, n - 1}))
+    dy = np.array(sorted({0, 3, 11, n // 2, n - 1}))
+    t1 = time.perf_counter()
+    To = G.table((n, n, n), SLAB_H, z0, k0, slab_ground_epsilon(), lateral=(dx, dy))
+    t_ora = time.perf_counter() - t1
+    Td = T[:, dx][:, :, dy]
+    err = float(np.max(np.abs(Td - To)) / np.max(np.abs(T)))
+    log(f"ground table: device {t_dev:.2f} s, oracle on {len(dx) * len(dy)} lateral offsets x {2 * n - 1} layer "
+        f"sums {t_ora:.1f} s, max difference {err:.2e} of the largest entry")
+    return {"max_rel_diff": err, "lateral_offsets": [dx.tolist(), dy.tolist()], "device_table_s": t_dev}
+
+
+def slab_cpu_leg(n, k0, eps_cell, blocks, x_sols, iterations_needed, restart, sample_iters, ground=False):
     """The reference's own FFT operator (oracle/_ref: vie_fft.c, vie_kernel.c,
     vie_iterative.c built from its sources with OpenMP) on the slab:
     vie_fft_create at 128^3 (timed, reported beside the value), the true
     relative residual of each GPU solution on that operator with the same
-    [C B; D] blocks, and the first ``sample_iters`` vie_gmres(restart)
-    iterations of the same system, scaled to ``iterations_needed``."""
+    [C B; D] blocks (free space only), and the first ``sample_iters``
+    vie_gmres(restart) iterations of the same system, scaled to
+    ``iterations_needed``.  Over a real ground the reference has no FFT
+    operator (dielectric_ground.c:28-29; its GN 2 path is a dense matrix,
+    (6.3 M)^2 entries here): its free-space FFT-GMRES on the same lattice and
+    blocks, the fastest path it has at this size, is timed instead."""
     from oracle import oracle as O
     if not O.reference_available():
         return None, {}
@@ -452,12 +489,16 @@ def slab_cpu_leg(n, k0, eps_cell, blocks, x_sols, iterations_needed, restart, sa
     bnorm = float(np.linalg.norm(b))
     resid = {}
     t_mv = None
-    for tag, x in x_sols.items():
+    for tag, x in (x_sols.items() if not ground else []):
         t1 = time.perf_counter()
         r = b - rc.apply(x)
         t_mv = time.perf_counter() - t1
         resid[tag] = float(np.linalg.norm(r) / bnorm)
         log(f"cpu: {tag} solution, relative residual on the reference operator {resid[tag]:.3e}")
+    if t_mv is None:
+        t1 = time.perf_counter()
+        rc.apply(b)
+        t_mv = time.perf_counter() - t1
     t2 = time.perf_counter()
     ok, x, rep = rc.gmres(b, restart=restart, maxit=sample_iters, rtol=1e-8)
     t_iters = time.perf_counter() - t2
@@ -467,7 +508,9 @@ def slab_cpu_leg(n, k0, eps_cell, blocks, x_sols, iterations_needed, restart, sa
         "unit": "solves/s",
         "cores": cores,
         "kind": "reference",
-        "sample": (f"the same {n}^3 slab + Yagi system on the reference's FFT operator (builtin FFT, OpenMP, "
+        "sample": (("the reference's free-space FFT operator (its FFT path rejects a real ground, "
+                    "dielectric_ground.c:28-29) on " if ground else "") +
+                   f"the same {n}^3 slab + Yagi system on the reference's FFT operator (builtin FFT, OpenMP, "
                    f"FP64): the first {rep.iterations} vie_gmres({restart}) iterations took {t_iters:.1f} s, "
                    f"scaled to the {iterations_needed} iterations the same FP64 algorithm needs on the GPU "
                    f"(optimistic for the CPU: later steps orthogonalise against more vectors); like the GPU "
@@ -484,18 +527,18 @@ def slab_cpu_leg(n, k0, eps_cell, blocks, x_sols, iterations_needed, restart, sa
 
 
 def bench_slab(args):
-    """The 128^3 slab (the headline): eps_r 6, 0.01 S/m, 1 cm voxels, driven
+    """The 128^3 slab (the headline): eps_r 6, 0.01 S/m, 1 cm voxels, 0.1 m
+    above a real ground (eps_r 13, 0.005 S/m; --free-space drops it), driven
     by a 4-element Yagi 0.1 m from its x = 0 face, at 200 MHz.  One step is one
     mixed-precision solve of that system to a 1e-8 true residual from x0 = 0.
     FP64 solves (the reference's restarted GMRES, and the same recycled
-    subspace as the mixed solve) are timed beside it.  The reference's FFT
-    path rejects a real ground (dielectric_ground.c:28-29), so the slab is in
-    free space."""
+    subspace as the mixed solve) are timed beside it."""
     from nec2c_amd import vie
     n = args.slab_size
     steps, warmup = args.steps, args.warmup
     opts = dict(restart=args.restart, maximum_iterations=args.max_iterations, relative_tolerance=1e-8)
-    s, yagi, k0, eps_cell, blocks, t_prep = slab_system(n, opts, keep_blocks=not args.no_cpu)
+    ground = not args.free_space
+    s, yagi, k0, eps_cell, blocks, t_prep = slab_system(n, opts, keep_blocks=not args.no_cpu, ground=ground)
     c = yagi.segments
 
     def timed(prec, recycle, count, warm):
@@ -535,8 +578,8 @@ def bench_slab(args):
     ms64, p64 = s.time_apply(vie.FP64, args.matvec_repeats)
     lib32, pb32 = s.apply_bytes(vie.MIXED)
     P = 2 * n
-    bmv32 = matvec_bytes_bmv(n, P, c, 8)
-    bmv64 = matvec_bytes_bmv(n, P, c, 16)
+    bmv32 = matvec_bytes_bmv(n, P, c, 8, ground)
+    bmv64 = matvec_bytes_bmv(n, P, c, 16, ground)
     names = ["zfwd", "yfwd", "xpass", "yinv", "zinv", "coupling"]
     gbs = bmv32["total"] / (ms32 * 1e-3) / 1e9
     # the solve's streaming Arnoldi kernels at the mean basis column of the mixed solves
@@ -549,10 +592,11 @@ def bench_slab(args):
     zm, zd = 1 / x_mixed[fi], 1 / x_fp64[fi]
     i_err = float(np.max(np.abs(x_mixed[:c] - x_fp64[:c])) / np.max(np.abs(x_fp64[:c])))
     from nec2c_amd import workloads as W
-    gain_db = gain_difference_db(x_mixed, x_fp64, k0, W.cube_cells(n, SLAB_H), SLAB_H, yagi.points(),
-                                 yagi.deltas())
+    gain_db = gain_difference_db(x_mixed, x_fp64, k0, W.cube_cells(n, SLAB_H, origin=W.slab_origin(ground)), SLAB_H,
+                                 yagi.points(), yagi.deltas())
+    table_check = slab_ground_table_check(n, k0) if ground else None
     log(f"slab far-field gain, mixed vs FP64: max {gain_db:.2e} dB over a 5-degree grid")
-    traffic = profiled_traffic_sum(f"{n}slab_matvec", "fp32")
+    traffic = profiled_traffic_sum(f"{n}slab{'_ground' if ground else ''}_matvec", "fp32")
     result = {
         "metric": METRIC,
         "value": steps / t_mixed,
@@ -568,10 +612,15 @@ def bench_slab(args):
         "data": "synthetic",
         "config": {"workload": (f"{n}^3 voxel slab (eps_r 6, 0.01 S/m, 1 cm voxels, {s.order} unknowns, FFT grid "
                                 f"padded to {P}^3) + 4-element 84-segment wire Yagi 0.1 m away, 200 MHz, "
-                                f"GMRES({args.restart}) to 1e-8; in FREE SPACE, not over the real ground the "
-                                f"configuration names: the reference's FFT path rejects real ground "
-                                f"(dielectric_ground.c:28-29)"),
-                   "grid": [n, n, n], "padded": [P, P, P], "unknowns": s.order, "conductors": c},
+                                f"GMRES({args.restart}) to 1e-8; " +
+                                (f"0.1 m above a real ground (eps_r 13, 0.005 S/m: eps {slab_ground_epsilon():.4f} "
+                                 f"by NEC's GN rule) through the reflected half-space tensor "
+                                 f"(vie_halfspace_cuboids, the reference's GN 2 treatment) as a second spectrum; "
+                                 f"synthetic free-space wire blocks (NEC's Sommerfeld wire-ground terms are the "
+                                 f"caller's)" if ground else "in free space")),
+                   "grid": [n, n, n], "padded": [P, P, P], "unknowns": s.order, "conductors": c,
+                   "ground": [W.SLAB_GROUND[0], W.SLAB_GROUND[1]] if ground else None},
+        "ground_table_check": table_check,
         "fp64_solves_per_s": fsteps / t_fp64,
         "speedup_mixed_vs_fp64": (t_fp64 / fsteps) / (t_mixed / steps),
         "fp64_recycled_solves_per_s": fsteps / t_fp64r,
@@ -626,9 +675,10 @@ def bench_slab(args):
     if not args.no_cpu:
         log("CPU baseline: the reference's FFT operator at the same size")
         cb, resid = slab_cpu_leg(n, k0, eps_cell, blocks, {"mixed": x_mixed, "fp64": x_fp64},
-                                 rep_fp64[0].iterations, args.restart, args.cpu_iterations)
+                                 rep_fp64[0].iterations, args.restart, args.cpu_iterations, ground=ground)
         result["cpu_baseline"] = cb
-        result["residual_on_reference_operator"] = resid
+        if not ground:
+            result["residual_on_reference_operator"] = resid
     print(json.dumps(result), flush=True)
     s.close()
     return 0
@@ -643,8 +693,8 @@ def bench_matvec(args):
     opts = dict(restart=4, maximum_iterations=4, relative_tolerance=1e-8)
     if args.geometry == "slab":
         n = args.slab_size
-        s, _, _, _, _, _ = slab_system(n, opts)
-        what = f"{n}^3 slab + 84-segment Yagi, 200 MHz"
+        s, _, _, _, _, _ = slab_system(n, opts, ground=not args.free_space)
+        what = f"{n}^3 slab + 84-segment Yagi, 200 MHz, {'free space' if args.free_space else 'over the real ground'}"
     else:
         n = args.size
         pitch = 0.5 / n
@@ -1087,6 +1137,8 @@ def main():
                     help="slab: timed FP64 solves of each FP64 variant (after one warm-up)")
     ap.add_argument("--cpu-iterations", type=int, default=2,
                     help="slab: reference GMRES iterations timed for the CPU baseline")
+    ap.add_argument("--free-space", action="store_true",
+                    help="slab: the slab in free space instead of over the real ground")
     ap.add_argument("--geometry", choices=("sweep", "slab"), default="sweep",
                     help="matvec: the geometry whose fused apply is timed (counter runs)")
     ap.add_argument("--size", type=int, default=64, help="voxels per cube edge (64 = the 64^3 sweep)")

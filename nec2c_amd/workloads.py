@@ -236,20 +236,39 @@ SLAB_FREQ = 200e6
 EPS0_REF = 8.8541878128e-12   # the reference's eps0 in dielectric_material_epsilon (dielectric.h:55)
 
 
-def yagi_slab(n):
+SLAB_GROUND_HEIGHT = 0.1     # m: the slab's bottom face above the real ground
+SLAB_GROUND = (13.0, 0.005)  # eps_r and sigma (S/m) of the ground
+
+
+def nec_ground_epsilon(epsr, sigma, frequency_mhz):
+    """NEC's complex ground permittivity for a GN card (dielectric_ground.c:97-100:
+    epsr - i sigma (CVEL / f) 59.96 with NEC's CVEL = 299.8, or the given
+    imaginary part when sigma is negative)."""
+    return complex(epsr, sigma if sigma < 0 else -sigma * (299.8 / frequency_mhz) * 59.96)
+
+
+def yagi_slab(n, ground=False):
     """The slab workload: an n^3 block of 1 cm voxels (eps_r 6, 0.01 S/m)
     driven by a 4-element Yagi (reflector 0.50 m, driven 0.47 m, directors
     0.44 m and 0.43 m, 21 segments each, 1 mm radius, 0.2 m spacing) 0.1 m
     from its x = 0 face, at 200 MHz.  Returns (yagi, k0, eps_cell) with the
     cell permittivity eps_r - i sigma / (omega eps0) evaluated as the
-    reference evaluates a conductive material."""
+    reference evaluates a conductive material.  With ground=True the block
+    sits 0.1 m above the real ground (its cells from z = 0.1 m up: use
+    slab_origin) and the Yagi at the block's mid-height."""
     edge = n * SLAB_PITCH
     yc = zc = 0.5 * edge
+    if ground:
+        zc += SLAB_GROUND_HEIGHT
     yagi = WireArray(tuple((-0.1, yc + (i - 1.5) * 0.2, zc, L) for i, L in enumerate((0.50, 0.47, 0.44, 0.43))),
                      segments_per_wire=21, radius=1e-3, feed_wire=1)
     k0 = k0_of(SLAB_FREQ)
     eps_cell = 6.0 - 1j * 0.01 / (k0 * C0 * EPS0_REF)
     return yagi, k0, eps_cell
+
+
+def slab_origin(ground=False):
+    return (0.0, 0.0, SLAB_GROUND_HEIGHT if ground else 0.0)
 
 
 def cube_cells(n, pitch, origin=(0.0, 0.0, 0.0)):

@@ -52,10 +52,12 @@ def _sinc(v):
     return out
 
 
-def table(shape, h, z0, k, eps, order=48):
+def table(shape, h, z0, k, eps, order=48, lateral=None):
     """T[6, nx, ny, 2 nz - 1]: the reflected tensor between cells dx, dy
     apart laterally (test minus source, in cells, >= 0) whose centre heights
-    add to 2 z0 + s h; z0 is the lowest layer's centre height."""
+    add to 2 z0 + s h; z0 is the lowest layer's centre height.  With
+    ``lateral`` = (dx array, dy array) only those offsets are evaluated:
+    T[6, len(dx), len(dy), 2 nz - 1] (the node set is still the lattice's)."""
     nx, ny, nz = shape
     eps = complex(eps)
     gap = 2 * z0 - h
@@ -71,9 +73,13 @@ def table(shape, h, z0, k, eps, order=48):
     rho = h * math.hypot(max(nx - 1, 0), max(ny - 1, 0))
     pref = -1j * k ** 3 / (8 * math.pi ** 2) * h ** 3
     L = 2 * nz - 1
-    T = np.zeros((6, nx, ny, L), complex)
-    dx = np.arange(nx)[:, None]
-    dy = np.arange(ny)[None, :]
+    if lateral is None:
+        dx, dy = np.arange(nx), np.arange(ny)
+    else:
+        dx, dy = np.asarray(lateral[0]), np.asarray(lateral[1])
+    T = np.zeros((6, len(dx), len(dy), L), complex)
+    dx = dx[:, None]
+    dy = dy[None, :]
     S = 2 * z0 + np.arange(L) * h
     for seg in range(-1, len(cut) - 1):
         lo = 0.0 if seg < 0 else cut[seg]
