@@ -82,6 +82,8 @@ typedef struct {
 static struct {
   void *handle;
   prepare_fn prepare, prepare_pec;
+  int (*prepare_ground)(void **, const dielectric_model *, const dielectric_mesh_result *, size_t, double, size_t,
+                        double, double, dielectric_diagnostic *);
   void (*release)(void *);
   complex double *(*entry)(void *, size_t, size_t);
   void (*get_operator)(void *, vie_linear_operator *);
@@ -131,6 +133,7 @@ static int load_library(void)
   if(!h) return 0;
 #define TAKE(field, name) (*(void **)&lib.field = dlsym(h, name))
   if(!TAKE(prepare, "dielectric_fft_prepare") || !TAKE(prepare_pec, "dielectric_fft_prepare_pec") ||
+     !TAKE(prepare_ground, "dielectric_fft_prepare_ground") ||
      !TAKE(release, "dielectric_fft_free") || !TAKE(entry, "dielectric_fft_entry") ||
      !TAKE(get_operator, "dielectric_fft_get_operator") || !TAKE(storage, "dielectric_fft_storage") ||
      !TAKE(solve, "dielectric_fft_solve") || !TAKE(last_error, "vie_hip_last_error") ||
@@ -215,6 +218,45 @@ int __wrap_dielectric_fft_prepare_pec(dielectric_fft_system **out, const dielect
   if(hip_requested)
     return prepare_hip(lib.prepare_pec, out, model, mesh, conductors, k0, extra, diagnostic);
   return __real_dielectric_fft_prepare_pec(out, model, mesh, conductors, k0, extra, diagnostic);
+}
+
+/* The FFT system over a real ground (GN 2), which nec2c's own FFT path
+   rejects (dielectric_ground.c:28-29): with "hip" selected, the library's
+   system with the reflected half-space tensor; ground permittivity as
+   dielectric_ground_epsilon(frequency) gives it.  Not wrapped: nec2c calls it
+   only where the patch at dielectric_wire.c:1121-1123 does. */
+int dielectric_fft_prepare_ground(dielectric_fft_system **out, const dielectric_model *model,
+                                  const dielectric_mesh_result *mesh, size_t conductors, double k0, size_t extra,
+                                  complex double ground, dielectric_diagnostic *diagnostic)
+{
+  hip_system *h;
+  *out = NULL;
+  if(!hip_requested) {
+    memset(diagnostic, 0, sizeof(*diagnostic));
+    snprintf(diagnostic->message, sizeof(diagnostic->message),
+             "FFT: real-ground FFT requires --fft-backend hip");
+    return 0;
+  }
+  h = calloc(1, sizeof(*h));
+  if(!h) {
+    memset(diagnostic, 0, sizeof(*diagnostic));
+    snprintf(diagnostic->message, sizeof(diagnostic->message), "FFT: dispatch allocation failed");
+    return 0;
+  }
+  if(!lib.prepare_ground(&h->system, model, mesh, conductors, k0, extra, creal(ground), cimag(ground), diagnostic)) {
+    free(h);
+    report_phases();
+    return 0;
+  }
+  report_phases();
+  lib.get_operator(h->system, &h->op);
+  pthread_mutex_lock(&lock);
+  h->next = systems;
+  systems = h;
+  hip_last_used = 1;
+  pthread_mutex_unlock(&lock);
+  *out = (dielectric_fft_system *)h;
+  return 1;
 }
 
 void __wrap_dielectric_fft_free(dielectric_fft_system *s)

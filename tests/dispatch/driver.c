@@ -5,8 +5,11 @@
  * dielectric_fft.c and the dispatcher under the --wrap flags the dispatcher
  * lists, so every call below is routed exactly as nec2c's would be.
  *
- *   driver BACKEND   (auto | builtin | fftw | hip)
+ *   driver BACKEND [ground]   (auto | builtin | fftw | hip)
  *
+ * With "ground" the system is prepared over a real ground of permittivity
+ * 13 - 0.45 i (dielectric_fft_prepare_ground, which only "hip" provides) and
+ * the cells are raised 0.1 m above it; "raised" raises them without ground.
  * Prepares a 5 x 3 x 2 lattice with two holes (two materials) and two
  * conductors written through dielectric_fft_entry, prints the backend names,
  * the operator product of a fixed vector, and the vie_gmres solve of that
@@ -39,6 +42,10 @@ void performance_add(const char *phase, double elapsed_seconds)
   phases[i].calls++;
 }
 
+int dielectric_fft_prepare_ground(dielectric_fft_system **out, const dielectric_model *model,
+                                  const dielectric_mesh_result *mesh, size_t conductors, double k0, size_t extra,
+                                  complex double ground, dielectric_diagnostic *diagnostic);
+
 int main(int argc, char **argv)
 {
   dielectric_model model;
@@ -54,6 +61,7 @@ int main(int argc, char **argv)
   size_t ncell = 0, i, j, c = 2, n;
   int ix, iy, iz, ok;
   double err = 0;
+  int ground = argc > 2 && !strcmp(argv[2], "ground"), raised = ground || (argc > 2 && !strcmp(argv[2], "raised"));
   if(argc < 2) return 2;
   if(!vie_fft_backend_set(argv[1])) { printf("backend %s refused\n", argv[1]); return 3; }
   printf("requested %s\n", vie_fft_backend_requested_name());
@@ -64,13 +72,18 @@ int main(int argc, char **argv)
   for(ix = 0; ix < 5; ix++) for(iy = 0; iy < 3; iy++) for(iz = 0; iz < 2; iz++) {
     if((ix == 2 && iy == 1 && iz == 0) || (ix == 4 && iy == 2 && iz == 1)) continue;   /* holes */
     cells[ncell].material_id = (ix + iy + iz) % 2 ? 2 : 1;
-    cells[ncell].center[0] = 0.25 * ix; cells[ncell].center[1] = 0.25 * iy; cells[ncell].center[2] = 0.25 * iz;
+    cells[ncell].center[0] = 0.25 * ix; cells[ncell].center[1] = 0.25 * iy;
+    cells[ncell].center[2] = 0.25 * iz + (raised ? 0.225 : 0.0);
     for(j = 0; j < 3; j++) cells[ncell].size[j] = 0.25;
     ncell++;
   }
   model.materials = mats; model.material_count = 2;
   mesh.cells = cells; mesh.cell_count = ncell;
-  if(!dielectric_fft_prepare(&s, &model, &mesh, c, 0.8, 0, &diag)) { printf("prepare refused: %s\n", diag.message); return 4; }
+  if(!(ground ? dielectric_fft_prepare_ground(&s, &model, &mesh, c, 0.8, 0, 13 - 0.45 * I, &diag)
+              : dielectric_fft_prepare(&s, &model, &mesh, c, 0.8, 0, &diag))) {
+    printf("prepare refused: %s\n", diag.message);
+    return 4;
+  }
   printf("backend %s last_used %s\n", dielectric_fft_backend(s),
          vie_fft_backend_last_used_name() ? vie_fft_backend_last_used_name() : "(none)");
   n = c + 3 * ncell;

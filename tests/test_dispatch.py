@@ -19,14 +19,15 @@ DRIVER = os.path.join(ROOT, "oracle", "_ref", "hip_dispatch_driver")
 LIBRARY = os.path.join(ROOT, "nec2c_amd", "lib", "libvie_hip.so")
 
 
-def run(backend, library=True):
+def run(backend, library=True, mode=None):
     if not os.path.exists(DRIVER):
         pytest.skip("oracle/_ref/hip_dispatch_driver not built")
     env = dict(os.environ)
     env.pop("VIE_HIP_LIBRARY", None)
     if library:
         env["VIE_HIP_LIBRARY"] = LIBRARY
-    r = subprocess.run([DRIVER, backend], capture_output=True, text=True, timeout=120, env=env)
+    r = subprocess.run([DRIVER, backend] + ([mode] if mode else []), capture_output=True, text=True, timeout=120,
+                       env=env)
     out = {"rc": r.returncode, "text": r.stdout + r.stderr, "y": {}, "phases": {}}
     for line in r.stdout.splitlines():
         w = line.split()
@@ -93,3 +94,43 @@ def test_hip_backend_matches_builtin_on_the_gpu():
     print("phases", ph, flush=True)
     assert ph["fft_build"][1] == 1 and ph["fft_gmres"][1] == 1 and ph["fft_matvec"][1] == 1
     assert all(sec > 0 for sec, _ in ph.values())
+
+
+def test_real_ground_is_refused_by_the_cpu_backends():
+    r = run("builtin", mode="ground")
+    assert r["rc"] == 4 and "requires --fft-backend hip" in r["text"]
+
+
+@pytest.mark.gpu
+def test_hip_real_ground_system_through_the_dispatcher():
+    """GN 2 through nec2c's link: the hip product over the ground minus
+    nec2c's own free-space product of the same raised cells is the reflected
+    part -(eps - 1) N_R x of the volume rows (N_R from the oracle, pinned to
+    the reference's vie_halfspace_cuboids) and nothing in the conductor rows."""
+    from oracle import ground as G
+    hg = run("hip", mode="ground")
+    bf = run("builtin", mode="raised")
+    assert hg["rc"] == 0 and bf["rc"] == 0, hg["text"]
+    assert hg["backend"] == "hip" and hg["solve"]["ok"]
+    xyz, mids = [], []
+    for ix in range(5):
+        for iy in range(3):
+            for iz in range(2):
+                if (ix, iy, iz) in ((2, 1, 0), (4, 2, 1)):
+                    continue
+                xyz.append((ix, iy, iz))
+                mids.append(2 if (ix + iy + iz) % 2 else 1)
+    k0 = 0.8
+    sig = 0.02 / (k0 * 299792458.0 * 8.8541878128e-12)
+    eps = np.array([3 - 0.4j if m == 1 else 2 - 1j * sig for m in mids])
+    T = G.table((5, 3, 2), 0.25, 0.225, k0, 13 - 0.45j)   # lowest cell centre 0.225 m
+    NR = G.reflected_matrix(T, xyz)
+    n = 2 + 3 * len(xyz)
+    x = np.array([np.sin(0.7 * i) + 1j * np.cos(0.3 * i) for i in range(n)])
+    expect = np.zeros(n, complex)
+    expect[2:] = -np.repeat(eps - 1, 3) * (NR @ x[2:])
+    diff = hg["yv"] - bf["yv"]
+    err = float(np.max(np.abs(diff - expect)) / np.max(np.abs(bf["yv"])))
+    print(f"ground - free vs oracle reflected part: {err:.2e} (reflected share "
+          f"{np.max(np.abs(expect)) / np.max(np.abs(bf['yv'])):.2e})", flush=True)
+    assert err < 1e-11

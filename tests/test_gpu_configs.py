@@ -203,3 +203,52 @@ def test_tabulated_material_outside_its_samples_is_refused_by_both(ref):
     with pytest.raises(ValueError) as eg:
         vie.System(cells, MATERIALS, 0, k0)
     assert str(eg.value) == str(er.value) == "FFT: unknown or invalid passive material"
+
+
+def test_c3_slab_over_the_real_ground_full_size():
+    """The 128^3 slab 0.1 m above the real ground (eps_r 13, 0.005 S/m by
+    NEC's GN rule) at full size: the device's reflected-tensor table against
+    the oracle (oracle/ground.py, pinned to the reference's
+    vie_halfspace_cuboids) on sampled lateral offsets and every layer sum,
+    and the mixed solve against the FP64 solve (impedance and currents within
+    1e-6).  The reference has no FFT operator over a real ground and its GN 2
+    matrix has (6.3 M)^2 entries, so no reference product exists here; the
+    assembled operator is pinned against dense products on smaller lattices
+    (tests/test_gpu_ground.py)."""
+    from oracle import ground as G
+    n, h = 128, W.SLAB_PITCH
+    yagi, k0, eps_cell = W.yagi_slab(n, ground=True)
+    eps_g = W.nec_ground_epsilon(*W.SLAB_GROUND, W.SLAB_FREQ / 1e6)
+    z0 = W.SLAB_GROUND_HEIGHT + 0.5 * h
+    T = vie.ground_table((n, n, n), h, z0, k0, eps_g)
+    dx, dy = np.array([0, 1, 9, 64, 127]), np.array([0, 2, 40, 127])
+    To = G.table((n, n, n), h, z0, k0, eps_g, lateral=(dx, dy))
+    terr = float(np.max(np.abs(T[:, dx][:, :, dy] - To)) / np.max(np.abs(T)))
+    log(f"C3 ground table vs oracle: {terr:.2e}")
+    assert terr < 1e-12
+    del T
+    c = yagi.segments
+    centers = W.cube_cells(n, h, origin=W.slab_origin(True))
+    C = yagi.impedance_matrix(k0)
+    B, D = W.coupling_blocks(yagi, centers, h, eps_cell, k0, threads=16)
+    s = vie.System(centers, [vie.MaterialSpec(1, 6.0, 0.0, 0.01)], c, k0, pitch=h, ground=eps_g)
+    s.set_blocks(C=C, B=B, D=D)
+    del centers, B, D
+    b = np.zeros(s.order, complex)
+    b[:c] = yagi.rhs()
+    sols = {}
+    for prec in (vie.MIXED, vie.FP64):
+        t = time.perf_counter()
+        ok, x, rep = s.solve(b, precision=prec, restart=120, maximum_iterations=3000, relative_tolerance=1e-8)
+        log(f"C3 ground precision {prec}: ok {ok}, {rep.iterations} iterations, residual {rep.relative_residual:.2e}, "
+            f"{time.perf_counter() - t:.1f} s")
+        assert ok and rep.relative_residual <= 1e-8
+        sols[prec] = x
+    # the FP64 solution's residual on the FP64 device operator, recomputed from the host
+    r = float(np.linalg.norm(b - s.apply_ex(sols[vie.FP64], vie.FP64)) / np.linalg.norm(b))
+    assert r <= 1.05e-8
+    fi = yagi.feed_index()
+    xm, xd = sols[vie.MIXED], sols[vie.FP64]
+    assert abs(1 / xm[fi] - 1 / xd[fi]) / abs(1 / xd[fi]) <= 1e-6
+    assert rel(xm[:c], xd[:c]) <= 1e-6
+    s.close()
