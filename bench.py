@@ -99,8 +99,8 @@ def build_case(n, freq, eps, dipole, cells_xyz, centers, pitch, opts):
     return sys_, (C, B, D, b, k0)
 
 
-def pass_kernel_name(pas, P, v, E=4096):
-    """Kernel instantiation of FP32 volume pass `pas` at launch variant v
+def pass_kernel_name(pas, P, v, E=4096, real="float"):
+    """Kernel instantiation of volume pass `pas` at launch variant v
     (lines per workgroup and workgroup size as the library's tile rules; the
     x pass takes twice the elements and threads)."""
     cap = (8, 16, 32, 16, 8)[pas]
@@ -108,11 +108,16 @@ def pass_kernel_name(pas, P, v, E=4096):
     l0 = min(wide * E // P, cap)
     lw = (l0, max(l0 // 2, 1), max(l0 // 4, 1), max(l0 // 2, 1))[v & 3]
     wg = wide * (256, 128, 64, 256)[v & 3]
-    pair = ",true" if pas == 2 and v >= 4 else ""
-    kern = ("k_zfwd<float,{P},{lw},{wg}>", "k_ypass<float,{P},{lw},{wg},false>",
-            "k_xpass<float,{P},{lw},{wg}{pair}> (X FFT + 3x3 spectral multiply + inverse X)",
-            "k_ypass<float,{P},{lw},{wg},true>", "k_zinv<float,{P},{lw},{wg}>")[pas]
-    return kern.format(P=P, lw=lw, wg=wg, pair=pair)
+    tf = lambda b: "true" if b else "false"
+    if pas == 2 and v >= 16:
+        lw, waves = (32, 2) if v & 1 else (16, 2 if v & 2 else 3)
+        return (f"k_xpass_r16<{lw},{tf(v & 4)},{tf(v & 8)},{waves}> (register-resident X FFT + 3x3 spectral "
+                f"multiply + inverse X)")
+    pair = f",{tf(v & 4)},{tf(v & 8)}" if pas == 2 else ""
+    kern = ("k_zfwd<{R},{P},{lw},{wg}>", "k_ypass<{R},{P},{lw},{wg},false>",
+            "k_xpass<{R},{P},{lw},{wg}{pair}> (X FFT + 3x3 spectral multiply + inverse X)",
+            "k_ypass<{R},{P},{lw},{wg},true>", "k_zinv<{R},{P},{lw},{wg}>")[pas]
+    return kern.format(R=real, P=P, lw=lw, wg=wg, pair=pair)
 
 
 def cpu_baseline(n, pitch, eps, case, iterations_needed, restart, sample_iters=10):
@@ -712,6 +717,22 @@ def bench_matvec(args):
         tot, pb = s.apply_bytes(prec)
         out[tag] = {"ms": ms, "bytes": tot, "pass_ms": dict(zip(names, passes)), "pass_bytes": dict(zip(names, pb)),
                     "pass_gbs": {k: (b / (t * 1e-3) / 1e9 if t > 0 else None) for k, b, t in zip(names, pb, passes)}}
+    if args.pass_sweep:
+        # every valid launch variant of the x pass, the others as tuned
+        out["x_variants"] = {}
+        for prec, tag in ((vie.MIXED, "fp32"), (vie.FP64, "fp64")):
+            rows = {}
+            for v in range(32):
+                f64, f32 = list(variants["fp64"]), list(variants["fp32"])
+                (f32 if prec == vie.MIXED else f64)[2] = v
+                if not s.set_variants(f64, f32):
+                    continue
+                _, passes = s.time_apply(prec, args.matvec_repeats)
+                rows[v] = {"kernel": pass_kernel_name(2, 2 * n, v, 4096 if prec == vie.MIXED else 2048,
+                                                      "float" if prec == vie.MIXED else "double").split(" ")[0],
+                           "ms": passes[2]}
+            out["x_variants"][tag] = rows
+        s.set_variants(variants["fp64"], variants["fp32"])
     if args.arnoldi_column > 0:
         s.reserve(restart=min(args.arnoldi_column + 1, vie.KMAX_RESTART), maximum_iterations=1000,
                   relative_tolerance=1e-8)
@@ -1168,6 +1189,8 @@ def main():
     ap.add_argument("--mesh-size", type=int, default=64, help="mesh: voxels per cube edge (one BOX per voxel)")
     ap.add_argument("--mesh-cpu-lefts", type=int, default=2048,
                     help="mesh: left boxes whose pairs the CPU baseline scans")
+    ap.add_argument("--pass-sweep", action="store_true",
+                    help="matvec: also time every launch variant of the x pass (counter runs then cover them all)")
     ap.add_argument("--arnoldi-column", type=int, default=0,
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")

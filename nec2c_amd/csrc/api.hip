@@ -564,8 +564,9 @@ static vie_fft_plan* plan_create(const size_t box[3], size_t kshape_z, int pec, 
     // the x pass must keep each line's kz mirror in its workgroup
     for (int slot = 0; slot < 2 && ok; slot++) {
       int v = 0;
-      while (v < 8 && !(slot ? vie::pass_variant_ok<float>(g, 2, v) : vie::pass_variant_ok<double>(g, 2, v))) v++;
-      if (v == 8) {
+      const int nv = slot ? vie::pass_variant_count<float>(g, 2) : vie::pass_variant_count<double>(g, 2);
+      while (v < nv && !(slot ? vie::pass_variant_ok<float>(g, 2, v) : vie::pass_variant_ok<double>(g, 2, v))) v++;
+      if (v == nv) {
         set_error("no x-pass launch shape keeps the kz mirrors of this grid together (Pz = %d)", g.Pz);
         ok = false;
       }

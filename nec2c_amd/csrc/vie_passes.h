@@ -16,6 +16,7 @@
 // inverse scaling lives in the spectrum.
 #pragma once
 #include <algorithm>
+#include <type_traits>
 
 #include "fft_core.h"
 #include "vie_kernels.h"
@@ -182,7 +183,31 @@ __device__ __forceinline__ long xline(const Geom& g, long blk, int l) {
   }
 }
 
-template <typename T, int PX, int LW, int WG, bool PAIR = false>
+// Tile order of the x pass.  Workgroups are dispatched round-robin over the
+// eight XCDs, each with its own L2: workgroup b runs on XCD b % 8 in that
+// XCD's (b / 8)-th slot.  Giving consecutive slots of one XCD the tiles that
+// read the same octant spectrum entries (rows ky and Py - ky; unpaired kz
+// tiles t and T - 1 - t, which hold each other's kz mirrors) lets every
+// reader after the first hit that XCD's L2 instead of the last-level cache or
+// HBM.  Returns the natural tile index (row-major over (ky, tile)); grids the
+// order does not cover keep the natural order.
+template <bool PAIR, int LW>
+__device__ __forceinline__ long xcd_tile(const Geom& g, long b, long nb) {
+  const int T = g.Pz / LW;  // tiles (paired: kz groups) per ky row
+  if (T < 1 || g.Pz % LW || g.Py < 2 || nb % 8 || nb != (long)g.Py * T) return b;
+  const long v = (b % 8) * (nb / 8) + b / 8;
+  const int TT = (!PAIR && T >= 2) ? 2 : 1;  // tiles of one row in a group
+  const int TP = T / TT;
+  const long rest = v / (2 * TT);
+  const int inner = (int)(v % (2 * TT));
+  const int tp = (int)(rest % TP), p = (int)(rest / TP);
+  const int ri = inner / TT, ti = inner % TT;
+  // row pairs (0, Py/2) and (p, Py - p): the same |ky| fold, bar the first
+  const int row = p == 0 ? (ri ? g.Py / 2 : 0) : (ri ? g.Py - p : p);
+  return (long)row * T + (ti ? T - 1 - tp : tp);
+}
+
+template <typename T, int PX, int LW, int WG, bool PAIR = false, bool XCD = false>
 __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   constexpr int E = LW * PX, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
@@ -191,7 +216,7 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.Py * g.Pz;
-  const long blk = blockIdx.x;
+  const long blk = XCD ? xcd_tile<PAIR, LW>(g, blockIdx.x, gridDim.x) : (long)blockIdx.x;
   const size_t plane = (size_t)g.Py * g.Pz;
   for (int i = tid; i < PX; i += WG) tw[i] = a.tw[i];
   cx<T> hold[3][EPT], next[EPT];
@@ -201,7 +226,8 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
     const int f_ = tid + (i) * WG, e_ = f_ / LW, l_ = f_ % LW;                               \
     const long line_ = xline<PAIR, LW>(g, blk, l_);                                          \
     next[i] = (E % WG == 0 || f_ < E) && line_ < lines && e_ < g.nx                          \
-                  ? a.B[((size_t)(comp) * g.nx + e_) * plane + line_]                         \
+                  ? (XCD ? ld_stream(&a.B[((size_t)(comp) * g.nx + e_) * plane + line_])      \
+                         : a.B[((size_t)(comp) * g.nx + e_) * plane + line_])                 \
                   : cx<T>{(T)0, (T)0};                                                       \
   } while (0)
 #pragma unroll
@@ -313,6 +339,211 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------- pass 3, registers
+// The x pass for P = 256 in FP32 as a two-level 16 x 16 transform held in
+// registers.  Thread (l, c) of a workgroup of 16 LW threads serves line l:
+//   forward, x index n = c + 16 n1, frequency k = k1 + 16 k2:
+//     a 16-point DFT over n1 (inputs n1 >= 8 are zero: n < P/2 >= nx),
+//     twiddle W256^(c k1), one LDS transpose, a 16-point DFT over n2 that
+//     leaves the thread holding X[c + 16 k2] for every k2;
+//   the 3x3 spectral multiply on those registers (the same entries as k_xpass);
+//   inverse, n = m2 + 16 m1: a 16-point DFT over k2 in the same registers,
+//     twiddle, one LDS transpose, a 16-point DFT over k1 forming only the
+//     outputs m1 < 8 (the nx physical rows live below P/2).
+// So each component crosses LDS twice per direction instead of once per
+// radix stage, with two barriers per crossing.  Inverse transforms use
+// conj(DFT(conj x)); the 1/P scaling is in the spectrum.
+template <typename T> __device__ __forceinline__ cx<T> w16(cx<T> x, int e) {
+  // x * exp(-2 pi i e / 16) for the e of a 4 x 4 split (e in 0..9)
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, r2 = (T)0.70710678118654752440;
+  switch (e) {
+    case 0: return x;
+    case 1: return cx<T>{c1 * x.re + s1 * x.im, c1 * x.im - s1 * x.re};
+    case 2: return cx<T>{r2 * (x.re + x.im), r2 * (x.im - x.re)};
+    case 3: return cx<T>{s1 * x.re + c1 * x.im, s1 * x.im - c1 * x.re};
+    case 4: return mul_mi(x);
+    case 6: return cx<T>{r2 * (x.im - x.re), -r2 * (x.re + x.im)};
+    default: return cx<T>{-c1 * x.re - s1 * x.im, s1 * x.re - c1 * x.im};  // 9
+  }
+}
+
+// 16-point DFT of v in place, natural order, split n = 4a + b, k = c + 4d.
+// IN8: v[8..15] are zero on entry; OUT8: only X[0..7] are formed.
+template <typename T, bool IN8, bool OUT8> __device__ __forceinline__ void dft16(cx<T>* v) {
+  cx<T> u[4][4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if constexpr (IN8) {
+      const cx<T> x0 = v[b], x1 = v[4 + b];
+      u[b][0] = x0 + x1;
+      u[b][1] = x0 + mul_mi(x1);
+      u[b][2] = x0 - x1;
+      u[b][3] = x0 - mul_mi(x1);
+    } else {
+      cx<T> w[4] = {v[b], v[4 + b], v[8 + b], v[12 + b]};
+      dft4(w);
+#pragma unroll
+      for (int c = 0; c < 4; c++) u[b][c] = w[c];
+    }
+  }
+#pragma unroll
+  for (int b = 1; b < 4; b++)
+#pragma unroll
+    for (int c = 1; c < 4; c++) u[b][c] = w16(u[b][c], b * c);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    if constexpr (OUT8) {
+      const cx<T> s02 = u[0][c] + u[2][c], d02 = u[0][c] - u[2][c];
+      const cx<T> s13 = u[1][c] + u[3][c], d13 = u[1][c] - u[3][c];
+      v[c] = s02 + s13;
+      v[c + 4] = d02 + mul_mi(d13);
+    } else {
+      cx<T> w[4] = {u[0][c], u[1][c], u[2][c], u[3][c]};
+      dft4(w);
+#pragma unroll
+      for (int d = 0; d < 4; d++) v[c + 4 * d] = w[d];
+    }
+  }
+}
+
+// WPE waves per SIMD: the register budget (168 at three, 256 at two) for the
+// three held components (96 registers) and one transform's temporaries.
+template <int LW, bool PAIR, bool XCD, int WPE>
+__global__ __launch_bounds__(16 * LW) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_xpass_r16(OpArgs<float> a) {
+  using T = float;
+  constexpr int PX = 256, WG = 16 * LW;
+  __shared__ cx<T> lds[LW * PX];
+  __shared__ cx<T> tw[16][16];  // tw[c][k] = W256^(c k): a thread's twiddles are one row
+  if (a.di.skipped()) return;
+  const int tid = threadIdx.x, l = tid % LW, c = tid / LW;
+  const Geom& g = a.g;
+  const long lines = (long)g.Py * g.Pz;
+  const long blk = XCD ? xcd_tile<PAIR, LW>(g, blockIdx.x, gridDim.x) : (long)blockIdx.x;
+  // element offsets fit 32 bits (checked on the host), so loads take a
+  // scalar base and one offset register
+  const unsigned plane = (unsigned)g.Py * g.Pz;
+  if (tid < 256) tw[tid / 16][tid % 16] = a.tw[(tid / 16) * (tid % 16)];
+  const long line = xline<PAIR, LW>(g, blk, l);
+  const bool live = line < lines;
+  const unsigned uline = live ? (unsigned)line : 0u;
+  cx<T> hold[3][16], nxt[8];
+  // loads are unconditional (from element 0 where there is no input) and
+  // selected afterwards: a branch per load would force spills around it
+  auto load = [&](int comp) {
+#pragma unroll
+    for (int n1 = 0; n1 < 8; n1++) {
+      const int n = c + 16 * n1;
+      const bool ok = live && n < g.nx;
+      const cx<T>* src = a.B + (ok ? ((unsigned)comp * g.nx + n) * plane + uline : 0u);
+      const cx<T> val = XCD ? ld_stream(src) : *src;
+      nxt[n1] = ok ? val : cx<T>{(T)0, (T)0};
+    }
+  };
+  load(0);
+#pragma unroll
+  for (int comp = 0; comp < 3; comp++) {
+    cx<T> v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 8; n1++) v[n1] = nxt[n1];
+#pragma unroll
+    for (int n1 = 8; n1 < 16; n1++) v[n1] = cx<T>{(T)0, (T)0};
+    if (comp < 2) load(comp + 1);  // next component in flight during this one
+    if (comp == 0) __syncthreads();  // twiddle table
+    dft16<T, true, false>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < 16; k1++) v[k1] = v[k1] * tw[c][k1];
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) lds[(k1 * 16 + c) * LW + l] = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 16; n2++) hold[comp][n2] = lds[(c * 16 + n2) * LW + l];
+    __syncthreads();
+    dft16<T, false, false>(hold[comp]);  // hold[comp][k2] = X[c + 16 k2]
+  }
+  // spectral multiply: dst_i = sum_j T^_ij src_j   (vie_fft.c:253-256)
+  const int Hx = g.Px / 2 + 1, Hy = g.Py / 2 + 1, Hz = g.Pz / 2 + 1;
+  const unsigned oct = (unsigned)Hx * Hy * Hz;
+  int mlane = 0;
+  if (g.ground) {  // the -kz line of this lane's line (see k_xpass)
+    int lm;
+    if constexpr (PAIR) {
+      constexpr int H = LW / 2;
+      lm = ((blk % (g.Pz / LW)) == 0 && (l % H) == 0) ? l : (l ^ H);
+    } else {
+      const int kz = l % g.Pz;
+      lm = l - kz + ((g.Pz - kz) & (g.Pz - 1));
+    }
+    mlane = (tid - l + lm) & 63;
+  }
+  const unsigned rstride = (unsigned)Hx * Hy * g.Pz;
+  // lanes past the last line compute on line 0 and store nothing
+  const int ky = live ? (int)(line / g.Pz) : 0, kz = live ? (int)(line % g.Pz) : 0;
+  const int oy = ky <= g.Py / 2 ? ky : g.Py - ky, oz = kz <= g.Pz / 2 ? kz : g.Pz - kz;
+  const T sy = ky <= g.Py / 2 ? (T)1 : (T)-1, sz = kz <= g.Pz / 2 ? (T)1 : (T)-1;
+#pragma unroll
+  for (int k2 = 0; k2 < 16; k2++) {
+    cx<T> m0 = {(T)0, (T)0}, m1 = m0, m2 = m0;
+    if (g.ground) {  // uniform branch: every lane takes part in the shuffles
+      m0 = cx<T>{__shfl(hold[0][k2].re, mlane), __shfl(hold[0][k2].im, mlane)};
+      m1 = cx<T>{__shfl(hold[1][k2].re, mlane), __shfl(hold[1][k2].im, mlane)};
+      m2 = cx<T>{__shfl(hold[2][k2].re, mlane), __shfl(hold[2][k2].im, mlane)};
+    }
+    {
+      const int kx = c + 16 * k2;
+      const int ox = kx <= PX / 2 ? kx : PX - kx;
+      const T sx = kx <= PX / 2 ? (T)1 : (T)-1;
+      const unsigned o = ((unsigned)ox * Hy + oy) * Hz + oz;
+      cx<T> txx = a.spec[o], tyy = a.spec[oct + o], tzz = a.spec[2 * oct + o];
+      cx<T> txy = (sx * sy) * a.spec[3 * oct + o];
+      cx<T> txz = (sx * sz) * a.spec[4 * oct + o];
+      cx<T> tyz = (sy * sz) * a.spec[5 * oct + o];
+      cx<T> s0 = hold[0][k2], s1 = hold[1][k2], s2 = hold[2][k2];
+      cx<T> d0 = txx * s0 + txy * s1 + txz * s2;
+      cx<T> d1 = txy * s0 + tyy * s1 + tyz * s2;
+      cx<T> d2 = txz * s0 + tyz * s1 + tzz * s2;
+      if (g.ground) {
+        const unsigned r = ((unsigned)ox * Hy + oy) * g.Pz + kz;
+        cx<T> rxx = a.rspec[r], ryy = a.rspec[rstride + r], rzz = a.rspec[2 * rstride + r];
+        cx<T> rxy = (sx * sy) * a.rspec[3 * rstride + r];
+        cx<T> rxz = sx * a.rspec[4 * rstride + r];
+        cx<T> ryz = sy * a.rspec[5 * rstride + r];
+        d0 = d0 + rxx * m0 + rxy * m1 + rxz * m2;
+        d1 = d1 + rxy * m0 + ryy * m1 + ryz * m2;
+        d2 = d2 - rxz * m0 - ryz * m1 + rzz * m2;
+      }
+      hold[0][k2] = d0;
+      hold[1][k2] = d1;
+      hold[2][k2] = d2;
+    }
+    // keep each k2's spectrum loads within its own step: hoisting them all
+    // would need far more registers than the held components leave
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int comp = 0; comp < 3; comp++) {
+    cx<T>* v = hold[comp];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) v[k2] = conj(v[k2]);
+    dft16<T, false, false>(v);  // conj Z[m2] for this k1 = c
+#pragma unroll
+    for (int m2 = 1; m2 < 16; m2++) v[m2] = v[m2] * tw[c][m2];
+#pragma unroll
+    for (int m2 = 0; m2 < 16; m2++) lds[(m2 * 16 + c) * LW + l] = v[m2];
+    __syncthreads();
+    cx<T> u[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) u[k1] = lds[(c * 16 + k1) * LW + l];
+    __syncthreads();
+    dft16<T, false, true>(u);  // conj x[c + 16 m1], m1 < 8
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) {
+      const int n = c + 16 * m1;
+      if (live && n < g.nx) a.B[((unsigned)comp * g.nx + n) * plane + uline] = conj(u[m1]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- pass 5
 template <typename T, int PZ, int LW, int WG>
 __global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
@@ -382,7 +613,8 @@ constexpr int kZCap = 8, kYCap = 16, kXCap = 32;
 
 template <typename T, int P> struct PassLauncher {
   static constexpr int E = PassShape<T>::E;
-  template <int V, bool PAIR = false> static void run(int pass, const OpArgs<T>& a, hipStream_t s) {
+  template <int V, bool PAIR = false, bool XCD = false>
+  static void run(int pass, const OpArgs<T>& a, hipStream_t s) {
     constexpr int WG = tile_wg(V);
     constexpr int LZ = tile_lw(tile_l0(E, P, kZCap), V);
     constexpr int LY = tile_lw(tile_l0(E, P, kYCap), V);
@@ -393,25 +625,57 @@ template <typename T, int P> struct PassLauncher {
     switch (pass) {
       case 0: hipLaunchKernelGGL((k_zfwd<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
       case 1: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, false>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
-      case 2: hipLaunchKernelGGL((k_xpass<T, P, LX, 2 * WG, PAIR>), dim3((xl + LX - 1) / LX), dim3(2 * WG), 0, s, a); break;
+      case 2: hipLaunchKernelGGL((k_xpass<T, P, LX, 2 * WG, PAIR, XCD>), dim3((xl + LX - 1) / LX), dim3(2 * WG), 0, s, a); break;
       case 3: hipLaunchKernelGGL((k_ypass<T, P, LY, WG, true>), dim3((yl + LY - 1) / LY), dim3(WG), 0, s, a); break;
       default: hipLaunchKernelGGL((k_zinv<T, P, LZ, WG>), dim3((zl + LZ - 1) / LZ), dim3(WG), 0, s, a); break;
     }
   }
   // x-pass line group of variant v (for the paired mapping's Pz % LX test)
   static constexpr int x_lines(int v) { return tile_lw(tile_l0(2 * E, P, kXCap), v & 3); }
+  template <int V> static void run_x(int pass, int v, const OpArgs<T>& a, hipStream_t s) {
+    switch (pass == 2 ? (v >> 2) & 3 : 0) {
+      case 1: run<V, true>(pass, a, s); return;
+      case 2: run<V, false, true>(pass, a, s); return;
+      case 3: run<V, true, true>(pass, a, s); return;
+      default: run<V>(pass, a, s); return;
+    }
+  }
+  template <int LW, bool PAIR, bool XCD> static void run_r16(int v, const OpArgs<T>& a, hipStream_t s) {
+    if constexpr (std::is_same<T, float>::value && P == 256) {
+      const long xl = (long)a.g.Py * a.g.Pz;
+      if (LW == 16 && !(v & 2))
+        hipLaunchKernelGGL((k_xpass_r16<16, PAIR, XCD, 3>), dim3((xl + 15) / 16), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_xpass_r16<LW, PAIR, XCD, 2>), dim3((xl + LW - 1) / LW), dim3(16 * LW), 0, s, a);
+    }
+  }
+  // x-pass variant bits: 0-1 the shape, 2 kz-mirror pairing, 3 the
+  // XCD-grouped tile order (with streaming loads of the pencils), 4 the
+  // register-resident 16 x 16 kernel (FP32, P = 256; bit 0 then picks 32
+  // lines per workgroup instead of 16, bit 1 two waves per SIMD instead of
+  // three at 16 lines)
   static void launch(int pass, int v, const OpArgs<T>& a, hipStream_t s) {
+    if (pass == 2 && v >= 16) {
+      switch (v & 13) {
+        case 0: run_r16<16, false, false>(v, a, s); return;
+        case 1: run_r16<32, false, false>(v, a, s); return;
+        case 4: run_r16<16, true, false>(v, a, s); return;
+        case 5: run_r16<32, true, false>(v, a, s); return;
+        case 8: run_r16<16, false, true>(v, a, s); return;
+        case 9: run_r16<32, false, true>(v, a, s); return;
+        case 12: run_r16<16, true, true>(v, a, s); return;
+        default: run_r16<32, true, true>(v, a, s); return;
+      }
+    }
     if constexpr (P >= 16) {
-      // variants 4-7 of the x pass: the shapes of 0-3 with kz-mirror pairing
-      const bool pair = pass == 2 && v >= 4;
       switch (v & 3) {
-        case 1: pair ? run<1, true>(pass, a, s) : run<1>(pass, a, s); return;
-        case 2: pair ? run<2, true>(pass, a, s) : run<2>(pass, a, s); return;
-        case 3: pair ? run<3, true>(pass, a, s) : run<3>(pass, a, s); return;
-        default: pair ? run<0, true>(pass, a, s) : run<0>(pass, a, s); return;
+        case 1: run_x<1>(pass, v, a, s); return;
+        case 2: run_x<2>(pass, v, a, s); return;
+        case 3: run_x<3>(pass, v, a, s); return;
+        default: run_x<0>(pass, v, a, s); return;
       }
     } else {
-      pass == 2 && v >= 4 ? run<0, true>(pass, a, s) : run<0>(pass, a, s);
+      run_x<0>(pass, v, a, s);
     }
   }
 };
@@ -463,7 +727,7 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
 
 template <typename T> int pass_variant_count(const Geom& g, int pass) {
   const int P = pass_extent(g, pass);
-  if (pass == 2) return 2 * kPassVariants;
+  if (pass == 2) return std::is_same<T, float>::value && P == 256 ? 8 * kPassVariants : 4 * kPassVariants;
   return P >= 16 ? kPassVariants : 1;
 }
 
@@ -473,8 +737,16 @@ template <typename T> bool pass_variant_ok(const Geom& g, int pass, int v) {
   if (pass != 2) return true;
   if (P < 16 && (v & 3) != 0) return false;
   int lx = 0;
-  dispatch_p<T>(P, [&](auto L) { lx = L.x_lines(v & 3); });
-  const bool paired = v >= 4;
+  if (v >= 16) {
+    if ((v & 3) == 3) return false;  // 32 lines run at two waves only
+    // element offsets of the pencils, spectrum and reflected spectrum in 32 bits
+    const double big = 3.0 * g.nx * g.Py * g.Pz + 6.0 * (g.Px / 2 + 1) * (g.Py / 2 + 1) * g.Pz;
+    if (big * 8 >= 4294967296.0) return false;
+    lx = (v & 1) ? 32 : 16;
+  } else {
+    dispatch_p<T>(P, [&](auto L) { lx = L.x_lines(v & 3); });
+  }
+  const bool paired = (v & 4) != 0;
   if (paired && !(lx >= 2 && g.Pz >= lx && g.Pz % lx == 0)) return false;
   if (g.ground && !paired && lx % g.Pz != 0) return false;
   return true;

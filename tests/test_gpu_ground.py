@@ -121,22 +121,28 @@ def test_vacuum_ground_is_free_space():
     sf.close()
 
 
-def test_every_x_pass_variant_gives_the_same_ground_product():
-    shape, h, z0 = (20, 12, 16), 0.02, 0.03
+@pytest.mark.parametrize("shape", [(20, 12, 16), (72, 6, 8)])
+def test_every_x_pass_variant_gives_the_same_ground_product(shape):
+    h, z0 = 0.02, 0.03
     xyz, eps = lattice(shape)
     s = ground_system(xyz, eps, h, z0, K)
     x = np.random.default_rng(6).standard_normal(s.order) + 0.5j
     base = {}
-    ok = [v for v in range(8) if s.set_variants([0, 0, v, 0, 0], [0, 0, v, 0, 0])]
-    assert any(v < 4 for v in ok) and any(v >= 4 for v in ok)   # whole ky rows and paired halves
-    for v in ok:
-        assert s.set_variants([0, 0, v, 0, 0], [0, 0, v, 0, 0])
+    ok = [v for v in range(16) if s.set_variants([0, 0, v, 0, 0], [0, 0, v, 0, 0])]
+    assert any(v & 4 == 0 for v in ok) and any(v & 4 for v in ok)   # whole ky rows and paired halves
+    assert any(v & 8 for v in ok)                                    # the XCD-grouped tile order
+    # the register-resident kernel exists in FP32 only
+    r16 = [v for v in range(16, 32) if s.set_variants([0, 0, ok[0], 0, 0], [0, 0, v, 0, 0])]
+    if shape[0] > 64:
+        assert r16
+    for v64, v32 in [(v, v) for v in ok] + [(ok[0], v) for v in r16]:
+        assert s.set_variants([0, 0, v64, 0, 0], [0, 0, v32, 0, 0])
         for prec, tol in ((vie.FP64, 1e-14), (vie.MIXED, 1e-6)):
             y = s.apply_ex(x, prec)
             if prec not in base:
                 base[prec] = y
-            assert rel(y, base[prec]) <= tol, (v, prec)
-    print("ground x-pass variants", ok, flush=True)
+            assert rel(y, base[prec]) <= tol, (v64, v32, prec)
+    print("ground x-pass variants", ok, "FP32 register-resident", r16, flush=True)
     s.close()
 
 
