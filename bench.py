@@ -113,6 +113,9 @@ def pass_kernel_name(pas, P, v, E=4096, real="float"):
         lw, waves = (32, 2) if v & 1 else (16, 2 if v & 2 else 3)
         return (f"k_xpass_r16<{lw},{tf(v & 4)},{tf(v & 8)},{waves}> (register-resident X FFT + 3x3 spectral "
                 f"multiply + inverse X)")
+    if pas in (1, 3) and v >= 4:
+        lw = (16 if real == "float" else 8) * (2 if v == 5 else 1)
+        return f"k_ypass_r16<{real},{lw},{tf(pas == 3)}>"
     pair = f",{tf(v & 4)},{tf(v & 8)}" if pas == 2 else ""
     kern = ("k_zfwd<{R},{P},{lw},{wg}>", "k_ypass<{R},{P},{lw},{wg},false>",
             "k_xpass<{R},{P},{lw},{wg}{pair}> (X FFT + 3x3 spectral multiply + inverse X)",
@@ -718,20 +721,22 @@ def bench_matvec(args):
         out[tag] = {"ms": ms, "bytes": tot, "pass_ms": dict(zip(names, passes)), "pass_bytes": dict(zip(names, pb)),
                     "pass_gbs": {k: (b / (t * 1e-3) / 1e9 if t > 0 else None) for k, b, t in zip(names, pb, passes)}}
     if args.pass_sweep:
-        # every valid launch variant of the x pass, the others as tuned
-        out["x_variants"] = {}
+        # every valid launch variant of the y and x passes, the others as tuned
+        out["pass_variants"] = {}
         for prec, tag in ((vie.MIXED, "fp32"), (vie.FP64, "fp64")):
-            rows = {}
-            for v in range(32):
-                f64, f32 = list(variants["fp64"]), list(variants["fp32"])
-                (f32 if prec == vie.MIXED else f64)[2] = v
-                if not s.set_variants(f64, f32):
-                    continue
-                _, passes = s.time_apply(prec, args.matvec_repeats)
-                rows[v] = {"kernel": pass_kernel_name(2, 2 * n, v, 4096 if prec == vie.MIXED else 2048,
-                                                      "float" if prec == vie.MIXED else "double").split(" ")[0],
-                           "ms": passes[2]}
-            out["x_variants"][tag] = rows
+            out["pass_variants"][tag] = {}
+            for pas in (1, 2, 3):
+                rows = {}
+                for v in range(32):
+                    f64, f32 = list(variants["fp64"]), list(variants["fp32"])
+                    (f32 if prec == vie.MIXED else f64)[pas] = v
+                    if not s.set_variants(f64, f32):
+                        continue
+                    _, passes = s.time_apply(prec, args.matvec_repeats)
+                    rows[v] = {"kernel": pass_kernel_name(pas, 2 * n, v, 4096 if prec == vie.MIXED else 2048,
+                                                          "float" if prec == vie.MIXED else "double").split(" ")[0],
+                               "ms": passes[pas]}
+                out["pass_variants"][tag][names[pas]] = rows
         s.set_variants(variants["fp64"], variants["fp32"])
     if args.arnoldi_column > 0:
         s.reserve(restart=min(args.arnoldi_column + 1, vie.KMAX_RESTART), maximum_iterations=1000,
@@ -1190,7 +1195,7 @@ def main():
     ap.add_argument("--mesh-cpu-lefts", type=int, default=2048,
                     help="mesh: left boxes whose pairs the CPU baseline scans")
     ap.add_argument("--pass-sweep", action="store_true",
-                    help="matvec: also time every launch variant of the x pass (counter runs then cover them all)")
+                    help="matvec: also time every launch variant of the y and x passes (counter runs then cover them all)")
     ap.add_argument("--arnoldi-column", type=int, default=0,
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")

@@ -156,6 +156,125 @@ __global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------- 16-point DFT in registers
+template <typename T> __device__ __forceinline__ cx<T> w16(cx<T> x, int e) {
+  // x * exp(-2 pi i e / 16) for the e of a 4 x 4 split (e in 0..9)
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, r2 = (T)0.70710678118654752440;
+  switch (e) {
+    case 0: return x;
+    case 1: return cx<T>{c1 * x.re + s1 * x.im, c1 * x.im - s1 * x.re};
+    case 2: return cx<T>{r2 * (x.re + x.im), r2 * (x.im - x.re)};
+    case 3: return cx<T>{s1 * x.re + c1 * x.im, s1 * x.im - c1 * x.re};
+    case 4: return mul_mi(x);
+    case 6: return cx<T>{r2 * (x.im - x.re), -r2 * (x.re + x.im)};
+    default: return cx<T>{-c1 * x.re - s1 * x.im, s1 * x.re - c1 * x.im};  // 9
+  }
+}
+
+// 16-point DFT of v in place, natural order, split n = 4a + b, k = c + 4d.
+// IN8: v[8..15] are zero on entry; OUT8: only X[0..7] are formed.
+template <typename T, bool IN8, bool OUT8> __device__ __forceinline__ void dft16(cx<T>* v) {
+  cx<T> u[4][4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if constexpr (IN8) {
+      const cx<T> x0 = v[b], x1 = v[4 + b];
+      u[b][0] = x0 + x1;
+      u[b][1] = x0 + mul_mi(x1);
+      u[b][2] = x0 - x1;
+      u[b][3] = x0 - mul_mi(x1);
+    } else {
+      cx<T> w[4] = {v[b], v[4 + b], v[8 + b], v[12 + b]};
+      dft4(w);
+#pragma unroll
+      for (int c = 0; c < 4; c++) u[b][c] = w[c];
+    }
+  }
+#pragma unroll
+  for (int b = 1; b < 4; b++)
+#pragma unroll
+    for (int c = 1; c < 4; c++) u[b][c] = w16(u[b][c], b * c);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    if constexpr (OUT8) {
+      const cx<T> s02 = u[0][c] + u[2][c], d02 = u[0][c] - u[2][c];
+      const cx<T> s13 = u[1][c] + u[3][c], d13 = u[1][c] - u[3][c];
+      v[c] = s02 + s13;
+      v[c + 4] = d02 + mul_mi(d13);
+    } else {
+      cx<T> w[4] = {u[0][c], u[1][c], u[2][c], u[3][c]};
+      dft4(w);
+#pragma unroll
+      for (int d = 0; d < 4; d++) v[c + 4 * d] = w[d];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- pass 2 / 4, registers
+// The y passes for P = 256 as the two-level 16 x 16 transform of the
+// register-resident x pass (one component per line, so one 16-entry register
+// set per thread): thread (l, c) serves line l of LW consecutive kz lines.
+//   forward: inputs y = c + 16 n1 with n1 < 8 (rows below ny <= P/2), DFT over
+//     n1, twiddle, LDS transpose, DFT over n2; stores X[c + 16 k2];
+//   inverse: loads X[c + 16 k2], DFT over k2, twiddle, LDS transpose, DFT
+//     over k1 forming only the rows m1 < 8; stores the ny physical rows.
+template <typename T, int LW, bool INV>
+__global__ __launch_bounds__(16 * LW) void k_ypass_r16(OpArgs<T> a) {
+  constexpr int PY = 256;
+  __shared__ cx<T> lds[LW * PY];
+  __shared__ cx<T> tw[16][16];
+  if (a.di.skipped()) return;
+  const int tid = threadIdx.x, l = tid % LW, c = tid / LW;
+  const Geom& g = a.g;
+  const long lines = 3L * g.nx * g.Pz;
+  const long line = (long)blockIdx.x * LW + l;
+  const bool live = line < lines;
+  const size_t cxi = live ? (size_t)(line / g.Pz) : 0, kz = live ? (size_t)(line % g.Pz) : 0;
+  for (int i = tid; i < 256; i += 16 * LW) tw[i / 16][i % 16] = a.tw[g.Px + (i / 16) * (i % 16)];
+  const cx<T>* src = INV ? a.B : a.A;
+  cx<T>* dst = INV ? a.A : a.B;
+  const int srows = INV ? g.Py : g.ny, drows = INV ? g.ny : g.Py;
+  cx<T> v[16];
+  if constexpr (!INV) {
+#pragma unroll
+    for (int n1 = 0; n1 < 8; n1++) {
+      const int e = c + 16 * n1;
+      const bool ok = live && e < g.ny;
+      const cx<T> val = src[ok ? ((cxi * srows + e) * g.Pz + kz) : 0];
+      v[n1] = ok ? val : cx<T>{(T)0, (T)0};
+    }
+#pragma unroll
+    for (int n1 = 8; n1 < 16; n1++) v[n1] = cx<T>{(T)0, (T)0};
+    __syncthreads();  // twiddle table
+    dft16<T, true, false>(v);
+  } else {
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) v[k2] = conj(src[(cxi * srows + c + 16 * k2) * g.Pz + kz]);
+    __syncthreads();  // twiddle table
+    dft16<T, false, false>(v);
+  }
+#pragma unroll
+  for (int k = 1; k < 16; k++) v[k] = v[k] * tw[c][k];
+#pragma unroll
+  for (int k = 0; k < 16; k++) lds[(k * 16 + c) * LW + l] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = lds[(c * 16 + k) * LW + l];
+  if constexpr (!INV) {
+    dft16<T, false, false>(v);
+    if (live)
+#pragma unroll
+      for (int k2 = 0; k2 < 16; k2++) dst[(cxi * drows + c + 16 * k2) * g.Pz + kz] = v[k2];
+  } else {
+    dft16<T, false, true>(v);
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) {
+      const int e = c + 16 * m1;
+      if (live && e < g.ny) dst[(cxi * drows + e) * g.Pz + kz] = conj(v[m1]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- pass 3
 // Lines are (ky, kz) with kz fastest.  Forward x FFT of each component is
 // staged through LDS and parked in registers; the 3x3 multiply runs in
@@ -350,62 +469,9 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
 //   inverse, n = m2 + 16 m1: a 16-point DFT over k2 in the same registers,
 //     twiddle, one LDS transpose, a 16-point DFT over k1 forming only the
 //     outputs m1 < 8 (the nx physical rows live below P/2).
-// So each component crosses LDS twice per direction instead of once per
-// radix stage, with two barriers per crossing.  Inverse transforms use
+// So each component crosses LDS once per direction (one transpose, two
+// barriers) instead of once per radix stage.  Inverse transforms use
 // conj(DFT(conj x)); the 1/P scaling is in the spectrum.
-template <typename T> __device__ __forceinline__ cx<T> w16(cx<T> x, int e) {
-  // x * exp(-2 pi i e / 16) for the e of a 4 x 4 split (e in 0..9)
-  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, r2 = (T)0.70710678118654752440;
-  switch (e) {
-    case 0: return x;
-    case 1: return cx<T>{c1 * x.re + s1 * x.im, c1 * x.im - s1 * x.re};
-    case 2: return cx<T>{r2 * (x.re + x.im), r2 * (x.im - x.re)};
-    case 3: return cx<T>{s1 * x.re + c1 * x.im, s1 * x.im - c1 * x.re};
-    case 4: return mul_mi(x);
-    case 6: return cx<T>{r2 * (x.im - x.re), -r2 * (x.re + x.im)};
-    default: return cx<T>{-c1 * x.re - s1 * x.im, s1 * x.re - c1 * x.im};  // 9
-  }
-}
-
-// 16-point DFT of v in place, natural order, split n = 4a + b, k = c + 4d.
-// IN8: v[8..15] are zero on entry; OUT8: only X[0..7] are formed.
-template <typename T, bool IN8, bool OUT8> __device__ __forceinline__ void dft16(cx<T>* v) {
-  cx<T> u[4][4];
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    if constexpr (IN8) {
-      const cx<T> x0 = v[b], x1 = v[4 + b];
-      u[b][0] = x0 + x1;
-      u[b][1] = x0 + mul_mi(x1);
-      u[b][2] = x0 - x1;
-      u[b][3] = x0 - mul_mi(x1);
-    } else {
-      cx<T> w[4] = {v[b], v[4 + b], v[8 + b], v[12 + b]};
-      dft4(w);
-#pragma unroll
-      for (int c = 0; c < 4; c++) u[b][c] = w[c];
-    }
-  }
-#pragma unroll
-  for (int b = 1; b < 4; b++)
-#pragma unroll
-    for (int c = 1; c < 4; c++) u[b][c] = w16(u[b][c], b * c);
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    if constexpr (OUT8) {
-      const cx<T> s02 = u[0][c] + u[2][c], d02 = u[0][c] - u[2][c];
-      const cx<T> s13 = u[1][c] + u[3][c], d13 = u[1][c] - u[3][c];
-      v[c] = s02 + s13;
-      v[c + 4] = d02 + mul_mi(d13);
-    } else {
-      cx<T> w[4] = {u[0][c], u[1][c], u[2][c], u[3][c]};
-      dft4(w);
-#pragma unroll
-      for (int d = 0; d < 4; d++) v[c + 4 * d] = w[d];
-    }
-  }
-}
-
 // WPE waves per SIMD: the register budget (168 at three, 256 at two) for the
 // three held components (96 registers) and one transform's temporaries.
 template <int LW, bool PAIR, bool XCD, int WPE>
@@ -654,7 +720,23 @@ template <typename T, int P> struct PassLauncher {
   // register-resident 16 x 16 kernel (FP32, P = 256; bit 0 then picks 32
   // lines per workgroup instead of 16, bit 1 two waves per SIMD instead of
   // three at 16 lines)
+  template <int LW> static void run_y16(int pass, const OpArgs<T>& a, hipStream_t s) {
+    if constexpr (P == 256) {
+      const long yl = 3L * a.g.nx * a.g.Pz;
+      if (pass == 1)
+        hipLaunchKernelGGL((k_ypass_r16<T, LW, false>), dim3((yl + LW - 1) / LW), dim3(16 * LW), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_ypass_r16<T, LW, true>), dim3((yl + LW - 1) / LW), dim3(16 * LW), 0, s, a);
+    }
+  }
+  // y-pass variants 4 and 5: the register-resident 16 x 16 kernel (P = 256)
+  // at 16 or 32 lines per workgroup in FP32, 8 or 16 in FP64
+  static constexpr int kY16 = std::is_same<T, float>::value ? 16 : 8;
   static void launch(int pass, int v, const OpArgs<T>& a, hipStream_t s) {
+    if ((pass == 1 || pass == 3) && v >= 4) {
+      v == 4 ? run_y16<kY16>(pass, a, s) : run_y16<2 * kY16>(pass, a, s);
+      return;
+    }
     if (pass == 2 && v >= 16) {
       switch (v & 13) {
         case 0: run_r16<16, false, false>(v, a, s); return;
@@ -727,6 +809,7 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
 
 template <typename T> int pass_variant_count(const Geom& g, int pass) {
   const int P = pass_extent(g, pass);
+  if ((pass == 1 || pass == 3) && P == 256) return kPassVariants + 2;
   if (pass == 2) return std::is_same<T, float>::value && P == 256 ? 8 * kPassVariants : 4 * kPassVariants;
   return P >= 16 ? kPassVariants : 1;
 }

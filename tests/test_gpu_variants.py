@@ -6,7 +6,8 @@ instantiations).  Each variant is forced in turn (vie_fft_set_variants) and
 the product compared with variant 0: within 1e-14 relative in FP64 and 1e-6
 in FP32, on geometries where the x pass also offers its kz-mirror-paired
 variants, its XCD-grouped tile order and (x extent 256) its register-resident
-FP32 kernel, with holes and over a PEC plane."""
+FP32 kernel, the register-resident y passes (y extent 256), with holes and
+over a PEC plane."""
 import numpy as np
 import pytest
 
@@ -30,7 +31,9 @@ def system(shape, pec, holes):
 @pytest.mark.parametrize("shape,pec,holes", [((20, 12, 16), False, True), ((9, 17, 12), True, False),
                                              ((64, 64, 64), False, False),
                                              # x extent 256: the register-resident FP32 x pass
-                                             ((100, 6, 5), False, True), ((90, 7, 8), True, False)])
+                                             ((100, 6, 5), False, True), ((90, 7, 8), True, False),
+                                             # y extent 256: the register-resident y passes
+                                             ((5, 90, 6), False, True)])
 def test_all_pass_variants_give_identical_products(shape, pec, holes):
     s = system(shape, pec, holes)
     x = np.random.default_rng(2).standard_normal(s.order) + 1j * np.random.default_rng(3).standard_normal(s.order)
