@@ -240,12 +240,13 @@ def bench_batch(args):
     del inputs
     log(f"batch: host inputs {t_host:.1f} s, device setup (spectra, blocks, tuning) {t_setup:.1f} s")
     biggest = max(systems, key=lambda q: q.order)
-    out = {}
+    out, group_steps = {}, {}
     for prec, name in ((vie.MIXED, "mixed"), (vie.FP64, "fp64")):
         b = vie.Batch()
         b.reserve(args.inflight, biggest, **opts)
         res, secs = serve.run_batch(jobs, precision=prec, inflight=args.inflight, batch=b,
                                     inner_reduction=args.inner_reduction, recycle=args.recycle, **opts)
+        group_steps[name] = b.group_stats()
         b.close()
         out[name] = (res, secs)
         log(f"batch {name}: {len(jobs)} solves in {secs:.2f} s "
@@ -290,6 +291,10 @@ def bench_batch(args):
         "residual_max_mixed": max(r[2].relative_residual for r in rm),
         "residual_max_fp64": max(r[2].relative_residual for r in rd),
         "port_impedance_rel_err_mixed_vs_fp64": z_err,
+        # multi-job steps: each issued step is one launch per kernel for all
+        # of its member jobs (job_steps / group_steps = jobs per launch)
+        "multi_job_steps": {k: {"group_steps": g, "job_steps": j, "jobs_per_launch": (j / g if g else None)}
+                            for k, (g, j) in group_steps.items()},
         "setup_seconds": {"host_inputs": t_host, "device": t_setup},
         "roofline": None,
         "cpu_baseline": None,

@@ -459,6 +459,13 @@ int vie_hip_batch_result(vie_hip_batch *batch, long long id, VIE_CDOUBLE *x, vie
 /* Returns a finished job's workspace to the pool. */
 int vie_hip_batch_release(vie_hip_batch *batch, long long id);
 
+/* Multi-job step counters of a batch: group_steps = multi-job Arnoldi steps
+ * issued (each one launch per kernel for all members), job_steps = the job
+ * steps they covered.  No counterpart in the reference (it solves one
+ * right-hand side per vie_gmres call, dielectric_wire.c:1296-1304). */
+int vie_hip_batch_group_stats(const vie_hip_batch *batch, unsigned long long *group_steps,
+                              unsigned long long *job_steps);
+
 #ifdef __cplusplus
 }
 #endif

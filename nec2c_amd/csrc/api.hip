@@ -725,6 +725,12 @@ struct SolverWork {
   bool rec_breakdown = false, rec_ready = false;
   DevBuf rcoef;            // combination coefficients of U and C
   c64* rec_host = nullptr; // pinned: first cycle's Hraw, dc_a, dc_alpha, then the coefficients
+  // Multi-job steps (a batch's step groups, by cycle precision: 0 FP64, 1 FP32)
+  struct StepGroup* groups[2] = {nullptr, nullptr};
+  bool grouped = false;         // the group issues and checks this cycle's matvec steps
+  bool group_nonfinite = false; // a grouped step reported non-finite values
+  DevBuf fftws;                 // own FFT workspace (group members run concurrently)
+  hipEvent_t ev_join = nullptr; // end of the cycle start, for the group stream
   ~SolverWork() {
     if (step_exec) (void)hipGraphExecDestroy(step_exec);
     if (pin) (void)hipHostFree(pin);
@@ -733,6 +739,7 @@ struct SolverWork {
     for (auto& e : ev_ring)
       if (e) (void)hipEventDestroy(e);
     if (ev_misc) (void)hipEventDestroy(ev_misc);
+    if (ev_join) (void)hipEventDestroy(ev_join);
   }
 };
 
@@ -1149,6 +1156,7 @@ static int ensure_scratch(SolverWork& w) {
     HIP_OK(hipHostGetDevicePointer((void**)&w.pin_dev, w.pin, 0));
     for (auto& e : w.ev_ring) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&w.ev_misc, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&w.ev_join, hipEventDisableTiming));
   }
   return 1;
 }
@@ -1418,6 +1426,287 @@ template <typename T> static int job_issue_step(SolverWork& w, int j) {
   return 1;
 }
 
+// ---------------------------------------------------------------- multi-job steps
+// A batch advances the Arnoldi steps of its jobs in groups, one per cycle
+// precision: a job whose cycle starts joins its group, and every group step
+// is one launch per kernel over all members -- per pass and line length for
+// the FFT passes (vie::volume_apply_mj), then the coupling and delayed-CGS2
+// kernels (vie::step_mj) -- instead of eleven launches per job.  Each job
+// keeps its own state machine: the group only issues the matvec steps of
+// running cycles and reads their statuses.  A job leaves when its cycle is
+// marked done, when it needs the closing step of a full cycle (its own
+// launch), or on non-finite values; it rejoins at its next cycle.  Members
+// run concurrently, so each uses its own FFT workspace.  Real-ground
+// operators, more than kMaxKeys conductors, the plain-CGS2 step and recorded
+// steps keep the single-job path.  VIE_HIP_MULTIJOB=0 turns grouping off.
+static bool multijob_enabled() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("VIE_HIP_MULTIJOB");
+    on = !(e && !strcmp(e, "0"));
+  }
+  return on == 1;
+}
+
+struct StepGroup {
+  bool fp32 = false;
+  hipStream_t stream = nullptr;
+  std::vector<SolverWork*> members;    // joined jobs
+  std::vector<SolverWork*> table;      // jobs of the device tables, in table order
+  std::vector<SolverWork*> stepped[2]; // jobs of the last two issued steps
+  std::vector<hipEvent_t> waits;       // cycle starts the next step waits for
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int issued = 0, checked = 0;
+  DevBuf dev;  // ops | blocks | lists | steps
+  struct Stage {
+    char* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool used = false;
+  };
+  Stage stage[3];
+  int stage_next = 0;
+  std::vector<vie::MjClass> classes;
+  unsigned total_nb[5] = {0, 0, 0, 0, 0};  // workgroups of the five step kernels
+  size_t off_blocks = 0, off_lists = 0, off_steps = 0, off_pre = 0, step_pre = 0;
+  unsigned long long launches = 0, steps = 0;  // group steps issued, jobs x steps covered
+  bool stale = true;  // a member joined: its cycle basis may have moved (recycled columns)
+  int init() {
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& st : stage) HIP_OK(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    return 1;
+  }
+  ~StepGroup() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& st : stage) {
+      if (st.host) (void)hipHostFree(st.host);
+      if (st.done) (void)hipEventDestroy(st.done);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+template <typename T> static bool mj_eligible(const SolverWork& w) {
+  const dielectric_fft_system* s = w.sys;
+  if (!(w.groups[sizeof(T) == sizeof(float)] && multijob_enabled() && w.dcgs2 && !graphs_enabled() && s->plan &&
+        !s->plan->g.ground && s->c <= (size_t)vie::kMaxKeys))
+    return false;
+  const int slot = sizeof(T) == sizeof(double) ? 0 : 1;
+  for (int p = 0; p < 5; p++)
+    if (!vie::pass_mj_ok<T>(s->plan->g, p, s->plan->var[slot][p])) return false;
+  return true;
+}
+
+// Called when a cycle of w has been set up on its system stream.
+template <typename T> static int group_join(SolverWork& w) {
+  StepGroup& g = *w.groups[sizeof(T) == sizeof(float)];
+  const vie::Geom& ge = w.sys->plan->g;
+  const size_t ws = (3 * (size_t)ge.nx * ge.ny * ge.Pz + 3 * (size_t)ge.nx * ge.Py * ge.Pz) * sizeof(vie::cx<T>);
+  if (!w.fftws.alloc(ws)) return 0;
+  HIP_OK(hipEventRecord(w.ev_join, w.sys->stream));
+  g.waits.push_back(w.ev_join);
+  w.grouped = true;
+  w.group_nonfinite = false;
+  g.members.push_back(&w);
+  g.stale = true;
+  return 1;
+}
+
+// Device tables of the jobs in g.table (host staging, then one copy on the
+// group stream, ordered after the steps that read the previous tables).
+template <typename T> static int group_upload(StepGroup& g) {
+  const size_t n = g.table.size();
+  const size_t b_ops = align256(n * sizeof(vie::OpArgs<T>)), b_blocks = align256(5 * n * sizeof(unsigned));
+  const size_t b_lists = align256(5 * n * sizeof(int)), b_steps = align256(n * sizeof(vie::MjStep<T>));
+  // workgroup prefixes: at most 5 n classes of count + 1 entries, then the step's five
+  const size_t b_pre = align256((5 * n + 5 * n + 5 * (n + 1)) * sizeof(unsigned));
+  const size_t total = b_ops + b_blocks + b_lists + b_steps + b_pre;
+  g.off_blocks = b_ops;
+  g.off_lists = b_ops + b_blocks;
+  g.off_steps = b_ops + b_blocks + b_lists;
+  g.off_pre = g.off_steps + b_steps;
+  StepGroup::Stage& st = g.stage[g.stage_next];
+  g.stage_next = (g.stage_next + 1) % 3;
+  if (st.used) HIP_OK(hipEventSynchronize(st.done));  // its last copy has read the staging area
+  if (st.cap < total) {
+    if (st.host) HIP_OK(hipHostFree(st.host));
+    st.host = nullptr;
+    st.cap = 0;
+    HIP_OK(hipHostMalloc((void**)&st.host, total, hipHostMallocDefault));
+    st.cap = total;
+  }
+  if (g.dev.bytes < total) {
+    // steps in flight still read the old tables
+    HIP_OK(hipStreamSynchronize(g.stream));
+    if (!g.dev.alloc(2 * total)) return 0;
+  }
+  auto* ops = (vie::OpArgs<T>*)st.host;
+  auto* blocks = (unsigned*)(st.host + g.off_blocks);
+  auto* lists = (int*)(st.host + g.off_lists);
+  auto* steps = (vie::MjStep<T>*)(st.host + g.off_steps);
+  auto* pre = (unsigned*)(st.host + g.off_pre);
+  for (size_t q = 0; q < n; q++) {
+    SolverWork& w = *g.table[q];
+    dielectric_fft_system* s = w.sys;
+    vie::VolumeView<T> vv = s->plan->view<T>();
+    const vie::Geom& ge = vv.g;
+    vv.A = w.fftws.as<vie::cx<T>>();
+    vv.B = vv.A + 3 * (size_t)ge.nx * ge.ny * ge.Pz;
+    vie::ArnoldiState* sd = w.state.as<vie::ArnoldiState>();
+    vie::DevIndex di;
+    di.j = &sd->j;
+    di.stride = w.ldv;
+    di.skip = &sd->done;
+    vie::cx<T>* V = cycle_basis<T>(w);
+    const size_t c = s->c;
+    ops[q] = vie::op_args<T>(vv, V + c, V + w.ldv + c, 1, di);
+    for (int p = 0; p < 5; p++) blocks[5 * q + p] = vie::pass_blocks_mj<T>(ge, p, vv.var[p]);
+    vie::MjStep<T> m{};
+    m.c = (int)c;
+    m.nv = 3 * s->nbox();
+    m.C = Prec<T>::C(s).template as<vie::cx<T>>();
+    m.B = Prec<T>::B(s).template as<vie::cx<T>>();
+    m.D = Prec<T>::D(s).template as<vie::cx<T>>();
+    m.x = V;
+    m.y = V + w.ldv;
+    m.J.nbox = s->nbox();
+    m.J.eps = Prec<T>::eps(s->plan).template as<vie::cx<T>>();
+    for (int i = 0; i < 3; i++) m.J.self[i] = vie::cx<T>{(T)s->plan->self[i].re, (T)s->plan->self[i].im};
+    m.di = di;
+    m.Q = V;
+    m.ldv = w.ldv;
+    m.n = w.n;
+    m.st = sd;
+    m.partial = w.partial.as<c64>();
+    m.dots = w.dots.as<c64>();
+    m.ring = w.pin_dev->ring;
+    m.eps_b = Prec<T>::eps_b;
+    m.max_j = w.m;
+    vie::step_blocks_mj<T>(m);
+    steps[q] = m;
+  }
+  // classes: per pass, the jobs with the same line length and variant, in pass order
+  g.classes.clear();
+  int off = 0, npre = 0;
+  const int slot = sizeof(T) == sizeof(double) ? 0 : 1;
+  for (int p = 0; p < 5; p++) {
+    std::map<std::pair<int, int>, std::vector<int>> byP;
+    for (size_t q = 0; q < n; q++) {
+      const vie_fft_plan* pl = g.table[q]->sys->plan;
+      const vie::Geom& ge = pl->g;
+      byP[{p == 2 ? ge.Px : (p == 1 || p == 3) ? ge.Py : ge.Pz, (int)pl->var[slot][p]}].push_back((int)q);
+    }
+    for (auto& kv : byP) {
+      vie::MjClass c{p, kv.first.first, kv.first.second, (int)kv.second.size(), off, npre, 0};
+      for (int q : kv.second) {
+        lists[off++] = q;
+        pre[npre++] = c.total_blocks;
+        c.total_blocks += blocks[5 * q + p];
+      }
+      pre[npre++] = c.total_blocks;
+      g.classes.push_back(c);
+    }
+  }
+  // the step kernels' prefixes over all jobs, in table order
+  g.step_pre = (size_t)npre;
+  for (int k = 0; k < 5; k++) {
+    unsigned acc = 0;
+    for (size_t q = 0; q < n; q++) {
+      pre[npre++] = acc;
+      const vie::MjStep<T>& m = steps[q];
+      acc += k == 0 ? m.nb_dots : k == 1 ? m.nb_out : k == 2 ? m.nb_dc : k == 3 ? m.nb_finish : m.nb_update;
+    }
+    pre[npre++] = acc;
+    g.total_nb[k] = acc;
+  }
+  HIP_OK(hipMemcpyAsync(g.dev.p, st.host, total, hipMemcpyHostToDevice, g.stream));
+  HIP_OK(hipEventRecord(st.done, g.stream));
+  st.used = true;
+  return 1;
+}
+
+// Reads the statuses of completed group steps, lets finished members go and
+// issues the next step.  Returns the number of state changes.
+template <typename T> static int group_progress(StepGroup& g) {
+  int changed = 0;
+  while (g.checked < g.issued) {
+    hipError_t r = hipEventQuery(g.ev[g.checked & 1]);
+    if (r == hipErrorNotReady) break;
+    if (r != hipSuccess) {
+      set_error("group step wait failed: %s", hipGetErrorString(r));
+      for (SolverWork* w : g.stepped[g.checked & 1]) w->group_nonfinite = true;  // ends the member's solve
+    }
+    for (SolverWork* w : g.stepped[g.checked & 1]) {
+      const StatusBlock& sb = w->pin->ring[w->checked & 1];
+      if (sb.nonfinite) w->group_nonfinite = true;
+      if (sb.done) w->cycle_done = true;
+      w->checked++;
+    }
+    g.checked++;
+    changed++;
+  }
+  // members whose issued steps are all checked leave when the group has no
+  // more matvec steps for them
+  size_t keep = 0;
+  for (size_t i = 0; i < g.members.size(); i++) {
+    SolverWork* w = g.members[i];
+    const bool more = !w->cycle_done && !w->group_nonfinite && w->issued < w->m_eff;
+    if (!more && w->checked == w->issued) {
+      w->grouped = false;
+      if (w->group_nonfinite) job_finish(*w, VIE_ITERATIVE_NONFINITE, 0);
+      changed++;
+      continue;
+    }
+    g.members[keep++] = w;
+  }
+  g.members.resize(keep);
+  if (g.issued - g.checked >= 2) return changed;
+  std::vector<SolverWork*> next;
+  for (SolverWork* w : g.members)
+    if (!w->cycle_done && !w->group_nonfinite && w->issued < w->m_eff) next.push_back(w);
+  if (next.empty()) return changed;
+  if (g.stale || next != g.table) {
+    g.stale = false;
+    g.table.swap(next);
+    if (!group_upload<T>(g)) {
+      for (SolverWork* w : g.table) w->group_nonfinite = true;
+      g.table.clear();
+      return changed + 1;
+    }
+  }
+  for (hipEvent_t e : g.waits) HIP_OK(hipStreamWaitEvent(g.stream, e, 0));
+  g.waits.clear();
+  const char* base = g.dev.as<char>();
+  const unsigned* pres = (const unsigned*)(base + g.off_pre);
+  const bool ok = vie::volume_apply_mj<T>((const vie::OpArgs<T>*)base, (const unsigned*)(base + g.off_blocks),
+                                          (const int*)(base + g.off_lists), pres, g.classes.data(),
+                                          (int)g.classes.size(), g.stream) != 0;
+  vie::step_mj<T>((const vie::MjStep<T>*)(base + g.off_steps), (int)g.table.size(), pres + g.step_pre, g.total_nb,
+                  g.stream);
+  if (!ok || hipGetLastError() != hipSuccess || hipEventRecord(g.ev[g.issued & 1], g.stream) != hipSuccess) {
+    set_error("group step failed to launch");
+    for (SolverWork* w : g.table) w->group_nonfinite = true;
+    g.table.clear();
+    return changed + 1;
+  }
+  g.stepped[g.issued & 1] = g.table;
+  for (SolverWork* w : g.table) {
+    w->rep.operator_applications++;
+    w->col_sum += w->issued + w->cyc_base;
+    w->col_steps++;
+    w->issued++;
+  }
+  g.issued++;
+  g.launches++;
+  g.steps += g.table.size();
+  return changed + 1;
+}
+
 // Builds the recycled subspace from the finished first cycle (its basis from
 // column 2 rk, its unrotated Hessenberg, and the last step's a and alpha in
 // w.rec_host): P = the rk harmonic Ritz vectors of smallest modulus,
@@ -1603,9 +1892,14 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
         w.issued = w.checked = 0;
         w.cycle_done = false;
         w.phase = PH_CYCLE;
+        if (mj_eligible<T>(w) && w.m_eff > 0 && !group_join<T>(w)) {
+          job_finish(w, VIE_ITERATIVE_ALLOCATION_FAILED, 0);
+          return 1;
+        }
         break;
       }
       case PH_CYCLE: {
+        if (w.grouped) return 0;  // the batch's step group issues and checks the matvec steps
         // one step of lookahead: step j is queued before the status of step
         // j-1 is read, so the GPU never idles on the host; the device marks
         // the cycle done and later queued steps leave the state untouched
@@ -3317,11 +3611,30 @@ struct vie_hip_batch {
   std::map<long long, BatchJob> jobs;
   std::vector<long long> order;  // submission order of unfinished jobs
   long long next_id = 1;
+  // multi-job step groups (0: FP64 cycles, 1: FP32 cycles); declared last,
+  // so they are destroyed (their streams drained) before the jobs
+  std::unique_ptr<StepGroup> groups[2];
 };
 
 extern "C" vie_hip_batch* vie_hip_batch_create(void) {
   g_last_error.clear();
   return new (std::nothrow) vie_hip_batch();
+}
+
+// The step groups are made with the first job (their streams need the device).
+static int batch_groups(vie_hip_batch* b) {
+  if (!multijob_enabled()) return 1;
+  for (int i = 0; i < 2; i++) {
+    if (b->groups[i]) continue;
+    std::unique_ptr<StepGroup> g(new (std::nothrow) StepGroup());
+    if (!g || !g->init()) {
+      set_error("batch: step group setup failed");
+      return 0;
+    }
+    g->fp32 = i == 1;
+    b->groups[i] = std::move(g);
+  }
+  return 1;
 }
 
 extern "C" void vie_hip_batch_destroy(vie_hip_batch* b) {
@@ -3402,6 +3715,7 @@ extern "C" long long vie_hip_batch_submit(vie_hip_batch* b, dielectric_fft_syste
     init_solve_defaults();
     so = &g_solve;
   }
+  if (!batch_groups(b)) return 0;
   BatchJob job;
   job.w = pool_take(b);
   if (!job.w) return 0;
@@ -3413,6 +3727,7 @@ extern "C" long long vie_hip_batch_submit(vie_hip_batch* b, dielectric_fft_syste
     b->pool.push_back(std::move(job.w));
     return 0;
   }
+  for (int i = 0; i < 2; i++) w.groups[i] = b->groups[i].get();
   // b: user layout -> pinned -> device -> system layout; x0 = 0
   memcpy(w.host_io, rhs, nu * 16);
   hipStream_t st = s->stream;
@@ -3471,9 +3786,16 @@ static int batch_step(BatchJob& j, bool blocking) {
   return 0;
 }
 
+static int groups_progress(vie_hip_batch* b) {
+  int changed = 0;
+  if (b->groups[0]) changed += group_progress<double>(*b->groups[0]);
+  if (b->groups[1]) changed += group_progress<float>(*b->groups[1]);
+  return changed;
+}
+
 // One non-blocking sweep over the unfinished jobs; returns how many changed.
 static int batch_sweep(vie_hip_batch* b) {
-  int changed = 0;
+  int changed = groups_progress(b);
   size_t keep = 0;
   for (size_t i = 0; i < b->order.size(); i++) {
     long long id = b->order[i];
@@ -3482,7 +3804,21 @@ static int batch_sweep(vie_hip_batch* b) {
     if (j.state != JB_READY) b->order[keep++] = id;
   }
   b->order.resize(keep);
-  return changed;
+  return changed + groups_progress(b);
+}
+
+extern "C" int vie_hip_batch_group_stats(const vie_hip_batch* b, unsigned long long* group_steps,
+                                         unsigned long long* job_steps) {
+  if (!b) return 0;
+  unsigned long long gs = 0, js = 0;
+  for (auto& g : b->groups)
+    if (g) {
+      gs += g->launches;
+      js += g->steps;
+    }
+  if (group_steps) *group_steps = gs;
+  if (job_steps) *job_steps = js;
+  return 1;
 }
 
 extern "C" size_t vie_hip_batch_progress(vie_hip_batch* b, long long* ready, size_t max_ready, size_t* running) {

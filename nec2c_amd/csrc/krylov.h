@@ -75,7 +75,9 @@ template <typename T> struct JacobiLite {
   __device__ inline cx<T> apply(size_t i, cx<T> v) const {
     size_t comp = i / nbox, cell = i - comp * nbox;
     cx<T> e = eps[cell], one = {(T)1, (T)0};
-    return cdiv(v, e - (e - one) * self[comp]);
+    // selects, not an indexed load: a copy of the struct stays in registers
+    const cx<T> sf = comp == 0 ? self[0] : comp == 1 ? self[1] : self[2];
+    return cdiv(v, e - (e - one) * sf);
   }
 };
 
@@ -162,5 +164,44 @@ void precondition_apply(const c64* x, c64* y, size_t n, const JacobiView& J, hip
 template <typename T>
 void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv, cx<T>* yc, cx<T>* yv, int precond,
                     ReduceScratch& rs, hipStream_t s, DevIndex di = {});
+
+// ---- multi-job Arnoldi steps ----
+// One job's share of a multi-job delayed-CGS2 matvec step: the arguments its
+// single-job coupling (Jacobi-scaled input) and delayed-CGS2 launches take,
+// and its own workgroup counts.  Every kernel of the step takes the job
+// table and walks the jobs with blockIdx.y; workgroups past a job's own count
+// return at once.
+template <typename T> struct MjStep {
+  // conductor coupling: c conductors (0: none, at most kMaxKeys), nv volume
+  // unknowns; x / y the step's input and output columns (offset by di)
+  int c;
+  size_t nv;
+  const cx<T>* C;
+  const cx<T>* B;
+  const cx<T>* D;
+  const cx<T>* x;
+  cx<T>* y;
+  JacobiLite<T> J;
+  DevIndex di;
+  // delayed CGS2 over the cycle basis Q (n unknowns, leading dimension ldv)
+  cx<T>* Q;
+  size_t ldv, n;
+  ArnoldiState* st;
+  c64* partial;  // reduction partials, kMaxDotBlocks per key
+  c64* dots;
+  StepStatus* ring;
+  double eps_b;
+  int max_j;
+  // workgroups: coupling reduction, coupling output, delayed-CGS2 reduction,
+  // its finishing sums, the fused update
+  unsigned nb_dots, nb_out, nb_dc, nb_finish, nb_update;
+};
+// Workgroup counts of one job (fills the nb_ fields from c, nv, n, max_j).
+template <typename T> void step_blocks_mj(MjStep<T>& m);
+// The coupling and delayed-CGS2 kernels of one step for count jobs (tab a
+// device array); pre: five workgroup prefixes of count + 1 entries (the nb_
+// fields in order, see mj_job), total the five totals.
+template <typename T>
+void step_mj(const MjStep<T>* tab, int count, const unsigned* pre, const unsigned total[5], hipStream_t s);
 
 }  // namespace vie

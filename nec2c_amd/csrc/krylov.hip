@@ -83,9 +83,9 @@ __device__ inline c64 block_reduce(c64 v, c64* lds) {
 template <typename T> struct DotShape { static constexpr int CPL = 128 / (int)sizeof(cx<T>); };
 
 template <typename T, bool CONJ, bool JAC>
-__global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n,
+__device__ __forceinline__ void k_dots_body(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n,
                                               int self_norm, c64* partial, int ldp, JacobiLite<T> J,
-                                              DevIndex di) {
+                                              DevIndex di, unsigned bx, unsigned nbx) {
   constexpr int CPL = DotShape<T>::CPL;
   constexpr int WAVES = KWG / 64;
   constexpr int CH = 64 * CPL;
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int
   const int tot = nk + (self_norm ? 1 : 0);
   for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
   const size_t nchunks = (n + CH - 1) / CH;
-  for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+  for (size_t ch = bx; ch < nchunks; ch += nbx) {
     const size_t base = ch * CH;
     cx<T> a[CPL], an[CPL];
     auto load_key = [&](int k, cx<T>* dst) {
@@ -140,7 +140,13 @@ __global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + blockIdx.x] = lds[k];
+  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + bx] = lds[k];
+}
+template <typename T, bool CONJ, bool JAC>
+__global__ __launch_bounds__(KWG, 4) void k_dots(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n,
+                                              int self_norm, c64* partial, int ldp, JacobiLite<T> J,
+                                              DevIndex di) {
+  k_dots_body<T, CONJ, JAC>(V, ldv, nk, w, n, self_norm, partial, ldp, J, di, blockIdx.x, gridDim.x);
 }
 
 // out[k] = sum_b partial[k * ldp + b] (key-major, contiguous per key): one
@@ -167,15 +173,19 @@ __global__ __launch_bounds__(KWG) void k_finish(const c64* partial, int ldp, uns
 }
 
 // k_finish for the delayed-CGS2 reduction, key count read from the state.
-__global__ __launch_bounds__(KWG) void k_finish_dc(const c64* partial, int ldp, unsigned nblk,
-                                                   const ArnoldiState* st, int has_w, c64* out) {
+__device__ __forceinline__ void k_finish_dc_body(const c64* partial, int ldp, unsigned nblk,
+                                                   const ArnoldiState* st, int has_w, c64* out, unsigned bx, unsigned nbx) {
   if (st->done) return;
   const int j = st->j, nk = has_w ? 2 * j + 3 : j + 1;
-  const int key = blockIdx.x * (KWG / 64) + (threadIdx.x >> 6);
+  const int key = bx * (KWG / 64) + (threadIdx.x >> 6);
   if (key >= nk) return;
   const int lane = threadIdx.x & 63;
   const c64 s = sum_partials(partial + (size_t)key * ldp, nblk, lane);
   if (lane == 0) out[key] = s;
+}
+__global__ __launch_bounds__(KWG) void k_finish_dc(const c64* partial, int ldp, unsigned nblk,
+                                                   const ArnoldiState* st, int has_w, c64* out) {
+  k_finish_dc_body(partial, ldp, nblk, st, has_w, out, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- update
@@ -384,8 +394,8 @@ __global__ void k_cycle_init(ArnoldiState* st, const c64* beta2, double inner_ta
 //   a_k = q_k^H u, b_k = q_k^H w (k < j), nu = u^H u, c = u^H w, d = w^H w.
 // Layout: [a_0..a_{j-1}, b_0..b_{j-1}, nu, c, d]; without w only [a, nu].
 template <typename T>
-__global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, const ArnoldiState* st, int has_w,
-                                                 size_t n, c64* partial, int ldp) {
+__device__ __forceinline__ void k_dots_dc_body(const cx<T>* Q, size_t ldv, const ArnoldiState* st, int has_w,
+                                                 size_t n, c64* partial, int ldp, unsigned bx, unsigned nbx) {
   constexpr int CPL = DcShape<T>::CPL;  // u, w and q in flight per lane
   constexpr int WAVES = KWG / 64;
   constexpr int CH = 64 * CPL;
@@ -401,7 +411,7 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
   for (int k = threadIdx.x; k < tot; k += KWG) lds[k] = c64{0, 0};
   // whole chunks: the basis padding past n is zero (krylov.h, kBasisAlign)
   const size_t nchunks = (n + CH - 1) / CH;
-  for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+  for (size_t ch = bx; ch < nchunks; ch += nbx) {
     const size_t base = ch * CH;
     __syncthreads();
     for (int e = threadIdx.x; e < CH; e += KWG) {
@@ -481,7 +491,12 @@ __global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, 
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + blockIdx.x] = lds[k];
+  for (int k = threadIdx.x; k < tot; k += KWG) partial[(size_t)k * ldp + bx] = lds[k];
+}
+template <typename T>
+__global__ __launch_bounds__(KWG, 4) void k_dots_dc(const cx<T>* Q, size_t ldv, const ArnoldiState* st, int has_w,
+                                                 size_t n, c64* partial, int ldp) {
+  k_dots_dc_body<T>(Q, ldv, st, has_w, n, partial, ldp, blockIdx.x, gridDim.x);
 }
 
 // End of an Arnoldi step (thread 0): status block to the host ring slot of
@@ -512,8 +527,8 @@ __device__ __forceinline__ void end_step(ArnoldiState* st, StepStatus* ring, int
 //     projections are h = [(b - t)/alpha ; ((c - a^H b)/alpha - t_j)/alpha];
 //  4. next candidate u' = (w - Q_j e - f u)/alpha with s = t + alpha h,
 //     e = s(0:j-1) - (s_j/alpha) a, f = s_j/alpha; q_j = (u - Q_j a)/alpha.
-__global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, const c64* dots,
-                                                double breakdown_eps, StepStatus* ring) {
+__device__ __forceinline__ void k_dc_mid_body(ArnoldiState* st, int has_w, const c64* dots,
+                                                double breakdown_eps, StepStatus* ring, unsigned bx, unsigned nbx) {
   __shared__ c64 a[kMaxRestart], b[kMaxRestart], t[kMaxRestart + 1], col[kMaxRestart + 1];
   __shared__ c64 tpart[2][kMaxRestart + 1];
   __shared__ c64 sn[kMaxRestart];
@@ -709,10 +724,14 @@ __global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, con
   for (int k = j + 1 + tid; k <= m; k += KWG) Hraw[(size_t)j * ld + k] = c64{0, 0};
   if (tid == 0) end_step(st, ring, j);
 }
+__global__ __launch_bounds__(KWG) void k_dc_mid(ArnoldiState* st, int has_w, const c64* dots,
+                                                double breakdown_eps, StepStatus* ring) {
+  k_dc_mid_body(st, has_w, dots, breakdown_eps, ring, blockIdx.x, gridDim.x);
+}
 
 // Fused update of step j: slot j <- (u - Q_j a)/alpha, slot j+1 <- (w - Q_j e - f u)/alpha.
 template <typename T>
-__global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t n, const ArnoldiState* st) {
+__device__ __forceinline__ void k_update_dc_body(cx<T>* Q, size_t ldv, size_t n, const ArnoldiState* st, unsigned bx, unsigned nbx) {
   __shared__ c64 ca[kMaxRestart], ce[kMaxRestart];
   if (st->done) return;
   const int j = st->ju;
@@ -728,8 +747,8 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
   cx<T>* w = Q + (size_t)(j + 1) * ldv;
   // whole KWG * UE groups: the basis padding past n is zero and stays zero
   const size_t npad = (n + KWG * UE - 1) / (KWG * UE) * (KWG * UE);
-  const size_t step = (size_t)gridDim.x * KWG * UE;
-  for (size_t base = (size_t)blockIdx.x * KWG * UE + threadIdx.x; base < npad; base += step) {
+  const size_t step = (size_t)nbx * KWG * UE;
+  for (size_t base = (size_t)bx * KWG * UE + threadIdx.x; base < npad; base += step) {
     double ur[UE], ui[UE], wr[UE], wi[UE];
 #pragma unroll
     for (int e = 0; e < UE; e++) {
@@ -803,6 +822,10 @@ __global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t 
       w[i] = cx<T>{(T)(wr[e] * ia), (T)(wi[e] * ia)};
     }
   }
+}
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_update_dc(cx<T>* Q, size_t ldv, size_t n, const ArnoldiState* st) {
+  k_update_dc_body<T>(Q, ldv, n, st, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- elementwise
@@ -919,16 +942,16 @@ __device__ __forceinline__ void stage_zc(cx<T>* zs, const cx<T>* vc, const cx<T>
 // y_c = C z_c + B z_v   (dielectric_fft.c:56-60) with B z_v summed from the
 // partials of the preceding B reduction (or taken from bz when given).
 template <typename T>
-__global__ __launch_bounds__(256) void k_couple_out(const cx<T>* vc, const cx<T>* Cm, int c, int precond,
+__device__ __forceinline__ void k_couple_out_body(const cx<T>* vc, const cx<T>* Cm, int c, int precond,
                                                     const cx<T>* D, cx<T>* yv, size_t nv, const c64* bpart, int ldp,
-                                                    unsigned nblk, const c64* bz, cx<T>* yc, DevIndex di) {
+                                                    unsigned nblk, const c64* bz, cx<T>* yc, DevIndex di, unsigned bx, unsigned nbx) {
   __shared__ cx<T> zs[kMaxConductorsShared];
   if (di.skipped()) return;
   vc += di.off();
   yv += di.off();
   yc += di.off();
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if ((size_t)blockIdx.x * 256 < nv) {
+  const size_t i = (size_t)bx * 256 + threadIdx.x;
+  if ((size_t)bx * 256 < nv) {
     c64 s = {0, 0};
     if (i < nv) s = c64{(double)yv[i].re, (double)yv[i].im};
     for (int j0 = 0; j0 < c; j0 += kMaxConductorsShared) {
@@ -944,8 +967,8 @@ __global__ __launch_bounds__(256) void k_couple_out(const cx<T>* vc, const cx<T>
     }
     if (i < nv) yv[i] = cx<T>{(T)s.re, (T)s.im};
   }
-  if ((int)blockIdx.x * 4 < c) {
-    const int lane = threadIdx.x & 63, row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if ((int)bx * 4 < c) {
+    const int lane = threadIdx.x & 63, row = bx * 4 + (threadIdx.x >> 6);
     c64 acc = {0, 0};
     if (row < c) acc = bz ? bz[row] : sum_partials(bpart + (size_t)row * ldp, nblk, lane);
     c64 part = {0, 0};
@@ -963,6 +986,12 @@ __global__ __launch_bounds__(256) void k_couple_out(const cx<T>* vc, const cx<T>
     part = wave_reduce(part);
     if (row < c && lane == 0) yc[row] = cx<T>{(T)(acc.re + part.re), (T)(acc.im + part.im)};
   }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_couple_out(const cx<T>* vc, const cx<T>* Cm, int c, int precond,
+                                                    const cx<T>* D, cx<T>* yv, size_t nv, const c64* bpart, int ldp,
+                                                    unsigned nblk, const c64* bz, cx<T>* yc, DevIndex di) {
+  k_couple_out_body<T>(vc, Cm, c, precond, D, yv, nv, bpart, ldp, nblk, bz, yc, di, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- layout
@@ -1251,6 +1280,91 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
                      cv.C, c, precond, cv.D, yv, cv.nv, bpart, kMaxDotBlocks, nb, bz, yc, di);
 }
 
+// ---------------------------------------------------------------- multi-job step
+template <typename T> void step_blocks_mj(MjStep<T>& m) {
+  m.nb_dots = m.c > 0 && m.nv > 0 ? dot_blocks<T>(m.nv) : 0;
+  m.nb_out = m.c > 0 ? (unsigned)std::max<size_t>((m.nv + 255) / 256, ((size_t)m.c + 3) / 4) : 0;
+  m.nb_dc = dc_blocks<T>(m.n);
+  m.nb_finish = (unsigned)((2 * m.max_j + 3 + 3) / 4);
+  m.nb_update = update_blocks(m.n, DcShape<T>::UE);
+}
+
+// One job's step arguments from the table, pointers marked global.
+template <typename T> __device__ __forceinline__ MjStep<T> step_global(const MjStep<T>& t) {
+  MjStep<T> m = t;
+  if (m.c > 0) {
+    m.C = to_global(m.C);
+    m.B = to_global(m.B);
+    m.D = to_global(m.D);
+  }
+  m.x = to_global(m.x);
+  m.y = to_global(m.y);
+  m.J.eps = to_global(m.J.eps);
+  m.di = to_global(m.di);
+  m.Q = to_global(m.Q);
+  m.st = to_global(m.st);
+  m.partial = to_global(m.partial);
+  m.dots = to_global(m.dots);
+  m.ring = to_global(m.ring);
+  return m;
+}
+
+// (the multi-job reductions carry no minimum occupancy: at four waves per
+// SIMD their chunk registers would spill to scratch)
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_dots_mj(const MjStep<T>* __restrict__ tab, const unsigned* __restrict__ pre,
+                                                 int count) {
+  const int q = mj_job(pre, count, blockIdx.x);
+  const MjStep<T> m = step_global(tab[q]);
+  k_dots_body<T, false, true>(m.B, m.nv, m.c, m.x + m.c, m.nv, 0, m.partial, kMaxDotBlocks, m.J, m.di,
+                              blockIdx.x - pre[q], m.nb_dots);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_couple_out_mj(const MjStep<T>* __restrict__ tab,
+                                                       const unsigned* __restrict__ pre, int count) {
+  const int q = mj_job(pre, count, blockIdx.x);
+  const MjStep<T> m = step_global(tab[q]);
+  k_couple_out_body<T>(m.x, m.C, m.c, 1, m.D, m.y + m.c, m.nv, m.partial, kMaxDotBlocks, m.nb_dots, nullptr, m.y,
+                       m.di, blockIdx.x - pre[q], m.nb_out);
+}
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_dots_dc_mj(const MjStep<T>* __restrict__ tab,
+                                                    const unsigned* __restrict__ pre, int count) {
+  const int q = mj_job(pre, count, blockIdx.x);
+  const MjStep<T> m = step_global(tab[q]);
+  k_dots_dc_body<T>(m.Q, m.ldv, m.st, 1, m.n, m.partial, kMaxDotBlocks, blockIdx.x - pre[q], m.nb_dc);
+}
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_finish_dc_mj(const MjStep<T>* __restrict__ tab,
+                                                      const unsigned* __restrict__ pre, int count) {
+  const int q = mj_job(pre, count, blockIdx.x);
+  const MjStep<T> m = step_global(tab[q]);
+  k_finish_dc_body(m.partial, kMaxDotBlocks, m.nb_dc, m.st, 1, m.dots, blockIdx.x - pre[q], m.nb_finish);
+}
+template <typename T> __global__ __launch_bounds__(KWG) void k_dc_mid_mj(const MjStep<T>* __restrict__ tab) {
+  const MjStep<T> m = step_global(tab[blockIdx.x]);
+  k_dc_mid_body(m.st, 1, m.dots, m.eps_b, m.ring, 0, 1);
+}
+template <typename T>
+__global__ __launch_bounds__(KWG) void k_update_dc_mj(const MjStep<T>* __restrict__ tab,
+                                                      const unsigned* __restrict__ pre, int count) {
+  const int q = mj_job(pre, count, blockIdx.x);
+  const MjStep<T> m = step_global(tab[q]);
+  k_update_dc_body<T>(m.Q, m.ldv, m.n, m.st, blockIdx.x - pre[q], m.nb_update);
+}
+
+template <typename T>
+void step_mj(const MjStep<T>* tab, int count, const unsigned* pre, const unsigned total[5], hipStream_t s) {
+  if (count <= 0) return;
+  const size_t P = (size_t)count + 1;
+  if (total[0]) hipLaunchKernelGGL((k_dots_mj<T>), dim3(total[0]), dim3(KWG), 0, s, tab, pre, count);
+  if (total[1]) hipLaunchKernelGGL((k_couple_out_mj<T>), dim3(total[1]), dim3(256), 0, s, tab, pre + P, count);
+  hipLaunchKernelGGL((k_dots_dc_mj<T>), dim3(total[2]), dim3(KWG), 0, s, tab, pre + 2 * P, count);
+  hipLaunchKernelGGL((k_finish_dc_mj<T>), dim3(total[3]), dim3(KWG), 0, s, tab, pre + 3 * P, count);
+  hipLaunchKernelGGL((k_dc_mid_mj<T>), dim3((unsigned)count), dim3(KWG), 0, s, tab);
+  hipLaunchKernelGGL((k_update_dc_mj<T>), dim3(total[4]), dim3(KWG), 0, s, tab, pre + 4 * P, count);
+}
+
 #define VIE_INST(T)                                                                                              \
   template void multidot<T>(const cx<T>*, size_t, int, const cx<T>*, size_t, int, ReduceScratch&, c64*,            \
                             hipStream_t);                                                                        \
@@ -1271,7 +1385,9 @@ void coupling_apply(const CouplingView<T>& cv, const cx<T>* vc, const cx<T>* vv,
   template void dcgs2_step<T>(cx<T>*, size_t, bool, size_t, ArnoldiState*, ReduceScratch&, c64*, double, int,    \
                               StepStatus*, hipStream_t);                                                         \
   template int dcgs2_time<T>(cx<T>*, size_t, size_t, ArnoldiState*, ReduceScratch&, int, int, hipStream_t, float*, \
-                             float*);
+                             float*);                                                                            \
+  template void step_blocks_mj<T>(MjStep<T>&);                                                                   \
+  template void step_mj<T>(const MjStep<T>*, int, const unsigned*, const unsigned*, hipStream_t);
 VIE_INST(float)
 VIE_INST(double)
 

@@ -42,9 +42,11 @@ template <typename To, typename From> __host__ __device__ inline cx<To> cast(cx<
 // Streaming loads for data read once per pass (Krylov basis, coupling
 // blocks): the nontemporal hint keeps them from displacing reused data in
 // the caches and lets the memory pipeline stream at a higher rate.
+// (device memory only: the load is a global one)
 template <typename T> __device__ __forceinline__ cx<T> ld_stream(const cx<T>* p) {
   typedef T v2 __attribute__((ext_vector_type(2)));
-  v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  typedef __attribute__((address_space(1))) const v2 gv2;
+  v2 v = __builtin_nontemporal_load((gv2*)(uintptr_t)p);
   return cx<T>{v.x, v.y};
 }
 
@@ -65,6 +67,35 @@ struct DevIndex {
   __device__ inline size_t off() const { return j ? (size_t)(*j) * stride : 0; }
   __device__ inline bool skipped() const { return skip && *skip; }
 };
+
+// A pointer read from device memory is generic (flat) to the compiler; one
+// that comes from an integer through the global address space is known to
+// be global, so accesses through it are global instructions with a scalar
+// base (fewer registers than flat ones).  For device pointers only.
+template <typename P> __device__ __forceinline__ P* to_global(P* p) {
+  typedef __attribute__((address_space(1))) P gP;
+  return (P*)(gP*)(uintptr_t)p;
+}
+__device__ __forceinline__ DevIndex to_global(DevIndex d) {
+  d.j = d.j ? to_global(d.j) : nullptr;
+  d.skip = d.skip ? to_global(d.skip) : nullptr;
+  return d;
+}
+
+// Multi-job launches run a compact 1-D grid: pre[i] is the first workgroup
+// of job i and pre[count] the total, so workgroup b serves job
+// mj_job(pre, count, b) (the last i with pre[i] <= b; jobs without work have
+// pre[i] == pre[i + 1] and are never chosen) as its block b - pre[i].  b is
+// uniform across the workgroup, so the search runs on scalar loads.
+__device__ __forceinline__ int mj_job(const unsigned* pre, int count, unsigned b) {
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 // Wave sum by DPP row operations (no LDS crossbar), total in lane 63 only:
 // quad swaps (xor 1, xor 2), half-row and row mirrors give every row its sum,

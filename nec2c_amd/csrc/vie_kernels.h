@@ -35,6 +35,73 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
 template <typename T> int pass_variant_count(const Geom& g, int pass);
 template <typename T> bool pass_variant_ok(const Geom& g, int pass, int variant);
 
+// Launch arguments of the five volume passes (one job).
+template <typename T> struct OpArgs {
+  Geom g;
+  const cx<T>* xin;   // [3][nbox] input (v when precond, else z)
+  cx<T>* yout;        // [3][nbox] output
+  cx<T>* A;           // [3][nx][ny][Pz]
+  cx<T>* B;           // [3][nx][Py][Pz]
+  const cx<T>* spec;  // [6][Px/2+1][Py/2+1][Pz/2+1] octant spectrum / G
+  const cx<T>* rspec; // real ground: [6][Px/2+1][Py/2+1][Pz] reflected spectrum / G
+  const cx<T>* eps;   // [nbox] relative permittivity (1 in holes)
+  const cx<T>* tw;    // twiddles: x table (Px), then y (Py), then z (Pz)
+  cx<T> self[3];      // self-term diagonal N_ii(0)
+  int precond;
+  DevIndex di;        // xin / yout offset and skip flag (Arnoldi step replay)
+};
+
+// A job's pass arguments as read from a multi-job table, its pointers marked global.
+template <typename T> __device__ __forceinline__ OpArgs<T> op_args_global(const OpArgs<T>& t) {
+  OpArgs<T> a = t;
+  a.xin = to_global(a.xin);
+  a.yout = to_global(a.yout);
+  a.A = to_global(a.A);
+  a.B = to_global(a.B);
+  a.spec = to_global(a.spec);
+  a.rspec = a.rspec ? to_global(a.rspec) : nullptr;
+  a.eps = to_global(a.eps);
+  a.tw = to_global(a.tw);
+  a.di = to_global(a.di);
+  return a;
+}
+
+template <typename T>
+inline OpArgs<T> op_args(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, DevIndex di = {}) {
+  OpArgs<T> a;
+  a.di = di;
+  a.g = v.g;
+  a.xin = xin;
+  a.yout = yout;
+  a.A = v.A;
+  a.B = v.B;
+  a.spec = v.spec;
+  a.rspec = v.rspec;
+  a.eps = v.eps;
+  a.tw = v.tw;
+  for (int i = 0; i < 3; i++) a.self[i] = v.self[i];
+  a.precond = precond;
+  return a;
+}
+
+// Multi-job launches: one launch per pass covers every job whose lines of
+// that pass have the same length P and the same launch variant (a class),
+// for P up to kMjMaxP and the variants of the LDS kernels (pass_mj_ok).
+// ops[q] and blocks[5 q + pass] describe job q; lists + offset holds the
+// job indices of a class, blockIdx.y walking them.
+// pres + pre holds the class's workgroup prefix (count + 1 entries; see mj_job).
+constexpr int kMjMaxP = 128;
+struct MjClass {
+  int pass, P, variant, count, offset, pre;
+  unsigned total_blocks;
+};
+// Workgroups of pass `pass` at `variant` for geometry g (as its single-job launch).
+template <typename T> unsigned pass_blocks_mj(const Geom& g, int pass, int variant);
+template <typename T> bool pass_mj_ok(const Geom& g, int pass, int variant);
+template <typename T>
+int volume_apply_mj(const OpArgs<T>* ops, const unsigned* blocks, const int* lists, const unsigned* pres,
+                    const MjClass* classes, int nclasses, hipStream_t s);
+
 // y = eps z - (eps-1) conv(z), z = precond ? D^{-1} xin : xin.  Volume part only.
 // ev (optional, 6 events): recorded before each of the five passes and after the last.
 // di (optional): xin / yout offset by (*di.j) * di.stride on the device, whole

@@ -23,20 +23,6 @@
 
 namespace vie {
 
-template <typename T> struct OpArgs {
-  Geom g;
-  const cx<T>* xin;   // [3][nbox] input (v when precond, else z)
-  cx<T>* yout;        // [3][nbox] output
-  cx<T>* A;           // [3][nx][ny][Pz]
-  cx<T>* B;           // [3][nx][Py][Pz]
-  const cx<T>* spec;  // [6][Px/2+1][Py/2+1][Pz/2+1] octant spectrum / G
-  const cx<T>* rspec; // real ground: [6][Px/2+1][Py/2+1][Pz] reflected spectrum / G
-  const cx<T>* eps;   // [nbox] relative permittivity (1 in holes)
-  const cx<T>* tw;    // twiddles: x table (Px), then y (Py), then z (Pz)
-  cx<T> self[3];      // self-term diagonal N_ii(0)
-  int precond;
-  DevIndex di;        // xin / yout offset and skip flag (Arnoldi step replay)
-};
 
 template <typename T> __device__ __forceinline__ cx<T> jacobi_scale(cx<T> v, cx<T> e, cx<T> s) {
   // v / (eps - (eps - 1) self)   (vie_fft.c:271)
@@ -51,7 +37,7 @@ template <typename T> __device__ __forceinline__ cx<T> ld(const cx<T>* p) { retu
 // transform so their HBM latency overlaps; each transformed component is
 // stored as soon as it is done.
 template <typename T, int PZ, int LW, int WG>
-__global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
+__device__ __forceinline__ void zfwd_body(const OpArgs<T>& a, unsigned bx, unsigned nbx) {
   constexpr int E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
@@ -59,7 +45,7 @@ __global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.nx * g.ny;  // (x, y) pencils
-  const long line0 = (long)blockIdx.x * LW;
+  const long line0 = (long)bx * LW;
   const cx<T>* twz = a.tw + g.Px + g.Py;
   for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
   const size_t nbox = g.nbox();
@@ -111,11 +97,24 @@ __global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
   }
 }
 
+template <typename T, int PZ, int LW, int WG>
+__global__ __launch_bounds__(WG) void k_zfwd(OpArgs<T> a) {
+  zfwd_body<T, PZ, LW, WG>(a, blockIdx.x, gridDim.x);
+}
+template <typename T, int PZ, int LW, int WG>
+__global__ __launch_bounds__(WG) void k_zfwd_mj(const OpArgs<T>* __restrict__ ops, const unsigned* __restrict__ blocks,
+                                                const int* __restrict__ list,
+                                                const unsigned* __restrict__ pre, int count) {
+  const int i = mj_job(pre, count, blockIdx.x), q = list[i];
+  zfwd_body<T, PZ, LW, WG>(op_args_global(ops[q]), blockIdx.x - pre[i], blocks[5 * q + 0]);
+}
+
+
 // ---------------------------------------------------------------- pass 2 / 4
 // Lines are (comp, x, kz) with kz fastest; a workgroup takes LW consecutive
 // lines so each y row it loads is LW contiguous values.
 template <typename T, int PY, int LW, int WG, bool INV>
-__global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
+__device__ __forceinline__ void ypass_body(const OpArgs<T>& a, unsigned bx, unsigned nbx) {
   constexpr int E = LW * PY;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PY];
@@ -123,7 +122,7 @@ __global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = 3L * g.nx * g.Pz;
-  const long line0 = (long)blockIdx.x * LW;
+  const long line0 = (long)bx * LW;
   const cx<T>* twy = a.tw + g.Px;
   for (int i = tid; i < PY; i += WG) tw[i] = twy[i];
   const int nin = INV ? PY : g.ny;      // rows read
@@ -155,6 +154,19 @@ __global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
     }
   }
 }
+
+template <typename T, int PY, int LW, int WG, bool INV>
+__global__ __launch_bounds__(WG) void k_ypass(OpArgs<T> a) {
+  ypass_body<T, PY, LW, WG, INV>(a, blockIdx.x, gridDim.x);
+}
+template <typename T, int PY, int LW, int WG, bool INV>
+__global__ __launch_bounds__(WG) void k_ypass_mj(const OpArgs<T>* __restrict__ ops, const unsigned* __restrict__ blocks,
+                                                const int* __restrict__ list,
+                                                 const unsigned* __restrict__ pre, int count) {
+  const int i = mj_job(pre, count, blockIdx.x), q = list[i];
+  ypass_body<T, PY, LW, WG, INV>(op_args_global(ops[q]), blockIdx.x - pre[i], blocks[5 * q + (INV ? 3 : 1)]);
+}
+
 
 // ---------------------------------------------------------------- 16-point DFT in registers
 template <typename T> __device__ __forceinline__ cx<T> w16(cx<T> x, int e) {
@@ -326,8 +338,8 @@ __device__ __forceinline__ long xcd_tile(const Geom& g, long b, long nb) {
   return (long)row * T + (ti ? T - 1 - tp : tp);
 }
 
-template <typename T, int PX, int LW, int WG, bool PAIR = false, bool XCD = false>
-__global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
+template <typename T, int PX, int LW, int WG, bool PAIR, bool XCD>
+__device__ __forceinline__ void xpass_body(const OpArgs<T>& a, unsigned bx, unsigned nbx) {
   constexpr int E = LW * PX, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PX];
@@ -335,7 +347,7 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.Py * g.Pz;
-  const long blk = XCD ? xcd_tile<PAIR, LW>(g, blockIdx.x, gridDim.x) : (long)blockIdx.x;
+  const long blk = XCD ? xcd_tile<PAIR, LW>(g, bx, nbx) : (long)bx;
   const size_t plane = (size_t)g.Py * g.Pz;
   for (int i = tid; i < PX; i += WG) tw[i] = a.tw[i];
   cx<T> hold[3][EPT], next[EPT];
@@ -457,6 +469,19 @@ __global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
     __syncthreads();
   }
 }
+
+template <typename T, int PX, int LW, int WG, bool PAIR = false, bool XCD = false>
+__global__ __launch_bounds__(WG) void k_xpass(OpArgs<T> a) {
+  xpass_body<T, PX, LW, WG, PAIR, XCD>(a, blockIdx.x, gridDim.x);
+}
+template <typename T, int PX, int LW, int WG, bool PAIR, bool XCD>
+__global__ __launch_bounds__(WG) void k_xpass_mj(const OpArgs<T>* __restrict__ ops, const unsigned* __restrict__ blocks,
+                                                const int* __restrict__ list,
+                                                 const unsigned* __restrict__ pre, int count) {
+  const int i = mj_job(pre, count, blockIdx.x), q = list[i];
+  xpass_body<T, PX, LW, WG, PAIR, XCD>(op_args_global(ops[q]), blockIdx.x - pre[i], blocks[5 * q + 2]);
+}
+
 
 // ---------------------------------------------------------------- pass 3, registers
 // The x pass for P = 256 in FP32 as a two-level 16 x 16 transform held in
@@ -612,7 +637,7 @@ void k_xpass_r16(OpArgs<float> a) {
 
 // ---------------------------------------------------------------- pass 5
 template <typename T, int PZ, int LW, int WG>
-__global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
+__device__ __forceinline__ void zinv_body(const OpArgs<T>& a, unsigned bx, unsigned nbx) {
   constexpr int E = LW * PZ, EPT = (E + WG - 1) / WG;
   __shared__ cx<T> lds[E];
   __shared__ cx<T> tw[PZ];
@@ -620,7 +645,7 @@ __global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
   const int tid = threadIdx.x;
   const Geom& g = a.g;
   const long lines = (long)g.nx * g.ny;
-  const long line0 = (long)blockIdx.x * LW;
+  const long line0 = (long)bx * LW;
   const cx<T>* twz = a.tw + g.Px + g.Py;
   for (int i = tid; i < PZ; i += WG) tw[i] = twz[i];
   const size_t nbox = g.nbox();
@@ -662,6 +687,19 @@ __global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
     __syncthreads();
   }
 }
+
+template <typename T, int PZ, int LW, int WG>
+__global__ __launch_bounds__(WG) void k_zinv(OpArgs<T> a) {
+  zinv_body<T, PZ, LW, WG>(a, blockIdx.x, gridDim.x);
+}
+template <typename T, int PZ, int LW, int WG>
+__global__ __launch_bounds__(WG) void k_zinv_mj(const OpArgs<T>* __restrict__ ops, const unsigned* __restrict__ blocks,
+                                                const int* __restrict__ list,
+                                                const unsigned* __restrict__ pre, int count) {
+  const int i = mj_job(pre, count, blockIdx.x), q = list[i];
+  zinv_body<T, PZ, LW, WG>(op_args_global(ops[q]), blockIdx.x - pre[i], blocks[5 * q + 4]);
+}
+
 
 // ---------------------------------------------------------------- launch
 // Launch shapes of a pass over lines of length P.  Variant 0 stages the
@@ -713,6 +751,74 @@ template <typename T, int P> struct PassLauncher {
         hipLaunchKernelGGL((k_xpass_r16<16, PAIR, XCD, 3>), dim3((xl + 15) / 16), dim3(256), 0, s, a);
       else
         hipLaunchKernelGGL((k_xpass_r16<LW, PAIR, XCD, 2>), dim3((xl + LW - 1) / LW), dim3(16 * LW), 0, s, a);
+    }
+  }
+  // Multi-job launches at the job's own tuned variant (the LDS kernels:
+  // shapes 0-3, and for the x pass pairing and the XCD order), for lines of
+  // at most kMjMaxP points; the register-resident kernels (P = 256) are not
+  // offered.  Same workgroup count per job as its single-job launch.
+  template <int V, bool PAIR = false, bool XCD = false> static unsigned blocks_v(int pass, const Geom& g) {
+    constexpr int LZ = tile_lw(tile_l0(E, P, kZCap), V), LY = tile_lw(tile_l0(E, P, kYCap), V);
+    constexpr int LX = tile_lw(tile_l0(2 * E, P, kXCap), V);
+    const long zl = (long)g.nx * g.ny, yl = 3L * g.nx * g.Pz, xl = (long)g.Py * g.Pz;
+    switch (pass) {
+      case 0: case 4: return (unsigned)((zl + LZ - 1) / LZ);
+      case 1: case 3: return (unsigned)((yl + LY - 1) / LY);
+      default: return (unsigned)((xl + LX - 1) / LX);
+    }
+  }
+  static unsigned blocks_mj(int pass, int v, const Geom& g) {
+    switch (v & 3) {
+      case 1: return blocks_v<1>(pass, g);
+      case 2: return blocks_v<2>(pass, g);
+      case 3: return blocks_v<3>(pass, g);
+      default: return blocks_v<0>(pass, g);
+    }
+  }
+  template <int V, bool PAIR, bool XCD>
+  static void run_mj(const MjClass& c, const OpArgs<T>* ops, const unsigned* blocks, const int* lists,
+                     const unsigned* pres, hipStream_t s) {
+    if constexpr (P <= kMjMaxP) {
+      constexpr int WG = tile_wg(V);
+      constexpr int LZ = tile_lw(tile_l0(E, P, kZCap), V), LY = tile_lw(tile_l0(E, P, kYCap), V);
+      constexpr int LX = tile_lw(tile_l0(2 * E, P, kXCap), V);
+      const dim3 grid(c.total_blocks);
+      const int* list = lists + c.offset;
+      const unsigned* pre = pres + c.pre;
+      const int n = c.count;
+      switch (c.pass) {
+        case 0: hipLaunchKernelGGL((k_zfwd_mj<T, P, LZ, WG>), grid, dim3(WG), 0, s, ops, blocks, list, pre, n); break;
+        case 1:
+          hipLaunchKernelGGL((k_ypass_mj<T, P, LY, WG, false>), grid, dim3(WG), 0, s, ops, blocks, list, pre, n);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_xpass_mj<T, P, LX, 2 * WG, PAIR, XCD>), grid, dim3(2 * WG), 0, s, ops, blocks, list,
+                             pre, n);
+          break;
+        case 3:
+          hipLaunchKernelGGL((k_ypass_mj<T, P, LY, WG, true>), grid, dim3(WG), 0, s, ops, blocks, list, pre, n);
+          break;
+        default: hipLaunchKernelGGL((k_zinv_mj<T, P, LZ, WG>), grid, dim3(WG), 0, s, ops, blocks, list, pre, n); break;
+      }
+    }
+  }
+  template <int V>
+  static void run_mj_x(const MjClass& c, const OpArgs<T>* ops, const unsigned* blocks, const int* lists,
+                       const unsigned* pres, hipStream_t s) {
+    switch (c.pass == 2 ? (c.variant >> 2) & 3 : 0) {
+      case 1: run_mj<V, true, false>(c, ops, blocks, lists, pres, s); return;
+      case 2: run_mj<V, false, true>(c, ops, blocks, lists, pres, s); return;
+      case 3: run_mj<V, true, true>(c, ops, blocks, lists, pres, s); return;
+      default: run_mj<V, false, false>(c, ops, blocks, lists, pres, s); return;
+    }
+  }
+  static void launch_mj(const MjClass& c, const OpArgs<T>* ops, const unsigned* blocks, const int* lists,
+                        const unsigned* pres, hipStream_t s) {
+    switch (c.variant & 3) {
+      case 1: run_mj_x<1>(c, ops, blocks, lists, pres, s); return;
+      case 2: run_mj_x<2>(c, ops, blocks, lists, pres, s); return;
+      case 3: run_mj_x<3>(c, ops, blocks, lists, pres, s); return;
+      default: run_mj_x<0>(c, ops, blocks, lists, pres, s); return;
     }
   }
   // x-pass variant bits: 0-1 the shape, 2 kz-mirror pairing, 3 the
@@ -779,23 +885,6 @@ template <typename T, typename F> static bool dispatch_p(int P, F&& f) {
   }
 }
 
-template <typename T>
-static OpArgs<T> op_args(const VolumeView<T>& v, const cx<T>* xin, cx<T>* yout, int precond, DevIndex di = {}) {
-  OpArgs<T> a;
-  a.di = di;
-  a.g = v.g;
-  a.xin = xin;
-  a.yout = yout;
-  a.A = v.A;
-  a.B = v.B;
-  a.spec = v.spec;
-  a.rspec = v.rspec;
-  a.eps = v.eps;
-  a.tw = v.tw;
-  for (int i = 0; i < 3; i++) a.self[i] = v.self[i];
-  a.precond = precond;
-  return a;
-}
 
 static int pass_extent(const Geom& g, int pass) { return pass == 2 ? g.Px : (pass == 1 || pass == 3) ? g.Py : g.Pz; }
 
@@ -833,6 +922,28 @@ template <typename T> bool pass_variant_ok(const Geom& g, int pass, int v) {
   if (paired && !(lx >= 2 && g.Pz >= lx && g.Pz % lx == 0)) return false;
   if (g.ground && !paired && lx % g.Pz != 0) return false;
   return true;
+}
+
+template <typename T> unsigned pass_blocks_mj(const Geom& g, int pass, int variant) {
+  unsigned nb = 0;
+  dispatch_p<T>(pass_extent(g, pass), [&](auto L) { nb = L.blocks_mj(pass, variant, g); });
+  return nb;
+}
+
+template <typename T> bool pass_mj_ok(const Geom& g, int pass, int variant) {
+  return pass_extent(g, pass) <= kMjMaxP && pass_variant_ok<T>(g, pass, variant) &&
+         variant < (pass == 2 ? 4 * kPassVariants : kPassVariants);
+}
+
+template <typename T>
+int volume_apply_mj(const OpArgs<T>* ops, const unsigned* blocks, const int* lists, const unsigned* pres,
+                    const MjClass* classes, int nclasses, hipStream_t s) {
+  bool ok = true;
+  // classes are in pass order, so every job's passes run in sequence
+  for (int i = 0; i < nclasses && ok; i++)
+    if (classes[i].count > 0 && classes[i].total_blocks > 0)
+      ok = dispatch_p<T>(classes[i].P, [&](auto L) { L.launch_mj(classes[i], ops, blocks, lists, pres, s); });
+  return ok && hipGetLastError() == hipSuccess;
 }
 
 template <typename T>

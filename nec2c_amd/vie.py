@@ -146,7 +146,8 @@ EXPORTS = (
     "vie_hip_voxels_info", "vie_hip_voxels_cells", "dielectric_fft_prepare_voxels", "dielectric_fft_reserve",
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
-    "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_fft_tune",
+    "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_hip_batch_group_stats",
+    "vie_fft_tune",
     "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
     "dielectric_fft_last_solve_escalated", "dielectric_fft_last_solve_recycled", "vie_hip_harmonic_ritz",
     "vie_hip_radiation_amplitudes", "vie_hip_scattered_power", "vie_hip_volume_conductor_rows",
@@ -246,6 +247,7 @@ def lib() -> ctypes.CDLL:
         "vie_hip_batch_result": (i, [vp, ctypes.c_longlong, vp, ctypes.POINTER(GmresReport),
                                      ctypes.POINTER(d)]),
         "vie_hip_batch_release": (i, [vp, ctypes.c_longlong]),
+        "vie_hip_batch_group_stats": (i, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]),
         "vie_fft_tune": (i, [vp, ctypes.POINTER(ctypes.c_int)]),
         "vie_hip_memory_info": (i, [szp, szp]),
         "dielectric_fft_time_arnoldi": (i, [vp, i, i, i] + [ctypes.POINTER(d)] * 4),
@@ -1114,6 +1116,13 @@ class Batch:
             lib().vie_hip_batch_release(self.b, jid)
             self._orders.pop(jid, None)
         return bool(ok), x, rep, sec.value
+
+    def group_stats(self):
+        """(multi-job steps issued, job steps they covered): one launch per
+        kernel served every member of a multi-job step."""
+        g, j = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        lib().vie_hip_batch_group_stats(self.b, ctypes.byref(g), ctypes.byref(j))
+        return g.value, j.value
 
     def close(self):
         if getattr(self, "b", None):

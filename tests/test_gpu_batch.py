@@ -4,7 +4,10 @@ right-hand sides each, FP64 and mixed precision, all in flight together.
 Each result must equal the same solve run alone through dielectric_fft_solve
 (same kernels, deterministic reductions: tolerance 1e-12 relative, and the
 iteration counts must match), and must satisfy the 1e-8 relative residual
-of the reference's own coupled operator (oracle/_ref) on the same system."""
+of the reference's own coupled operator (oracle/_ref) on the same system.
+The batch advances the matvec steps of its running cycles as multi-job steps
+(one launch per kernel for every member), so the same comparison covers the
+multi-job kernels against the single-job ones."""
 import numpy as np
 import pytest
 
@@ -133,6 +136,10 @@ def test_batch_at_configuration_size(ref):
         print(f"{len(jobs)} jobs of {len(cs)} models in flight: {batch.running}", flush=True)
         batch.wait_all()
         results = {jid: batch.result(jid) for *_, jid in jobs}
+        # the matvec steps ran as multi-job steps: one launch per kernel for many jobs
+        group_steps, job_steps = batch.group_stats()
+        print(f"multi-job steps {group_steps} covering {job_steps} job steps", flush=True)
+        assert group_steps > 0 and job_steps >= 4 * group_steps
     finally:
         batch.close()
     ops = {}
