@@ -931,16 +931,23 @@ def bench_mesh(args):
                          f"({B * 21} pairs) {t_cl:.2f} s",
                "clearance_s": t_cl}
         log(f"mesh CPU port: {cpu_rate:.3e} pairs/s on one core (full scan {pairs / cpu_rate:.0f} s)")
-    # VALU issue: the pair loop of k_box_overlap is 20 VALU instructions per
-    # pair in the AMD assembly of this build (9 v_max_f64, 3 v_min_f64,
-    # 3 v_cmp_nlt_f64, a move and the loop's 4 integer ops) beside 3 LDS reads;
-    # peak = 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s.
-    per_pair, peak = 20, 256 * 4 * 16 * 2.4e9
-    achieved = rate * per_pair
-    roof = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "VALU lane-instructions/s",
+    # LDS issue: every pair of k_box_overlap reads the right-hand box's six
+    # coordinates from LDS as 3 broadcast ds_read2st64_b64, beside 20
+    # VALU instructions (9 v_max_f64, 3 v_min_f64, 3 v_cmp_nlt_f64, a move and
+    # the loop's 4 integer ops).  A build with 14 VALU instructions per pair ran
+    # no faster, so the LDS reads bound the scan: one LDS instruction per CU per
+    # cycle, peak = 256 CUs x 2.4 GHz wave-level LDS instructions/s, each pair
+    # costing 3/64 of a wave-level read.  The VALU issue fraction is reported
+    # beside it.
+    lds_per_pair, peak = 3 / 64, 256 * 2.4e9
+    achieved = rate * lds_per_pair
+    valu_per_pair, valu_peak = 20, 256 * 4 * 16 * 2.4e9
+    roof = {"bound": "lds", "achieved": achieved, "peak": peak, "unit": "wave-level LDS read instructions/s",
             "frac": achieved / peak, "traffic": None, "kernel": "k_box_overlap (all box pairs)",
-            "instructions_per_pair": per_pair}
-    log(f"mesh roofline: {achieved:.3e} of {peak:.3e} VALU lane-instructions/s ({achieved / peak:.1%})")
+            "lds_reads_per_pair": 3, "valu_instructions_per_pair": valu_per_pair,
+            "valu_issue_frac": rate * valu_per_pair / valu_peak}
+    log(f"mesh roofline: {achieved:.3e} of {peak:.3e} LDS read instructions/s ({achieved / peak:.1%}); "
+        f"VALU issue {rate * valu_per_pair / valu_peak:.1%}")
     print(json.dumps({
         "metric": "box pairs/s of the mesher's interior-overlap scan (validate_box_relationships), per-voxel model",
         "value": rate, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,

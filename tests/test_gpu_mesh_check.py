@@ -34,8 +34,9 @@ def test_first_pair_matches_oracle(count, seed):
 
 @pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
 def test_non_finite_corners_take_the_exact_scan(bad):
-    """A non-finite corner sends the scan to the IEEE build, whose fmax / fmin
-    treat NaN as the reference's C library does (the other operand)."""
+    """A non-finite corner goes through the same single scan kernel, whose
+    fmax / fmin treat NaN as the reference's C library does (the other
+    operand), so the first overlapping pair is the reference's."""
     lo, hi = lattice(8, 1.0)  # touching only: no pair overlaps until a corner goes bad
     for corner, k, a in ((lo, 300, 0), (hi, 77, 1), (lo, 450, 2)):
         c = corner.copy()
