@@ -22,9 +22,11 @@ lines it follows):
 Parity status: pinned in tests/test_conductor_rows_oracle.py by numpy's
 Gauss-Legendre rule, finite differences of the scalar Green function for
 the E dyadic (k^2 + grad grad) g and for grad g x e_q, the point-source limit
-of a small cell far away, image theory over a PEC plane, and a loop that
-follows the row assembly statement by statement; the C restatement entry for
-entry against this module.
+of a small cell far away, image theory over a PEC plane, the wire rows
+against the mixed-potential field (vector potential plus face charges) of the
+cell current, the patch rows against NEC's patch equations (matrix.c cmss)
+formed from the Biot-Savart field; the C restatement entry for entry against
+this module.
 """
 from __future__ import annotations
 

@@ -91,27 +91,110 @@ def test_pec_is_image_theory(magnetic):
         assert np.allclose(got @ J, expect, rtol=1e-13, atol=0)
 
 
-def test_rows_follow_the_assembly():
+MU_0 = 1.0 / (R.EPSILON_0 * R.C0 ** 2)  # consistent with k0 = omega / c0
+
+
+def _gl(n, a, b):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (b - a) * x + 0.5 * (b + a), 0.5 * (b - a) * w
+
+
+def _mixed_potential_field(obs, center, size, q, k0, n=14):
+    """E at obs of the uniform current density e_q (A/m^2) in a cuboid, in the
+    mixed-potential form E = -i w A - grad(phi) under exp(+i w t): the vector
+    potential A = mu0 int g J dV over the volume, and the scalar potential of
+    the surface charge sigma = (J . n) / (i w) on the two faces normal to q
+    (the current has no divergence inside).  Independent of the (k^2 + grad
+    grad) dyadic and of the 1/(i w eps0) the row assembly applies to it."""
+    omega = k0 * R.C0
+    axes = [_gl(n, center[a] - size[a] / 2, center[a] + size[a] / 2) for a in range(3)]
+    X, Y, Z = np.meshgrid(axes[0][0], axes[1][0], axes[2][0], indexing="ij")
+    W = np.einsum("i,j,k->ijk", axes[0][1], axes[1][1], axes[2][1])
+    Rv = np.stack([obs[0] - X, obs[1] - Y, obs[2] - Z], -1)
+    r = np.linalg.norm(Rv, axis=-1)
+    A = MU_0 * np.sum(W * np.exp(-1j * k0 * r) / (4 * math.pi * r))
+    E = np.zeros(3, complex)
+    E[q] = -1j * omega * A
+    others = [a for a in range(3) if a != q]
+    (u, wu), (v, wv) = axes[others[0]][0:2], axes[others[1]][0:2]
+    U, V = np.meshgrid(u, v, indexing="ij")
+    WF = np.outer(wu, wv)
+    for side in (+1, -1):
+        pts = np.zeros(U.shape + (3,))
+        pts[..., q] = center[q] + side * size[q] / 2
+        pts[..., others[0]], pts[..., others[1]] = U, V
+        Rf = obs - pts
+        rf = np.linalg.norm(Rf, axis=-1)
+        # grad_obs of exp(-ikR)/(4 pi R)
+        dg = (-(1 + 1j * k0 * rf) * np.exp(-1j * k0 * rf) / (4 * math.pi * rf ** 3))[..., None] * Rf
+        sigma = side / (1j * omega)
+        E -= (sigma / R.EPSILON_0) * np.sum(WF[..., None] * dg, axis=(0, 1))
+    return E
+
+
+def _biot_savart_field(obs, center, size, q, k0, n=14):
+    """H at obs of the uniform current density e_q in a cuboid:
+    int (1 + ikR) exp(-ikR) / (4 pi R^2) (J x R_hat) dV, R = obs - r'."""
+    axes = [_gl(n, center[a] - size[a] / 2, center[a] + size[a] / 2) for a in range(3)]
+    X, Y, Z = np.meshgrid(axes[0][0], axes[1][0], axes[2][0], indexing="ij")
+    W = np.einsum("i,j,k->ijk", axes[0][1], axes[1][1], axes[2][1])
+    Rv = np.stack([obs[0] - X, obs[1] - Y, obs[2] - Z], -1)
+    r = np.linalg.norm(Rv, axis=-1)
+    f = W * (1 + 1j * k0 * r) * np.exp(-1j * k0 * r) / (4 * math.pi * r ** 3)
+    J = np.eye(3)[q]
+    return np.sum(f[..., None] * np.cross(J, Rv), axis=(0, 1, 2))
+
+
+def test_wire_rows_are_the_tested_field_of_the_cell_current():
+    """A wire row entry is t . E(r_w) in V/m of the cell's uniform current
+    density e_q (the volume unknown is a current density: the volume rows
+    scale the wire field by -i w eps0 chi).  E comes from the mixed
+    potentials above, so a wrong 1/(i w eps0) (sign, eps0 vs mu0, omega
+    power) or a dropped charge term fails."""
     rng = np.random.default_rng(3)
     k0 = 2 * math.pi * 300e6 / R.C0
-    centers = rng.uniform(0.1, 0.3, (4, 3))
-    sizes = rng.uniform(0.01, 0.02, (4, 3))
-    wp = rng.uniform(-0.2, 0.0, (2, 3))
+    centers = rng.uniform(0.1, 0.3, (3, 3))
+    sizes = rng.uniform(0.01, 0.03, (3, 3))
+    wp = rng.uniform(-0.25, -0.05, (2, 3))
     wt = rng.standard_normal((2, 3))
     wt /= np.linalg.norm(wt, axis=1, keepdims=True)
-    pp = rng.uniform(-0.2, 0.0, (1, 3))
-    t1, t2 = np.eye(3)[0:1], np.eye(3)[1:2]
-    rows = R.conductor_rows(k0, centers, sizes, wp, wt, pp, t1, t2, [0.7], ground=0)
-    omega = k0 * R.C0
+    rows = R.conductor_rows(k0, centers, sizes, wp, wt, ground=0)
     for w in range(2):
-        for cc in range(4):
-            block = R.point_cuboid_n(wp[w], centers[cc], sizes[cc], k0)
+        for cc in range(3):
             for q in range(3):
-                tested = sum(wt[w][p] * block[p][q] for p in range(3))
-                assert abs(rows[w, 3 * cc + q] - tested / (1j * omega * R.EPSILON_0)) <= 1e-14 * abs(rows[w]).max()
-    block = R.point_cuboid_h(pp[0], centers[1], sizes[1], k0)
-    assert np.allclose(rows[2, 3:6], -0.7 * (t2[0] @ block))
-    assert np.allclose(rows[3, 3:6], -0.7 * (t1[0] @ block))
+                want = wt[w] @ _mixed_potential_field(wp[w], centers[cc], sizes[cc], q, k0)
+                got = rows[w, 3 * cc + q]
+                assert abs(got - want) <= 1e-8 * abs(rows[w]).max(), (w, cc, q, got, want)
+
+
+def test_patch_rows_follow_nec_patch_equations():
+    """NEC's patch equations (matrix.c:426-434, cmss) are psalp t1 . (n x H)
+    for the first and -psalp t2 . (n x H) for the second (n = t1 x t2: the
+    self terms -1/2 and +1/2 carry those signs).  H here is the Biot-Savart
+    field of the cell's uniform current, and n x H is formed explicitly, so a
+    swapped row order, a sign or a wrong H fails."""
+    rng = np.random.default_rng(5)
+    k0 = 2 * math.pi * 250e6 / R.C0
+    centers = rng.uniform(0.1, 0.3, (2, 3))
+    sizes = rng.uniform(0.01, 0.03, (2, 3))
+    pp = rng.uniform(-0.25, -0.05, (2, 3))
+    t1 = rng.standard_normal((2, 3))
+    t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+    t2 = np.cross(rng.standard_normal((2, 3)), t1)
+    t2 /= np.linalg.norm(t2, axis=1, keepdims=True)
+    scale = np.array([0.7, 1.3])
+    rows = R.conductor_rows(k0, centers, sizes, patch_points=pp, patch_t1=t1, patch_t2=t2, patch_scale=scale,
+                            ground=0)
+    for p in range(2):
+        n = np.cross(t1[p], t2[p])
+        for cc in range(2):
+            for q in range(3):
+                H = _biot_savart_field(pp[p], centers[cc], sizes[cc], q, k0)
+                nxh = np.cross(n, H)
+                first, second = scale[p] * (t1[p] @ nxh), -scale[p] * (t2[p] @ nxh)
+                tol = 1e-8 * abs(rows[2 * p:2 * p + 2]).max()
+                assert abs(rows[2 * p, 3 * cc + q] - first) <= tol
+                assert abs(rows[2 * p + 1, 3 * cc + q] - second) <= tol
 
 
 @pytest.mark.parametrize("ground", [0, 1])
