@@ -574,34 +574,36 @@ void k_xpass_r16(OpArgs<float> a) {
   const T sy = ky <= g.Py / 2 ? (T)1 : (T)-1, sz = kz <= g.Pz / 2 ? (T)1 : (T)-1;
 #pragma unroll
   for (int k2 = 0; k2 < 16; k2++) {
+    const int kx = c + 16 * k2;
+    const int ox = kx <= PX / 2 ? kx : PX - kx;
+    const T sx = kx <= PX / 2 ? (T)1 : (T)-1;
+    const unsigned o = ((unsigned)ox * Hy + oy) * Hz + oz;
+    // every load of the step is issued before any of its arithmetic, so the
+    // step waits for memory once (the reflected loads of a ground are not
+    // held back behind the octant arithmetic)
+    cx<T> t[6], rr[6] = {};
+#pragma unroll
+    for (int q = 0; q < 6; q++) t[q] = a.spec[q * oct + o];
     cx<T> m0 = {(T)0, (T)0}, m1 = m0, m2 = m0;
     if (g.ground) {  // uniform branch: every lane takes part in the shuffles
+      const unsigned r = ((unsigned)ox * Hy + oy) * g.Pz + kz;
+#pragma unroll
+      for (int q = 0; q < 6; q++) rr[q] = a.rspec[q * rstride + r];
       m0 = cx<T>{__shfl(hold[0][k2].re, mlane), __shfl(hold[0][k2].im, mlane)};
       m1 = cx<T>{__shfl(hold[1][k2].re, mlane), __shfl(hold[1][k2].im, mlane)};
       m2 = cx<T>{__shfl(hold[2][k2].re, mlane), __shfl(hold[2][k2].im, mlane)};
     }
     {
-      const int kx = c + 16 * k2;
-      const int ox = kx <= PX / 2 ? kx : PX - kx;
-      const T sx = kx <= PX / 2 ? (T)1 : (T)-1;
-      const unsigned o = ((unsigned)ox * Hy + oy) * Hz + oz;
-      cx<T> txx = a.spec[o], tyy = a.spec[oct + o], tzz = a.spec[2 * oct + o];
-      cx<T> txy = (sx * sy) * a.spec[3 * oct + o];
-      cx<T> txz = (sx * sz) * a.spec[4 * oct + o];
-      cx<T> tyz = (sy * sz) * a.spec[5 * oct + o];
+      cx<T> txy = (sx * sy) * t[3], txz = (sx * sz) * t[4], tyz = (sy * sz) * t[5];
       cx<T> s0 = hold[0][k2], s1 = hold[1][k2], s2 = hold[2][k2];
-      cx<T> d0 = txx * s0 + txy * s1 + txz * s2;
-      cx<T> d1 = txy * s0 + tyy * s1 + tyz * s2;
-      cx<T> d2 = txz * s0 + tyz * s1 + tzz * s2;
+      cx<T> d0 = t[0] * s0 + txy * s1 + txz * s2;
+      cx<T> d1 = txy * s0 + t[1] * s1 + tyz * s2;
+      cx<T> d2 = txz * s0 + tyz * s1 + t[2] * s2;
       if (g.ground) {
-        const unsigned r = ((unsigned)ox * Hy + oy) * g.Pz + kz;
-        cx<T> rxx = a.rspec[r], ryy = a.rspec[rstride + r], rzz = a.rspec[2 * rstride + r];
-        cx<T> rxy = (sx * sy) * a.rspec[3 * rstride + r];
-        cx<T> rxz = sx * a.rspec[4 * rstride + r];
-        cx<T> ryz = sy * a.rspec[5 * rstride + r];
-        d0 = d0 + rxx * m0 + rxy * m1 + rxz * m2;
-        d1 = d1 + rxy * m0 + ryy * m1 + ryz * m2;
-        d2 = d2 - rxz * m0 - ryz * m1 + rzz * m2;
+        cx<T> rxy = (sx * sy) * rr[3], rxz = sx * rr[4], ryz = sy * rr[5];
+        d0 = d0 + rr[0] * m0 + rxy * m1 + rxz * m2;
+        d1 = d1 + rxy * m0 + rr[1] * m1 + ryz * m2;
+        d2 = d2 - rxz * m0 - ryz * m1 + rr[2] * m2;
       }
       hold[0][k2] = d0;
       hold[1][k2] = d1;
