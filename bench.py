@@ -231,26 +231,37 @@ def bench_batch(args):
         s.set_blocks(C=C, B=B, D=D)
         s.reserve(restart=1, maximum_iterations=1, relative_tolerance=1e-8)  # blocks resident, passes tuned
         systems.append(s)
-        for port in range(len(case.ports)):
-            b = np.zeros(s.order, dtype=np.complex128)
-            b[: case.dipole.segments] = case.rhs(port)
-            jobs.append((s, b))
-            feeds.append(case.ports[port])
+    # port-major order: the first port of every model (its seed), then the
+    # second ports, ...: with shared subspaces the later ports of a model
+    # start after its seed has published its recycled subspace
+    for port in range(max(len(c.ports) for c in cases)):
+        for case, s in zip(cases, systems):
+            if port < len(case.ports):
+                b = np.zeros(s.order, dtype=np.complex128)
+                b[: case.dipole.segments] = case.rhs(port)
+                jobs.append((s, b))
+                feeds.append(case.ports[port])
     t_setup = time.perf_counter() - t1
     del inputs
     log(f"batch: host inputs {t_host:.1f} s, device setup (spectra, blocks, tuning) {t_setup:.1f} s")
     biggest = max(systems, key=lambda q: q.order)
-    out, group_steps = {}, {}
-    for prec, name in ((vie.MIXED, "mixed"), (vie.FP64, "fp64")):
+    out, group_steps, shared = {}, {}, {}
+    k_fp64 = args.recycle if args.recycle > 0 else 20
+    runs = (("mixed", vie.MIXED, True, args.recycle),          # the batched form: shared recycled subspaces
+            ("fp64", vie.FP64, False, 0),                       # the reference's GMRES, one job per port
+            ("fp64_shared", vie.FP64, True, k_fp64),            # FP64 with the same shared subspaces
+            ("mixed_unshared", vie.MIXED, False, args.recycle))  # mixed, each job on its own
+    for name, prec, share, rk in runs:
         b = vie.Batch()
         b.reserve(args.inflight, biggest, **opts)
         res, secs = serve.run_batch(jobs, precision=prec, inflight=args.inflight, batch=b,
-                                    inner_reduction=args.inner_reduction, recycle=args.recycle, **opts)
+                                    inner_reduction=args.inner_reduction, recycle=rk, share=share, **opts)
         group_steps[name] = b.group_stats()
+        shared[name] = b.shared_stats()
         b.close()
         out[name] = (res, secs)
-        log(f"batch {name}: {len(jobs)} solves in {secs:.2f} s "
-            f"({sum(r[0] for r in res)} converged)")
+        log(f"batch {name}: {len(jobs)} solves in {secs:.2f} s ({sum(r[0] for r in res)} converged, "
+            f"{sum(r[2].iterations for r in res)} Arnoldi steps, subspaces published/adopted {shared[name]})")
     rm, tm = out["mixed"]
     rd, td = out["fp64"]
     z_err = max(abs(1 / xm[1][f] - 1 / xd[1][f]) / abs(1 / xd[1][f]) for xm, xd, f in zip(rm, rd, feeds))
@@ -272,12 +283,21 @@ def bench_batch(args):
         "config": {
             "workload": f"batched: {len(cases)} models, grids 8-48 per axis (cubes/cuboids), eps_r U[2,12], "
                         f"50-500 MHz, k0 sqrt(eps_r) h = {args.batch_kh:g}, dipoles of 11-31 segments, 1-8 ports each; "
-                        f"one GMRES({args.restart}) job per port to 1e-8, {args.inflight} jobs in flight",
+                        f"one GMRES({args.restart}) job per port to 1e-8, {args.inflight} jobs in flight, ports "
+                        "submitted port-major with the recycled subspace of each model's first port shared by its "
+                        "other ports (mixed); fp64 = the reference's GMRES per port",
             "models": len(cases), "right_hand_sides": len(jobs),
             "cells_total": int(sum(np.prod(c.shape) for c in cases)),
         },
         "fp64_solves_per_s": len(jobs) / td,
         "speedup_mixed_vs_fp64": td / tm,
+        "fp64_shared_solves_per_s": len(jobs) / out["fp64_shared"][1],
+        "speedup_mixed_vs_fp64_shared": out["fp64_shared"][1] / tm,
+        "mixed_unshared_solves_per_s": len(jobs) / out["mixed_unshared"][1],
+        "iterations_total": {k: int(sum(r[2].iterations for r in v[0])) for k, v in out.items()},
+        "converged": {k: int(sum(r[0] for r in v[0])) for k, v in out.items()},
+        "residual_max": {k: max(r[2].relative_residual for r in v[0]) for k, v in out.items()},
+        "shared_subspaces_published_adopted": shared,
         "p50_ms_mixed": 1e3 * float(np.percentile(lat_m, 50)),
         "p99_ms_mixed": 1e3 * float(np.percentile(lat_m, 99)),
         "p50_ms_fp64": 1e3 * float(np.percentile(lat_d, 50)),

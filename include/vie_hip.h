@@ -459,6 +459,27 @@ int vie_hip_batch_result(vie_hip_batch *batch, long long id, VIE_CDOUBLE *x, vie
 /* Returns a finished job's workspace to the pool. */
 int vie_hip_batch_release(vie_hip_batch *batch, long long id);
 
+/* Recycled subspace shared among the right-hand sides of one system (new;
+   the reference solves one right-hand side per vie_gmres call,
+   dielectric_wire.c:1296-1304, and restarts from nothing).  With on != 0,
+   jobs submitted afterwards take part: the first of them on a system whose
+   first cycle ends (converged or not) and that solves with a recycled
+   subspace (vie_hip_solve_options.recycle, the MIXED default) publishes its
+   U and C (A M^-1 U = C, C orthonormal: they depend on the operator only);
+   a job of the same system, precision and recycled size that starts later,
+   on an unchanged operator, adopts them, so its first cycle already
+   minimises over span(U) + its own Krylov space (GCRO with a fixed subspace,
+   Parks et al., SIAM J. Sci. Comput. 28, 2006: recycling across a sequence
+   of right-hand sides).  Results differ from solo solves by the solver
+   tolerance, not bitwise; acceptance stays the true residual.  The system
+   holds the shared subspace (2 k basis vectors per precision) until it is
+   freed.  Submit one right-hand side per system first (a seed) and the rest
+   after it for the others to benefit. */
+int vie_hip_batch_share_subspaces(vie_hip_batch *batch, int on);
+/* Subspaces published and adopted by the batch's finished jobs. */
+int vie_hip_batch_shared_stats(const vie_hip_batch *batch, unsigned long long *published,
+                               unsigned long long *adopted);
+
 /* Multi-job step counters of a batch: group_steps = multi-job Arnoldi steps
  * issued (each one launch per kernel for all members), job_steps = the job
  * steps they covered.  No counterpart in the reference (it solves one

@@ -184,6 +184,7 @@ static bool add_ok(size_t a, size_t b, size_t* out) {
 // ------------------------------------------------------------------ plan
 struct vie_fft_plan {
   vie::Geom g{};
+  unsigned long long gen = 1;  // permittivity generation: bumped by each upload
   size_t cells = 0;
   size_t bytes = 0;
   double h = 0, k = 0;
@@ -287,6 +288,7 @@ static int plan_upload_eps(vie_fft_plan* p) {
   HIP_OK(hipMemcpyAsync(p->eps32.p, box32.data(), nbox * sizeof(c32), hipMemcpyHostToDevice, p->stream));
   HIP_OK(hipStreamSynchronize(p->stream));
   p->eps_dirty = false;
+  p->gen++;
   return 1;
 }
 
@@ -727,6 +729,12 @@ struct SolverWork {
   c64* rec_host = nullptr; // pinned: first cycle's Hraw, dc_a, dc_alpha, then the coefficients
   // Multi-job steps (a batch's step groups, by cycle precision: 0 FP64, 1 FP32)
   struct StepGroup* groups[2] = {nullptr, nullptr};
+  // Sharing of the recycled subspace among a batch's jobs of one system: the
+  // first job whose first cycle ends publishes its U and C; a job of the same
+  // system, operator generation, precision and k that starts later adopts
+  // them, so its first cycle is already a recycled one.
+  bool share = false, adopted = false, published = false;
+  unsigned long long op_gen = 0, op_eps_gen = 0;  // the operator this job solves
   bool grouped = false;         // the group issues and checks this cycle's matvec steps
   bool group_nonfinite = false; // a grouped step reported non-finite values
   DevBuf fftws;                 // own FFT workspace (group members run concurrently)
@@ -761,6 +769,19 @@ struct dielectric_fft_system {
   DevBuf C64, B64, D64, C32, B32, D32, bz, cdiag, stage;
   SolverWork sw;
   bool rhs_loaded = false;
+  // Operator generation: bumped whenever the device blocks change (uploads,
+  // rows or columns assembled on the device).
+  unsigned long long gen = 1;
+  // Recycled subspace published by a batch job for the other right-hand
+  // sides of this system, per cycle precision (0 FP64, 1 FP32): [U | C],
+  // 2 k columns of stride ldv, for the operator generations it was built on.
+  struct SharedSubspace {
+    DevBuf uc;
+    int k = 0;
+    size_t ldv = 0;
+    unsigned long long gen = 0, eps_gen = 0;
+    bool ok = false;
+  } shared[2];
   ~dielectric_fft_system() {
     free(blocks);
     if (owns_plan) delete plan;
@@ -1055,6 +1076,7 @@ static int upload_blocks(dielectric_fft_system* s) {
     HIP_OK(hipStreamSynchronize(st));
   }
   s->blocks_dirty = false;
+  s->gen++;
   return 1;
 }
 
@@ -1707,6 +1729,48 @@ template <typename T> static int group_progress(StepGroup& g) {
   return changed + 1;
 }
 
+// The shared subspace of w's system and precision, when it was built for the
+// operator w solves with w's k and basis stride.
+template <typename T> static dielectric_fft_system::SharedSubspace* shared_for(SolverWork& w) {
+  dielectric_fft_system::SharedSubspace& sh = w.sys->shared[sizeof(T) == sizeof(float)];
+  const bool match = sh.ok && sh.k <= w.rk && sh.ldv == w.ldv && sh.gen == w.op_gen && sh.eps_gen == w.op_eps_gen;
+  return match ? &sh : nullptr;
+}
+
+// Publishes w's freshly built U and C (basis columns 0 .. 2 rk - 1) for the
+// other jobs of its system, unless an equal subspace is already there.  The
+// copy is queued on the system stream, which every job of the system uses
+// for its cycle starts, so an adopting job reads it complete.
+template <typename T> static void shared_publish(SolverWork& w) {
+  if (shared_for<T>(w)) return;
+  dielectric_fft_system::SharedSubspace& sh = w.sys->shared[sizeof(T) == sizeof(float)];
+  sh.ok = false;
+  const size_t bytes = (size_t)2 * w.rk * w.ldv * sizeof(vie::cx<T>);
+  if (!sh.uc.alloc(bytes) ||
+      hipMemcpyAsync(sh.uc.p, w.V.p, bytes, hipMemcpyDeviceToDevice, w.sys->stream) != hipSuccess)
+    return;
+  sh.k = w.rk;
+  sh.ldv = w.ldv;
+  sh.gen = w.op_gen;
+  sh.eps_gen = w.op_eps_gen;
+  sh.ok = true;
+  w.published = true;
+}
+
+// At the start of w's first cycle: take the system's shared U and C, if any
+// fit, so that this cycle minimises over span(U) + its Krylov space already.
+// A published subspace may hold fewer vectors than w reserved (a short seed
+// cycle): w then recycles that many.
+template <typename T> static void shared_adopt(SolverWork& w) {
+  dielectric_fft_system::SharedSubspace* sh = shared_for<T>(w);
+  if (!sh) return;
+  const size_t bytes = (size_t)2 * sh->k * w.ldv * sizeof(vie::cx<T>);
+  if (hipMemcpyAsync(w.V.p, sh->uc.p, bytes, hipMemcpyDeviceToDevice, w.sys->stream) != hipSuccess) return;
+  w.rk = sh->k;
+  w.rec_ready = true;
+  w.adopted = true;
+}
+
 // Builds the recycled subspace from the finished first cycle (its basis from
 // column 2 rk, its unrotated Hessenberg, and the last step's a and alpha in
 // w.rec_host): P = the rk harmonic Ritz vectors of smallest modulus,
@@ -1720,8 +1784,10 @@ template <typename T> static int group_progress(StepGroup& g) {
 // or the small problems fail.
 template <typename T> static bool recycle_build(SolverWork& w) {
   namespace D = vie::dense;
-  const int J = w.rec_used, k = w.rk, m = w.m;
-  if (J < k + 1 || w.rec_breakdown) return false;
+  const int J = w.rec_used, m = w.m, k0 = w.rk;
+  // a job that shares its subspace takes what a short first cycle offers
+  const int k = (w.share && J < k0 + 1) ? J - 1 : k0;
+  if (k < 1 || J < k + 1 || w.rec_breakdown) return false;
   const double t0 = now_s();
   const c64* hr = w.rec_host;  // column-major, ld m + 1
   const c64* a = hr + (size_t)(m + 1) * m;
@@ -1765,14 +1831,16 @@ template <typename T> static bool recycle_build(SolverWork& w) {
   if (hipMemcpyAsync(w.rcoef.p, cc, nc * sizeof(c64), hipMemcpyHostToDevice, st) != hipSuccess) return false;
   vie::cx<T>* V = w.V.as<vie::cx<T>>();
   const size_t ldv = w.ldv;
-  const vie::cx<T>* first = V + (size_t)2 * k * ldv;
+  const vie::cx<T>* first = V + (size_t)2 * k0 * ldv;  // the first cycle's basis
   vie::block_combine<T>(V + (size_t)k * ldv, k, first, J + 1, ldv, w.rcoef.as<c64>(), w.n, st);
   vie::block_combine<T>(V, k, first, J, ldv, w.rcoef.as<c64>() + (size_t)(J + 1) * k, w.n, st);
   if (hipGetLastError() != hipSuccess) return false;
+  w.rk = k;
   w.rec_ready = true;
   if (trace_enabled())
     fprintf(stderr, "[vie_hip] recycled subspace of %d vectors from a %d-column cycle (host %.2f ms)\n", k, J,
             (now_s() - t0) * 1e3);
+  if (w.share) shared_publish<T>(w);
   return true;
 }
 
@@ -1835,6 +1903,9 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           if (w.beta > w.target && rep.iterations < w.o.maximum_iterations) rep.restarts++;
         }
         if (w.beta <= w.target) {
+          // a first cycle that converged still gives the other jobs of its
+          // system their recycled subspace
+          if (w.share && w.rk && !w.rec_ready && w.cycles == 1 && !shared_for<T>(w)) (void)recycle_build<T>(w);
           job_finish(w, VIE_ITERATIVE_CONVERGED, 1);
           return 1;
         }
@@ -1863,6 +1934,7 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           return job_advance<double>(w, blocking);
         }
         w.cycle_beta = w.beta;
+        if (w.share && w.rk && !w.rec_ready && w.cycles == 0) shared_adopt<T>(w);
         // new cycle; the second one first forms the recycled subspace
         if (w.rk && !w.rec_ready && w.cycles == 1 && !recycle_build<T>(w)) {
           if (trace_enabled()) fprintf(stderr, "[vie_hip] no recycled subspace: plain cycles\n");
@@ -2028,6 +2100,9 @@ static int job_prepare(dielectric_fft_system* s, SolverWork& w, const vie_gmres_
     rep->status = VIE_ITERATIVE_CALLBACK_FAILED;
     return 0;
   }
+  w.op_gen = s->gen;
+  w.op_eps_gen = s->plan ? s->plan->gen : 0;
+  w.share = w.adopted = w.published = false;
   return 1;
 }
 
@@ -3077,6 +3152,7 @@ extern "C" int dielectric_fft_assemble_conductor_rows(dielectric_fft_system* s, 
                    patches, patch_points, patch_t1, patch_t2, patch_scale, ground};
   // until the rows are in place the block is neither the host's nor the device's
   s->rows_on_device = true;
+  s->gen++;  // the device operator changes
   if (!rows_run(a, nullptr, 0, 0, nullptr, system_block_sink(s, b64, b32))) {
     s->rows_on_device = false;
     s->blocks_dirty = true;  // the next upload restores the host block
@@ -3103,6 +3179,7 @@ extern "C" int dielectric_fft_assemble_conductor_columns(
                       z,   si,   bi,       cab,     sab,      salp,         patches,    patch_points, patch_area,
                       patch_t1, patch_t2, s->c, sources, rkh, ground};
   s->cols_on_device = true;
+  s->gen++;  // the device operator changes
   if (!columns_run(a, nullptr, 0, nullptr, system_block_sink(s, b64, b32))) {
     s->cols_on_device = false;
     s->blocks_dirty = true;
@@ -3611,6 +3688,9 @@ struct vie_hip_batch {
   std::map<long long, BatchJob> jobs;
   std::vector<long long> order;  // submission order of unfinished jobs
   long long next_id = 1;
+  // recycled subspaces shared among the jobs of one system (off by default)
+  bool share = false;
+  unsigned long long shared_published = 0, shared_adopted = 0;
   // multi-job step groups (0: FP64 cycles, 1: FP32 cycles); declared last,
   // so they are destroyed (their streams drained) before the jobs
   std::unique_ptr<StepGroup> groups[2];
@@ -3728,6 +3808,7 @@ extern "C" long long vie_hip_batch_submit(vie_hip_batch* b, dielectric_fft_syste
     return 0;
   }
   for (int i = 0; i < 2; i++) w.groups[i] = b->groups[i].get();
+  w.share = b->share;
   // b: user layout -> pinned -> device -> system layout; x0 = 0
   memcpy(w.host_io, rhs, nu * 16);
   hipStream_t st = s->stream;
@@ -3752,10 +3833,12 @@ extern "C" long long vie_hip_batch_submit(vie_hip_batch* b, dielectric_fft_syste
 }
 
 // Advances one job; returns 1 when it changed state.
-static int batch_step(BatchJob& j, bool blocking) {
+static int batch_step(vie_hip_batch* b, BatchJob& j, bool blocking) {
   SolverWork& w = *j.w;
   if (j.state == JB_RUNNING) {
     if (!job_advance_any(w, blocking)) return 0;
+    b->shared_published += w.published;
+    b->shared_adopted += w.adopted;
     // solution: system layout -> user layout -> pinned
     dielectric_fft_system* s = w.sys;
     hipStream_t st = s->stream;
@@ -3800,11 +3883,25 @@ static int batch_sweep(vie_hip_batch* b) {
   for (size_t i = 0; i < b->order.size(); i++) {
     long long id = b->order[i];
     BatchJob& j = b->jobs[id];
-    while (j.state != JB_READY && batch_step(j, false)) changed++;
+    while (j.state != JB_READY && batch_step(b, j, false)) changed++;
     if (j.state != JB_READY) b->order[keep++] = id;
   }
   b->order.resize(keep);
   return changed + groups_progress(b);
+}
+
+extern "C" int vie_hip_batch_share_subspaces(vie_hip_batch* b, int on) {
+  if (!b) return 0;
+  b->share = on != 0;
+  return 1;
+}
+
+extern "C" int vie_hip_batch_shared_stats(const vie_hip_batch* b, unsigned long long* published,
+                                          unsigned long long* adopted) {
+  if (!b) return 0;
+  if (published) *published = b->shared_published;
+  if (adopted) *adopted = b->shared_adopted;
+  return 1;
 }
 
 extern "C" int vie_hip_batch_group_stats(const vie_hip_batch* b, unsigned long long* group_steps,

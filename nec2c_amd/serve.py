@@ -238,15 +238,21 @@ class Server:
         self.cache.clear()
 
 
-def run_batch(jobs, precision=vie.MIXED, inflight=128, inner_reduction=0.0, batch=None, recycle=0, **opts):
+def run_batch(jobs, precision=vie.MIXED, inflight=128, inner_reduction=0.0, batch=None, recycle=0, share=False,
+              **opts):
     """Solves a closed batch: `jobs` is a list of (system, rhs).  At most
     `inflight` jobs are on the device at once; a finished job's slot is refilled
     at once (continuous batching).  `batch` may be a caller's ``vie.Batch``
-    (e.g. with reserved workspaces).  Returns (results in job order, seconds);
-    each result is (converged, x, report, seconds in flight)."""
+    (e.g. with reserved workspaces).  With `share`, jobs of one system share
+    the recycled subspace of the first of them to finish a cycle
+    (vie_hip_batch_share_subspaces): put one right-hand side per system first.
+    Returns (results in job order, seconds); each result is (converged, x,
+    report, seconds in flight)."""
     own = batch is None
     if own:
         batch = vie.Batch()
+    if share:
+        batch.share_subspaces(True)
     try:
         results = [None] * len(jobs)
         ids = {}

@@ -147,6 +147,7 @@ EXPORTS = (
     "vie_hip_batch_create", "vie_hip_batch_destroy", "vie_hip_batch_job_bytes", "vie_hip_batch_reserve",
     "vie_hip_batch_submit", "vie_hip_batch_progress", "vie_hip_batch_running", "vie_hip_batch_wait",
     "vie_hip_batch_wait_all", "vie_hip_batch_result", "vie_hip_batch_release", "vie_hip_batch_group_stats",
+    "vie_hip_batch_share_subspaces", "vie_hip_batch_shared_stats",
     "vie_fft_tune",
     "vie_hip_memory_info", "dielectric_fft_time_arnoldi", "dielectric_fft_solve_stats",
     "dielectric_fft_last_solve_escalated", "dielectric_fft_last_solve_recycled", "vie_hip_harmonic_ritz",
@@ -248,6 +249,8 @@ def lib() -> ctypes.CDLL:
                                      ctypes.POINTER(d)]),
         "vie_hip_batch_release": (i, [vp, ctypes.c_longlong]),
         "vie_hip_batch_group_stats": (i, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]),
+        "vie_hip_batch_share_subspaces": (i, [vp, i]),
+        "vie_hip_batch_shared_stats": (i, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]),
         "vie_fft_tune": (i, [vp, ctypes.POINTER(ctypes.c_int)]),
         "vie_hip_memory_info": (i, [szp, szp]),
         "dielectric_fft_time_arnoldi": (i, [vp, i, i, i] + [ctypes.POINTER(d)] * 4),
@@ -1060,11 +1063,24 @@ class Batch:
     their Arnoldi steps and overlaps jobs of different systems on the GPU.
     """
 
-    def __init__(self):
+    def __init__(self, share_subspaces=False):
         self.b = lib().vie_hip_batch_create()
         if not self.b:
             raise RuntimeError(f"batch create failed: {last_error()}")
         self._orders = {}
+        if share_subspaces:
+            self.share_subspaces(True)
+
+    def share_subspaces(self, on=True):
+        """vie_hip_batch_share_subspaces: jobs of one system submitted from now
+        on share the recycled subspace of the first of them to finish a cycle."""
+        lib().vie_hip_batch_share_subspaces(self.b, 1 if on else 0)
+
+    def shared_stats(self):
+        """(subspaces published, subspaces adopted) by the finished jobs."""
+        p, a = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        lib().vie_hip_batch_shared_stats(self.b, ctypes.byref(p), ctypes.byref(a))
+        return p.value, a.value
 
     @staticmethod
     def job_bytes(system, precision=FP64, recycle=0, **opts):

@@ -159,3 +159,64 @@ def test_batch_at_configuration_size(ref):
         assert r <= 1.05e-8
     for s, _, _ in built:
         s.close()
+
+
+def test_shared_subspaces_across_ports(ref):
+    """vie_hip_batch_share_subspaces: the seed port of each model publishes the
+    recycled subspace of its first cycle, the model's other ports adopt it and
+    start with a recycled cycle.  Every job converges, meets 1e-8 on the
+    reference operator and agrees with its solo (unshared) solve within the
+    solver tolerance; the adopting jobs need fewer Arnoldi steps than alone.
+    After the operator changes, the old subspace is not adopted."""
+    cs = [c for c in W.batch_cases(256, seed=4, kh=0.1) if len(c.ports) >= 3 and np.prod(c.shape) <= 24 ** 3][:5]
+    opts = dict(restart=120, maximum_iterations=3000, relative_tolerance=1e-8)
+    built = [build(c) for c in cs]
+    try:
+        for prec, rk in ((vie.MIXED, 0), (vie.FP64, 20)):
+            batch = vie.Batch(share_subspaces=True)
+            try:
+                seeds = [batch.submit(s, full_rhs(case, s, 0), precision=prec, recycle=rk, **opts)
+                         for case, (s, _, _) in zip(cs, built)]
+                batch.wait_all()
+                seed_res = [batch.result(j) for j in seeds]
+                rest = [(ci, port, batch.submit(s, full_rhs(case, s, port), precision=prec, recycle=rk, **opts))
+                        for ci, (case, (s, _, _)) in enumerate(zip(cs, built)) for port in range(1, len(case.ports))]
+                batch.wait_all()
+                rest_res = {jid: batch.result(jid) for *_, jid in rest}
+                published, adopted = batch.shared_stats()
+            finally:
+                batch.close()
+            print(f"precision {prec}: published {published}, adopted {adopted} of {len(rest)}", flush=True)
+            assert published == len(cs) and adopted == len(rest)
+            assert all(r[0] for r in seed_res)
+            steps_shared = steps_solo = 0
+            for ci, port, jid in rest:
+                case, (s, blocks, eps) = cs[ci], built[ci]
+                ok, x, rep, _ = rest_res[jid]
+                b = full_rhs(case, s, port)
+                ok1, x1, rep1 = s.solve(b, precision=prec, recycle=rk, **opts)
+                fi = case.ports[port]
+                zerr = abs(1 / x[fi] - 1 / x1[fi]) / abs(1 / x1[fi])
+                r = ref_residual(ref, case, blocks, eps, b, x)
+                print(f"model {ci} {case.shape} port {port}: shared {rep.iterations} steps, alone {rep1.iterations}; "
+                      f"reference residual {r:.2e}, impedance vs alone {zerr:.1e}", flush=True)
+                assert ok and ok1 and r <= 1.05e-8 and zerr <= 1e-6
+                steps_shared += rep.iterations
+                steps_solo += rep1.iterations
+            assert steps_shared < 0.9 * steps_solo
+        # a changed operator (new permittivity) does not adopt the old subspace
+        case, (s, blocks, eps) = cs[0], built[0]
+        eps2 = np.full_like(eps, 0.8 * case.eps_r + 0j)
+        s.set_permittivities(eps2)
+        batch = vie.Batch(share_subspaces=True)
+        try:
+            jid = batch.submit(s, full_rhs(case, s, 1), precision=vie.MIXED, **opts)
+            batch.wait_all()
+            ok, x, rep, _ = batch.result(jid)
+            assert batch.shared_stats() == (1, 0)
+        finally:
+            batch.close()
+        assert ok and ref_residual(ref, case, blocks, eps2, full_rhs(case, s, 1), x) <= 1.05e-8
+    finally:
+        for s, _, _ in built:
+            s.close()
