@@ -387,6 +387,7 @@ def bench_serve(args):
     srv = serve.Server(budget, precision=vie.MIXED, restart=args.restart, maximum_iterations=args.max_iterations,
                        inner_reduction=args.inner_reduction, recycle=args.recycle)
     rep = srv.run(requests)
+    rep["shared_subspaces_published_adopted"] = srv.batch.shared_stats()
     srv.close()
     result = {
         "metric": METRIC,
@@ -404,7 +405,8 @@ def bench_serve(args):
         "config": {
             "workload": f"served: {args.requests} Poisson requests ({args.rate:g}/s, seed 5), grids 16^3-96^3 "
                         "log-uniform, random eps_r fields (as the sweep), 100-400 MHz, 1-4 ports, 30% key reuse, "
-                        f"mixed GMRES({args.restart}) to 1e-8, HBM budget {budget / 1e9:.0f} GB",
+                        f"mixed GMRES({args.restart}) to 1e-8, HBM budget {budget / 1e9:.0f} GB; each model's "
+                        "recycled subspace shared by its ports and later requests (a new model's first port first)",
             "distinct_models": len(keys),
         },
         "serve": rep,

@@ -13,6 +13,12 @@ one or more feed ports (one right-hand side each).  The server
   ``vie.Batch`` (the ``vie_hip_batch_*`` C-ABI), so the Arnoldi steps of all
   requests in flight are interleaved on the device while new requests keep
   arriving;
+* lets the ports of one model share a recycled Krylov subspace
+  (``vie_hip_batch_share_subspaces``): the first port solved on a model
+  publishes the subspace of its first cycle, later ports and later requests
+  for the same cached model start from it; a request whose model has no
+  subspace yet runs its first port alone and submits the others when it
+  finishes (``seed_first``);
 * records per-request latency (arrival -> last port finished) and service time
   (admission -> last port finished).
 
@@ -75,7 +81,8 @@ class Request:
 
 class Server:
     def __init__(self, budget_bytes: int, precision=vie.MIXED, restart=120, maximum_iterations=2000,
-                 relative_tolerance=1e-8, inner_reduction=0.0, recycle=0, clock=time.perf_counter):
+                 relative_tolerance=1e-8, inner_reduction=0.0, recycle=0, share=True, seed_first=True,
+                 clock=time.perf_counter):
         self.budget = int(budget_bytes)
         self.precision = precision
         self.inner_reduction = inner_reduction
@@ -83,8 +90,14 @@ class Server:
         self.opts = dict(restart=restart, maximum_iterations=maximum_iterations,
                          relative_tolerance=relative_tolerance)
         self.restart = restart
-        self.batch = vie.Batch()
-        self.cache: "collections.OrderedDict[Hashable, list]" = collections.OrderedDict()  # key -> [system, bytes, users]
+        # recycled vectors of a solve (the library's default: 20 for MIXED, none for FP64)
+        self.rk = recycle if recycle > 0 else (20 if precision == vie.MIXED and recycle == 0 else 0)
+        self.share = share and self.rk > 0
+        self.seed_first = seed_first and self.share
+        self.batch = vie.Batch(share_subspaces=self.share)
+        # key -> [system, bytes, users, seeded (a port has finished on it)]
+        self.cache: "collections.OrderedDict[Hashable, list]" = collections.OrderedDict()
+        self.deferred = {}       # rid -> ports waiting for the model's first port
         self.queue: "collections.deque[Request]" = collections.deque()
         self.inflight = {}       # job id -> (request, port)
         self.pending = {}        # rid -> ports still running
@@ -103,8 +116,17 @@ class Server:
     def _job_bytes(self, req):
         return job_device_bytes(req.shape, req.conductors, self.restart, self.precision)
 
+    def _system_bytes(self, req):
+        """A cached system, with the shared subspace it holds once a port has
+        published one (2 k FP32 basis vectors)."""
+        extra = 0
+        if self.share:
+            n = req.conductors + 3 * int(np.prod(req.shape))
+            extra = 2 * self.rk * ((n + 511) // 512 * 512) * 8
+        return system_device_bytes(req.shape, req.conductors) + extra
+
     def _footprint(self, req):
-        sys_bytes = 0 if req.key in self.cache else system_device_bytes(req.shape, req.conductors)
+        sys_bytes = 0 if req.key in self.cache else self._system_bytes(req)
         return sys_bytes + len(req.rhs) * self._job_bytes(req)
 
     def _evict_for(self, need, keep=None):
@@ -114,7 +136,7 @@ class Server:
         for key in list(self.cache):
             if self.used + need <= self.budget:
                 return True
-            s, nbytes, users = self.cache[key]
+            s, nbytes, users, _ = self.cache[key]
             if users == 0 and key != keep:
                 s.close()
                 del self.cache[key]
@@ -129,7 +151,7 @@ class Server:
             s = req.make_system()
             self.build_seconds += self.clock() - t
             self.builds += 1
-            entry = [s, system_device_bytes(req.shape, req.conductors), 0]
+            entry = [s, self._system_bytes(req), 0, False]
             self.cache[req.key] = entry
             self.used += entry[1]
         else:
@@ -139,18 +161,31 @@ class Server:
         req.admitted = now
         req.results = [None] * len(req.rhs)
         self.pending[req.rid] = len(req.rhs)
-        for port, b in enumerate(req.rhs):
-            jid = self.batch.submit(entry[0], b, precision=self.precision, inner_reduction=self.inner_reduction,
-                                    recycle=self.recycle,
-                                    **self.opts)
-            self.inflight[jid] = (req, port)
+        ports = list(range(len(req.rhs)))
+        if self.seed_first and len(ports) > 1 and not entry[3]:
+            # the model's first port publishes its recycled subspace; the
+            # others start from it (their memory is reserved now)
+            self.deferred[req.rid] = ports[1:]
+            self.used += len(ports[1:]) * self._job_bytes(req)
+            ports = ports[:1]
+        for port in ports:
+            self._submit(req, entry, port)
             self.used += self._job_bytes(req)
+
+    def _submit(self, req, entry, port):
+        jid = self.batch.submit(entry[0], req.rhs[port], precision=self.precision,
+                                inner_reduction=self.inner_reduction, recycle=self.recycle, **self.opts)
+        self.inflight[jid] = (req, port)
 
     def _collect(self, jid):
         req, port = self.inflight.pop(jid)
         ok, x, rep, sec = self.batch.result(jid)
         req.results[port] = (ok, x, rep, sec)
         self.used -= self._job_bytes(req)
+        entry = self.cache[req.key]
+        entry[3] = True
+        for q in self.deferred.pop(req.rid, ()):
+            self._submit(req, entry, q)
         self.pending[req.rid] -= 1
         if self.pending[req.rid] == 0:
             del self.pending[req.rid]
@@ -233,7 +268,7 @@ class Server:
 
     def close(self):
         self.batch.close()
-        for s, _, _ in self.cache.values():
+        for s, *_ in self.cache.values():
             s.close()
         self.cache.clear()
 

@@ -146,3 +146,51 @@ def test_returning_key_that_is_lru_is_not_evicted_for_itself():
     assert srv.done[-1].cache_hit
     assert max(used_max) <= srv.budget
     assert list(srv.cache) == ["K1"]
+
+
+class RecordingBatch(FakeBatch):
+    """FakeBatch that records submissions and collected results in order."""
+
+    def __init__(self, delay=3):
+        super().__init__(delay)
+        self.log = []
+
+    def submit(self, system, b, **kw):
+        jid = super().submit(system, b, **kw)
+        self.log.append(("submit", system.key, jid))
+        return jid
+
+    def result(self, jid):
+        self.log.append(("result", None, jid))
+        return super().result(jid)
+
+
+def test_seed_port_runs_first_on_a_new_model():
+    """On a model without a published subspace, a request's first port runs
+    alone and the others are submitted when it finishes; a later request for
+    the cached (seeded) model submits all its ports at once."""
+    built = []
+    srv = make_server(budget=10 ** 12)
+    srv.batch = RecordingBatch(delay=3)
+    reqs = [request(0, "A", 16, 3, built), request(1, "A", 16, 2, built, arrival=0.005)]
+    rep = srv.run(reqs)
+    assert rep["requests"] == 2 and rep["solves"] == 5 and built == ["A"]
+    log = srv.batch.log
+    # the seed (job 1) alone, its result, then ports 2 and 3 of the first request
+    assert log[:4] == [("submit", "A", 1), ("result", None, 1), ("submit", "A", 2), ("submit", "A", 3)]
+    # the second request's two ports go in together
+    second = [e for e in log if e[0] == "submit" and e[2] >= 4]
+    first_result_after = min(i for i, e in enumerate(log) if e[0] == "result" and e[2] >= 4)
+    assert len(second) == 2 and log.index(second[1]) < first_result_after
+    assert srv.used == sum(e[1] for e in srv.cache.values())   # only the cached model remains
+
+
+def test_held_ports_count_against_the_budget_at_admission():
+    built = []
+    srv = make_server(budget=10 ** 12)
+    srv.batch = RecordingBatch(delay=50)
+    srv.t0 = srv.clock()
+    req = request(0, "A", 16, 3, built)
+    srv._admit(req)
+    assert len(srv.inflight) == 1
+    assert srv.used == srv._system_bytes(req) + 3 * srv._job_bytes(req)
