@@ -202,6 +202,10 @@ struct vie_fft_plan {
   DevBuf io_user, io_sys_in, io_sys_out;  // host-pointer apply staging
   dielectric_fft_system* wrap = nullptr;  // c = 0 system around this plan, for solves
   unsigned char var[2][5] = {};          // pass launch variants: [0] FP64, [1] FP32
+  // the variants of multi-job launches: the separate passes (many jobs per
+  // launch hide the small passes' latency, so the fused ones, tuned on one
+  // job, measured slower there)
+  unsigned char var_mj[2][5] = {};
   bool tuned[2] = {false, false};
   ~vie_fft_plan();
   size_t nbox() const { return g.nbox(); }
@@ -300,13 +304,13 @@ static int plan_upload_eps(vie_fft_plan* p) {
 // keeps variant 0 everywhere.
 namespace {
 struct TuneKey {
-  int prec, nx, ny, nz, Px, Py, Pz, pec;
+  int prec, nx, ny, nz, Px, Py, Pz, pec, fuse;
   bool operator<(const TuneKey& o) const {
     return memcmp(this, &o, sizeof(TuneKey)) < 0;
   }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::array<unsigned char, 5>> g_tune_cache;
+std::map<TuneKey, std::array<unsigned char, 10>> g_tune_cache;  // var, then var_mj
 }  // namespace
 
 static bool tuning_enabled() {
@@ -323,12 +327,15 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
   }
   TuneKey key;
   memset(&key, 0, sizeof(key));
-  key = TuneKey{slot, p->g.nx, p->g.ny, p->g.nz, p->g.Px, p->g.Py, p->g.Pz, p->g.pec + 2 * p->g.ground};
+  const char* fz = getenv("VIE_HIP_FUSE");
+  const bool fuse = !(fz && !strcmp(fz, "0"));
+  key = TuneKey{slot, p->g.nx, p->g.ny, p->g.nz, p->g.Px, p->g.Py, p->g.Pz, p->g.pec + 2 * p->g.ground, fuse};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
       memcpy(p->var[slot], it->second.data(), 5);
+      memcpy(p->var_mj[slot], it->second.data() + 5, 5);
       p->tuned[slot] = true;
       return 1;
     }
@@ -373,6 +380,36 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
     }
     p->var[slot][pass] = (unsigned char)best_v;
   }
+  memcpy(p->var_mj[slot], p->var[slot], 5);
+  // the fused z+y passes against the chosen pass pairs (0 + 1 forward,
+  // 3 + 4 inverse): the fused one must win by 2% as well (VIE_HIP_FUSE=0
+  // keeps the separate passes)
+  if (ok && vie::fused_ok(p->g) && fuse) {
+    const int code = vie::fused_code(p->g);
+    for (int dir = 0; dir < 2 && ok; dir++) {
+      const int pa = dir ? 3 : 0, pb = dir ? 4 : 1, fused = dir ? 4 : 0;
+      float ms[2] = {0, 0};
+      for (int alt = 0; alt < 2 && ok; alt++) {
+        auto run = [&]() {
+          if (alt == 0) {
+            ok = ok && vie::pass_launch<T>(v, pa, p->var[slot][pa], xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st);
+            ok = ok && vie::pass_launch<T>(v, pb, p->var[slot][pb], xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st);
+          } else {
+            ok = ok && vie::pass_launch<T>(v, fused, code, xin.as<vie::cx<T>>(), yout.as<vie::cx<T>>(), 1, st);
+          }
+        };
+        for (int r = 0; r < 2; r++) run();
+        ok = ok && hipEventRecord(e0, st) == hipSuccess;
+        for (int r = 0; r < reps; r++) run();
+        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        ok = ok && hipEventElapsedTime(&ms[alt], e0, e1) == hipSuccess;
+      }
+      if (ok && ms[1] < 0.98f * ms[0]) {
+        p->var[slot][fused] = (unsigned char)code;
+        p->var[slot][dir ? 3 : 1] = (unsigned char)vie::kFuseSkip;
+      }
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (!ok) {
@@ -380,8 +417,9 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
     return 0;
   }
   p->tuned[slot] = true;
-  std::array<unsigned char, 5> a;
+  std::array<unsigned char, 10> a;
   memcpy(a.data(), p->var[slot], 5);
+  memcpy(a.data() + 5, p->var_mj[slot], 5);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune_cache[key] = a;
   return 1;
@@ -437,9 +475,19 @@ extern "C" int vie_fft_set_variants(vie_fft_plan* p, const int* variants) {
       return 0;
     }
   }
+  for (int k = 0; k < 2; k++) {
+    unsigned char v[5];
+    for (int i = 0; i < 5; i++) v[i] = (unsigned char)variants[5 * k + i];
+    if (!vie::variants_consistent(v)) {
+      set_error("vie_fft_set_variants: a fused z+y pass needs its y pass skipped, and only then");
+      return 0;
+    }
+  }
   for (int i = 0; i < 5; i++) {
     p->var[0][i] = (unsigned char)variants[i];
     p->var[1][i] = (unsigned char)variants[5 + i];
+    for (int k = 0; k < 2; k++)  // multi-job launches run forced fused passes as the separate variant 0
+      p->var_mj[k][i] = p->var[k][i] >= vie::kFuseBase ? 0 : p->var[k][i];
   }
   p->tuned[0] = p->tuned[1] = true;
   return 1;
@@ -721,6 +769,7 @@ struct SolverWork {
   // 2 rk on, U and C are then formed from it in place, and every later cycle
   // sees C as its columns 0..rk-1 (basis pointer at column rk).
   int rk = 0;              // recycled vectors of this solve (0: plain restarted GMRES)
+  int rk_req = 0;          // the recycled vectors the solve was set up with
   int cyc_base = 0;        // first Krylov column of the current cycle
   int cycles = 0;          // cycles started in this solve
   int rec_used = 0;        // columns of the first cycle
@@ -1288,6 +1337,7 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   const char* ortho = getenv("VIE_HIP_ORTHO");
   w.dcgs2 = !(ortho && !strcmp(ortho, "cgs2"));
   if (!w.dcgs2) w.rk = 0;  // the recycled cycles are built on the delayed-CGS2 step
+  w.rk_req = w.rk;
   // with a recycled subspace a restart loses little, so FP32 cycles may end earlier
   w.reduction = so->inner_reduction > 0 ? so->inner_reduction : (w.rk ? 1e-4 : 1e-5);
   w.cyc_base = w.cycles = w.rec_used = 0;
@@ -1520,7 +1570,7 @@ template <typename T> static bool mj_eligible(const SolverWork& w) {
     return false;
   const int slot = sizeof(T) == sizeof(double) ? 0 : 1;
   for (int p = 0; p < 5; p++)
-    if (!vie::pass_mj_ok<T>(s->plan->g, p, s->plan->var[slot][p])) return false;
+    if (!vie::pass_mj_ok<T>(s->plan->g, p, s->plan->var_mj[slot][p])) return false;
   return true;
 }
 
@@ -1587,7 +1637,8 @@ template <typename T> static int group_upload(StepGroup& g) {
     vie::cx<T>* V = cycle_basis<T>(w);
     const size_t c = s->c;
     ops[q] = vie::op_args<T>(vv, V + c, V + w.ldv + c, 1, di);
-    for (int p = 0; p < 5; p++) blocks[5 * q + p] = vie::pass_blocks_mj<T>(ge, p, vv.var[p]);
+    for (int p = 0; p < 5; p++)
+      blocks[5 * q + p] = vie::pass_blocks_mj<T>(ge, p, s->plan->var_mj[sizeof(T) == sizeof(float)][p]);
     vie::MjStep<T> m{};
     m.c = (int)c;
     m.nv = 3 * s->nbox();
@@ -1621,7 +1672,7 @@ template <typename T> static int group_upload(StepGroup& g) {
     for (size_t q = 0; q < n; q++) {
       const vie_fft_plan* pl = g.table[q]->sys->plan;
       const vie::Geom& ge = pl->g;
-      byP[{p == 2 ? ge.Px : (p == 1 || p == 3) ? ge.Py : ge.Pz, (int)pl->var[slot][p]}].push_back((int)q);
+      byP[{p == 2 ? ge.Px : (p == 1 || p == 3) ? ge.Py : ge.Pz, (int)pl->var_mj[slot][p]}].push_back((int)q);
     }
     for (auto& kv : byP) {
       vie::MjClass c{p, kv.first.first, kv.first.second, (int)kv.second.size(), off, npre, 0};
@@ -1917,14 +1968,25 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
         // hit the FP32 floor of this system (its solution is too large against
         // the tolerance for FP32 products): the remaining cycles run in FP64.
         if (mixed && w.after_cycle && (w.breakdown || w.beta > 0.5 * w.cycle_beta)) {
-          if (!w.V.alloc((size_t)(w.m + 1) * w.ldv * sizeof(c64))) {
+          // The FP32 recycled subspace does not carry over, but the FP64
+          // cycles recycle their own: they are what the hard systems need
+          // (restarted GMRES alone can stagnate on them).  The first FP64
+          // cycle is a first cycle again: it builds its subspace from
+          // column 2 k, and a shared FP64 subspace of the system is adopted.
+          // a system FP32 could not advance is a hard one: the FP64 cycles
+          // keep the largest subspace the basis allows
+          if (w.rk_req > 0) w.rk_req = std::max(w.rk_req, std::min<int>(vie::kMaxRecycle, w.m / 2));
+          if (!w.V.alloc((size_t)(w.m + 1 + 2 * w.rk_req) * w.ldv * sizeof(c64))) {
             job_finish(w, VIE_ITERATIVE_ALLOCATION_FAILED, 0);
             return 1;
           }
           w.prec = VIE_HIP_FP64;
           w.escalated = true;
-          w.rk = 0;  // the FP32 recycled subspace does not carry over
+          w.rk = w.rk_req;
           w.rec_ready = false;
+          w.cycles = 0;
+          w.rec_used = 0;
+          w.rec_breakdown = false;
           if (trace_enabled())
             fprintf(stderr, "[vie_hip] fp32 cycle stalled at %.3e: continuing in fp64\n", rep.relative_residual);
           w.cycle_beta = w.beta;
@@ -3653,6 +3715,17 @@ extern "C" int dielectric_fft_apply_bytes(const dielectric_fft_system* s, vie_hi
     p[2] = (2 * b + oct + (g.ground ? 6.0 * (g.Px / 2 + 1) * (g.Py / 2 + 1) * g.Pz : 0.0)) * S;
     p[3] = (b + a) * S;
     p[4] = (a + 3 * N + N + 3 * N) * S;
+    // fused z+y passes move no intermediate workspace: their bytes are the
+    // fused kernel's, and the skipped y pass moves nothing
+    const unsigned char* var = s->plan->var[prec == VIE_HIP_MIXED ? 1 : 0];
+    if (var[1] == vie::kFuseSkip) {
+      p[0] = (3 * N + N + b) * S;
+      p[1] = 0;
+    }
+    if (var[3] == vie::kFuseSkip) {
+      p[3] = 0;
+      p[4] = (b + 3 * N + N + 3 * N) * S;
+    }
   }
   double c = (double)s->c, nv = s->plan ? 3.0 * s->plan->g.nbox() : 0.0;
   if (s->c) p[5] = (c * c + 2 * c * nv + (nv ? 3 * nv : 0)) * S;

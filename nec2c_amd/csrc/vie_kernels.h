@@ -35,6 +35,25 @@ int pass_launch(const VolumeView<T>& v, int pass, int variant, const cx<T>* xin,
 template <typename T> int pass_variant_count(const Geom& g, int pass);
 template <typename T> bool pass_variant_ok(const Geom& g, int pass, int variant);
 
+// Fused-pass variant codes: pass 0 (forward) or 4 (inverse) at kFuseBase +
+// log2(Py) does both of its transforms, and passes 1 and 3 at kFuseSkip then
+// launch nothing.  The y extent rides in the code so that multi-job classes
+// (keyed by pass extent and variant) hold one (Pz, Py) pair each.
+constexpr int kFuseBase = 96, kFuseSkip = 120;
+__host__ __device__ inline bool fused_extent_ok(int P) { return P == 32 || P == 64 || P == 128; }
+inline bool fused_ok(const Geom& g) {
+  return fused_extent_ok(g.Pz) && fused_extent_ok(g.Py) && 2 * g.ny <= g.Py && g.nx >= 1;
+}
+inline int fused_code(const Geom& g) { return kFuseBase + (g.Py == 32 ? 5 : g.Py == 64 ? 6 : 7); }
+
+// A fused forward (pass 0) goes with a skipped pass 1 and a fused inverse
+// (pass 4) with a skipped pass 3, and the other way round.
+inline bool variants_consistent(const unsigned char* var) {
+  const bool f0 = var[0] >= kFuseBase && var[0] != kFuseSkip, s1 = var[1] == kFuseSkip;
+  const bool f4 = var[4] >= kFuseBase && var[4] != kFuseSkip, s3 = var[3] == kFuseSkip;
+  return f0 == s1 && f4 == s3;
+}
+
 // Launch arguments of the five volume passes (one job).
 template <typename T> struct OpArgs {
   Geom g;

@@ -287,6 +287,187 @@ __global__ __launch_bounds__(16 * LW) void k_ypass_r16(OpArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------- passes 1+2 and 4+5 fused
+// For lattices whose padded y and z extents are at most 128, one workgroup
+// owns a whole x plane of one component, so the z and y transforms of that
+// plane meet in LDS and the [3][nx][ny][Pz] workspace between them is never
+// written or read (a round trip of 3 nx ny Pz values per direction, and one
+// launch fewer).  Forward: the ny rows of the plane (scatter, Jacobi scaling,
+// PEC image as pass 1) are loaded and z-transformed in LDS; then C kz columns
+// at a time are copied into a padded scratch (line stride PY + 1 against bank
+// conflicts), zero-extended to PY and y-transformed, and stored as kz-
+// contiguous rows of B.  Inverse: C kz columns of B at a time (the next
+// chunk's loads in flight) are y-transformed and their ny physical rows kept
+// in LDS; then the rows are z-transformed and gathered with the contrast as
+// pass 5.  The butterflies are those of the separate passes (same radix plan
+// and twiddles), so the products agree with them to rounding.
+// One workgroup holds a whole plane, so a plane of 128 x 128 takes the CU
+// alone: it gets 1024 threads (FP32, 32 columns per y chunk) or 512 (FP64,
+// whose plane leaves LDS room for 8 columns) to keep the CU's SIMDs busy;
+// smaller planes run 256 threads and several workgroups per CU.
+template <typename T, int PZ, int PY> struct FusedShape {
+  static constexpr bool big = PZ * PY >= 128 * 128;
+  static constexpr int C = std::is_same<T, float>::value ? (big ? 32 : 16) : 8;
+  static constexpr int WG = big ? (std::is_same<T, float>::value ? 1024 : 512) : 256;
+};
+
+template <typename T, int PZ, int PY>
+__device__ __forceinline__ void zy_fwd_body(const OpArgs<T>& a, unsigned bx) {
+  constexpr int NY = PY / 2, C = FusedShape<T, PZ, PY>::C, WG = FusedShape<T, PZ, PY>::WG, LS = PY + 1;
+  constexpr int E1 = NY * PZ, EPT1 = (E1 + WG - 1) / WG, E2 = C * PY, EPT2 = (E2 + WG - 1) / WG;
+  __shared__ cx<T> zs[E1];
+  __shared__ cx<T> ys[C * LS];
+  __shared__ cx<T> twz[PZ], twy[PY];
+  if (a.di.skipped()) return;
+  const int tid = threadIdx.x;
+  const Geom& g = a.g;
+  const int comp = (int)(bx / g.nx), x = (int)(bx % g.nx);
+  for (int i = tid; i < PZ; i += WG) twz[i] = a.tw[g.Px + g.Py + i];
+  for (int i = tid; i < PY; i += WG) twy[i] = a.tw[g.Px + i];
+  const size_t nbox = g.nbox();
+  const cx<T>* xin = a.xin + a.di.off();
+  const cx<T> sf = a.self[comp];
+  cx<T> in[EPT1];
+#pragma unroll
+  for (int i = 0; i < EPT1; i++) {
+    const int f = tid + i * WG;
+    const int y = f / PZ, e = f % PZ;
+    int z = -1;
+    bool image = false;
+    if ((E1 % WG == 0 || f < E1) && y < g.ny) {
+      if (e >= g.zoff && e - g.zoff < g.nz) z = e - g.zoff;
+      else if (g.pec && e < g.nz) {
+        z = g.nz - 1 - e;
+        image = true;
+      }
+    }
+    cx<T> v = {(T)0, (T)0};
+    if (z >= 0) {
+      const size_t cell = ((size_t)x * g.ny + y) * g.nz + z;
+      v = xin[comp * nbox + cell];
+      if (a.precond) v = jacobi_scale(v, a.eps[cell], sf);
+      if (image && comp != 2) v = cx<T>{-v.re, -v.im};
+    }
+    in[i] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < EPT1; i++) {
+    const int f = tid + i * WG;
+    if (E1 % WG == 0 || f < E1) zs[f] = in[i];  // row y at zs[y PZ]
+  }
+  __syncthreads();
+  fft_lines<T, PZ, NY, WG, 1, PZ, false>(zs, twz, tid);
+  cx<T>* out = a.B + ((size_t)comp * g.nx + x) * PY * PZ;
+  for (int k0 = 0; k0 < PZ; k0 += C) {
+#pragma unroll
+    for (int i = 0; i < EPT2; i++) {
+      const int f = tid + i * WG;
+      if (E2 % WG == 0 || f < E2) {
+        const int c = f % C, y = f / C;
+        ys[c * LS + y] = y < g.ny ? zs[y * PZ + k0 + c] : cx<T>{(T)0, (T)0};
+      }
+    }
+    __syncthreads();
+    fft_lines<T, PY, C, WG, 1, LS, true>(ys, twy, tid);
+#pragma unroll
+    for (int i = 0; i < EPT2; i++) {
+      const int f = tid + i * WG;
+      if (E2 % WG == 0 || f < E2) {
+        const int c = f % C, ky = f / C;
+        out[(size_t)ky * PZ + k0 + c] = ys[c * LS + ky];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int PY, int PZ>
+__device__ __forceinline__ void yz_inv_body(const OpArgs<T>& a, unsigned bx) {
+  constexpr int NY = PY / 2, C = FusedShape<T, PZ, PY>::C, WG = FusedShape<T, PZ, PY>::WG, LS = PY + 1;
+  constexpr int E2 = C * PY, EPT2 = (E2 + WG - 1) / WG;
+  __shared__ cx<T> zs[NY * PZ];
+  __shared__ cx<T> ys[C * LS];
+  __shared__ cx<T> twz[PZ], twy[PY];
+  if (a.di.skipped()) return;
+  const int tid = threadIdx.x;
+  const Geom& g = a.g;
+  const int comp = (int)(bx / g.nx), x = (int)(bx % g.nx);
+  for (int i = tid; i < PZ; i += WG) twz[i] = a.tw[g.Px + g.Py + i];
+  for (int i = tid; i < PY; i += WG) twy[i] = a.tw[g.Px + i];
+  const cx<T>* src = a.B + ((size_t)comp * g.nx + x) * PY * PZ;
+  // y inverse of C kz columns at a time: conj(FFT(conj B)); its conjugate is
+  // what the z inverse takes in, so the rows are kept unconjugated
+  cx<T> nxt[EPT2];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < EPT2; i++) {
+      const int f = tid + i * WG;
+      const int c = f % C, ky = f / C;
+      nxt[i] = (E2 % WG == 0 || f < E2) ? conj(src[(size_t)ky * PZ + k0 + c]) : cx<T>{(T)0, (T)0};
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < PZ; k0 += C) {
+#pragma unroll
+    for (int i = 0; i < EPT2; i++) {
+      const int f = tid + i * WG;
+      if (E2 % WG == 0 || f < E2) ys[(f % C) * LS + f / C] = nxt[i];
+    }
+    if (k0 + C < PZ) load(k0 + C);
+    __syncthreads();
+    fft_lines<T, PY, C, WG, 1, LS, true>(ys, twy, tid);
+#pragma unroll
+    for (int i = 0; i < EPT2; i++) {
+      const int f = tid + i * WG;
+      if (E2 % WG == 0 || f < E2) {
+        const int c = f % C, y = f / C;
+        if (y < g.ny) zs[y * PZ + k0 + c] = ys[c * LS + y];
+      }
+    }
+    __syncthreads();
+  }
+  // rows y >= ny are not written and not read back: they only feed lines
+  // whose results are discarded
+  fft_lines<T, PZ, NY, WG, 1, PZ, false>(zs, twz, tid);
+  // gather + contrast: y = eps z - (eps - 1) dst   (vie_fft.c:261-262)
+  const size_t nbox = g.nbox();
+  const cx<T> one = {(T)1, (T)0}, sf = a.self[comp];
+  const cx<T>* xin = a.xin + a.di.off();
+  cx<T>* yout = a.yout + a.di.off();
+  for (int f = tid; f < g.ny * g.nz; f += WG) {
+    const int y = f / g.nz, z = f % g.nz;
+    const size_t cell = ((size_t)x * g.ny + y) * g.nz + z;
+    const cx<T> e = a.eps[cell];
+    cx<T> v = xin[comp * nbox + cell];
+    if (a.precond) v = jacobi_scale(v, e, sf);
+    const cx<T> d = conj(zs[y * PZ + g.zoff + z]);
+    yout[comp * nbox + cell] = e * v - (e - one) * d;
+  }
+}
+
+template <typename T, int PZ, int PY>
+__global__ __launch_bounds__((FusedShape<T, PZ, PY>::WG)) void k_zyfwd(OpArgs<T> a) {
+  zy_fwd_body<T, PZ, PY>(a, blockIdx.x);
+}
+template <typename T, int PY, int PZ>
+__global__ __launch_bounds__((FusedShape<T, PZ, PY>::WG)) void k_yzinv(OpArgs<T> a) {
+  yz_inv_body<T, PY, PZ>(a, blockIdx.x);
+}
+template <typename T, int PZ, int PY>
+__global__ __launch_bounds__((FusedShape<T, PZ, PY>::WG)) void k_zyfwd_mj(const OpArgs<T>* __restrict__ ops,
+                                                                const int* __restrict__ list,
+                                                                const unsigned* __restrict__ pre, int count) {
+  const int i = mj_job(pre, count, blockIdx.x), q = list[i];
+  zy_fwd_body<T, PZ, PY>(op_args_global(ops[q]), blockIdx.x - pre[i]);
+}
+template <typename T, int PY, int PZ>
+__global__ __launch_bounds__((FusedShape<T, PZ, PY>::WG)) void k_yzinv_mj(const OpArgs<T>* __restrict__ ops,
+                                                                const int* __restrict__ list,
+                                                                const unsigned* __restrict__ pre, int count) {
+  const int i = mj_job(pre, count, blockIdx.x), q = list[i];
+  yz_inv_body<T, PY, PZ>(op_args_global(ops[q]), blockIdx.x - pre[i]);
+}
+
 // ---------------------------------------------------------------- pass 3
 // Lines are (ky, kz) with kz fastest.  Forward x FFT of each component is
 // staged through LDS and parked in registers; the 3x3 multiply runs in
@@ -770,6 +951,8 @@ template <typename T, int P> struct PassLauncher {
     }
   }
   static unsigned blocks_mj(int pass, int v, const Geom& g) {
+    if (v == kFuseSkip) return 0;
+    if (v >= kFuseBase) return 3u * (unsigned)g.nx;
     switch (v & 3) {
       case 1: return blocks_v<1>(pass, g);
       case 2: return blocks_v<2>(pass, g);
@@ -816,6 +999,10 @@ template <typename T, int P> struct PassLauncher {
   }
   static void launch_mj(const MjClass& c, const OpArgs<T>* ops, const unsigned* blocks, const int* lists,
                         const unsigned* pres, hipStream_t s) {
+    if (c.variant >= kFuseBase) {
+      (void)fused_launch(c.pass, c.variant, nullptr, &c, ops, lists, pres, s);
+      return;
+    }
     switch (c.variant & 3) {
       case 1: run_mj_x<1>(c, ops, blocks, lists, pres, s); return;
       case 2: run_mj_x<2>(c, ops, blocks, lists, pres, s); return;
@@ -840,7 +1027,39 @@ template <typename T, int P> struct PassLauncher {
   // y-pass variants 4 and 5: the register-resident 16 x 16 kernel (P = 256)
   // at 16 or 32 lines per workgroup in FP32, 8 or 16 in FP64
   static constexpr int kY16 = std::is_same<T, float>::value ? 16 : 8;
+  // Fused z+y passes (pass 0 forward, pass 4 inverse; P is Pz), one
+  // workgroup per (component, x plane), the y extent from the variant code.
+  template <int PY> static void run_fused(int pass, const OpArgs<T>& a, hipStream_t s) {
+    if constexpr (P == 32 || P == 64 || P == 128) {
+      const dim3 grid(3u * (unsigned)a.g.nx), blk(FusedShape<T, P, PY>::WG);
+      if (pass == 0) hipLaunchKernelGGL((k_zyfwd<T, P, PY>), grid, blk, 0, s, a);
+      else hipLaunchKernelGGL((k_yzinv<T, PY, P>), grid, blk, 0, s, a);
+    }
+  }
+  template <int PY>
+  static void run_fused_mj(const MjClass& c, const OpArgs<T>* ops, const int* lists, const unsigned* pres,
+                           hipStream_t s) {
+    if constexpr (P == 32 || P == 64 || P == 128) {
+      const dim3 grid(c.total_blocks), blk(FusedShape<T, P, PY>::WG);
+      const int* list = lists + c.offset;
+      const unsigned* pre = pres + c.pre;
+      if (c.pass == 0) hipLaunchKernelGGL((k_zyfwd_mj<T, P, PY>), grid, blk, 0, s, ops, list, pre, c.count);
+      else hipLaunchKernelGGL((k_yzinv_mj<T, PY, P>), grid, blk, 0, s, ops, list, pre, c.count);
+    }
+  }
+  static bool fused_launch(int pass, int v, const OpArgs<T>* a, const MjClass* c, const OpArgs<T>* ops,
+                           const int* lists, const unsigned* pres, hipStream_t s) {
+    if (v == kFuseSkip) return true;  // done by the fused pass 0 or 4
+    if (v < kFuseBase + 5 || v > kFuseBase + 7) return false;
+    switch (v - kFuseBase) {
+      case 5: a ? run_fused<32>(pass, *a, s) : run_fused_mj<32>(*c, ops, lists, pres, s); break;
+      case 6: a ? run_fused<64>(pass, *a, s) : run_fused_mj<64>(*c, ops, lists, pres, s); break;
+      default: a ? run_fused<128>(pass, *a, s) : run_fused_mj<128>(*c, ops, lists, pres, s); break;
+    }
+    return true;
+  }
   static void launch(int pass, int v, const OpArgs<T>& a, hipStream_t s) {
+    if (v >= kFuseBase && fused_launch(pass, v, &a, nullptr, nullptr, nullptr, nullptr, s)) return;
     if ((pass == 1 || pass == 3) && v >= 4) {
       v == 4 ? run_y16<kY16>(pass, a, s) : run_y16<2 * kY16>(pass, a, s);
       return;
@@ -907,6 +1126,10 @@ template <typename T> int pass_variant_count(const Geom& g, int pass) {
 
 template <typename T> bool pass_variant_ok(const Geom& g, int pass, int v) {
   const int P = pass_extent(g, pass);
+  // fused z+y passes: pass 0 / 4 carry both transforms, 1 / 3 launch nothing
+  // (whether the two agree is variants_consistent's to check)
+  if (v == kFuseSkip) return (pass == 1 || pass == 3) && fused_ok(g);
+  if (v >= kFuseBase) return (pass == 0 || pass == 4) && fused_ok(g) && v == fused_code(g);
   if (v < 0 || v >= pass_variant_count<T>(g, pass)) return false;
   if (pass != 2) return true;
   if (P < 16 && (v & 3) != 0) return false;
@@ -933,9 +1156,12 @@ template <typename T> unsigned pass_blocks_mj(const Geom& g, int pass, int varia
 }
 
 template <typename T> bool pass_mj_ok(const Geom& g, int pass, int variant) {
+  if (variant >= kFuseBase) return pass_variant_ok<T>(g, pass, variant);
   return pass_extent(g, pass) <= kMjMaxP && pass_variant_ok<T>(g, pass, variant) &&
          variant < (pass == 2 ? 4 * kPassVariants : kPassVariants);
 }
+
+
 
 template <typename T>
 int volume_apply_mj(const OpArgs<T>* ops, const unsigned* blocks, const int* lists, const unsigned* pres,

@@ -55,12 +55,15 @@ def system_device_bytes(shape, conductors, pec=False):
 
 def job_device_bytes(shape, conductors, restart, precision=vie.MIXED):
     """Device bytes one in-flight solve holds (Krylov basis, FP64 vectors,
-    reduction scratch); the pooled basis is FP64-sized so it serves either
+    reduction scratch).  The pooled basis is FP64-sized, so it serves either
     precision (a mixed basis with its recycled subspace, restart + 1 + 2k FP32
-    columns with k <= restart / 2, fits in it)."""
+    columns with k <= restart / 2, fits in it); a mixed solve that hands over
+    to FP64 cycles keeps a recycled subspace of up to 32 vectors in FP64
+    (krylov.h kMaxRecycle), so its basis may grow by 2 x 32 FP64 columns."""
     n = conductors + 3 * int(np.prod(shape))
     ldv = (n + 511) // 512 * 512   # krylov.h basis_ld
-    return (restart + 1) * ldv * 16 + 4 * n * 16 + KRYLOV_SCRATCH
+    extra = 2 * min(32, restart // 2) if precision == vie.MIXED else 0
+    return (restart + 1 + extra) * ldv * 16 + 4 * n * 16 + KRYLOV_SCRATCH
 
 
 @dataclass

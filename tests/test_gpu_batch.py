@@ -15,6 +15,22 @@ from nec2c_amd import vie, workloads as W
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, scope="module")
+def separate_passes():
+    """Solo solves may tune to the fused z+y passes, which batches (multi-job
+    launches) do not use; the bit-for-bit comparisons here are between the
+    multi-job and the single-job kernels of the same passes, so these
+    systems tune without fusion."""
+    import os
+    old = os.environ.get("VIE_HIP_FUSE")
+    os.environ["VIE_HIP_FUSE"] = "0"
+    yield
+    if old is None:
+        del os.environ["VIE_HIP_FUSE"]
+    else:
+        os.environ["VIE_HIP_FUSE"] = old
+
 OPTS = dict(restart=30, maximum_iterations=600, relative_tolerance=1e-8)
 
 

@@ -59,3 +59,37 @@ def test_all_pass_variants_give_identical_products(shape, pec, holes):
     print(f"{shape} PEC {pec}: {tried} variants agree, {same} products bit-identical", flush=True)
     assert tried >= 12
     s.close()
+
+
+@pytest.mark.parametrize("shape,pec,holes", [((20, 12, 16), False, True), ((9, 17, 12), True, False),
+                                             ((64, 64, 64), False, False), ((30, 40, 10), False, False),
+                                             ((7, 50, 20), True, True)])
+def test_fused_z_y_passes_give_the_same_products(shape, pec, holes):
+    """The fused z+y forward (pass 0 carries pass 1) and y+z inverse (pass 4
+    carries pass 3) for padded y and z extents up to 128: the same product as
+    the separate passes, alone and together, FP64 and FP32; a fused pass
+    without its skipped partner (or the other way round) is refused."""
+    s = system(shape, pec, holes)
+    x = np.random.default_rng(2).standard_normal(s.order) + 1j * np.random.default_rng(3).standard_normal(s.order)
+    assert s.set_variants([0] * 5, [0] * 5)
+    base = {prec: s.apply_ex(x, prec, jacobi=True) for prec in (vie.FP64, vie.MIXED)}
+    P = [1 << max(0, int(np.ceil(np.log2(2 * e - 1)))) for e in shape]
+    if pec:
+        P[2] = 1 << int(np.ceil(np.log2(2 * (2 * shape[2]) - 1)))
+    code = 96 + int(np.log2(P[1]))
+    tol = {vie.FP64: 1e-14, vie.MIXED: 1e-6}
+    tried = 0
+    for var in ([code, 120, 0, 0, 0], [0, 0, 0, 120, code], [code, 120, 0, 120, code]):
+        if not s.set_variants(var, var):
+            continue
+        tried += 1
+        for prec in (vie.FP64, vie.MIXED):
+            y = s.apply_ex(x, prec, jacobi=True)
+            err = float(np.max(np.abs(y - base[prec])) / np.max(np.abs(base[prec])))
+            print(f"{shape} PEC {pec} variants {var} precision {prec}: {err:.2e}", flush=True)
+            assert err <= tol[prec], (shape, var, prec, err)
+    assert tried == 3
+    assert not s.set_variants([code, 0, 0, 0, 0], [0] * 5)
+    assert not s.set_variants([0, 120, 0, 0, 0], [0] * 5)
+    assert not s.set_variants([0, 0, 0, 0, code], [0] * 5)
+    s.close()
