@@ -255,9 +255,13 @@ def bench_batch(args):
         systems.append(s)
     # port-major order: the first port of every model (its seed), then the
     # second ports, ...: with shared subspaces the later ports of a model
-    # start after its seed has published its recycled subspace
+    # start after its seed has published its recycled subspace.  Within a
+    # port, the largest models first (longest jobs first: the batch drains
+    # with small jobs, not with a few large ones running alone).
+    by_size = sorted(range(len(cases)), key=lambda i: -int(np.prod(cases[i].shape)))
     for port in range(max(len(c.ports) for c in cases)):
-        for case, s in zip(cases, systems):
+        for i in by_size:
+            case, s = cases[i], systems[i]
             if port < len(case.ports):
                 b = np.zeros(s.order, dtype=np.complex128)
                 b[: case.dipole.segments] = case.rhs(port)
