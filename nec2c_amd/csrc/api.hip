@@ -329,7 +329,10 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
   memset(&key, 0, sizeof(key));
   const char* fz = getenv("VIE_HIP_FUSE");
   const bool fuse = !(fz && !strcmp(fz, "0"));
-  key = TuneKey{slot, p->g.nx, p->g.ny, p->g.nz, p->g.Px, p->g.Py, p->g.Pz, p->g.pec + 2 * p->g.ground, fuse};
+  const char* mt = getenv("VIE_HIP_MJ_TUNED");
+  const bool mj_tuned = mt && !strcmp(mt, "1");
+  key = TuneKey{slot, p->g.nx, p->g.ny, p->g.nz, p->g.Px, p->g.Py, p->g.Pz, p->g.pec + 2 * p->g.ground,
+                fuse + 2 * mj_tuned};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -380,7 +383,17 @@ template <typename T> static int plan_tune(vie_fft_plan* p) {
     }
     p->var[slot][pass] = (unsigned char)best_v;
   }
+  // Multi-job launches take shape 1 (the widest tile's elements per thread
+  // at 128 threads) where it applies: their grids are the sum of many jobs',
+  // so smaller workgroups pack the CUs better than the shapes tuned on a
+  // lone job (C4: 2.37 against 2.45 s mixed and 3.80 against 3.92 s FP64;
+  // shape 0 measured 2.35 / 3.98, shape 2 2.61 / 4.02, shape 3 2.44 / 4.31).
+  // VIE_HIP_MJ_TUNED=1 keeps the solo-tuned shapes (so that batched and solo
+  // solves run the same arithmetic, as tests compare them bit for bit).
   memcpy(p->var_mj[slot], p->var[slot], 5);
+  if (!mj_tuned)
+    for (int pass = 0; pass < 5; pass++)
+      if (vie::pass_mj_ok<T>(p->g, pass, 1)) p->var_mj[slot][pass] = 1;
   // the fused z+y passes against the chosen pass pairs (0 + 1 forward,
   // 3 + 4 inverse): the fused one must win by 2% as well (VIE_HIP_FUSE=0
   // keeps the separate passes)

@@ -18,18 +18,21 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(autouse=True, scope="module")
 def separate_passes():
-    """Solo solves may tune to the fused z+y passes, which batches (multi-job
-    launches) do not use; the bit-for-bit comparisons here are between the
-    multi-job and the single-job kernels of the same passes, so these
-    systems tune without fusion."""
+    """Solo solves may tune to the fused z+y passes, and batches (multi-job
+    launches) run their own launch shape; the bit-for-bit comparisons here
+    are between the multi-job and the single-job kernels of the same passes
+    and shapes, so these systems tune without fusion and batches keep the
+    solo-tuned shapes."""
     import os
-    old = os.environ.get("VIE_HIP_FUSE")
-    os.environ["VIE_HIP_FUSE"] = "0"
+    keys = {"VIE_HIP_FUSE": "0", "VIE_HIP_MJ_TUNED": "1"}
+    old = {k: os.environ.get(k) for k in keys}
+    os.environ.update(keys)
     yield
-    if old is None:
-        del os.environ["VIE_HIP_FUSE"]
-    else:
-        os.environ["VIE_HIP_FUSE"] = old
+    for k, v in old.items():
+        if v is None:
+            del os.environ[k]
+        else:
+            os.environ[k] = v
 
 OPTS = dict(restart=30, maximum_iterations=600, relative_tolerance=1e-8)
 
