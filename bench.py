@@ -272,7 +272,7 @@ def bench_batch(args):
     log(f"batch: host inputs {t_host:.1f} s, device setup (spectra, blocks, tuning) {t_setup:.1f} s")
     biggest = max(systems, key=lambda q: q.order)
     out, group_steps, shared = {}, {}, {}
-    k_fp64 = args.recycle if args.recycle > 0 else 20
+    k_fp64 = args.recycle if args.recycle > 0 else 16
     runs = (("mixed", vie.MIXED, True, args.recycle),          # the batched form: shared recycled subspaces
             ("fp64", vie.FP64, False, 0),                       # the reference's GMRES, one job per port
             ("fp64_shared", vie.FP64, True, k_fp64),            # FP64 with the same shared subspaces
@@ -629,7 +629,7 @@ def bench_slab(args):
     t_fp64, lat_fp64, rep_fp64 = timed(vie.FP64, 0, fsteps, 1)
     x_fp64 = s.download_solution()
     log(f"slab fp64: {rep_fp64[0].iterations} iterations, {t_fp64 / fsteps:.3f} s per solve")
-    k_r = args.recycle if args.recycle > 0 else 20
+    k_r = args.recycle if args.recycle > 0 else 16
     t_fp64r, _, rep_fp64r = timed(vie.FP64, k_r, fsteps, 1)
     log(f"slab fp64 recycled: {rep_fp64r[0].iterations} iterations, {t_fp64r / fsteps:.3f} s per solve")
 
@@ -1368,7 +1368,7 @@ def bench_sweep(args):
     t_fp64, lat_fp64, rep_fp64 = run(vie.FP64)
     x_fp64 = [s.download_solution() for s in timed]
     # the same recycled cycles in FP64, to separate the precision gain from the recycling gain
-    k_fp64r = args.recycle if args.recycle > 0 else 20
+    k_fp64r = args.recycle if args.recycle > 0 else 16
     log(f"timing FP64 solves with a recycled subspace of {k_fp64r}")
     for q in timed:  # untimed: sizes each basis for FP64 plus the recycled columns
         q.solve_resident(vie.FP64, 0.0, k_fp64r, **dict(opts, relative_tolerance=1.0))

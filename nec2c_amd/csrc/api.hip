@@ -1287,7 +1287,7 @@ static size_t solver_bytes(size_t n, int m, int prec, int rk = 0) {
 // Recycled vectors a solve keeps: the option's k (0: default per precision,
 // -1: none), capped at kMaxRecycle and at half the restart so every later
 // cycle still has a Krylov space of its own.
-constexpr int kDefaultRecycleMixed = 20;
+constexpr int kDefaultRecycleMixed = 16;
 static int recycle_count(const vie_hip_solve_options* so, int prec, int m) {
   int k = so ? so->recycle : 0;
   if (k < 0) return 0;

@@ -93,8 +93,8 @@ class Server:
         self.opts = dict(restart=restart, maximum_iterations=maximum_iterations,
                          relative_tolerance=relative_tolerance)
         self.restart = restart
-        # recycled vectors of a solve (the library's default: 20 for MIXED, none for FP64)
-        self.rk = recycle if recycle > 0 else (20 if precision == vie.MIXED and recycle == 0 else 0)
+        # recycled vectors of a solve (the library's default: 16 for MIXED, none for FP64)
+        self.rk = recycle if recycle > 0 else (16 if precision == vie.MIXED and recycle == 0 else 0)
         self.share = share and self.rk > 0
         self.seed_first = seed_first and self.share
         self.batch = vie.Batch(share_subspaces=self.share)
