@@ -246,7 +246,7 @@ def bench_batch(args):
     with ThreadPoolExecutor(16) as ex:
         inputs = list(ex.map(_case_blocks, cases))
     t_host = time.perf_counter() - t0
-    systems, jobs, feeds = [], [], []
+    systems, jobs, feeds, job_case = [], [], [], []
     t1 = time.perf_counter()
     for case, (centers, C, B, D) in zip(cases, inputs):
         s = vie.System(centers, [vie.MaterialSpec(1, case.eps_r)], case.dipole.segments, case.k0, pitch=case.pitch)
@@ -267,7 +267,13 @@ def bench_batch(args):
                 b[: case.dipole.segments] = case.rhs(port)
                 jobs.append((s, b))
                 feeds.append(case.ports[port])
+                job_case.append(i)
     t_setup = time.perf_counter() - t1
+    # the CPU baseline's sample of jobs keeps its models' host blocks
+    sample = [] if args.no_cpu else sorted(np.random.default_rng(7).choice(len(jobs), size=min(args.batch_cpu_jobs,
+                                                                                                  len(jobs)),
+                                                                        replace=False).tolist())
+    kept = {job_case[j]: inputs[job_case[j]][1:] for j in sample}
     del inputs
     log(f"batch: host inputs {t_host:.1f} s, device setup (spectra, blocks, tuning) {t_setup:.1f} s")
     biggest = max(systems, key=lambda q: q.order)
@@ -345,10 +351,71 @@ def bench_batch(args):
         "roofline": None,
         "cpu_baseline": None,
     }
+    if sample:
+        cpu, check = batch_cpu_baseline(cases, kept, jobs, job_case, feeds, sample, rm, rd, args.restart)
+        result["cpu_baseline"], result["oracle_check"] = cpu, check
     print(json.dumps(result), flush=True)
     for s in systems:
         s.close()
     return 0
+
+
+def batch_cpu_baseline(cases, blocks, jobs, job_case, feeds, sample, res_mixed, res_fp64, restart):
+    """The reference's own FFT operator and vie_gmres (oracle/_ref, OpenMP)
+    solving a seeded random sample of the batch's jobs to 1e-8, one at a
+    time: solves/s over the sample's GMRES time (plan creation excluded, as
+    the GPU value excludes its setup).  The same run checks the sampled jobs'
+    GPU solutions: their true residual on the reference operator and their
+    port impedance against the reference's solution."""
+    from oracle import oracle as O
+    if not O.reference_available():
+        return None, None
+    R = O.ReferenceLibrary()
+    cores = R.threads(0)
+    t_solve, t_plan, iters = 0.0, 0.0, 0
+    worst = {"residual_on_reference_operator_mixed": 0.0, "residual_on_reference_operator_fp64": 0.0,
+             "port_impedance_rel_err_mixed_vs_reference": 0.0, "port_impedance_rel_err_fp64_vs_reference": 0.0}
+    cells = []
+    for j in sample:
+        case = cases[job_case[j]]
+        C, B, D = blocks[job_case[j]]
+        b = jobs[j][1]
+        xyz = np.stack(np.meshgrid(*[np.arange(v) for v in case.shape], indexing="ij"), -1).reshape(-1, 3)
+        t0 = time.perf_counter()
+        plan = R.plan(case.shape, xyz, case.pitch, case.k0, complex(case.eps_r))
+        op = plan.coupled(C, B, D)
+        t1 = time.perf_counter()
+        ok, xr, rep = op.gmres(b, restart=restart, maxit=2000, rtol=1e-8)
+        t2 = time.perf_counter()
+        if not ok:
+            log(f"batch cpu baseline: reference solve of job {j} did not converge")
+            return None, None
+        t_plan += t1 - t0
+        t_solve += t2 - t1
+        iters += rep.iterations
+        cells.append(int(np.prod(case.shape)))
+        bn = float(np.linalg.norm(b))
+        f = feeds[j]
+        for tag, res in (("mixed", res_mixed), ("fp64", res_fp64)):
+            x = res[j][1]
+            worst[f"residual_on_reference_operator_{tag}"] = max(
+                worst[f"residual_on_reference_operator_{tag}"], float(np.linalg.norm(b - op.apply(x)) / bn))
+            worst[f"port_impedance_rel_err_{tag}_vs_reference"] = max(
+                worst[f"port_impedance_rel_err_{tag}_vs_reference"],
+                float(abs(1 / x[f] - 1 / xr[f]) / abs(1 / xr[f])))
+        del op, plan
+    log("batch reference check: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()))
+    cpu = {
+        "value": len(sample) / t_solve,
+        "unit": "solves/s",
+        "cores": cores,
+        "kind": "reference",
+        "sample": (f"{len(sample)} of the {len(jobs)} jobs (seeded uniform draw; models of {min(cells)}-{max(cells)} "
+                   f"cells) solved one at a time by the reference's vie_gmres({restart}) on its FFT operator "
+                   f"(builtin FFT, OpenMP, FP64) to 1e-8: {iters} iterations in {t_solve:.2f} s of GMRES; plan "
+                   f"creation ({t_plan:.2f} s) excluded like the GPU's setup"),
+    }
+    return cpu, {"jobs_checked": len(sample), **worst}
 
 
 def bench_serve(args):
@@ -1273,6 +1340,7 @@ def main():
                     help="matvec: also run the Arnoldi step kernels at this basis column (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256, help="batch: number of models")
     ap.add_argument("--inflight", type=int, default=96, help="batch: jobs on the device at once")
+    ap.add_argument("--batch-cpu-jobs", type=int, default=24, help="batch: jobs in the CPU-baseline sample")
     ap.add_argument("--batch-kh", type=float, default=0.1,
                     help="batch: k0 sqrt(eps_r) h of every model (the workload bounds it by 0.5)")
     ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")

