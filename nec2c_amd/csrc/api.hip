@@ -773,6 +773,7 @@ struct SolverWork {
   bool after_cycle = false, breakdown = false, cycle_done = false, dcgs2 = true;
   double cycle_beta = 0;    // true residual at the start of the current cycle
   bool escalated = false;   // a MIXED solve that switched to FP64 cycles
+  bool resid_known = false; // the residual of PH_RESID is already in pin->scal[2]
   double col_sum = 0;       // sum of basis columns j over the matvec Arnoldi steps issued
   unsigned long long col_steps = 0;
   double bnorm = 0, target = 0, beta = 0, reduction = 1e-5, t_submit = 0, t_end = 0;
@@ -1355,7 +1356,7 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   w.reduction = so->inner_reduction > 0 ? so->inner_reduction : (w.rk ? 1e-4 : 1e-5);
   w.cyc_base = w.cycles = w.rec_used = 0;
   w.rec_breakdown = w.rec_ready = false;
-  w.after_cycle = w.breakdown = w.cycle_done = w.escalated = false;
+  w.after_cycle = w.breakdown = w.cycle_done = w.escalated = w.resid_known = false;
   w.cycle_beta = INFINITY;
   w.result = 0;
   w.t_submit = now_s();
@@ -1933,6 +1934,22 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           job_finish(w, VIE_ITERATIVE_NONFINITE, 0);
           return 1;
         }
+        if (xnorm == 0) {
+          // x = 0 exactly: r = b and ||r|| = ||b|| (the norm kernels sum in
+          // the same order), so the start needs no operator application and
+          // no second round trip
+          // (the cycle start reads ||r||^2 from the device copy, scal[2])
+          if (hipMemcpyAsync(w.r.p, w.b.p, w.n * sizeof(c64), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(w.scal.as<c64>() + 2, w.scal.as<c64>(), sizeof(c64), hipMemcpyDeviceToDevice, st) !=
+                  hipSuccess) {
+            job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+            return 1;
+          }
+          w.pin->scal[2] = w.pin->scal[0];
+          w.phase = PH_RESID;
+          w.resid_known = true;
+          break;
+        }
         if (!job_issue_residual(w)) {
           job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
           return 1;
@@ -1941,11 +1958,14 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
         break;
       }
       case PH_RESID: {
-        int r = job_ready(w.ev_misc, blocking);
-        if (r <= 0) {
-          if (r < 0) job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
-          return r < 0;
+        if (!w.resid_known) {
+          int r = job_ready(w.ev_misc, blocking);
+          if (r <= 0) {
+            if (r < 0) job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
+            return r < 0;
+          }
         }
+        w.resid_known = false;
         w.beta = sqrt(w.pin->scal[2].re);
         rep.residual_norm = w.beta;
         if (!std::isfinite(w.beta)) {
