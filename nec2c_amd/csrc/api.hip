@@ -774,6 +774,7 @@ struct SolverWork {
   double cycle_beta = 0;    // true residual at the start of the current cycle
   bool escalated = false;   // a MIXED solve that switched to FP64 cycles
   bool resid_known = false; // the residual of PH_RESID is already in pin->scal[2]
+  bool check_singular = false;  // PH_RESID follows a cycle end: test its back substitution first
   double col_sum = 0;       // sum of basis columns j over the matvec Arnoldi steps issued
   unsigned long long col_steps = 0;
   double bnorm = 0, target = 0, beta = 0, reduction = 1e-5, t_submit = 0, t_end = 0;
@@ -1306,7 +1307,7 @@ static int recycle_count(const vie_hip_solve_options* so, int prec, int m) {
 // true residual acceptance).  MIXED: the same outer loop with FP32 cycles that
 // stop once |g| <= max(target, inner_reduction * beta); each restart's FP64
 // true residual refines the solution.
-enum { PH_IDLE, PH_NORMS, PH_RESID, PH_CYCLE, PH_BACKSOLVE, PH_FINISHED };
+enum { PH_IDLE, PH_NORMS, PH_RESID, PH_CYCLE, PH_FINISHED };
 
 static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -1356,7 +1357,7 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   w.reduction = so->inner_reduction > 0 ? so->inner_reduction : (w.rk ? 1e-4 : 1e-5);
   w.cyc_base = w.cycles = w.rec_used = 0;
   w.rec_breakdown = w.rec_ready = false;
-  w.after_cycle = w.breakdown = w.cycle_done = w.escalated = w.resid_known = false;
+  w.after_cycle = w.breakdown = w.cycle_done = w.escalated = w.resid_known = w.check_singular = false;
   w.cycle_beta = INFINITY;
   w.result = 0;
   w.t_submit = now_s();
@@ -1966,6 +1967,14 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           }
         }
         w.resid_known = false;
+        if (w.check_singular) {
+          w.check_singular = false;
+          if (w.pin->fin.singular) {
+            rep.operator_applications--;  // the residual of the unchanged x was not needed
+            job_finish(w, VIE_ITERATIVE_BREAKDOWN, 0);
+            return 1;
+          }
+        }
         w.beta = sqrt(w.pin->scal[2].re);
         rep.residual_norm = w.beta;
         if (!std::isfinite(w.beta)) {
@@ -2121,20 +2130,10 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
                                   sizeof(double), hipMemcpyDeviceToHost, st));
           }
         }
-        HIP_OK(hipEventRecord(w.ev_misc, st));
-        w.phase = PH_BACKSOLVE;
-        break;
-      }
-      case PH_BACKSOLVE: {
-        int r = job_ready(w.ev_misc, blocking);
-        if (r <= 0) {
-          if (r < 0) job_finish(w, VIE_ITERATIVE_CALLBACK_FAILED, 0);
-          return r < 0;
-        }
-        if (w.pin->fin.singular) {
-          job_finish(w, VIE_ITERATIVE_BREAKDOWN, 0);
-          return 1;
-        }
+        // The solution update and the true residual are queued behind the
+        // back substitution at once (one host round trip per cycle end):
+        // k_combine leaves x as it is when the back substitution found no
+        // least-squares solution, and PH_RESID reports that case first.
         vie::combine<T>(w.x.as<c64>(), cycle_basis<T>(w), w.ldv, w.n, sd, jacobi_view(s), 1, st,
                         w.cyc_base ? w.V.as<vie::cx<T>>() : nullptr);
         if (!job_issue_residual(w)) {
@@ -2142,6 +2141,7 @@ template <typename T> static int job_advance(SolverWork& w, bool blocking) {
           return 1;
         }
         w.after_cycle = true;
+        w.check_singular = true;
         w.phase = PH_RESID;
         break;
       }
