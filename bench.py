@@ -499,7 +499,8 @@ def bench_serve(args):
             "workload": f"served: {args.requests} Poisson requests ({args.rate:g}/s, seed 5), grids 16^3-96^3 "
                         "log-uniform, random eps_r fields (as the sweep), 100-400 MHz, 1-4 ports, 30% key reuse, "
                         f"mixed GMRES({args.restart}) to 1e-8, HBM budget {budget / 1e9:.0f} GB; each model's "
-                        "recycled subspace shared by its ports and later requests (a new model's first port first)",
+                        "recycled subspace shared by its ports and later requests (a new model's first port first); "
+                        "new systems run the default launch shapes (no per-system tuning)",
             "distinct_models": len(keys),
         },
         "serve": rep,

@@ -19,6 +19,9 @@ one or more feed ports (one right-hand side each).  The server
   for the same cached model start from it; a request whose model has no
   subspace yet runs its first port alone and submits the others when it
   finishes (``seed_first``);
+* runs new systems with the default launch shapes instead of tuning them
+  (``tune``; a served model gets a few solves, fewer than tuning needs to
+  pay for itself);
 * records per-request latency (arrival -> last port finished) and service time
   (admission -> last port finished).
 
@@ -66,6 +69,17 @@ def job_device_bytes(shape, conductors, restart, precision=vie.MIXED):
     return (restart + 1 + extra) * ldv * 16 + 4 * n * 16 + KRYLOV_SCRATCH
 
 
+def ready_system(system, tune=False):
+    """What the server does to a freshly built system before its first job:
+    without `tune` it fixes the default launch shape of every pass
+    (vie_fft_set_variants with variant 0), so the system's first solve runs
+    no tuning timings.  A served model gets a few solves, and tuning one costs
+    more than it saves over them."""
+    if not tune and hasattr(system, "set_variants"):
+        system.set_variants([0] * 5, [0] * 5)
+    return system
+
+
 @dataclass
 class Request:
     rid: int
@@ -85,7 +99,7 @@ class Request:
 class Server:
     def __init__(self, budget_bytes: int, precision=vie.MIXED, restart=120, maximum_iterations=2000,
                  relative_tolerance=1e-8, inner_reduction=0.0, recycle=0, share=True, seed_first=True,
-                 clock=time.perf_counter):
+                 tune=False, clock=time.perf_counter):
         self.budget = int(budget_bytes)
         self.precision = precision
         self.inner_reduction = inner_reduction
@@ -111,6 +125,7 @@ class Server:
         self.t0 = None
         self.builds = 0
         self.build_seconds = 0.0
+        self.tune = tune         # tune each new system's passes (see ready_system)
 
     # ---------------------------------------------------------------- helpers
     def now(self):
@@ -151,7 +166,7 @@ class Server:
         entry = self.cache.get(req.key)
         if entry is None:
             t = self.clock()
-            s = req.make_system()
+            s = ready_system(req.make_system(), self.tune)
             self.build_seconds += self.clock() - t
             self.builds += 1
             entry = [s, self._system_bytes(req), 0, False]

@@ -88,7 +88,7 @@ def alone(workload):
     keys, spec, arrivals, data = workload
     out = {}
     for key in keys:
-        s = make_system(data[key])
+        s = serve.ready_system(make_system(data[key]))  # as the server readies its systems
         for p in sorted({p for k, ports in spec if k == key for p in ports}):
             batch = vie.Batch()
             jid = batch.submit(s, rhs_for(key, p), precision=vie.MIXED, **OPTS)
