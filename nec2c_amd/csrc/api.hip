@@ -1341,8 +1341,10 @@ static int job_issue_residual(SolverWork& w) {
   return 1;
 }
 
-// Queues the first work of a solve whose b and x are staged in w.
-static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_solve_options* so) {
+// Queues the first work of a solve whose b and x are staged in w (x_zero:
+// the caller has just zeroed x, so its norm is not computed).
+static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_solve_options* so,
+                     bool x_zero = false) {
   dielectric_fft_system* s = w.sys;
   memset(&w.rep, 0, sizeof(w.rep));
   w.rep.status = VIE_ITERATIVE_INVALID_INPUT;
@@ -1367,8 +1369,13 @@ static int job_start(SolverWork& w, const vie_gmres_options* o, const vie_hip_so
   HIP_OK(hipMemcpyAsync((char*)w.state.p + offsetof(vie::ArnoldiState, m), &mm, sizeof(int), hipMemcpyHostToDevice,
                         s->stream));
   vie::norm2<double>(w.b.as<c64>(), w.n, rs, w.scal.as<c64>() + 0, s->stream);
-  vie::norm2<double>(w.x.as<c64>(), w.n, rs, w.scal.as<c64>() + 1, s->stream);
-  HIP_OK(hipMemcpyAsync(&w.pin->scal[0], w.scal.p, 2 * sizeof(c64), hipMemcpyDeviceToHost, s->stream));
+  if (x_zero) {
+    w.pin->scal[1] = c64{0, 0};
+    HIP_OK(hipMemcpyAsync(&w.pin->scal[0], w.scal.p, sizeof(c64), hipMemcpyDeviceToHost, s->stream));
+  } else {
+    vie::norm2<double>(w.x.as<c64>(), w.n, rs, w.scal.as<c64>() + 1, s->stream);
+    HIP_OK(hipMemcpyAsync(&w.pin->scal[0], w.scal.p, 2 * sizeof(c64), hipMemcpyDeviceToHost, s->stream));
+  }
   HIP_OK(hipEventRecord(w.ev_misc, s->stream));
   w.phase = PH_NORMS;
   return 1;
@@ -3927,7 +3934,7 @@ extern "C" long long vie_hip_batch_submit(vie_hip_batch* b, dielectric_fft_syste
   vie::to_box<double>(w.ax.as<c64>(), s->c, s->cells, s->plan ? s->plan->map_dev.as<size_t>() : nullptr, s->nbox(),
                       w.b.as<c64>(), st);
   vie::vec_zero<double>(w.x.as<c64>(), n, st);
-  if (!job_start(w, options, so)) {
+  if (!job_start(w, options, so, true)) {
     b->pool.push_back(std::move(job.w));
     return 0;
   }
